@@ -1,0 +1,268 @@
+"""ctypes binding of ``libanuga_sw.so`` (declarations in ``include/anuga_sw.h``).
+
+The library is built in-tree by ``anuga_cuda_amd/csrc/build.sh``.  There is no
+CPU fallback: if the shared library is missing or a GPU is not present, the
+first use raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libanuga_sw.so")
+
+SW_STAGE, SW_XMOM, SW_YMOM, SW_ELEVATION, SW_FRICTION, SW_HEIGHT, SW_XVELOCITY, SW_YVELOCITY = range(8)
+SW_CENTROID, SW_EDGE, SW_VERTEX, SW_BOUNDARY, SW_EXPLICIT_UPDATE, SW_SEMI_IMPLICIT_UPDATE = range(6)
+SW_BC_REFLECTIVE, SW_BC_DIRICHLET, SW_BC_TRANSMISSIVE = range(3)
+SW_PRECISION_FP64, SW_PRECISION_MIXED = 0, 1
+SW_EVENT_NONE, SW_EVENT_YIELD, SW_EVENT_FINAL = 0, 1, 2
+
+SW_ERR_UPDATE = -1
+SW_ERR_SMALLSTEPS = -2
+SW_ERR_NEIGHBOUR = -3
+
+
+class SwParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "g", "epsilon", "H0", "minimum_allowed_height", "maximum_allowed_speed",
+        "beta_w", "beta_w_dry", "beta_uh", "beta_uh_dry", "beta_vh", "beta_vh_dry",
+        "alpha_balance", "CFL", "evolve_max_timestep", "evolve_min_timestep")] + [
+        (n, C.c_int32) for n in (
+            "optimise_dry_cells", "extrapolate_velocity_second_order",
+            "tight_slope_limiters", "use_centroid_velocities", "use_sloped_mannings",
+            "max_smallsteps", "default_order", "precision")]
+
+
+_I64P = C.POINTER(C.c_int64)
+_F64P = C.POINTER(C.c_double)
+
+
+class SwMesh(C.Structure):
+    _fields_ = [
+        ("number_of_elements", C.c_int64),
+        ("number_of_boundary_elements", C.c_int64),
+        ("neighbours", _I64P),
+        ("neighbour_edges", _I64P),
+        ("surrogate_neighbours", _I64P),
+        ("number_of_boundaries", _I64P),
+        ("tri_full_flag", _I64P),
+        ("normals", _F64P),
+        ("edgelengths", _F64P),
+        ("radii", _F64P),
+        ("areas", _F64P),
+        ("centroid_coordinates", _F64P),
+        ("vertex_coordinates", _F64P),
+        ("boundary_cells", _I64P),
+        ("boundary_edges", _I64P),
+    ]
+
+
+class SwState(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "time", "starttime", "finaltime", "yieldtime", "yieldstep", "timestep", "flux_timestep",
+        "recorded_max_timestep", "recorded_min_timestep")] + [
+        (n, C.c_int64) for n in (
+            "smallsteps", "order", "number_of_steps", "number_of_first_order_steps",
+            "has_finaltime", "event", "error")]
+
+
+class SwError(RuntimeError):
+    """Error reported by the HIP library (message from sw_last_error)."""
+
+    def __init__(self, code, message):
+        super().__init__("anuga_sw error %d: %s" % (code, message))
+        self.code = code
+
+
+_lib = None
+
+# (name, argtypes) of every exported entry point
+_SIGNATURES = {
+    "sw_create": ([C.POINTER(SwMesh), C.POINTER(SwParams), C.c_int], C.c_void_p),
+    "sw_destroy": ([C.c_void_p], C.c_int),
+    "sw_last_error": ([C.c_void_p], C.c_char_p),
+    "sw_set_params": ([C.c_void_p, C.POINTER(SwParams)], C.c_int),
+    "sw_get_params": ([C.c_void_p, C.POINTER(SwParams)], C.c_int),
+    "sw_set_boundary": ([C.c_void_p, C.POINTER(C.c_int32), _F64P], C.c_int),
+    "sw_upload": ([C.c_void_p, C.c_int, C.c_int, _F64P, C.c_int64], C.c_int),
+    "sw_download": ([C.c_void_p, C.c_int, C.c_int, _F64P, C.c_int64], C.c_int),
+    "sw_get_state": ([C.c_void_p, C.POINTER(SwState)], C.c_int),
+    "sw_set_state": ([C.c_void_p, C.POINTER(SwState)], C.c_int),
+    "sw_download_max_speed": ([C.c_void_p, _F64P, C.c_int64], C.c_int),
+    "sw_compute_fluxes": ([C.c_void_p, _F64P], C.c_int),
+    "sw_gravity_wb": ([C.c_void_p], C.c_int),
+    "sw_manning_friction": ([C.c_void_p], C.c_int),
+    "sw_update_timestep": ([C.c_void_p, C.c_double, C.c_double, C.c_int32], C.c_int),
+    "sw_update_conserved_quantities": ([C.c_void_p], C.c_int),
+    "sw_protect": ([C.c_void_p], C.c_int),
+    "sw_extrapolate_second_order_sw": ([C.c_void_p], C.c_int),
+    "sw_extrapolate_first_order": ([C.c_void_p], C.c_int),
+    "sw_balance_deep_and_shallow": ([C.c_void_p], C.c_int),
+    "sw_interpolate_from_vertices_to_edges": ([C.c_void_p], C.c_int),
+    "sw_distribute_to_vertices_and_edges": ([C.c_void_p], C.c_int),
+    "sw_update_boundary": ([C.c_void_p], C.c_int),
+    "sw_update_other_quantities": ([C.c_void_p], C.c_int),
+    "sw_evolve_prepare": ([C.c_void_p, C.c_double, C.c_double, C.c_int32], C.c_int),
+    "sw_evolve_steps": ([C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32)], C.c_int),
+    "sw_advance_yield": ([C.c_void_p], C.c_int),
+    "sw_evolve_one_step_unfused": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
+    "sw_synchronize": ([C.c_void_p], C.c_int),
+    "sw_enable_timing": ([C.c_void_p, C.c_int32], C.c_int),
+    "sw_kernel_times": ([C.c_void_p, _F64P, _I64P, C.c_int32], C.c_int),
+}
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libanuga_sw.so not built (run anuga_cuda_amd/csrc/build.sh or "
+            "__graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    for name, (argtypes, restype) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+def f64p(a):
+    return a.ctypes.data_as(_F64P)
+
+
+def i64p(a):
+    return a.ctypes.data_as(_I64P)
+
+
+class Handle:
+    """Owner of one ``sw_handle``: device buffers of one domain."""
+
+    def __init__(self, mesh, params: SwParams, device=0):
+        self.lib = load()
+        self._keep = {}
+        m = SwMesh()
+        m.number_of_elements = mesh.number_of_elements
+        m.number_of_boundary_elements = mesh.number_of_boundary_elements
+
+        def ia(name, arr):
+            a = np.ascontiguousarray(arr, dtype=np.int64)
+            if a.size == 0:
+                a = np.zeros(1, dtype=np.int64)
+            self._keep[name] = a
+            setattr(m, name, i64p(a))
+
+        def fa(name, arr):
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            self._keep[name] = a
+            setattr(m, name, f64p(a))
+
+        ia("neighbours", mesh.neighbours)
+        ia("neighbour_edges", mesh.neighbour_edges)
+        ia("surrogate_neighbours", mesh.surrogate_neighbours)
+        ia("number_of_boundaries", mesh.number_of_boundaries)
+        ia("tri_full_flag", mesh.tri_full_flag)
+        fa("normals", mesh.normals)
+        fa("edgelengths", mesh.edgelengths)
+        fa("radii", mesh.radii)
+        fa("areas", mesh.areas)
+        fa("centroid_coordinates", mesh.centroid_coordinates)
+        fa("vertex_coordinates", mesh.vertex_coordinates)
+        ia("boundary_cells", mesh.boundary_cells)
+        ia("boundary_edges", mesh.boundary_edges)
+        self.params = params
+        h = self.lib.sw_create(C.byref(m), C.byref(params), int(device))
+        self._keep.clear()
+        if not h:
+            raise SwError(-10, self.lib.sw_last_error(None).decode())
+        self.h = C.c_void_p(h)
+        self.N = mesh.number_of_elements
+        self.Nb = mesh.number_of_boundary_elements
+
+    # ------------------------------------------------------------------
+    def check(self, rc):
+        if rc != 0:
+            raise SwError(rc, self.lib.sw_last_error(self.h).decode())
+
+    def call(self, name, *args):
+        self.check(getattr(self.lib, name)(self.h, *args))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        self.params = params
+        self.call("sw_set_params", C.byref(params))
+
+    def set_boundary(self, kinds, dirichlet):
+        k = np.ascontiguousarray(kinds, dtype=np.int32)
+        d = np.ascontiguousarray(dirichlet, dtype=np.float64)
+        if k.size == 0:
+            return
+        self.call("sw_set_boundary", k.ctypes.data_as(C.POINTER(C.c_int32)), f64p(d))
+
+    def upload(self, q, loc, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if a.size == 0:
+            return
+        self.call("sw_upload", int(q), int(loc), f64p(a), a.size)
+
+    def download(self, q, loc, out):
+        if out.size == 0:
+            return out
+        assert out.flags.c_contiguous and out.dtype == np.float64
+        self.call("sw_download", int(q), int(loc), f64p(out), out.size)
+        return out
+
+    def get_state(self):
+        s = SwState()
+        self.call("sw_get_state", C.byref(s))
+        return s
+
+    def set_state(self, s):
+        self.call("sw_set_state", C.byref(s))
+
+    def max_speed(self):
+        out = np.empty(self.N)
+        self.call("sw_download_max_speed", f64p(out), out.size)
+        return out
+
+    def compute_fluxes(self):
+        ts = C.c_double()
+        self.call("sw_compute_fluxes", C.byref(ts))
+        return ts.value
+
+    def evolve_steps(self, max_steps):
+        done = C.c_int64()
+        ev = C.c_int32()
+        self.call("sw_evolve_steps", int(max_steps), C.byref(done), C.byref(ev))
+        return done.value, ev.value
+
+    def evolve_one_step_unfused(self):
+        ev = C.c_int32()
+        self.call("sw_evolve_one_step_unfused", C.byref(ev))
+        return ev.value
+
+    def kernel_times(self, n=4):
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        self.call("sw_kernel_times", f64p(ms), i64p(cnt), n)
+        return ms, cnt

@@ -1,0 +1,16 @@
+#!/bin/bash
+# Builds libanuga_sw.so (gfx950) in-tree from sw_capi.hip.
+# -ffp-contract=off: no FMA contraction, so FP64 results follow the C oracle's
+#                    operation order bit for bit.
+# -packed-fp32-ops disabled: hipcc must not emit packed FP32 instructions.
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+OUT="${HERE}/../libanuga_sw.so"
+ARCH="${SW_OFFLOAD_ARCH:-gfx950}"
+/opt/rocm/bin/hipcc --offload-arch="${ARCH}" -O3 -std=c++17 -fPIC -shared \
+    -ffp-contract=off -fno-fast-math \
+    -Xclang -target-feature -Xclang -packed-fp32-ops \
+    -Wall -Wno-unused-function \
+    -o "${OUT}.tmp" "${HERE}/sw_capi.hip"
+mv -f "${OUT}.tmp" "${OUT}"
+echo "built ${OUT}"

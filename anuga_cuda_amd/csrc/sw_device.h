@@ -1,0 +1,358 @@
+// Device-side numerics of the shallow-water step for gfx950 (CDNA4).
+//
+// Every helper evaluates the same IEEE operations, in the same order, as
+// ANUGA's FP64 C kernels in the reference (src/anuga_HMPP/shallow_water_ext.c,
+// quantity.c, util_ext.h).  The library is compiled with -ffp-contract=off so
+// that no a*b+c is fused; FP64 division and square root lower to correctly
+// rounded instruction sequences on gfx950, so a kernel built from these
+// helpers reproduces the CPU results bit for bit (pow() in Manning friction is
+// the one libm call whose last bit may differ).
+//
+// The helpers are templated on the arithmetic type R of the flux and the
+// reconstruction: R = double is the FP64 path; R = float is the mixed path,
+// where depths, wet/dry tests, accumulation and the timestep stay double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SW_DEV __device__ __forceinline__
+
+namespace sw {
+
+// util_ext.h:42-56: comparisons, not fmax/fmin (NaN propagates the same way)
+template <typename T> SW_DEV T omax(T x, T y) { return (x > y) ? x : y; }
+template <typename T> SW_DEV T omin(T x, T y) { return (x < y) ? x : y; }
+
+struct FluxParams {
+    double epsilon, h0, limiting_threshold, g;
+};
+
+// shallow_water_ext.c:214-240 (_compute_speed)
+template <typename R>
+SW_DEV R compute_speed(R &uh, R &h, R epsilon, R h0, R limiting_threshold) {
+    R u;
+    if (h < limiting_threshold) {
+        if (h < epsilon) {
+            h = R(0);
+            u = R(0);
+        } else {
+            u = uh / (h + h0 / h);
+        }
+        uh = u * h;
+    } else {
+        u = uh / h;
+    }
+    return u;
+}
+
+// shallow_water_ext.c:302-442 (_flux_function_central, Kurganov-Noelle-Petrova
+// central-upwind flux); ql/qr = (w, uh, vh) in x/y frame, result in x/y frame.
+template <typename R>
+SW_DEV void flux_central(const R ql_in[3], const R qr_in[3], R z_left, R z_right, R n1, R n2,
+                         R epsilon, R h0, R limiting_threshold, R g, R ef[3], R &max_speed) {
+    R ql0 = ql_in[0];
+    R ql1 = n1 * ql_in[1] + n2 * ql_in[2];
+    R ql2 = -n2 * ql_in[1] + n1 * ql_in[2];
+    R qr0 = qr_in[0];
+    R qr1 = n1 * qr_in[1] + n2 * qr_in[2];
+    R qr2 = -n2 * qr_in[1] + n1 * qr_in[2];
+
+    R z = R(0.5) * (z_left + z_right);
+
+    R h_left = ql0 - z;
+    R uh_left = ql1;
+    R u_left = compute_speed<R>(uh_left, h_left, epsilon, h0, limiting_threshold);
+    R h_right = qr0 - z;
+    R uh_right = qr1;
+    R u_right = compute_speed<R>(uh_right, h_right, epsilon, h0, limiting_threshold);
+    R vh_left = ql2;
+    R vh_right = qr2;
+    compute_speed<R>(vh_left, h_left, epsilon, h0, limiting_threshold);
+    compute_speed<R>(vh_right, h_right, epsilon, h0, limiting_threshold);
+
+    R soundspeed_left = sqrt(g * h_left);
+    R soundspeed_right = sqrt(g * h_right);
+
+    R s_max = omax<R>(u_left + soundspeed_left, u_right + soundspeed_right);
+    if (s_max < R(0)) s_max = R(0);
+    R s_min = omin<R>(u_left - soundspeed_left, u_right - soundspeed_right);
+    if (s_min > R(0)) s_min = R(0);
+
+    R fl0 = u_left * h_left;
+    R fl1 = u_left * uh_left + R(0.5) * g * h_left * h_left;
+    R fl2 = u_left * vh_left;
+    R fr0 = u_right * h_right;
+    R fr1 = u_right * uh_right + R(0.5) * g * h_right * h_right;
+    R fr2 = u_right * vh_right;
+
+    R denom = s_max - s_min;
+    if (denom < epsilon) {
+        ef[0] = R(0);
+        ef[1] = R(0);
+        ef[2] = R(0);
+        max_speed = R(0);
+    } else {
+        R inv = R(1) / denom;
+        R smm = s_max * s_min;
+        R e0 = s_max * fl0 - s_min * fr0;
+        e0 += smm * (qr0 - ql0);
+        e0 *= inv;
+        R e1 = s_max * fl1 - s_min * fr1;
+        e1 += smm * (qr1 - ql1);
+        e1 *= inv;
+        R e2 = s_max * fl2 - s_min * fr2;
+        e2 += smm * (qr2 - ql2);
+        e2 *= inv;
+        max_speed = omax<R>(fabs(s_max), fabs(s_min));
+        R m2 = -n2;
+        ef[0] = e0;
+        ef[1] = n1 * e1 + m2 * e2;
+        ef[2] = -m2 * e1 + n1 * e2;
+    }
+}
+
+// util_ext.h:66-109 (_gradient)
+SW_DEV void gradient3(double x0, double y0, double x1, double y1, double x2, double y2,
+                      double q0, double q1, double q2, double &a, double &b) {
+    double det = (y2 - y0) * (x1 - x0) - (y1 - y0) * (x2 - x0);
+    a = (y2 - y0) * (q1 - q0) - (y1 - y0) * (q2 - q0);
+    a /= det;
+    b = (x1 - x0) * (q2 - q0) - (x2 - x0) * (q1 - q0);
+    b /= det;
+}
+
+// shallow_water_ext.c:98-149 (find_qmin_and_qmax)
+template <typename R>
+SW_DEV void find_qmin_and_qmax(R dq0, R dq1, R dq2, R &qmin, R &qmax) {
+    if (dq0 >= R(0)) {
+        if (dq1 >= dq2) {
+            qmax = (dq1 >= R(0)) ? dq0 + dq1 : dq0;
+            qmin = dq0 + dq2;
+            if (qmin >= R(0)) qmin = R(0);
+        } else {
+            qmax = (dq2 > R(0)) ? dq0 + dq2 : dq0;
+            qmin = dq0 + dq1;
+            if (qmin >= R(0)) qmin = R(0);
+        }
+    } else {
+        if (dq1 <= dq2) {
+            qmin = (dq1 < R(0)) ? dq0 + dq1 : dq0;
+            qmax = dq0 + dq2;
+            if (qmax <= R(0)) qmax = R(0);
+        } else {
+            qmin = (dq2 < R(0)) ? dq0 + dq2 : dq0;
+            qmax = dq0 + dq1;
+            if (qmax <= R(0)) qmax = R(0);
+        }
+    }
+}
+
+// shallow_water_ext.c:151-183 (limit_gradient); r0 carries over between i.
+template <typename R>
+SW_DEV void limit_gradient(R dqv[3], R qmin, R qmax, R beta) {
+    R r = R(1000.0), r0 = R(1.0);
+    const R TINY = R(1.0e-100);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (dqv[i] < -TINY) r0 = qmin / dqv[i];
+        if (dqv[i] > TINY) r0 = qmax / dqv[i];
+        r = omin<R>(r0, r);
+    }
+    R phi = omin<R>(r * beta, R(1.0));
+    dqv[0] = dqv[0] * phi;
+    dqv[1] = dqv[1] * phi;
+    dqv[2] = dqv[2] * phi;
+}
+
+// Auxiliary-triangle reconstruction of one quantity (shallow_water_ext.c:3023-3062).
+// Returns the three limited jumps from the centroid.
+template <typename R>
+SW_DEV void extrap_aux(R qk, R q0, R q1, R q2, R dx1, R dx2, R dy1, R dy2, R inv_area2,
+                       const R dxv[3], const R dyv[3], R beta, R dqv[3]) {
+    R dq0 = q0 - qk;
+    R dq1 = q1 - q0;
+    R dq2 = q2 - q0;
+    R a = dy2 * dq1 - dy1 * dq2;
+    a *= inv_area2;
+    R b = dx1 * dq2 - dx2 * dq1;
+    b *= inv_area2;
+    dqv[0] = a * dxv[0] + b * dyv[0];
+    dqv[1] = a * dxv[1] + b * dyv[1];
+    dqv[2] = a * dxv[2] + b * dyv[2];
+    R qmin, qmax;
+    find_qmin_and_qmax<R>(dq0, dq1, dq2, qmin, qmax);
+    limit_gradient<R>(dqv, qmin, qmax, beta);
+}
+
+// Reconstruction toward the single interior neighbour (shallow_water_ext.c:3174-3213).
+template <typename R>
+SW_DEV void extrap_one(R qk, R q1, R dx2, R dy2, const R dxv[3], const R dyv[3], R beta,
+                       R dqv[3]) {
+    R dq1 = q1 - qk;
+    R a = dq1 * dx2;
+    R b = dq1 * dy2;
+    dqv[0] = a * dxv[0] + b * dyv[0];
+    dqv[1] = a * dxv[1] + b * dyv[1];
+    dqv[2] = a * dxv[2] + b * dyv[2];
+    R qmin, qmax;
+    if (dq1 >= R(0)) {
+        qmin = R(0);
+        qmax = dq1;
+    } else {
+        qmin = dq1;
+        qmax = R(0);
+    }
+    limit_gradient<R>(dqv, qmin, qmax, beta);
+}
+
+// quantity.c:769-820 (_update) for one triangle.  Returns false when the
+// semi-implicit denominator is <= 0 (the C returns -1 there).
+SW_DEV bool update_one(double &q, double eu, double &si, double dt) {
+    double x = q;
+    if (x == 0.0)
+        si = 0.0;
+    else
+        si /= x;
+    q += dt * eu;
+    double denominator = 1.0 - dt * si;
+    if (denominator <= 0.0) return false;
+    q /= denominator;
+    return true;
+}
+
+// shallow_water_ext.c:1638-1705 (_protect) for one triangle.
+SW_DEV void protect_one(double &wc, double zc, double &xmomc, double &ymomc, double mah,
+                        double mas, double epsilon) {
+    double hc = wc - zc;
+    if (mas < epsilon) {
+        if (hc < mah) {
+            xmomc = 0.0;
+            ymomc = 0.0;
+            if (hc <= 0.0) wc = zc;
+        }
+    } else {
+        if (hc < mah) {
+            if (hc <= 0.0) {
+                wc = zc;
+                xmomc = 0.0;
+                ymomc = 0.0;
+            } else {
+                double u = xmomc / hc;
+                if (fabs(u) > mas) {
+                    double reduced_speed = mas * u / fabs(u);
+                    xmomc = reduced_speed * hc;
+                }
+                double v = ymomc / hc;
+                if (fabs(v) > mas) {
+                    double reduced_speed = mas * v / fabs(v);
+                    ymomc = reduced_speed * hc;
+                }
+            }
+        }
+    }
+}
+
+struct BalanceParams {
+    double H0, alpha_balance;
+    int tight_slope_limiters, use_centroid_velocities;
+};
+
+// shallow_water_ext.c:1462-1636 (_balance_deep_and_shallow) for one triangle.
+SW_DEV void balance_one(double wc, double zc, double wv[3], const double zv[3], double xmomc,
+                        double ymomc, double xmomv[3], double ymomv[3], const BalanceParams &p) {
+    const double epsilon = 1.0e-6;
+    double hc_k = wc - zc;
+    double dz = 0.0, alpha, hv[3];
+    if (p.tight_slope_limiters == 0) {
+        for (int i = 0; i < 3; i++) dz = omax<double>(dz, fabs(zv[i] - zc));
+    }
+    hv[0] = wv[0] - zv[0];
+    hv[1] = wv[1] - zv[1];
+    hv[2] = wv[2] - zv[2];
+    double hmin = omin<double>(hv[0], omin<double>(hv[1], hv[2]));
+    if (p.tight_slope_limiters == 0) {
+        if (dz > 0.0)
+            alpha = omax<double>(omin<double>(p.alpha_balance * hmin / dz, 1.0), 0.0);
+        else
+            alpha = 1.0;
+    } else {
+        if (hmin < p.H0) {
+            alpha = 1.0;
+            for (int i = 0; i < 3; i++) {
+                double h_diff = hc_k - hv[i];
+                if (!(h_diff <= 0)) alpha = omin<double>(alpha, (hc_k - p.H0) / h_diff);
+            }
+            if (alpha > 1.0) alpha = 1.0;
+            if (alpha < 0.0) alpha = 0.0;
+        } else {
+            alpha = 1.0;
+        }
+    }
+    if (alpha < 1) {
+        for (int i = 0; i < 3; i++) {
+            wv[i] = zv[i] + (1 - alpha) * hc_k + alpha * hv[i];
+            if (p.use_centroid_velocities == 1) {
+                double uc, vc;
+                if (hc_k > epsilon) {
+                    uc = xmomc / hc_k;
+                    vc = ymomc / hc_k;
+                } else {
+                    uc = 0.0;
+                    vc = 0.0;
+                }
+                hv[i] = wv[i] - zv[i];
+                xmomv[i] = uc * hv[i];
+                ymomv[i] = vc * hv[i];
+            } else {
+                xmomv[i] = (1 - alpha) * xmomc + alpha * xmomv[i];
+                ymomv[i] = (1 - alpha) * ymomc + alpha * ymomv[i];
+            }
+        }
+    }
+}
+
+// quantity.c:687-707 (_interpolate_from_vertices_to_edges) for one triangle.
+SW_DEV void interpolate_one(const double v[3], double e[3]) {
+    e[0] = 0.5 * (v[1] + v[2]);
+    e[1] = 0.5 * (v[0] + v[2]);
+    e[2] = 0.5 * (v[0] + v[1]);
+}
+
+// evaluate_segment.cu:36-63: reflective boundary momentum (normal component negated).
+SW_DEV void reflect_momentum(double q1, double q2, double n1, double n2, double &b1, double &b2) {
+    double r1 = -q1 * n1 - q2 * n2;
+    double r2 = -q1 * n2 + q2 * n1;
+    b1 = n1 * r1 - n2 * r2;
+    b2 = n2 * r1 + n1 * r2;
+}
+
+// Positive doubles order like their bit patterns, so the CFL minimum is an
+// unsigned 64-bit atomic min of the bits.
+SW_DEV unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+SW_DEV double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Wavefront (64-lane) minimum with cross-lane shuffles, then one LDS pass
+// across the block's wavefronts; one atomic per block.
+template <int BLOCK>
+SW_DEV void block_min_atomic(double v, unsigned long long *target) {
+    unsigned long long b = dbits(v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = (o < b) ? o : b;
+    }
+    constexpr int NW = BLOCK / 64;
+    __shared__ unsigned long long wmin[NW];
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wmin[wid] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = wmin[0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) m = (wmin[w] < m) ? wmin[w] : m;
+        atomicMin(target, m);
+    }
+}
+
+}  // namespace sw

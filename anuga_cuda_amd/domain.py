@@ -1,0 +1,617 @@
+"""ANUGA-compatible shallow-water ``Domain`` running its step on the MI355X.
+
+Mirrors the user-facing API of the reference's device-resident domain
+(``src/anuga_cuda/gpu_domain_advanced.py``, itself a subclass of ANUGA's
+``Domain``): construction from points / triangles / boundary tags,
+``set_quantity``, ``set_boundary`` with ``Reflective_boundary``,
+``Dirichlet_boundary`` and ``Transmissive_boundary``, the step operators under
+their ANUGA names, and ``evolve(yieldstep, finaltime, duration,
+skip_initial_step)`` as a generator.  Every operator runs as HIP kernels
+through the C ABI (``include/anuga_sw.h``); host arrays are numpy copies that
+are synchronised with the device on access.
+
+Default parameters are ANUGA's shallow-water defaults as the reference passes
+them to its kernels (g, epsilon, H0 = minimum_allowed_height, beta_*,
+alpha_balance, tight_slope_limiters, use_centroid_velocities,
+optimise_dry_cells, extrapolate_velocity_second_order, CFL, evolve_max/min
+timestep, max_smallsteps, default order 2); ANUGA itself is not vendored by the
+reference, so these values are stated here and every one can be overridden.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import (SW_BOUNDARY, SW_CENTROID, SW_EDGE, SW_ELEVATION, SW_EXPLICIT_UPDATE,
+                   SW_FRICTION, SW_HEIGHT, SW_SEMI_IMPLICIT_UPDATE, SW_STAGE, SW_VERTEX,
+                   SW_XMOM, SW_XVELOCITY, SW_YMOM, SW_YVELOCITY)
+from .mesh import Mesh
+
+__all__ = [
+    "Domain",
+    "Quantity",
+    "Reflective_boundary",
+    "Dirichlet_boundary",
+    "Transmissive_boundary",
+    "DEFAULT_PARAMETERS",
+]
+
+DEFAULT_PARAMETERS = dict(
+    g=9.8,
+    epsilon=1.0e-12,
+    H0=1.0e-3,
+    minimum_allowed_height=1.0e-3,
+    maximum_allowed_speed=0.0,
+    beta_w=0.9,
+    beta_w_dry=0.9,
+    beta_uh=0.9,
+    beta_uh_dry=0.9,
+    beta_vh=0.9,
+    beta_vh_dry=0.9,
+    alpha_balance=2.0,
+    CFL=1.0,
+    evolve_max_timestep=1.0e100,
+    evolve_min_timestep=1.0e-6,
+    optimise_dry_cells=1,
+    extrapolate_velocity_second_order=1,
+    tight_slope_limiters=1,
+    use_centroid_velocities=1,
+    use_sloped_mannings=0,
+    max_smallsteps=50,
+    default_order=2,
+)
+
+_QID = {
+    "stage": SW_STAGE,
+    "xmomentum": SW_XMOM,
+    "ymomentum": SW_YMOM,
+    "elevation": SW_ELEVATION,
+    "friction": SW_FRICTION,
+    "height": SW_HEIGHT,
+    "xvelocity": SW_XVELOCITY,
+    "yvelocity": SW_YVELOCITY,
+}
+CONSERVED = ("stage", "xmomentum", "ymomentum")
+_LOC = {
+    "centroid_values": SW_CENTROID,
+    "edge_values": SW_EDGE,
+    "vertex_values": SW_VERTEX,
+    "boundary_values": SW_BOUNDARY,
+    "explicit_update": SW_EXPLICIT_UPDATE,
+    "semi_implicit_update": SW_SEMI_IMPLICIT_UPDATE,
+}
+
+
+# ---------------------------------------------------------------------------
+# Boundary objects (ANUGA generic_boundary_conditions; kernels in the
+# reference: src/anuga_cuda/evaluate_segment/evaluate_segment.cu:3-119)
+class Reflective_boundary:
+    """Solid wall: stage copied, normal momentum negated."""
+
+    kind = _lib.SW_BC_REFLECTIVE
+
+    def __init__(self, domain=None):
+        self.domain = domain
+
+    def __repr__(self):
+        return "Reflective_boundary"
+
+
+class Transmissive_boundary:
+    """Boundary state equals the inside edge state."""
+
+    kind = _lib.SW_BC_TRANSMISSIVE
+
+    def __init__(self, domain=None):
+        self.domain = domain
+
+    def __repr__(self):
+        return "Transmissive_boundary"
+
+
+class Dirichlet_boundary:
+    """Fixed (stage, xmomentum, ymomentum) on the boundary."""
+
+    kind = _lib.SW_BC_DIRICHLET
+
+    def __init__(self, dirichlet_values=None):
+        if dirichlet_values is None:
+            raise ValueError("Dirichlet_boundary needs [stage, xmomentum, ymomentum]")
+        v = np.asarray(dirichlet_values, dtype=np.float64).ravel()
+        if v.size < 3:
+            raise ValueError("Dirichlet_boundary needs at least 3 values")
+        self.dirichlet_values = v
+
+    def __repr__(self):
+        return "Dirichlet_boundary(%s)" % self.dirichlet_values[:3].tolist()
+
+
+# ---------------------------------------------------------------------------
+class Quantity:
+    """Values of one quantity at centroids, edges, vertices and boundary edges.
+
+    Array attributes are numpy arrays synchronised with the device: reading one
+    after a device operation fetches the current device values; arrays read or
+    written on the host are sent back before the next device operation.
+    """
+
+    def __init__(self, domain, name):
+        self.domain = domain
+        self.name = name
+        N = domain.number_of_elements
+        Nb = domain.number_of_boundary_elements
+        self._arrays = {
+            "centroid_values": np.zeros(N),
+            "edge_values": np.zeros((N, 3)),
+            "vertex_values": np.zeros((N, 3)),
+            "boundary_values": np.zeros(Nb),
+        }
+        if name in CONSERVED:
+            self._arrays["explicit_update"] = np.zeros(N)
+            self._arrays["semi_implicit_update"] = np.zeros(N)
+        if name == "friction":
+            self._arrays = {"centroid_values": np.zeros(N), "vertex_values": np.zeros((N, 3)),
+                            "edge_values": np.zeros((N, 3))}
+
+    def _get(self, key):
+        self.domain._pull()
+        self.domain._touched.add((self.name, key))
+        return self._arrays[key]
+
+    def _set(self, key, value):
+        arr = self._get(key)
+        arr[...] = value
+
+    centroid_values = property(lambda s: s._get("centroid_values"), lambda s, v: s._set("centroid_values", v))
+    edge_values = property(lambda s: s._get("edge_values"), lambda s, v: s._set("edge_values", v))
+    vertex_values = property(lambda s: s._get("vertex_values"), lambda s, v: s._set("vertex_values", v))
+    boundary_values = property(lambda s: s._get("boundary_values"), lambda s, v: s._set("boundary_values", v))
+    explicit_update = property(lambda s: s._get("explicit_update"), lambda s, v: s._set("explicit_update", v))
+    semi_implicit_update = property(lambda s: s._get("semi_implicit_update"),
+                                    lambda s, v: s._set("semi_implicit_update", v))
+
+    # quantity.c:662-684 (_interpolate): centroid = (q0+q1+q2)/3, edge i = mean of the other two
+    def interpolate(self):
+        v = self.vertex_values
+        self.centroid_values[:] = (v[:, 0] + v[:, 1] + v[:, 2]) / 3.0
+        self.interpolate_from_vertices_to_edges()
+
+    def interpolate_from_vertices_to_edges(self):
+        v = self.vertex_values
+        e = self.edge_values
+        e[:, 0] = 0.5 * (v[:, 1] + v[:, 2])
+        e[:, 1] = 0.5 * (v[:, 0] + v[:, 2])
+        e[:, 2] = 0.5 * (v[:, 0] + v[:, 1])
+
+    def extrapolate_first_order(self):
+        c = self.centroid_values
+        self.edge_values[:] = c[:, None]
+        self.vertex_values[:] = c[:, None]
+
+    def set_values(self, numeric=None, location="vertices", indices=None):
+        """Constant, array or callable f(x, y) at 'vertices' (then interpolated
+        to centroids and edges, ANUGA's default) or 'centroids' (then first
+        order extrapolated)."""
+        d = self.domain
+        if location not in ("vertices", "centroids", "edges"):
+            raise ValueError("unknown location %r" % location)
+        if indices is not None:
+            indices = np.asarray(indices, dtype=np.int64)
+        if location == "vertices":
+            if callable(numeric):
+                xy = d.mesh.vertex_coordinates.reshape(-1, 2)
+                vals = np.asarray(numeric(xy[:, 0], xy[:, 1]), dtype=np.float64).reshape(-1, 3)
+            else:
+                vals = np.broadcast_to(np.asarray(numeric, dtype=np.float64),
+                                       (d.number_of_elements, 3)) if np.ndim(numeric) < 2 else \
+                    np.asarray(numeric, dtype=np.float64).reshape(-1, 3)
+            if indices is None:
+                self.vertex_values[:] = vals
+            else:
+                self.vertex_values[indices] = vals[indices] if vals.shape[0] == d.number_of_elements else vals
+            self.interpolate()
+        elif location == "centroids":
+            if callable(numeric):
+                c = d.mesh.centroid_coordinates
+                vals = np.asarray(numeric(c[:, 0], c[:, 1]), dtype=np.float64)
+            else:
+                vals = np.broadcast_to(np.asarray(numeric, dtype=np.float64), (d.number_of_elements,))
+            if indices is None:
+                self.centroid_values[:] = vals
+            else:
+                self.centroid_values[indices] = vals[indices] if vals.shape[0] == d.number_of_elements else vals
+            self.extrapolate_first_order()
+        else:
+            if callable(numeric):
+                xy = d.mesh.edge_midpoint_coordinates.reshape(-1, 2)
+                vals = np.asarray(numeric(xy[:, 0], xy[:, 1]), dtype=np.float64).reshape(-1, 3)
+            else:
+                vals = np.broadcast_to(np.asarray(numeric, dtype=np.float64), (d.number_of_elements, 3))
+            self.edge_values[:] = vals
+
+    def get_integral(self):
+        return float(np.sum(self.centroid_values * self.domain.mesh.areas))
+
+
+# ---------------------------------------------------------------------------
+class Domain:
+    """Shallow-water domain on an unstructured triangle mesh.
+
+    ``Domain(coordinates, vertices, boundary)`` as in ANUGA, or
+    ``Domain(mesh=Mesh(...))``.  Keyword parameters override
+    ``DEFAULT_PARAMETERS``; ``precision='mixed'`` selects FP32 flux and
+    reconstruction with FP64 accumulation, timestep and wet/dry tests.
+    """
+
+    def __init__(self, coordinates=None, vertices=None, boundary=None, mesh=None,
+                 precision="fp64", device=0, **parameters):
+        if mesh is None:
+            if coordinates is None or vertices is None:
+                raise ValueError("need coordinates and vertices, or mesh=")
+            mesh = Mesh(coordinates, vertices, boundary=boundary)
+        self.mesh = mesh
+        self.number_of_elements = mesh.number_of_elements
+        self.number_of_boundary_elements = mesh.number_of_boundary_elements
+        self.boundary = mesh.boundary
+        unknown = set(parameters) - set(DEFAULT_PARAMETERS)
+        if unknown:
+            raise TypeError("unknown parameters: %s" % sorted(unknown))
+        self._params = dict(DEFAULT_PARAMETERS)
+        self._params.update(parameters)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.device = device
+        self.quantities = {name: Quantity(self, name) for name in _QID}
+        self.conserved_quantities = list(CONSERVED)
+        self.evolved_quantities = ["stage", "xmomentum", "ymomentum", "elevation",
+                                   "height", "xvelocity", "yvelocity"]
+        self.other_quantities = ["elevation", "friction", "height", "xvelocity", "yvelocity"]
+        self.boundary_map = None
+        self._kinds = np.zeros(self.number_of_boundary_elements, dtype=np.int32)
+        self._dirichlet = np.zeros((self.number_of_boundary_elements, 3))
+        self._handle = None
+        self._host_stale = False
+        self._touched = set()
+        self._all_dirty = True
+        self._time = 0.0
+        self.starttime = 0.0
+        self.finaltime = None
+        self.yieldtime = None
+        self.timestep = 0.0
+        self.flux_timestep = 0.0
+        self.number_of_steps = 0
+        self.number_of_first_order_steps = 0
+        self.recorded_max_timestep = self._params["evolve_min_timestep"]
+        self.recorded_min_timestep = self._params["evolve_max_timestep"]
+        self._order_ = self._params["default_order"]
+        self.smallsteps = 0
+        self.max_speed = np.zeros(self.number_of_elements)
+
+    # ------------------------------------------------------------------ params
+    def __getattr__(self, name):
+        params = self.__dict__.get("_params")
+        if params is not None and name in params:
+            return params[name]
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        params = self.__dict__.get("_params")
+        if params is not None and name in params:
+            params[name] = value
+            if self.__dict__.get("_handle") is not None:
+                self._handle.set_params(self._sw_params())
+            return
+        object.__setattr__(self, name, value)
+
+    def __len__(self):
+        return self.number_of_elements
+
+    def set_default_order(self, n):
+        if n not in (1, 2):
+            raise ValueError("order must be 1 or 2")
+        self.default_order = n
+        self._order_ = n
+
+    def set_CFL(self, cfl):
+        self.CFL = float(cfl)
+
+    def set_beta(self, beta):
+        for n in ("beta_w", "beta_w_dry", "beta_uh", "beta_uh_dry", "beta_vh", "beta_vh_dry"):
+            setattr(self, n, float(beta))
+
+    def set_minimum_allowed_height(self, h):
+        self.minimum_allowed_height = float(h)
+        self.H0 = float(h)
+
+    def get_time(self):
+        if self._handle is not None:
+            return self._handle.get_state().time
+        return self._time
+
+    def set_time(self, t):
+        t = float(t)
+        if self._handle is not None:
+            s = self._handle.get_state()
+            s.time = t
+            self._handle.set_state(s)
+        self._time = t
+
+    def get_starttime(self):
+        return self.starttime
+
+    def set_starttime(self, t):
+        self.starttime = float(t)
+        self.set_time(t)
+
+    def get_quantity(self, name):
+        return self.quantities[name]
+
+    def get_timestepping_method(self):
+        return "euler"
+
+    def set_timestepping_method(self, method):
+        if method != "euler":
+            raise NotImplementedError("only Euler timestepping is built (the reference's profiled path)")
+
+    def set_quantities_to_be_stored(self, *args, **kwargs):
+        return None
+
+    def set_name(self, name):
+        self.name = name
+
+    def set_datadir(self, name):
+        self.datadir = name
+
+    def set_store(self, flag=True):
+        self.store = flag
+
+    # ------------------------------------------------------------ quantities
+    def set_quantity(self, name, numeric=None, function=None, expression=None,
+                     location="vertices", indices=None, **kwargs):
+        if name not in self.quantities:
+            raise KeyError("unknown quantity %r" % name)
+        value = numeric if numeric is not None else function
+        if expression is not None:
+            env = {n: self._expr_values(n, location) for n in self.quantities}
+            env["np"] = np
+            value = eval(expression, {"__builtins__": {}}, env)
+            if location == "vertices":
+                value = np.asarray(value).reshape(-1, 3)
+        if value is None:
+            raise ValueError("set_quantity needs numeric, function or expression")
+        if name == "elevation":
+            self._all_dirty = True
+        self.quantities[name].set_values(value, location=location, indices=indices)
+
+    def _expr_values(self, name, location):
+        q = self.quantities[name]
+        return q.vertex_values.copy() if location == "vertices" else q.centroid_values.copy()
+
+    def set_boundary(self, boundary_map):
+        """Bind each boundary tag to a boundary object (ANUGA Generic_Domain.set_boundary)."""
+        tags = self.mesh.boundary_tags
+        kinds = np.zeros(len(tags), dtype=np.int32)
+        dirichlet = np.zeros((len(tags), 3))
+        for tag in set(tags.tolist()):
+            if tag not in boundary_map or boundary_map[tag] is None:
+                raise ValueError("Tag '%s' has not been bound to a boundary object" % tag)
+        for m, tag in enumerate(tags.tolist()):
+            B = boundary_map[tag]
+            kinds[m] = B.kind
+            if B.kind == _lib.SW_BC_DIRICHLET:
+                dirichlet[m] = B.dirichlet_values[:3]
+        self.boundary_map = dict(boundary_map)
+        self._kinds = kinds
+        self._dirichlet = dirichlet
+        if self._handle is not None:
+            self._handle.set_boundary(kinds, dirichlet)
+
+    # ---------------------------------------------------------- host/device
+    def _sw_params(self):
+        p = _lib.SwParams()
+        for k, v in self._params.items():
+            setattr(p, k, v)
+        p.precision = _lib.SW_PRECISION_MIXED if self.precision == "mixed" else _lib.SW_PRECISION_FP64
+        return p
+
+    def _ensure_device(self):
+        if self._handle is None:
+            self._handle = _lib.Handle(self.mesh, self._sw_params(), self.device)
+            self._handle.set_boundary(self._kinds, self._dirichlet)
+            s = self._handle.get_state()
+            s.time = self._time
+            s.starttime = self.starttime
+            s.order = self._order_
+            self._handle.set_state(s)
+            self._all_dirty = True
+        return self._handle
+
+    def _each_array(self):
+        for name, q in self.quantities.items():
+            for key, arr in q._arrays.items():
+                if name == "friction" and key != "centroid_values":
+                    continue
+                yield name, key, arr
+
+    def _pull(self):
+        if self._host_stale and self._handle is not None:
+            self._host_stale = False
+            for name, key, arr in self._each_array():
+                self._handle.download(_QID[name], _LOC[key], arr.reshape(-1))
+            self.max_speed = self._handle.max_speed()
+
+    def _push(self):
+        h = self._ensure_device()
+        if self._all_dirty:
+            items = list(self._each_array())
+        else:
+            items = [(n, k, self.quantities[n]._arrays[k]) for (n, k) in self._touched
+                     if not (n == "friction" and k != "centroid_values")]
+        # elevation vertex values last: the device derives the mean bed from them
+        items.sort(key=lambda t: (t[0] == "elevation" and t[1] == "vertex_values"))
+        for name, key, arr in items:
+            h.upload(_QID[name], _LOC[key], arr)
+        self._touched.clear()
+        self._all_dirty = False
+
+    def _dev(self):
+        """Device handle with host changes applied; host copies marked stale."""
+        self._pull()
+        self._push()
+        self._host_stale = True
+        return self._handle
+
+    def _sync_scalars(self):
+        s = self._handle.get_state()
+        self._time = s.time
+        self.timestep = s.timestep
+        self.flux_timestep = s.flux_timestep
+        self.number_of_steps = s.number_of_steps
+        self.number_of_first_order_steps = s.number_of_first_order_steps
+        self.recorded_max_timestep = s.recorded_max_timestep
+        self.recorded_min_timestep = s.recorded_min_timestep
+        self._order_ = s.order
+        self.smallsteps = s.smallsteps
+        self.yieldtime = s.yieldtime
+        return s
+
+    # --------------------------------------------------------- operators
+    def compute_fluxes(self):
+        """compute_fluxes_central_structure + gravity_wb (the reference's
+        'wb_2' compute_fluxes, gpu_domain_advanced.py:1595-1696)."""
+        h = self._dev()
+        self.flux_timestep = h.compute_fluxes()
+        h.call("sw_gravity_wb")
+        return self.flux_timestep
+
+    def compute_fluxes_central_structure(self):
+        h = self._dev()
+        self.flux_timestep = h.compute_fluxes()
+        return self.flux_timestep
+
+    def gravity_wb(self):
+        self._dev().call("sw_gravity_wb")
+
+    def manning_friction_implicit(self):
+        self._dev().call("sw_manning_friction")
+
+    def compute_forcing_terms(self):
+        self.manning_friction_implicit()
+
+    def update_timestep(self, yieldstep, finaltime):
+        h = self._dev()
+        has_final = finaltime is not None
+        h.call("sw_update_timestep", float(yieldstep or 0.0), float(finaltime or 0.0), int(has_final))
+        self._sync_scalars()
+
+    def update_conserved_quantities(self):
+        self._dev().call("sw_update_conserved_quantities")
+
+    def protect_against_infinitesimal_and_negative_heights(self):
+        self._dev().call("sw_protect")
+
+    def extrapolate_second_order_sw(self):
+        self._dev().call("sw_extrapolate_second_order_sw")
+
+    def extrapolate_first_order(self):
+        self._dev().call("sw_extrapolate_first_order")
+
+    def balance_deep_and_shallow(self):
+        self._dev().call("sw_balance_deep_and_shallow")
+
+    def interpolate_from_vertices_to_edges(self):
+        self._dev().call("sw_interpolate_from_vertices_to_edges")
+
+    def distribute_to_vertices_and_edges(self):
+        h = self._dev()
+        s = h.get_state()
+        if s.order != self._order_:
+            s.order = self._order_
+            h.set_state(s)
+        h.call("sw_distribute_to_vertices_and_edges")
+
+    def update_boundary(self):
+        self._dev().call("sw_update_boundary")
+
+    def update_other_quantities(self):
+        self._dev().call("sw_update_other_quantities")
+
+    def evolve_one_euler_step(self, yieldstep, finaltime):
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+
+    def update_ghosts(self):
+        return None
+
+    def update_extrema(self):
+        return None
+
+    def apply_fractional_steps(self):
+        return None
+
+    # ------------------------------------------------------------ evolve
+    def evolve(self, yieldstep=None, finaltime=None, duration=None, skip_initial_step=False,
+               fused=True):
+        """Generator yielding the model time at every yieldstep and at the end
+        (gpu_domain_advanced.py:2296-2698).  ``fused=False`` runs the
+        one-launch-per-operator step instead of the fused device loop."""
+        h = self._dev()
+        if yieldstep is None:
+            yieldstep = self.evolve_max_timestep
+        else:
+            yieldstep = float(yieldstep)
+        if finaltime is not None and duration is not None:
+            raise ValueError("give finaltime or duration, not both")
+        if duration is not None:
+            finaltime = self.starttime + float(duration)
+        if finaltime is not None:
+            finaltime = float(finaltime)
+            if finaltime < self.starttime:
+                raise ValueError("finaltime precedes starttime")
+        self.finaltime = finaltime
+        s = h.get_state()
+        s.starttime = self.starttime
+        h.set_state(s)
+        h.call("sw_evolve_prepare", yieldstep, finaltime if finaltime is not None else 0.0,
+               int(finaltime is not None))
+        self._host_stale = True
+        self._sync_scalars()
+        if not skip_initial_step:
+            yield self._time
+        while True:
+            h = self._dev()
+            if fused:
+                _, event = h.evolve_steps(1 << 40)
+            else:
+                event = _lib.SW_EVENT_NONE
+                while event == _lib.SW_EVENT_NONE:
+                    event = h.evolve_one_step_unfused()
+            self._sync_scalars()
+            if event == _lib.SW_EVENT_FINAL:
+                yield self._time
+                break
+            yield self._time
+            self._handle.call("sw_advance_yield")
+            self._sync_scalars()
+
+    # ------------------------------------------------------------ reports
+    def timestepping_statistics(self):
+        return ("Time = %.4f, delta t in [%.8f, %.8f], steps=%d (%d)" % (
+            self.get_time(), self.recorded_min_timestep, self.recorded_max_timestep,
+            self.number_of_steps, self.number_of_first_order_steps))
+
+    def write_time(self):
+        print(self.timestepping_statistics())
+
+    def get_water_volume(self):
+        h = self.quantities["stage"].centroid_values - self.quantities["elevation"].centroid_values
+        return float(np.sum(h * self.mesh.areas))
+
+    def close(self):
+        if self._handle is not None:
+            self._pull()
+            self._handle.close()
+            self._handle = None

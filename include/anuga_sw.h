@@ -1,0 +1,181 @@
+/* anuga_sw.h -- C ABI of the MI355X (gfx950) shallow-water evolve() library.
+ *
+ * Drop-in boundary for ANUGA's shallow-water step operators.  Every entry point
+ * below replaces one operator of the reference GPU domain driver
+ * (src/anuga_cuda/gpu_domain_advanced.py) or the reference's C evolve loop
+ * (src/anuga_HMPP/evolve.c); the citation on each declaration names the
+ * interface it replaces.  Plain pointers and sizes only: the library owns its
+ * device buffers; host pointers are borrowed for the duration of a call.
+ *
+ * Conventions
+ *   - All arithmetic is IEEE FP64 unless the handle was created with
+ *     SW_PRECISION_MIXED (FP32 flux and reconstruction, FP64 accumulation,
+ *     timestep and wet/dry tests).
+ *   - Arrays use the reference layout (src/anuga_HMPP/sw_domain.h:60-180):
+ *     per-triangle [N], per-edge / per-vertex [N][3], normals and vertex
+ *     coordinates [N][6], centroids [N][2], boundary [Nb].
+ *   - Index arrays are int64 on the host (as the reference's `long*`); they
+ *     are stored as int32 on the device.
+ *   - Every function returns 0 on success or a negative SW_ERR_* code;
+ *     sw_last_error() gives the message.  A handle is not re-entrant.
+ */
+#ifndef ANUGA_SW_H
+#define ANUGA_SW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sw_handle sw_handle;
+
+/* Error codes. */
+#define SW_OK 0
+#define SW_ERR_UPDATE (-1)       /* semi-implicit denominator <= 0 (quantity.c:803-805) */
+#define SW_ERR_SMALLSTEPS (-2)   /* timestep below evolve_min_timestep at first order */
+#define SW_ERR_NEIGHBOUR (-3)    /* two-boundary triangle without interior neighbour */
+#define SW_ERR_ARG (-10)         /* invalid argument */
+#define SW_ERR_HIP (-11)         /* HIP runtime error */
+#define SW_ERR_STATE (-12)       /* call out of sequence */
+
+/* Quantity ids, locations and boundary kinds. */
+enum sw_quantity {
+    SW_STAGE = 0, SW_XMOM = 1, SW_YMOM = 2, SW_ELEVATION = 3, SW_FRICTION = 4,
+    SW_HEIGHT = 5, SW_XVELOCITY = 6, SW_YVELOCITY = 7
+};
+enum sw_location {
+    SW_CENTROID = 0, SW_EDGE = 1, SW_VERTEX = 2, SW_BOUNDARY = 3,
+    SW_EXPLICIT_UPDATE = 4, SW_SEMI_IMPLICIT_UPDATE = 5
+};
+enum sw_boundary_kind { SW_BC_REFLECTIVE = 0, SW_BC_DIRICHLET = 1, SW_BC_TRANSMISSIVE = 2 };
+enum sw_precision { SW_PRECISION_FP64 = 0, SW_PRECISION_MIXED = 1 };
+enum sw_event { SW_EVENT_NONE = 0, SW_EVENT_YIELD = 1, SW_EVENT_FINAL = 2 };
+
+/* Domain parameters; names follow struct domain (src/anuga_HMPP/sw_domain.h:16-60)
+ * and the attributes the reference driver passes to its kernels. */
+typedef struct sw_params {
+    double g, epsilon, H0, minimum_allowed_height, maximum_allowed_speed;
+    double beta_w, beta_w_dry, beta_uh, beta_uh_dry, beta_vh, beta_vh_dry;
+    double alpha_balance, CFL, evolve_max_timestep, evolve_min_timestep;
+    int32_t optimise_dry_cells, extrapolate_velocity_second_order;
+    int32_t tight_slope_limiters, use_centroid_velocities, use_sloped_mannings;
+    int32_t max_smallsteps, default_order, precision;
+} sw_params;
+
+/* Mesh arrays (host, borrowed during sw_create). */
+typedef struct sw_mesh {
+    int64_t number_of_elements;          /* N */
+    int64_t number_of_boundary_elements; /* Nb */
+    const int64_t *neighbours;           /* [N][3], boundary edges hold -(m+1) */
+    const int64_t *neighbour_edges;      /* [N][3] */
+    const int64_t *surrogate_neighbours; /* [N][3] */
+    const int64_t *number_of_boundaries; /* [N] */
+    const int64_t *tri_full_flag;        /* [N] */
+    const double *normals;               /* [N][6] */
+    const double *edgelengths;           /* [N][3] */
+    const double *radii;                 /* [N] */
+    const double *areas;                 /* [N] */
+    const double *centroid_coordinates;  /* [N][2] */
+    const double *vertex_coordinates;    /* [N][6] */
+    const int64_t *boundary_cells;       /* [Nb] vol_id of boundary edge m */
+    const int64_t *boundary_edges;       /* [Nb] edge_id of boundary edge m */
+} sw_mesh;
+
+/* Scalar evolve state (time bookkeeping of ANUGA's Generic_Domain, restated
+ * in src/anuga_HMPP/sw_domain.h:29-58). */
+typedef struct sw_state {
+    double time, starttime, finaltime, yieldtime, yieldstep, timestep, flux_timestep;
+    double recorded_max_timestep, recorded_min_timestep;
+    int64_t smallsteps, order, number_of_steps, number_of_first_order_steps;
+    int64_t has_finaltime, event, error;
+} sw_state;
+
+/* Handle lifetime.  Replaces allocate_device_array + asynchronous_transfer
+ * (gpu_domain_advanced.py:582-906) and the HMPP get_python_domain marshalling
+ * (src/anuga_HMPP/sw_domain_fun.h:57-329).  Returns NULL on failure
+ * (message via sw_last_error(NULL)). */
+sw_handle *sw_create(const sw_mesh *mesh, const sw_params *params, int device);
+int sw_destroy(sw_handle *h);
+const char *sw_last_error(const sw_handle *h);
+int sw_set_params(sw_handle *h, const sw_params *params);
+int sw_get_params(const sw_handle *h, sw_params *params);
+
+/* Boundary objects: per boundary edge kind and Dirichlet values [Nb][3]
+ * (stage, xmom, ymom).  Replaces set_boundary + boundary_map dispatch
+ * (gpu_domain_advanced.py:2006-2108).  dirichlet may be NULL if unused. */
+int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet);
+
+/* Host <-> device transfer of one quantity array (length N, 3N or Nb).
+ * Replaces the per-array cpy_back / memcpy_htod of the reference driver
+ * (src/utilities/utility.py:134-243). */
+int sw_upload(sw_handle *h, int quantity, int location, const double *host, int64_t count);
+int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t count);
+int sw_get_state(sw_handle *h, sw_state *state);
+int sw_set_state(sw_handle *h, const sw_state *state);
+int sw_download_max_speed(sw_handle *h, double *host, int64_t count);
+
+/* Step operators, one per overridden ANUGA method. */
+/* compute_fluxes_central_structure (gpu_domain_advanced.py:1612-1646;
+ * src/anuga_HMPP/shallow_water_ext.c:4519-4696).  Writes explicit updates and
+ * max_speed; *flux_timestep receives the CFL minimum (host sync). */
+int sw_compute_fluxes(sw_handle *h, double *flux_timestep);
+/* gravity_wb (gpu_domain_advanced.py:1656-1673; shallow_water_ext.c:1942-2037) */
+int sw_gravity_wb(sw_handle *h);
+/* manning_friction_implicit (gpu_domain_advanced.py:1255-1324;
+ * shallow_water_ext.c:1238-1317), flat or sloped per params */
+int sw_manning_friction(sw_handle *h);
+/* update_timestep (ANUGA Generic_Domain; src/anuga_HMPP/evolve.c:115-160) */
+int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime);
+/* update_conserved_quantities (gpu_domain_advanced.py:1412-1442; quantity.c:769-820) */
+int sw_update_conserved_quantities(sw_handle *h);
+/* protect_against_infinitesimal_and_negative_heights
+ * (gpu_domain_advanced.py:1003-1055; shallow_water_ext.c:1638-1705) */
+int sw_protect(sw_handle *h);
+/* extrapolate_second_order_sw (gpu_domain_advanced.py:1061-1161;
+ * shallow_water_ext.c:2717-3324) */
+int sw_extrapolate_second_order_sw(sw_handle *h);
+/* extrapolate_first_order for the conserved quantities
+ * (gpu_domain_advanced.py:1888-1907; extrapolate_first_order.cu:3-33) */
+int sw_extrapolate_first_order(sw_handle *h);
+/* balance_deep_and_shallow (gpu_domain_advanced.py:957-993;
+ * shallow_water_ext.c:1462-1636) */
+int sw_balance_deep_and_shallow(sw_handle *h);
+/* interpolate_from_vertices_to_edges for the conserved quantities
+ * (gpu_domain_advanced.py:1975-1986; quantity.c:687-707) */
+int sw_interpolate_from_vertices_to_edges(sw_handle *h);
+/* distribute_to_vertices_and_edges (gpu_domain_advanced.py:1744-1992) */
+int sw_distribute_to_vertices_and_edges(sw_handle *h);
+/* update_boundary (gpu_domain_advanced.py:2006-2108; evaluate_segment.cu:3-119;
+ * set_boundary.cu:1-18) */
+int sw_update_boundary(sw_handle *h);
+/* update_other_quantities (gpu_domain_advanced.py:2114-2142;
+ * update_centroids_of_velocities_and_height.cu:1-52) */
+int sw_update_other_quantities(sw_handle *h);
+
+/* evolve(): preamble up to the first yield (gpu_domain_advanced.py:2473-2545). */
+int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime);
+/* evolve() loop body (gpu_domain_advanced.py:2555-2663 / evolve.c:1466-1937):
+ * runs up to max_steps Euler steps on the device with fused kernels replayed
+ * from a hipGraph, stopping at the first yield or the final time.  Returns the
+ * number of steps taken and the event (SW_EVENT_*). */
+int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_t *event);
+/* Bookkeeping when evolve() resumes after a yield (gpu_domain_advanced.py:2649-2656). */
+int sw_advance_yield(sw_handle *h);
+/* Unfused reference-order step (one launch per operator, as the reference
+ * driver's evolve_one_euler_step + distribute + update_boundary). */
+int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event);
+
+/* Synchronise the handle's stream. */
+int sw_synchronize(sw_handle *h);
+/* Kernel timing of the fused step: when enabled, sw_evolve_steps records HIP
+ * events around every launch on the handle's stream; sw_kernel_times returns
+ * accumulated milliseconds and launch counts per fused kernel (flux, update,
+ * distribute, timestep) and resets them. */
+int sw_enable_timing(sw_handle *h, int32_t enable);
+int sw_kernel_times(sw_handle *h, double *ms, int64_t *launches, int32_t nkernels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
