@@ -1,0 +1,17 @@
+"""MI355X-native (gfx950) shallow-water evolve() for ANUGA-style domains.
+
+A drop-in for the step path of the reference GPU domain
+(src/anuga_cuda/gpu_domain_advanced.py): ANUGA's Domain API on top of a
+C ABI (include/anuga_sw.h) whose every stage is a hand-written HIP kernel.
+"""
+from .domain import (DEFAULT_PARAMETERS, Dirichlet_boundary, Domain, Quantity,
+                     Reflective_boundary, Transmissive_boundary)
+from .mesh import Mesh, delaunay_random, read_tsh, rectangular_cross
+
+GPU_domain = Domain
+
+__all__ = [
+    "Domain", "GPU_domain", "Quantity", "Mesh", "Reflective_boundary", "Dirichlet_boundary",
+    "Transmissive_boundary", "rectangular_cross", "read_tsh", "delaunay_random",
+    "DEFAULT_PARAMETERS",
+]

@@ -662,42 +662,38 @@ __global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, D
     for (int i = 0; i < 3; i++) F.vh.e[k3 + i] = e[i];
 }
 
-// update_boundary (evaluate_segment.cu:3-119, set_boundary.cu:1-18)
+// update_boundary (evaluate_segment.cu:3-119, set_boundary.cu:1-18).  Each
+// branch only computes the six boundary values; the stores are done once,
+// unconditionally and in a fixed order (no divergent stores to different
+// arrays for the compiler to merge across branches).
 __global__ __launch_bounds__(kBlock) void k_update_boundary(DevMesh M, DevFields F) {
     const int m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= M.Nb) return;
     const int vol = M.bcell[m], edge = M.bedge[m];
     const int e = 3 * vol + edge;
-    F.z.b[m] = F.z.e[e];
     const int kind = M.bkind[m];
-    if (kind == 0) {
-        const double n1 = M.normals[6 * vol + 2 * edge], n2 = M.normals[6 * vol + 2 * edge + 1];
-        double b1, b2;
-        F.w.b[m] = F.w.e[e];
-        F.h.b[m] = F.h.e[e];
-        reflect_momentum(F.uh.e[e], F.vh.e[e], n1, n2, b1, b2);
-        F.uh.b[m] = b1;
-        F.vh.b[m] = b2;
-        const double q1 = F.u.e[e], q2 = F.v.e[e];
-        const double r1 = q1 * n1 + q2 * n2;
-        const double r2 = q1 * n2 - q2 * n1;
-        F.u.b[m] = n1 * r1 - n2 * r2;
-        F.v.b[m] = n2 * r1 + n1 * r2;
-    } else if (kind == 1) {
-        F.h.b[m] = F.h.e[e];
-        F.u.b[m] = F.u.e[e];
-        F.v.b[m] = F.v.e[e];
-        F.w.b[m] = M.bdir[3 * m];
-        F.uh.b[m] = M.bdir[3 * m + 1];
-        F.vh.b[m] = M.bdir[3 * m + 2];
-    } else {
-        F.w.b[m] = F.w.e[e];
-        F.uh.b[m] = F.uh.e[e];
-        F.vh.b[m] = F.vh.e[e];
-        F.h.b[m] = F.h.e[e];
-        F.u.b[m] = F.u.e[e];
-        F.v.b[m] = F.v.e[e];
+    const double we = F.w.e[e], uhe = F.uh.e[e], vhe = F.vh.e[e];
+    const double he = F.h.e[e], ue = F.u.e[e], ve = F.v.e[e];
+    const double n1 = M.normals[6 * vol + 2 * edge], n2 = M.normals[6 * vol + 2 * edge + 1];
+    double wb = we, uhb = uhe, vhb = vhe, hb = he, ub = ue, vb = ve;  // transmissive
+    if (kind == 0) {  // reflective
+        reflect_momentum(uhe, vhe, n1, n2, uhb, vhb);
+        const double r1 = ue * n1 + ve * n2;
+        const double r2 = ue * n2 - ve * n1;
+        ub = n1 * r1 - n2 * r2;
+        vb = n2 * r1 + n1 * r2;
+    } else if (kind == 1) {  // Dirichlet: conserved quantities fixed, others edge copies
+        wb = M.bdir[3 * m];
+        uhb = M.bdir[3 * m + 1];
+        vhb = M.bdir[3 * m + 2];
     }
+    F.z.b[m] = F.z.e[e];
+    F.w.b[m] = wb;
+    F.uh.b[m] = uhb;
+    F.vh.b[m] = vhb;
+    F.h.b[m] = hb;
+    F.u.b[m] = ub;
+    F.v.b[m] = vb;
 }
 
 // update_other_quantities (update_centroids_of_velocities_and_height.cu:1-52

@@ -152,16 +152,27 @@ def read_tsh(path):
 
 
 def delaunay_random(npoints, extent=(10000.0, 10000.0), seed=12345):
-    """Delaunay triangulation of ``npoints`` seeded uniform points plus the four
-    corners of ``[0, Lx] x [0, Ly]``.  Returns ``(points, triangles)`` with
-    counter-clockwise triangles; all hull edges are boundary ('exterior')."""
+    """Delaunay triangulation of ``npoints`` seeded uniform points in
+    ``[0, Lx] x [0, Ly]`` plus evenly spaced points on its sides (spacing equal
+    to the mean point spacing, corners included), so the hull is the square
+    and its triangles are well shaped.  Returns ``(points, triangles)``; all
+    hull edges are boundary ('exterior')."""
     from scipy.spatial import Delaunay  # setup-time only
 
     rng = np.random.default_rng(seed)
     lx, ly = extent
     pts = rng.random((int(npoints), 2)) * np.array([lx, ly])
-    corners = np.array([[0.0, 0.0], [lx, 0.0], [lx, ly], [0.0, ly]])
-    pts = np.concatenate([corners, pts], axis=0)
+    spacing = np.sqrt(lx * ly / max(int(npoints), 1))
+    nx = max(int(round(lx / spacing)), 1)
+    ny = max(int(round(ly / spacing)), 1)
+    sx = np.linspace(0.0, lx, nx + 1)
+    sy = np.linspace(0.0, ly, ny + 1)[1:-1]
+    side = np.concatenate([
+        np.stack([sx, np.zeros_like(sx)], 1), np.stack([sx, np.full_like(sx, ly)], 1),
+        np.stack([np.zeros_like(sy), sy], 1), np.stack([np.full_like(sy, lx), sy], 1)])
+    # keep interior points off the sides
+    pts = np.clip(pts, 1e-6 * lx, (1 - 1e-6) * np.array([lx, ly]))
+    pts = np.concatenate([side, pts], axis=0)
     tri = Delaunay(pts).simplices.astype(np.int64)
     return pts, tri
 

@@ -1,0 +1,184 @@
+#!/usr/bin/env python
+"""Throughput of the shallow-water evolve() step on one MI355X.
+
+Workload: seeded Delaunay mesh of 500k uniform points in a 10 km x 10 km
+square (~1.0M triangles, Morton order), 5-octave fBm terrain (amplitude 20 m),
+stage max(z, 5 m) + 3 m mound of radius 500 m (wet/dry fronts), Manning
+n = 0.025, transmissive boundaries, FP64, second order, Euler steps.
+
+A step is one full Euler step of the evolve loop (compute_fluxes + gravity +
+friction + CFL timestep + update + protect + extrapolation + balance +
+vertex->edge interpolation) over the whole mesh, with every input resident in
+HBM.  The timed region runs --steps steps of the fused device loop (hipGraph
+replay, no host synchronisation between steps).  A second, untimed-for-value
+pass records HIP events around each kernel on the library's stream to get the
+per-kernel durations behind the roofline numbers.  The CPU baseline runs the
+FP64 C oracle (serial) on a bounded sample of steps of the same workload.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Unique HBM footprint of one Euler step per triangle (FP64 values, int32
+# indices, neighbour gathers as cache hits): geometry 196 + static quantities
+# 64 + centroid state 48 + vertex state 144 + edge exchange 144 + max_speed 8.
+STEP_BYTES_PER_TRIANGLE = 604
+# Per-launch algorithmic bytes per triangle of the fused kernels (arrays each
+# kernel reads or writes, counted once; gathers of neighbour data as hits).
+KERNEL_BYTES_PER_TRIANGLE = {
+    # edge w/uh/vh 72, bed edge 24, nbr+nbr_edge 24, normals 48, lengths 24,
+    # radii 8, area 8, full flag 4, stage vertex 24, vertex xy 48, centroid
+    # w/z/uh/vh 32, friction 8, mean bed 8 | writes: explicit 24, semi-impl 16, max_speed 8
+    "flux": 72 + 24 + 24 + 48 + 24 + 8 + 8 + 4 + 24 + 48 + 32 + 8 + 8 + 24 + 16 + 8,
+    # centroid w/uh/vh 24, explicit 24, semi-implicit 24, bed centroid 8 | write centroid 24
+    "update": 24 + 24 + 24 + 8 + 24,
+    # centroid w/uh/vh/z 32, centroid xy 16, surrogate 12, nob 4, vertex xy 48,
+    # bed vertex 24 | writes: vertex w/uh/vh 72, edge w/uh/vh 72
+    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 72,
+}
+KERNELS = ("flux", "timestep", "update", "distribute")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--points", type=int, default=500000)
+    ap.add_argument("--order", default="morton", choices=["morton", "random", "none"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=64)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (scale for rocprof FETCH/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (each GPU box has one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from anuga_cuda_amd import scenarios
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB read + %d MiB written per copy kernel" % (args.copy_calibration,
+                                                                             args.copy_calibration))
+        del a, b
+
+    t0 = time.time()
+    d = scenarios.delaunay_terrain(npoints=args.points, order=args.order, precision=args.precision)
+    N = d.number_of_elements
+    log("mesh: %d triangles, %d boundary edges, built in %.1f s" % (N, d.number_of_boundary_elements,
+                                                                  time.time() - t0))
+    cpu_snapshot = None
+    if not args.no_cpu:
+        from oracle.oracle import OracleDomain
+        cpu_snapshot = OracleDomain.from_domain(d)
+
+    h = d._dev()
+    h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    log("device prepared")
+    done, ev = h.evolve_steps(args.warmup)
+    h.call("sw_synchronize")
+    log("warmup %d steps done" % done)
+
+    t_start = time.perf_counter()
+    done, ev = h.evolve_steps(args.steps)
+    h.call("sw_synchronize")
+    elapsed = time.perf_counter() - t_start
+    if done != args.steps:
+        raise SystemExit("expected %d steps, ran %d (event %d)" % (args.steps, done, ev))
+    s = h.get_state()
+    value = N * args.steps / elapsed
+    log("timed: %d steps in %.4f s -> %.3e triangle-steps/s (t=%.3f s, dt=%.4e)" % (
+        args.steps, elapsed, value, s.time, s.timestep))
+
+    # per-kernel durations from HIP events on the library stream
+    h.call("sw_enable_timing", 1)
+    h.evolve_steps(args.timing_steps)
+    h.call("sw_enable_timing", 0)
+    ms, cnt = h.kernel_times(4)
+    avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
+    log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
+    dominant = max(("flux", "update", "distribute"), key=lambda k: avg_us[k] or 0.0)
+    bytes_launch = KERNEL_BYTES_PER_TRIANGLE[dominant] * N
+    achieved = bytes_launch / (avg_us[dominant] * 1e-6) / 1e9
+    step_gbs = STEP_BYTES_PER_TRIANGLE * N * args.steps / elapsed / 1e9
+    traffic = None
+    tfile = os.environ.get("SW_TRAFFIC_JSON")
+    if tfile and os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(dominant)
+
+    cpu = None
+    if cpu_snapshot is not None:
+        od = cpu_snapshot
+        od.evolve_prepare(1e9, None)
+        od.evolve_step()  # first step outside the sample
+        c0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            od.evolve_step()
+        c_el = time.perf_counter() - c0
+        cpu_model = platform.processor() or ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": N * args.cpu_steps / c_el, "unit": "triangle-steps/s", "cores": 1, "kind": "port",
+               "sample": "%d Euler steps of the same %d-triangle workload, serial FP64 C oracle "
+                         "(restated reference C), %.2f s on %s (nproc %d)" % (
+                             args.cpu_steps, N, c_el, cpu_model, os.cpu_count() or 0)}
+        log("cpu baseline: %.3e triangle-steps/s" % cpu["value"])
+
+    out = {
+        "metric": "triangle-steps/s at 1M tris, FV vs learned at stated error; % HBM peak",
+        "value": value,
+        "unit": "triangle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "plain FV, seeded Delaunay %d points in 10 km x 10 km, %s order, fBm terrain "
+                               "20 m, wet/dry mound, Manning 0.025, transmissive" % (args.points, args.order),
+                   "triangles": N, "precision": args.precision, "order": 2, "timestepping": "euler"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_triangle": KERNEL_BYTES_PER_TRIANGLE[dominant],
+                     "avg_launch_us": avg_us[dominant]},
+        "step_roofline": {"bytes_per_triangle_step": STEP_BYTES_PER_TRIANGLE, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS, "kernel_avg_us": avg_us},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    d.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,101 @@
+"""CPU checks of the C ABI library (loads, exports every declared symbol;
+no compute without a GPU), of the host mesh geometry and of the scenario
+builders."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from anuga_cuda_amd import _lib, scenarios
+from anuga_cuda_amd.mesh import Mesh, delaunay_random, morton_order, read_tsh, rectangular_cross
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "anuga_sw.h")).read()
+    return sorted(set(re.findall(r"\b(sw_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libanuga_sw.so not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared()
+    assert len(declared) >= 30
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_lib.exported_symbols())
+
+
+def test_struct_layouts_match_header():
+    # sizes of the three ABI structs as laid out by the C compiler
+    assert ctypes.sizeof(_lib.SwParams) == 15 * 8 + 8 * 4
+    assert ctypes.sizeof(_lib.SwMesh) == 2 * 8 + 13 * 8
+    assert ctypes.sizeof(_lib.SwState) == 9 * 8 + 7 * 8
+
+
+def test_rectangular_cross_geometry():
+    p, t, b = rectangular_cross(4, 3, 8.0, 6.0)
+    m = Mesh(p, t, boundary=b)
+    assert m.number_of_elements == 48
+    assert m.number_of_boundary_elements == 2 * (4 + 3)
+    np.testing.assert_allclose(m.areas.sum(), 48.0)
+    # sum of l_i n_i over a closed triangle vanishes
+    ln = m.normals.reshape(-1, 3, 2) * m.edgelengths[:, :, None]
+    np.testing.assert_allclose(ln.sum(1), 0.0, atol=1e-12)
+    # neighbour relation is symmetric and shared edges have opposite normals
+    N = m.number_of_elements
+    for k in range(N):
+        for i in range(3):
+            n = m.neighbours[k, i]
+            if n >= 0:
+                e = m.neighbour_edges[k, i]
+                assert m.neighbours[n, e] == k and m.neighbour_edges[n, e] == i
+                np.testing.assert_array_equal(m.normals[n, 2 * e:2 * e + 2], -m.normals[k, 2 * i:2 * i + 2])
+                assert m.edgelengths[n, e] == m.edgelengths[k, i]
+            else:
+                b_ = -n - 1
+                assert m.boundary_cells[b_] == k and m.boundary_edges[b_] == i
+    assert set(m.get_tag_names()) == {"left", "right", "top", "bottom"}
+    # radii: smallest centroid-to-midpoint distance
+    assert np.all(m.radii > 0) and np.all(m.radii <= np.sqrt(m.areas))
+
+
+def test_orientation_is_fixed_and_renumbering_is_a_relabelling():
+    p, t = delaunay_random(400, extent=(100.0, 100.0), seed=3)
+    t2 = t.copy()
+    t2[::2, 1], t2[::2, 2] = t[::2, 2], t[::2, 1]  # half clockwise
+    m = Mesh(p, t2)
+    v = m.vertex_coordinates
+    cross = (v[:, 2] - v[:, 0]) * (v[:, 5] - v[:, 1]) - (v[:, 4] - v[:, 0]) * (v[:, 3] - v[:, 1])
+    assert np.all(cross > 0)
+    mm = m.morton_renumbered()
+    perm = morton_order(*m.centroid_coordinates.T)
+    np.testing.assert_array_equal(mm.areas, m.areas[perm])
+    np.testing.assert_array_equal(mm.normals, m.normals[perm])
+    assert mm.number_of_boundary_elements == m.number_of_boundary_elements
+
+
+def test_merimbula_fixture_reads():
+    p, t, s = read_tsh(scenarios.MERIMBULA_TSH)
+    assert p.shape == (22137, 2) and t.shape == (43200, 3) and len(s) == 1072
+    m = Mesh(p, t, segments=s)
+    assert m.number_of_boundary_elements == 1072
+    assert set(m.get_tag_names()) == {"exterior", "open"}
+
+
+def test_dam_break_scenario_has_wet_and_dry_cells():
+    d = scenarios.dam_break_channel(20, 20)
+    h = d.quantities["stage"].centroid_values - d.quantities["elevation"].centroid_values
+    assert (h <= 0).any() and (h > 0.1).any()
+    assert d.number_of_elements == 1600
+
+
+def test_product_path_has_no_cpu_fallback(monkeypatch):
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libanuga_sw.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(ImportError):
+        _lib.load()

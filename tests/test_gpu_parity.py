@@ -1,0 +1,261 @@
+"""HIP path vs the CPU FP64 oracle, through the C ABI, on the same inputs.
+
+Bar (stated per comparison): stage / xmomentum / ymomentum within 1e-12
+normwise relative error per step (bit mismatches are counted and reported),
+identical wet/dry masks, identical step counts and timesteps within 1e-12.
+Manning friction calls pow(), whose last bit may differ between libm and the
+device math library; everything else is expected bit-identical.
+"""
+import numpy as np
+import pytest
+
+from anuga_cuda_amd import _lib, scenarios
+from anuga_cuda_amd.domain import Dirichlet_boundary, Domain, Reflective_boundary, Transmissive_boundary
+from anuga_cuda_amd.mesh import Mesh, rectangular_cross
+from oracle.oracle import OracleDomain
+from tests.helpers import compare, compare_state, device_array, wet_mask
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _dam(**kw):
+    return scenarios.dam_break_channel(24, 24, **kw)
+
+
+def _delaunay(order):
+    return scenarios.delaunay_terrain(npoints=3000, order=order, extent=2000.0, mound_radius=200.0,
+                                      seed=7, terrain_seed=11)
+
+
+SCENARIOS = {
+    "dam_rectcross": lambda: _dam(),
+    "delaunay_morton": lambda: _delaunay("morton"),
+    "delaunay_random": lambda: _delaunay("random"),
+}
+
+
+def _diagnose(domain, od, step):
+    m = domain.mesh
+    print("mismatch at step %d" % step)
+    for q in ("stage", "xmomentum", "ymomentum"):
+        for loc in ("explicit_update", "centroid_values", "vertex_values", "edge_values", "boundary_values"):
+            g = device_array(domain, q, loc)
+            e = od.q(q, loc).ravel()
+            bad = np.nonzero(np.abs(g - e) > 1e-12 * max(np.max(np.abs(e)), 1e-300))[0]
+            if bad.size:
+                k = bad[:6] if loc in ("centroid_values", "explicit_update") else bad[:6]
+                print("  %s.%s: %d bad, first %s" % (q, loc, bad.size, k.tolist()))
+                for i in k[:4]:
+                    tri = i if loc in ("centroid_values", "explicit_update") else (
+                        m.boundary_cells[i] if loc == "boundary_values" else i // 3)
+                    print("    idx %d tri %d nob %d nbr %s got %.17g exp %.17g" % (
+                        i, tri, m.number_of_boundaries[tri], m.neighbours[tri].tolist(), g[i], e[i]))
+
+
+def _lockstep(domain, nsteps, fused, stats, yieldstep=1e9, finaltime=None):
+    od = OracleDomain.from_domain(domain)
+    h = domain._dev()
+    h.call("sw_evolve_prepare", yieldstep, finaltime or 0.0, int(finaltime is not None))
+    od.evolve_prepare(yieldstep, finaltime)
+    compare_state(domain, od, TOL, stats=stats, tag="prepare.")
+    eps = domain.minimum_allowed_height
+    bed = od.q("elevation", "centroid_values")
+    dts = []
+    for step in range(nsteps):
+        if fused:
+            done, ev = h.evolve_steps(1)
+            assert done == 1
+        else:
+            ev = h.evolve_one_step_unfused()
+        ev_o = od.evolve_step()
+        s = h.get_state()
+        assert ev == ev_o, "event mismatch at step %d" % step
+        compare([s.timestep], [od.d.timestep], "timestep", TOL, stats)
+        compare([s.time], [od.d.time], "time", TOL, stats)
+        dts.append((s.timestep, od.d.timestep))
+        try:
+            compare_state(domain, od, TOL, stats=stats, tag="step.")
+            for q in ("stage", "xmomentum", "ymomentum"):
+                compare(device_array(domain, q, "explicit_update"), od.q(q, "explicit_update"),
+                        "step.%s.explicit_update" % q, TOL, stats)
+                if not fused:
+                    compare(device_array(domain, q, "boundary_values"), od.q(q, "boundary_values"),
+                            "step.%s.boundary_values" % q, TOL, stats)
+        except AssertionError:
+            _diagnose(domain, od, step)
+            raise
+        w = device_array(domain, "stage", "centroid_values")
+        assert np.array_equal(wet_mask(w, bed, eps), wet_mask(od.q("stage", "centroid_values"), bed, eps)), \
+            "wet/dry mask differs at step %d" % step
+        if ev != _lib.SW_EVENT_NONE:
+            break
+    return od, dts
+
+
+@pytest.mark.parametrize("name", sorted(SCENARIOS))
+def test_fused_evolve_matches_oracle(name):
+    d = SCENARIOS[name]()
+    stats = {}
+    od, dts = _lockstep(d, 25, fused=True, stats=stats)
+    bits = {k: v[0] for k, v in stats.items() if v[0]}
+    print("%s: N=%d steps=%d bit-mismatching fields %s" % (name, d.number_of_elements, len(dts), bits))
+    ms = d._handle.max_speed()
+    compare(ms, od.max_speed, "max_speed", TOL)
+
+
+@pytest.mark.parametrize("name", ["dam_rectcross", "delaunay_random"])
+def test_unfused_evolve_matches_oracle(name):
+    d = SCENARIOS[name]()
+    stats = {}
+    _lockstep(d, 8, fused=False, stats=stats)
+    for name_q in ("stage", "xmomentum", "ymomentum"):
+        compare(device_array(d, name_q, "boundary_values"), OracleDomain.from_domain(d).q(name_q, "boundary_values"),
+                name_q + ".boundary", TOL)
+
+
+def test_operators_match_oracle_one_by_one():
+    d = _dam()
+    # advance a few steps so momenta and wet/dry fronts are non-trivial
+    for t in d.evolve(yieldstep=0.5, finaltime=2.0):
+        pass
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    s = h.get_state()
+    s.yieldtime = 1e9
+    s.event = 0
+    h.set_state(s)
+    od.d.yieldtime = 1e9
+    st = {}
+    ft = h.compute_fluxes()
+    ft_o = od.compute_fluxes()
+    assert ft == ft_o, (ft, ft_o)
+    for q in ("stage", "xmomentum", "ymomentum"):
+        compare(device_array(d, q, "explicit_update"), od.q(q, "explicit_update"), q + ".flux", TOL, st)
+    compare(h.max_speed(), od.max_speed, "max_speed", 0.0)
+    h.call("sw_gravity_wb")
+    od.gravity_wb()
+    for q in ("xmomentum", "ymomentum"):
+        compare(device_array(d, q, "explicit_update"), od.q(q, "explicit_update"), q + ".gravity", 0.0, st)
+    h.call("sw_manning_friction")
+    od.manning_friction_implicit()
+    for q in ("xmomentum", "ymomentum"):
+        compare(device_array(d, q, "semi_implicit_update"), od.q(q, "semi_implicit_update"), q + ".friction",
+                TOL, st)
+    h.call("sw_update_timestep", 1e9, 0.0, 0)
+    od.update_timestep(None)
+    assert h.get_state().timestep == od.d.timestep
+    h.call("sw_update_conserved_quantities")
+    od.update_conserved_quantities()
+    for q in ("stage", "xmomentum", "ymomentum"):
+        compare(device_array(d, q, "centroid_values"), od.q(q, "centroid_values"), q + ".update", TOL, st)
+    for op, oop in (("sw_protect", od.protect),
+                    ("sw_extrapolate_second_order_sw", od.extrapolate_second_order_sw),
+                    ("sw_balance_deep_and_shallow", od.balance_deep_and_shallow),
+                    ("sw_interpolate_from_vertices_to_edges", od.interpolate_from_vertices_to_edges),
+                    ("sw_update_boundary", od.update_boundary),
+                    ("sw_update_other_quantities", od.update_other_quantities)):
+        h.call(op)
+        oop()
+        compare_state(d, od, TOL, locs=("centroid_values", "vertex_values", "edge_values", "boundary_values"),
+                      stats=st, tag=op + ".")
+    for q in ("height", "xvelocity", "yvelocity"):
+        for loc in ("centroid_values", "edge_values", "vertex_values", "boundary_values"):
+            compare(device_array(d, q, loc), od.q(q, loc), "%s.%s" % (q, loc), TOL, st)
+    print({k: v for k, v in st.items() if v[0]})
+
+
+def test_evolve_generator_yields_match_oracle():
+    d = _dam()
+    od = OracleDomain.from_domain(d)
+    times = []
+    steps = []
+    for t in d.evolve(yieldstep=0.7, finaltime=3.0):
+        times.append(t)
+        steps.append(d.number_of_steps)
+    times_o = []
+    steps_o = []
+    for t in od.evolve(0.7, 3.0):
+        times_o.append(t)
+        steps_o.append(od.d.number_of_steps)
+    assert steps == steps_o
+    compare(times, times_o, "yield times", TOL)
+    for q in ("stage", "xmomentum", "ymomentum"):
+        compare(d.quantities[q].centroid_values, od.q(q, "centroid_values"), q, TOL)
+        compare(d.quantities[q].vertex_values, od.q(q, "vertex_values"), q + ".vertex", TOL)
+    compare(d.quantities["height"].edge_values, od.q("height", "edge_values"), "height.edge", TOL)
+
+
+def test_merimbula_reference_step_count_on_gpu():
+    """The reference's published Merimbula run takes 153 steps to t = 50 s
+    (docs/profilling/CUDA/profile_on_GTX480_prepared_50_0.256246s)."""
+    d = scenarios.merimbula()
+    od = OracleDomain.from_domain(d)
+    total = 0
+    for t in d.evolve(yieldstep=50, finaltime=50):
+        total += d.number_of_steps
+    total_o = 0
+    for t in od.evolve(50, 50):
+        total_o += od.d.number_of_steps
+    assert total == total_o == 153
+    for q in ("stage", "xmomentum", "ymomentum"):
+        compare(d.quantities[q].centroid_values, od.q(q, "centroid_values"), q, TOL)
+
+
+def test_mixed_boundary_kinds():
+    points, tris, boundary = rectangular_cross(20, 10, 200.0, 100.0)
+    mesh = Mesh(points, tris, boundary=boundary)
+    d = Domain(mesh=mesh)
+    d.set_quantity("elevation", lambda x, y: -x / 100.0)
+    d.set_quantity("stage", lambda x, y: np.where(x < 50, 1.0, 0.2))
+    d.set_quantity("friction", 0.01, location="centroids")
+    d.set_boundary({"left": Dirichlet_boundary([1.5, 0.2, 0.0]), "right": Transmissive_boundary(d),
+                    "top": Reflective_boundary(d), "bottom": Reflective_boundary(d)})
+    stats = {}
+    _lockstep(d, 20, fused=True, stats=stats)
+
+
+def test_first_order_fallback_and_smallsteps_error():
+    """evolve_min_timestep above every CFL step forces ANUGA's fallback to first
+    order after max_smallsteps, then the too-small-timestep error."""
+    d = _dam(evolve_min_timestep=100.0, max_smallsteps=3)
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    od.evolve_prepare(1e9, None)
+    orders = []
+    err_dev = err_or = None
+    for step in range(12):
+        try:
+            h.evolve_steps(1)
+        except _lib.SwError as e:
+            err_dev = (step, e.code)
+        try:
+            od.evolve_step()
+        except Exception as e:  # OracleError
+            err_or = (step, e.code)
+        if err_dev or err_or:
+            break
+        orders.append(h.get_state().order)
+        assert h.get_state().order == od.d._order_
+        compare_state(d, od, TOL)
+    assert 1 in orders
+    assert err_dev is not None and err_or is not None
+    assert err_dev == (err_or[0], _lib.SW_ERR_SMALLSTEPS)
+
+
+def test_update_error_path_forced_from_host():
+    """A semi-implicit update with 1 - dt*si <= 0 is reported as an error, as
+    the C update returns -1 (quantity.c:803-805)."""
+    d = _dam()
+    h = d._dev()
+    s = h.get_state()
+    s.timestep = 1.0
+    h.set_state(s)
+    si = np.full(d.number_of_elements, 5.0)
+    stage = device_array(d, "stage", "centroid_values")
+    h.upload(_lib.SW_STAGE, _lib.SW_SEMI_IMPLICIT_UPDATE, si * stage)  # si/q = 5 -> 1 - 5 < 0
+    with pytest.raises(_lib.SwError) as exc:
+        h.call("sw_update_conserved_quantities")
+    assert exc.value.code == _lib.SW_ERR_UPDATE
