@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/anuga_sw.h"
@@ -23,7 +24,7 @@ namespace {
 
 constexpr int kGraphSteps = 16;
 constexpr int kBatchSteps = 8 * kGraphSteps;  // steps between host checks
-constexpr int kNumTimed = 4;                  // flux, timestep, update, distribute
+constexpr int kNumTimed = 3;                  // flux, update, distribute
 
 thread_local std::string g_create_error;
 
@@ -44,8 +45,9 @@ struct sw_handle {
     hipGraph_t graph = nullptr;
     bool timing = false;
     std::vector<hipEvent_t> ev;
-    double times_ms[kNumTimed] = {0, 0, 0, 0};
-    int64_t launches[kNumTimed] = {0, 0, 0, 0};
+    double times_ms[kNumTimed] = {0, 0, 0};
+    int64_t launches[kNumTimed] = {0, 0, 0};
+    double *staging = nullptr;  // [3N] transfers of packed records
 };
 
 namespace {
@@ -177,37 +179,69 @@ int sync_state(sw_handle *h, DevState *out) {
     return SW_OK;
 }
 
-double *field_ptr(sw_handle *h, int q, int loc, int64_t *count) {
-    DevQ *Q = nullptr;
+// Where one (quantity, location) array of the API lives on the device: a
+// per-triangle [N] array or a slot-major [3][N] array (slots = 1 or 3), either
+// plain doubles or component `comp` of the {w, uh, vh, z} records.
+struct Loc {
+    double *ptr = nullptr;
+    int slots = 1;
+    int comp = -1;
+    int64_t count = 0;  // values in the host array
+};
+
+bool locate(sw_handle *h, int q, int loc, Loc *out) {
+    DevFields &F = h->f;
+    const int64_t N = h->N, Nb = h->Nb;
+    Loc r;
+    auto set = [&](void *p, int slots, int comp, int64_t count) {
+        r.ptr = static_cast<double *>(p);
+        r.slots = slots;
+        r.comp = comp;
+        r.count = count;
+    };
     switch (q) {
-        case SW_STAGE: Q = &h->f.w; break;
-        case SW_XMOM: Q = &h->f.uh; break;
-        case SW_YMOM: Q = &h->f.vh; break;
-        case SW_ELEVATION: Q = &h->f.z; break;
-        case SW_HEIGHT: Q = &h->f.h; break;
-        case SW_XVELOCITY: Q = &h->f.u; break;
-        case SW_YVELOCITY: Q = &h->f.v; break;
+        case SW_STAGE: case SW_XMOM: case SW_YMOM: case SW_ELEVATION: {
+            const int comp = (q == SW_STAGE) ? 0 : (q == SW_XMOM) ? 1 : (q == SW_YMOM) ? 2 : 3;
+            double *vert = (q == SW_STAGE) ? F.wv : (q == SW_XMOM) ? F.uhv : (q == SW_YMOM) ? F.vhv : F.zv;
+            double *bnd = (q == SW_STAGE) ? F.wb : (q == SW_XMOM) ? F.uhb : (q == SW_YMOM) ? F.vhb : F.zb;
+            double *eu = (q == SW_STAGE) ? F.weu : (q == SW_XMOM) ? F.uheu : (q == SW_YMOM) ? F.vheu : nullptr;
+            double *si = (q == SW_STAGE) ? F.wsi : (q == SW_XMOM) ? F.uhsi : (q == SW_YMOM) ? F.vhsi : nullptr;
+            switch (loc) {
+                case SW_CENTROID: set(F.cq, 1, comp, N); break;
+                case SW_EDGE: set(F.eq, 3, comp, 3 * N); break;
+                case SW_VERTEX: set(vert, 3, -1, 3 * N); break;
+                case SW_BOUNDARY: set(bnd, 1, -1, Nb); break;
+                case SW_EXPLICIT_UPDATE: if (!eu) return false; set(eu, 1, -1, N); break;
+                case SW_SEMI_IMPLICIT_UPDATE: if (!si) return false; set(si, 1, -1, N); break;
+                default: return false;
+            }
+            break;
+        }
+        case SW_HEIGHT: case SW_XVELOCITY: case SW_YVELOCITY: {
+            double *c = (q == SW_HEIGHT) ? F.hc : (q == SW_XVELOCITY) ? F.uc : F.vc;
+            double *e = (q == SW_HEIGHT) ? F.he : (q == SW_XVELOCITY) ? F.ue : F.ve;
+            double *v = (q == SW_HEIGHT) ? F.hv : (q == SW_XVELOCITY) ? F.uv : F.vv;
+            double *b = (q == SW_HEIGHT) ? F.hb : (q == SW_XVELOCITY) ? F.ub : F.vb;
+            switch (loc) {
+                case SW_CENTROID: set(c, 1, -1, N); break;
+                case SW_EDGE: set(e, 3, -1, 3 * N); break;
+                case SW_VERTEX: set(v, 3, -1, 3 * N); break;
+                case SW_BOUNDARY: set(b, 1, -1, Nb); break;
+                default: return false;
+            }
+            break;
+        }
         case SW_FRICTION:
-            if (loc != SW_CENTROID) return nullptr;
-            *count = h->N;
-            return h->f.friction;
-        default: return nullptr;
+            if (loc != SW_CENTROID) return false;
+            set(F.friction, 1, -1, N);
+            break;
+        default: return false;
     }
-    switch (loc) {
-        case SW_CENTROID: *count = h->N; return Q->c;
-        case SW_EDGE: *count = 3 * h->N; return Q->e;
-        case SW_VERTEX: *count = 3 * h->N; return Q->v;
-        case SW_BOUNDARY: *count = h->Nb; return Q->b;
-        case SW_EXPLICIT_UPDATE: *count = h->N; return Q->eu;
-        case SW_SEMI_IMPLICIT_UPDATE: *count = h->N; return Q->si;
-        default: return nullptr;
-    }
+    *out = r;
+    return true;
 }
 
-__global__ void k_zbar(int N, const double *zv, double *zbar) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k < N) zbar[k] = (zv[3 * k] + zv[3 * k + 1] + zv[3 * k + 2]) / 3.0;
-}
+__global__ void k_set_bed_interp(DevState *st, long long v) { st->bed_interp = v; }
 
 // Enqueues one fused step.  slot >= 0 records HIP events around every kernel
 // into event group `slot` (timing mode).
@@ -223,15 +257,13 @@ int enqueue_fused_step(sw_handle *h, int slot) {
     else
         hipLaunchKernelGGL((k_flux<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     rec(1);
-    hipLaunchKernelGGL(k_step_timestep, dim3(1), dim3(1), 0, h->stream, h->st, h->dp);
-    rec(2);
     hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    rec(3);
+    rec(2);
     if (mixed)
         hipLaunchKernelGGL((k_distribute<float, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     else
         hipLaunchKernelGGL((k_distribute<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    rec(4);
+    rec(3);
     return launch_check(h, "fused step");
 }
 
@@ -281,8 +313,8 @@ int epilogue(sw_handle *h) {
     hipLaunchKernelGGL(k_update_boundary, dim3(grid_for(h->Nb) ? grid_for(h->Nb) : 1), dim3(kBlock), 0,
                        h->stream, h->mesh, h->f);
     hipLaunchKernelGGL(k_other_quantities, dim3(gb), dim3(kBlock), 0, h->stream, h->mesh, h->f);
-    hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, h->stream, h->f.uh.si, (long long)h->N, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, h->stream, h->f.vh.si, (long long)h->N, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, h->stream, h->f.uhsi, (long long)h->N, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(g), dim3(kBlock), 0, h->stream, h->f.vhsi, (long long)h->N, 0.0);
     return launch_check(h, "evolve epilogue");
 }
 
@@ -303,6 +335,85 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     if (N <= 0 || N > (int64_t)0x2AAAAAAA || Nb < 0 || Nb > 3 * N) {
         fail(nullptr, SW_ERR_ARG, "bad sizes N=%lld Nb=%lld", (long long)N, (long long)Nb);
         return nullptr;
+    }
+    const int64_t *arrays_i[] = {m->neighbours, m->neighbour_edges, m->surrogate_neighbours,
+                                 m->number_of_boundaries, m->tri_full_flag};
+    const double *arrays_f[] = {m->normals, m->edgelengths, m->radii, m->areas,
+                                m->centroid_coordinates, m->vertex_coordinates};
+    for (auto a : arrays_i)
+        if (!a) { fail(nullptr, SW_ERR_ARG, "a mesh index array is NULL"); return nullptr; }
+    for (auto a : arrays_f)
+        if (!a) { fail(nullptr, SW_ERR_ARG, "a mesh geometry array is NULL"); return nullptr; }
+    if (Nb > 0 && (!m->boundary_cells || !m->boundary_edges)) {
+        fail(nullptr, SW_ERR_ARG, "boundary arrays are NULL");
+        return nullptr;
+    }
+    // Host-side validation, packing of the neighbour structure and transposition
+    // of the per-edge / per-vertex geometry to the slot-major [3][N] layout.
+    std::vector<int> nbv(3 * N), flagsv(N), surv(3 * N);
+    std::vector<double> nrm(6 * N), lenv(3 * N), vcv(6 * N), ccv(2 * N);
+    bool own_normals = true;
+    for (int64_t k = 0; k < N; k++) {
+        int flags = 0;
+        int64_t key[3];
+        for (int i = 0; i < 3; i++) {
+            const int64_t ki = 3 * k + i;
+            const int64_t n = m->neighbours[ki];
+            int64_t computing = k;
+            key[i] = (int64_t)1 << 62 | i;  // own edges after lower neighbours, in local order
+            if (n < 0) {
+                const int64_t b = -n - 1;
+                if (b >= Nb || m->boundary_cells[b] != k || m->boundary_edges[b] != i) {
+                    fail(nullptr, SW_ERR_ARG, "neighbours[%lld] = %lld does not match boundary entry %lld",
+                         (long long)ki, (long long)n, (long long)b);
+                    return nullptr;
+                }
+                nbv[i * N + k] = (int)n;
+            } else {
+                const int64_t e = m->neighbour_edges[ki];
+                if (n >= N || e < 0 || e > 2 || m->neighbours[3 * n + e] != k) {
+                    fail(nullptr, SW_ERR_ARG, "neighbour structure is not symmetric at edge %lld", (long long)ki);
+                    return nullptr;
+                }
+                nbv[i * N + k] = (int)(e * N + n);
+                if (n < k) {
+                    flags |= 1 << i;
+                    computing = n;
+                    key[i] = n;
+                }
+                const int64_t ne = 3 * n + e;
+                if (m->normals[2 * ne] != -m->normals[2 * ki] || m->normals[2 * ne + 1] != -m->normals[2 * ki + 1] ||
+                    m->edgelengths[ne] != m->edgelengths[ki])
+                    own_normals = false;
+            }
+            if (m->tri_full_flag[computing] == 1) flags |= 1 << (3 + i);
+            const int64_t s = m->surrogate_neighbours[ki];
+            if (s < 0 || s >= N) {
+                fail(nullptr, SW_ERR_ARG, "surrogate_neighbours[%lld] out of range", (long long)ki);
+                return nullptr;
+            }
+            surv[i * N + k] = (int)s;
+            nrm[2 * (i * N + k)] = m->normals[2 * ki];
+            nrm[2 * (i * N + k) + 1] = m->normals[2 * ki + 1];
+            lenv[i * N + k] = m->edgelengths[ki];
+            vcv[2 * (i * N + k)] = m->vertex_coordinates[6 * k + 2 * i];
+            vcv[2 * (i * N + k) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
+        }
+        // summation order of the C edge-once loop: lower neighbours ascending, then own edges
+        int ord[3] = {0, 1, 2};
+        for (int a = 0; a < 3; a++)
+            for (int b = a + 1; b < 3; b++)
+                if (key[ord[b]] < key[ord[a]]) std::swap(ord[a], ord[b]);
+        flags |= (ord[0] | ord[1] << 2 | ord[2] << 4) << 6;
+        const int64_t nob = m->number_of_boundaries[k];
+        if (nob < 0 || nob > 3) {
+            fail(nullptr, SW_ERR_ARG, "number_of_boundaries[%lld] = %lld", (long long)k, (long long)nob);
+            return nullptr;
+        }
+        flags |= (int)nob << 12;
+        flagsv[k] = flags;
+        ccv[2 * k] = m->centroid_coordinates[2 * k];
+        ccv[2 * k + 1] = m->centroid_coordinates[2 * k + 1];
     }
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
@@ -327,18 +438,18 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     DevMesh &M = h->mesh;
     M.N = (int)N;
     M.Nb = (int)Nb;
-    int *nbr, *nbr_edge, *sur, *nob, *full, *bcell, *bedge, *bkind;
-    double *normals, *len, *radii, *areas, *cc, *vc, *bdir;
-    if (upload_i32(h, &nbr, m->neighbours, 3 * N, -Nb, N - 1, "neighbours") ||
-        upload_i32(h, &nbr_edge, m->neighbour_edges, 3 * N, -1, 2, "neighbour_edges") ||
-        upload_i32(h, &sur, m->surrogate_neighbours, 3 * N, 0, N - 1, "surrogate_neighbours") ||
-        upload_i32(h, &nob, m->number_of_boundaries, N, 0, 3, "number_of_boundaries") ||
-        upload_i32(h, &full, m->tri_full_flag, N, -1, 1, "tri_full_flag") ||
-        upload_f64(h, &normals, m->normals, 6 * N, "normals") ||
-        upload_f64(h, &len, m->edgelengths, 3 * N, "edgelengths") ||
-        upload_f64(h, &radii, m->radii, N, "radii") || upload_f64(h, &areas, m->areas, N, "areas") ||
-        upload_f64(h, &cc, m->centroid_coordinates, 2 * N, "centroid_coordinates") ||
-        upload_f64(h, &vc, m->vertex_coordinates, 6 * N, "vertex_coordinates"))
+    M.own_normals = own_normals ? 1 : 0;
+    int *dnb, *dflags, *dsur, *bcell, *bedge, *bkind;
+    double *dnrm, *dlen, *dvc, *dcc, *radii, *areas, *bdir;
+    auto put = [&](auto **dst, const auto &vec) -> int {
+        if (dalloc(h, dst, vec.size())) return -1;
+        if (hipMemcpy(*dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(h, SW_ERR_HIP, "mesh upload failed");
+        return 0;
+    };
+    if (put(&dnb, nbv) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dnrm, nrm) || put(&dlen, lenv) ||
+        put(&dvc, vcv) || put(&dcc, ccv) ||
+        upload_f64(h, &radii, m->radii, N, "radii") || upload_f64(h, &areas, m->areas, N, "areas"))
         return bail();
     if (Nb > 0) {
         if (upload_i32(h, &bcell, m->boundary_cells, Nb, 0, N - 1, "boundary_cells") ||
@@ -347,57 +458,37 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     } else {
         if (dalloc(h, &bcell, 1) || dalloc(h, &bedge, 1)) return bail();
     }
-    // every boundary neighbour index must refer to a listed boundary edge
-    for (int64_t i = 0; i < 3 * N; i++) {
-        const int64_t n = m->neighbours[i];
-        if (n < 0) {
-            const int64_t b = -n - 1;
-            if (m->boundary_cells[b] != i / 3 || m->boundary_edges[b] != i % 3) {
-                fail(h, SW_ERR_ARG, "neighbours[%lld] = %lld does not match boundary entry %lld",
-                     (long long)i, (long long)n, (long long)b);
-                return bail();
-            }
-        } else if (m->neighbour_edges[i] < 0) {
-            fail(h, SW_ERR_ARG, "neighbour_edges[%lld] is negative for an interior edge", (long long)i);
-            return bail();
-        }
-    }
     if (dalloc(h, &bkind, Nb) || dalloc(h, &bdir, 3 * Nb)) return bail();
-    M.nbr = nbr;
-    M.nbr_edge = nbr_edge;
-    M.sur = sur;
-    M.nob = nob;
-    M.full = full;
-    M.normals = normals;
-    M.len = len;
+    M.nb = dnb;
+    M.flags = dflags;
+    M.sur = dsur;
+    M.normals = reinterpret_cast<const double2 *>(dnrm);
+    M.len = dlen;
+    M.vc = reinterpret_cast<const double2 *>(dvc);
+    M.cc = reinterpret_cast<const double2 *>(dcc);
     M.radii = radii;
     M.areas = areas;
-    M.cc = cc;
-    M.vc = vc;
     M.bcell = bcell;
     M.bedge = bedge;
     M.bkind = bkind;
     M.bdir = bdir;
 
     DevFields &F = h->f;
-    DevQ *conserved[3] = {&F.w, &F.uh, &F.vh};
-    for (DevQ *Q : conserved) {
-        if (dalloc(h, &Q->c, N) || dalloc(h, &Q->e, 3 * N) || dalloc(h, &Q->v, 3 * N) ||
-            dalloc(h, &Q->b, Nb) || dalloc(h, &Q->eu, N) || dalloc(h, &Q->si, N))
-            return bail();
-    }
-    DevQ *others[4] = {&F.z, &F.h, &F.u, &F.v};
-    for (DevQ *Q : others) {
-        if (dalloc(h, &Q->c, N) || dalloc(h, &Q->e, 3 * N) || dalloc(h, &Q->v, 3 * N) ||
-            dalloc(h, &Q->b, Nb))
-            return bail();
-        Q->eu = Q->si = nullptr;
-    }
-    if (dalloc(h, &F.friction, N) || dalloc(h, &F.max_speed, N) || dalloc(h, &F.zbar, N)) return bail();
+    double **perN[] = {&F.weu, &F.uheu, &F.vheu, &F.wsi, &F.uhsi, &F.vhsi, &F.hc, &F.uc, &F.vc,
+                       &F.friction, &F.zbar, &F.max_speed};
+    double **per3N[] = {&F.wv, &F.uhv, &F.vhv, &F.zv, &F.ze, &F.he, &F.hv, &F.ue, &F.uv, &F.ve, &F.vv};
+    double **perNb[] = {&F.wb, &F.uhb, &F.vhb, &F.zb, &F.hb, &F.ub, &F.vb};
+    for (auto pp : perN)
+        if (dalloc(h, pp, N)) return bail();
+    for (auto pp : per3N)
+        if (dalloc(h, pp, 3 * N)) return bail();
+    for (auto pp : perNb)
+        if (dalloc(h, pp, Nb)) return bail();
+    if (dalloc(h, &F.cq, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &h->staging, 3 * N)) return bail();
     if (dalloc(h, &h->st, 1)) return bail();
     DevState s0{};
     s0.order = p->default_order;
-    s0.flux_min_bits = 0;
+    s0.bed_interp = 1;
     std::memcpy(&s0.flux_min_bits, &p->evolve_max_timestep, sizeof(double));
     if (hipMemcpy(h->st, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
         fail(h, SW_ERR_HIP, "state upload failed");
@@ -445,6 +536,7 @@ int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet) 
             return fail(h, SW_ERR_ARG, "Dirichlet boundary without values");
         k[i] = kind[i];
     }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
     HIPCHECK(h, hipMemcpy((void *)h->mesh.bkind, k.data(), h->Nb * sizeof(int), hipMemcpyHostToDevice));
     if (dirichlet)
         HIPCHECK(h, hipMemcpy((void *)h->mesh.bdir, dirichlet, 3 * h->Nb * sizeof(double), hipMemcpyHostToDevice));
@@ -453,28 +545,55 @@ int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet) 
 
 int sw_upload(sw_handle *h, int quantity, int location, const double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
-    int64_t n = 0;
-    double *d = field_ptr(h, quantity, location, &n);
-    if (!d) return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
-    if (count != n) return fail(h, SW_ERR_ARG, "upload of %lld values into an array of %lld", (long long)count, (long long)n);
+    Loc L;
+    if (!locate(h, quantity, location, &L))
+        return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
+    if (count != L.count)
+        return fail(h, SW_ERR_ARG, "upload of %lld values into an array of %lld", (long long)count, (long long)L.count);
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    if (n) HIPCHECK(h, hipMemcpy(d, host, n * sizeof(double), hipMemcpyHostToDevice));
-    if (quantity == SW_ELEVATION && location == SW_VERTEX) {
-        hipLaunchKernelGGL(k_zbar, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, (int)h->N, h->f.z.v, h->f.zbar);
-        if (int rc = launch_check(h, "k_zbar")) return rc;
-        HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (L.count == 0) return SW_OK;
+    if (L.slots == 1 && L.comp < 0) {
+        HIPCHECK(h, hipMemcpy(L.ptr, host, L.count * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        const long long n = L.count / L.slots;
+        HIPCHECK(h, hipMemcpy(h->staging, host, L.count * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, n, L.slots, L.comp,
+                           h->staging, L.ptr);
+        if (int rc = launch_check(h, "k_to_device")) return rc;
+        if (quantity == SW_ELEVATION && location == SW_EDGE) {
+            hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, n, 3, -1,
+                               h->staging, h->f.ze);
+            if (int rc = launch_check(h, "k_to_device")) return rc;
+        }
     }
+    if (quantity == SW_ELEVATION && (location == SW_VERTEX || location == SW_EDGE)) {
+        hipLaunchKernelGGL(k_set_bed_interp, dim3(1), dim3(1), 0, h->stream, h->st, 1ll);
+        hipLaunchKernelGGL(k_bed_derived, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, (int)h->N, h->f.zv,
+                           h->f.ze, h->f.zbar, h->st);
+        if (int rc = launch_check(h, "k_bed_derived")) return rc;
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
     return SW_OK;
 }
 
 int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
-    int64_t n = 0;
-    double *d = field_ptr(h, quantity, location, &n);
-    if (!d) return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
-    if (count != n) return fail(h, SW_ERR_ARG, "download of %lld values from an array of %lld", (long long)count, (long long)n);
+    Loc L;
+    if (!locate(h, quantity, location, &L))
+        return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
+    if (count != L.count)
+        return fail(h, SW_ERR_ARG, "download of %lld values from an array of %lld", (long long)count, (long long)L.count);
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    if (n) HIPCHECK(h, hipMemcpy(host, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (L.count == 0) return SW_OK;
+    if (L.slots == 1 && L.comp < 0) {
+        HIPCHECK(h, hipMemcpy(host, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)(L.count / L.slots),
+                           L.slots, L.comp, L.ptr, h->staging);
+        if (int rc = launch_check(h, "k_to_host")) return rc;
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        HIPCHECK(h, hipMemcpy(host, h->staging, L.count * sizeof(double), hipMemcpyDeviceToHost));
+    }
     return SW_OK;
 }
 
@@ -500,8 +619,7 @@ int sw_set_state(sw_handle *h, const sw_state *s) {
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     HIPCHECK(h, hipMemcpy(&d, h->st, sizeof d, hipMemcpyDeviceToHost));
     std::memcpy(&d, s, sizeof(sw_state));
-    d.halt = d.halt_next = 0;
-    if (s->event != SW_EVENT_NONE) d.halt = d.halt_next = 1;
+    d.halt = (s->event != SW_EVENT_NONE) ? 1 : 0;
     HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     return SW_OK;
 }
@@ -574,10 +692,7 @@ int sw_extrapolate_second_order_sw(sw_handle *h) {
 
 int sw_extrapolate_first_order(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
-    const unsigned g = grid_for(h->N);
-    DevQ *qs[3] = {&h->f.w, &h->f.uh, &h->f.vh};
-    for (DevQ *Q : qs)
-        hipLaunchKernelGGL(k_first_order, dim3(g), dim3(kBlock), 0, h->stream, (int)h->N, Q->c, Q->e, Q->v);
+    hipLaunchKernelGGL(k_first_order_conserved, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "extrapolate_first_order");
 }
 
@@ -628,6 +743,7 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
     d.order = h->params.default_order;
     HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
+    if (int rc = sync_state(h, &d)) return rc;
     d.time = d.starttime;
     d.yieldstep = yieldstep;
     d.order = h->params.default_order;
@@ -641,9 +757,9 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
     d.smallsteps = 0;
     d.event = 0;
     d.error = 0;
-    d.halt = d.halt_next = 0;
+    d.halt = 0;
+    d.blocks_done = 0;
     std::memcpy(&d.flux_min_bits, &h->params.evolve_max_timestep, sizeof(double));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
     HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
     if (int rc = sw_update_boundary(h)) return rc;
@@ -654,7 +770,7 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     if (!h || max_steps < 0) return SW_ERR_ARG;
     DevState d0, d;
     if (int rc = sync_state(h, &d0)) return rc;
-    if (d0.event != SW_EVENT_NONE || d0.halt || d0.halt_next)
+    if (d0.event != SW_EVENT_NONE || d0.halt)
         return fail(h, SW_ERR_STATE, "evolve_steps called while a yield is pending (call sw_advance_yield)");
     int64_t launched = 0;
     int rc = SW_OK;
@@ -679,7 +795,7 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         launched += done_here;
         if (rc) break;
         rc = sync_state(h, &d);
-        if (rc || d.event != SW_EVENT_NONE || d.halt_next) break;
+        if (rc || d.event != SW_EVENT_NONE || d.halt) break;
     }
     int rc2 = epilogue(h);
     int rc3 = sync_state(h, &d);
@@ -700,7 +816,7 @@ int sw_advance_yield(sw_handle *h) {
     d.number_of_steps = 0;
     d.number_of_first_order_steps = 0;
     d.event = 0;
-    d.halt = d.halt_next = 0;
+    d.halt = 0;
     HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     return SW_OK;
 }
@@ -724,7 +840,7 @@ int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event) {
     if (int rc = launch_check(h, "step bookkeeping")) return rc;
     int rc = sync_state(h, &d);
     if (d.event != SW_EVENT_NONE) {
-        d.halt = d.halt_next = 1;
+        d.halt = 1;
         HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     }
     if (event) *event = (int32_t)d.event;
@@ -749,11 +865,16 @@ int sw_enable_timing(sw_handle *h, int32_t enable) {
 
 int sw_kernel_times(sw_handle *h, double *ms, int64_t *launches, int32_t nkernels) {
     if (!h || nkernels < 0) return SW_ERR_ARG;
-    for (int i = 0; i < nkernels && i < kNumTimed; i++) {
-        if (ms) ms[i] = h->times_ms[i];
-        if (launches) launches[i] = h->launches[i];
-        h->times_ms[i] = 0;
-        h->launches[i] = 0;
+    for (int i = 0; i < nkernels; i++) {
+        if (i < kNumTimed) {
+            if (ms) ms[i] = h->times_ms[i];
+            if (launches) launches[i] = h->launches[i];
+            h->times_ms[i] = 0;
+            h->launches[i] = 0;
+        } else {
+            if (ms) ms[i] = 0;
+            if (launches) launches[i] = 0;
+        }
     }
     return SW_OK;
 }

@@ -20,6 +20,17 @@
 
 namespace sw {
 
+// Loads / stores of data a kernel touches once (mesh geometry, per-triangle
+// inputs and outputs): non-temporal, so they stream past the L2 and leave it to
+// the edge and centroid records that neighbouring triangles gather.
+template <typename T> SW_DEV T ld_s(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename T> SW_DEV void st_s(T *p, T v) { __builtin_nontemporal_store(v, p); }
+typedef double sw_f64x2 __attribute__((ext_vector_type(2)));
+SW_DEV double2 ld_s(const double2 *p) {
+    const sw_f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const sw_f64x2 *>(p));
+    return make_double2(v.x, v.y);
+}
+
 // util_ext.h:42-56: comparisons, not fmax/fmin (NaN propagates the same way)
 template <typename T> SW_DEV T omax(T x, T y) { return (x > y) ? x : y; }
 template <typename T> SW_DEV T omin(T x, T y) { return (x < y) ? x : y; }
@@ -122,30 +133,34 @@ SW_DEV void gradient3(double x0, double y0, double x1, double y1, double x2, dou
     b /= det;
 }
 
-// shallow_water_ext.c:98-149 (find_qmin_and_qmax)
+// shallow_water_ext.c:98-149 (find_qmin_and_qmax); results returned by value
+// so they stay in registers.
 template <typename R>
-SW_DEV void find_qmin_and_qmax(R dq0, R dq1, R dq2, R &qmin, R &qmax) {
+struct QRange {
+    R qmin, qmax;
+};
+
+template <typename R>
+SW_DEV QRange<R> find_qmin_and_qmax(R dq0, R dq1, R dq2) {
+    R qmin, qmax;
     if (dq0 >= R(0)) {
-        if (dq1 >= dq2) {
-            qmax = (dq1 >= R(0)) ? dq0 + dq1 : dq0;
-            qmin = dq0 + dq2;
-            if (qmin >= R(0)) qmin = R(0);
-        } else {
-            qmax = (dq2 > R(0)) ? dq0 + dq2 : dq0;
-            qmin = dq0 + dq1;
-            if (qmin >= R(0)) qmin = R(0);
-        }
+        const bool a = dq1 >= dq2;
+        const R hi = a ? dq1 : dq2;
+        const R lo = a ? dq2 : dq1;
+        // dq1 >= dq2: qmax = dq1 >= 0 ? dq0 + dq1 : dq0;  else qmax = dq2 > 0 ? dq0 + dq2 : dq0
+        const bool take = a ? (hi >= R(0)) : (hi > R(0));
+        qmax = take ? dq0 + hi : dq0;
+        qmin = dq0 + lo;
+        if (qmin >= R(0)) qmin = R(0);
     } else {
-        if (dq1 <= dq2) {
-            qmin = (dq1 < R(0)) ? dq0 + dq1 : dq0;
-            qmax = dq0 + dq2;
-            if (qmax <= R(0)) qmax = R(0);
-        } else {
-            qmin = (dq2 < R(0)) ? dq0 + dq2 : dq0;
-            qmax = dq0 + dq1;
-            if (qmax <= R(0)) qmax = R(0);
-        }
+        const bool a = dq1 <= dq2;
+        const R lo = a ? dq1 : dq2;
+        const R hi = a ? dq2 : dq1;
+        qmin = (lo < R(0)) ? dq0 + lo : dq0;
+        qmax = dq0 + hi;
+        if (qmax <= R(0)) qmax = R(0);
     }
+    return {qmin, qmax};
 }
 
 // shallow_water_ext.c:151-183 (limit_gradient); r0 carries over between i.
@@ -180,9 +195,8 @@ SW_DEV void extrap_aux(R qk, R q0, R q1, R q2, R dx1, R dx2, R dy1, R dy2, R inv
     dqv[0] = a * dxv[0] + b * dyv[0];
     dqv[1] = a * dxv[1] + b * dyv[1];
     dqv[2] = a * dxv[2] + b * dyv[2];
-    R qmin, qmax;
-    find_qmin_and_qmax<R>(dq0, dq1, dq2, qmin, qmax);
-    limit_gradient<R>(dqv, qmin, qmax, beta);
+    const QRange<R> q = find_qmin_and_qmax<R>(dq0, dq1, dq2);
+    limit_gradient<R>(dqv, q.qmin, q.qmax, beta);
 }
 
 // Reconstruction toward the single interior neighbour (shallow_water_ext.c:3174-3213).
@@ -195,14 +209,8 @@ SW_DEV void extrap_one(R qk, R q1, R dx2, R dy2, const R dxv[3], const R dyv[3],
     dqv[0] = a * dxv[0] + b * dyv[0];
     dqv[1] = a * dxv[1] + b * dyv[1];
     dqv[2] = a * dxv[2] + b * dyv[2];
-    R qmin, qmax;
-    if (dq1 >= R(0)) {
-        qmin = R(0);
-        qmax = dq1;
-    } else {
-        qmin = dq1;
-        qmax = R(0);
-    }
+    const R qmin = (dq1 >= R(0)) ? R(0) : dq1;
+    const R qmax = (dq1 >= R(0)) ? dq1 : R(0);
     limit_gradient<R>(dqv, qmin, qmax, beta);
 }
 
@@ -331,6 +339,28 @@ SW_DEV void reflect_momentum(double q1, double q2, double n1, double n2, double 
 // unsigned 64-bit atomic min of the bits.
 SW_DEV unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 SW_DEV double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Workgroup minimum of positive doubles as bit patterns: wavefront (64-lane)
+// shuffle reduction, then one LDS pass across the wavefronts.  The result is
+// valid in thread 0.
+template <int BLOCK>
+SW_DEV unsigned long long block_min_bits(double v) {
+    unsigned long long b = dbits(v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = (o < b) ? o : b;
+    }
+    constexpr int NW = BLOCK / 64;
+    __shared__ unsigned long long wmin[NW];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wmin[wid] = b;
+    __syncthreads();
+    unsigned long long m = wmin[0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) m = (wmin[w] < m) ? wmin[w] : m;
+    return m;
+}
 
 // Wavefront (64-lane) minimum with cross-lane shuffles, then one LDS pass
 // across the block's wavefronts; one atomic per block.

@@ -3,21 +3,35 @@
 // Two families share the numerics in sw_device.h:
 //   * operator kernels, one per overridden ANUGA operator (the drop-in API:
 //     each mirrors one reference kernel / C routine and its array contract);
-//   * fused step kernels used by evolve(): per Euler step
-//       flux   = compute_fluxes + boundary evaluation + gravity_wb + Manning
-//                friction + CFL minimum (wavefront/LDS reduction, one atomic
-//                per workgroup),
-//       tstep  = update_timestep and time bookkeeping (one lane),
-//       update = update_conserved_quantities + protect,
+//   * fused step kernels used by evolve(); per Euler step
+//       flux       = compute_fluxes + boundary evaluation + gravity_wb +
+//                    Manning friction + CFL minimum (wavefront shuffles, one
+//                    atomic per workgroup) + update_timestep and the time
+//                    bookkeeping, done by the workgroup that finishes last;
+//       update     = update_conserved_quantities + protect;
 //       distribute = extrapolate_second_order_sw + balance_deep_and_shallow +
-//                interpolate_from_vertices_to_edges,
-//     so the explicit / semi-implicit updates are the only intermediates that
-//     round-trip through HBM and the host is not involved between steps.
+//                    interpolate_from_vertices_to_edges;
+//     the explicit / semi-implicit updates are the only intermediates that go
+//     through HBM and the host is not involved between steps.
 //
-// One lane per triangle; 256-lane workgroups (4 wavefronts of 64); the
-// workgroup index is remapped so that each of the 8 XCDs works on one
-// contiguous range of triangles and neighbour gathers of a locality-ordered
-// mesh hit that XCD's own L2.
+// Data layout in HBM (one lane per triangle, 256-lane workgroups).  Per-edge
+// and per-vertex data are slot-major: slot i (edge i faces vertex i) of
+// triangle k lives at i*N + k, so one 64-lane load instruction reads whole
+// 128-byte lines that no later instruction needs again:
+//   * centroid state  cq[k]     = {w, uh, vh, z}  32-byte record
+//   * edge state      eq[i*N+k] = {w, uh, vh, z}  32-byte record; a
+//     neighbour's edge is a single 32-byte gather;
+//   * nb[i*N+k] = m*N+n, the slot of the neighbour's matching edge, or
+//     -(b+1) for boundary edge b;
+//   * flags[k]: bit i edge i is evaluated from the lower-numbered neighbour;
+//     bit 3+i the evaluating triangle is full; bits 6-11 the order in which
+//     the three edge fluxes are summed (2 bits per position); bits 12-13
+//     number_of_boundaries;
+//   * normals (double2), edge lengths, vertex coordinates (double2), vertex
+//     values, surrogate neighbours: [3][N]; centroids (double2), radii,
+//     areas, updates: [N].
+// Workgroup indices are remapped so each of the 8 XCDs sweeps one contiguous
+// range of a locality-ordered mesh and neighbour gathers hit its own L2.
 #pragma once
 
 #include "sw_device.h"
@@ -25,6 +39,10 @@
 namespace sw {
 
 constexpr int kBlock = 256;
+
+struct alignas(32) Q4 {
+    double w, uh, vh, z;
+};
 
 struct DevParams {
     double g, epsilon, H0, h0, limiting_threshold, minimum_allowed_height, maximum_allowed_speed;
@@ -36,21 +54,31 @@ struct DevParams {
 
 struct DevMesh {
     int N, Nb;
-    const int *nbr, *nbr_edge, *sur, *nob, *full;
-    const double *normals, *len, *radii, *areas, *cc, *vc;
-    const int *bcell, *bedge, *bkind;
-    const double *bdir;
-};
-
-struct DevQ {
-    double *c, *e, *v, *b, *eu, *si;
+    int own_normals;  // shared-edge normals are exact negations, lengths equal
+    const int *nb;    // [3][N]
+    const int *flags; // [N]
+    const int *sur;   // [3][N]
+    const double2 *normals;  // [3][N]
+    const double *len;       // [3][N]
+    const double2 *vc;       // [3][N] vertex coordinates
+    const double2 *cc;       // [N]    centroid coordinates
+    const double *radii, *areas;  // [N]
+    const int *bcell, *bedge, *bkind;  // [Nb]
+    const double *bdir;                // [Nb][3]
 };
 
 struct DevFields {
-    DevQ w, uh, vh, z, h, u, v;
-    double *friction;
-    double *max_speed;
-    double *zbar;  // (z0 + z1 + z2) / 3 of the bed vertex values, precomputed
+    Q4 *cq;                               // [N]     centroid w, uh, vh, z
+    Q4 *eq;                               // [3][N]  edge w, uh, vh, z
+    double *wv, *uhv, *vhv, *zv;          // [3][N]  vertex values
+    double *ze;                           // [3][N]  bed edge values (static copy)
+    double *wb, *uhb, *vhb, *zb;          // [Nb]    boundary values
+    double *weu, *uheu, *vheu;            // [N]     explicit updates
+    double *wsi, *uhsi, *vhsi;            // [N]     semi-implicit updates
+    double *hc, *he, *hv, *hb;            // height    ([N], [3][N], [3][N], [Nb])
+    double *uc, *ue, *uv, *ub;            // xvelocity
+    double *vc, *ve, *vv, *vb;            // yvelocity
+    double *friction, *zbar, *max_speed;  // [N]
 };
 
 // Scalar evolve state kept in device memory.  The first block of fields has
@@ -61,8 +89,9 @@ struct DevState {
     long long smallsteps, order, number_of_steps, number_of_first_order_steps;
     long long has_finaltime, event, error;
     // device-only
-    long long halt, halt_next, total_steps;
+    long long halt, total_steps, bed_interp;
     unsigned long long flux_min_bits;
+    unsigned int blocks_done, pad_;
 };
 
 enum : long long { kErrNone = 0, kErrUpdate = 1, kErrSmallsteps = 2, kErrNeighbour = 4, kErrDiscontinuous = 8 };
@@ -78,254 +107,8 @@ SW_DEV void set_error(DevState *st, long long code) {
     atomicOr((unsigned long long *)&st->error, (unsigned long long)code);
 }
 
-// ---------------------------------------------------------------------------
-// compute_fluxes (shallow_water_ext.c:4519-4696).
-//
-// The C routine visits triangles in index order and computes every edge once,
-// from the lower-numbered side, adding the flux to the neighbour with the
-// opposite sign.  Here triangle k evaluates all three of its edges itself, but
-// an edge shared with a lower-numbered neighbour n is evaluated exactly as the
-// C evaluates it (left state from n, n's normal and length), and the sum is
-// formed in the C's order: contributions of lower neighbours in ascending n,
-// then k's own edges in local order, then the 1/area scaling.  The CFL
-// candidates radii[k]/max_speed of all edges of k (gated by tri_full_flag of
-// the computing triangle) give the same minimum as the C's edge-once loop.
-//
-// FUSED adds: boundary values evaluated on the fly (reflective / Dirichlet /
-// transmissive), gravity_wb, Manning friction into the semi-implicit update,
-// and the halt guard of the device-side evolve loop.
-template <typename R, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st) {
-    if (FUSED && st->halt_next) return;
-    const int k = tri_index();
-    double dtk = P.evolve_max_timestep;
-    if (k < M.N) {
-        int nb[3], lowerflag[3];
-        double ef[3][3];
-        bool have[3];
-        double max_speed_total = 0.0;
-        const double eps = P.epsilon;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int ki = 3 * k + i;
-            const int n = M.nbr[ki];
-            nb[i] = n;
-            const bool lower = (n >= 0 && n < k);
-            lowerflag[i] = lower;
-            double ql[3], qr[3], zl, zr, n1, n2, len;
-            int c;
-            if (lower) {
-                const int nm = 3 * n + M.nbr_edge[ki];
-                ql[0] = F.w.e[nm];
-                ql[1] = F.uh.e[nm];
-                ql[2] = F.vh.e[nm];
-                zl = F.z.e[nm];
-                qr[0] = F.w.e[ki];
-                qr[1] = F.uh.e[ki];
-                qr[2] = F.vh.e[ki];
-                zr = F.z.e[ki];
-                n1 = M.normals[2 * nm];
-                n2 = M.normals[2 * nm + 1];
-                len = M.len[nm];
-                c = n;
-            } else {
-                ql[0] = F.w.e[ki];
-                ql[1] = F.uh.e[ki];
-                ql[2] = F.vh.e[ki];
-                zl = F.z.e[ki];
-                n1 = M.normals[2 * ki];
-                n2 = M.normals[2 * ki + 1];
-                len = M.len[ki];
-                c = k;
-                if (n < 0) {
-                    const int m = -n - 1;
-                    if (FUSED) {
-                        const int kind = M.bkind[m];
-                        if (kind == 0) {  // reflective (evaluate_segment.cu:36-63)
-                            qr[0] = ql[0];
-                            reflect_momentum(ql[1], ql[2], n1, n2, qr[1], qr[2]);
-                        } else if (kind == 1) {  // Dirichlet constants
-                            qr[0] = M.bdir[3 * m];
-                            qr[1] = M.bdir[3 * m + 1];
-                            qr[2] = M.bdir[3 * m + 2];
-                        } else {  // transmissive: inside edge state
-                            qr[0] = ql[0];
-                            qr[1] = ql[1];
-                            qr[2] = ql[2];
-                        }
-                    } else {
-                        qr[0] = F.w.b[m];
-                        qr[1] = F.uh.b[m];
-                        qr[2] = F.vh.b[m];
-                    }
-                    zr = zl;
-                } else {
-                    const int nm = 3 * n + M.nbr_edge[ki];
-                    qr[0] = F.w.e[nm];
-                    qr[1] = F.uh.e[nm];
-                    qr[2] = F.vh.e[nm];
-                    zr = F.z.e[nm];
-                }
-            }
-            have[i] = false;
-            ef[i][0] = ef[i][1] = ef[i][2] = 0.0;
-            if (P.optimise_dry_cells && fabs(ql[0] - zl) < eps && fabs(qr[0] - zr) < eps) continue;
-            if (fabs(zl - zr) > 1.0e-10) set_error(st, kErrDiscontinuous);
-            R qlr[3] = {R(ql[0]), R(ql[1]), R(ql[2])};
-            R qrr[3] = {R(qr[0]), R(qr[1]), R(qr[2])};
-            R efr[3], msr;
-            flux_central<R>(qlr, qrr, R(zl), R(zr), R(n1), R(n2), R(eps), R(P.h0),
-                            R(P.limiting_threshold), R(P.g), efr, msr);
-            const double ms = double(msr);
-            ef[i][0] = double(efr[0]) * len;
-            ef[i][1] = double(efr[1]) * len;
-            ef[i][2] = double(efr[2]) * len;
-            have[i] = true;
-            if (M.full[c] == 1 && ms > eps) dtk = omin<double>(dtk, M.radii[k] / ms);
-            if (!lower) max_speed_total = omax<double>(max_speed_total, ms);
-        }
-        // order: lower neighbours by ascending index, then own edges 0,1,2
-        int o0 = 0, o1 = 1, o2 = 2;
-        auto key = [&](int i) { return lowerflag[i] ? nb[i] : 0x7fffffff; };
-        if (key(o1) < key(o0)) { int t = o0; o0 = o1; o1 = t; }
-        if (key(o2) < key(o1)) { int t = o1; o1 = o2; o2 = t; }
-        if (key(o1) < key(o0)) { int t = o0; o0 = o1; o1 = t; }
-        const int ord[3] = {o0, o1, o2};
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int i = ord[j];
-            if (lowerflag[i] && have[i]) {
-                s0 += ef[i][0];
-                s1 += ef[i][1];
-                s2 += ef[i][2];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            if (!lowerflag[i] && have[i]) {
-                s0 -= ef[i][0];
-                s1 -= ef[i][1];
-                s2 -= ef[i][2];
-            }
-        }
-        const double inv_area = 1.0 / M.areas[k];
-        s0 *= inv_area;
-        s1 *= inv_area;
-        s2 *= inv_area;
-        if (FUSED) {
-            // gravity_wb (shallow_water_ext.c:1942-2037)
-            const int k3 = 3 * k, k6 = 6 * k;
-            const double g = P.g;
-            double wx, wy;
-            gradient3(M.vc[k6], M.vc[k6 + 1], M.vc[k6 + 2], M.vc[k6 + 3], M.vc[k6 + 4],
-                      M.vc[k6 + 5], F.w.v[k3], F.w.v[k3 + 1], F.w.v[k3 + 2], wx, wy);
-            const double wc = F.w.c[k];
-            const double avg_h = wc - F.z.c[k];
-            s1 += -g * wx * avg_h;
-            s2 += -g * wy * avg_h;
-            double sidex = 0.0, sidey = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const double hh = F.w.e[k3 + i] - F.z.e[k3 + i];
-                const double fact = -0.5 * g * hh * hh * M.len[k3 + i];
-                sidex = sidex + fact * M.normals[k6 + 2 * i];
-                sidey = sidey + fact * M.normals[k6 + 2 * i + 1];
-            }
-            const double area = M.areas[k];
-            s1 += -sidex / area;
-            s2 += -sidey / area;
-            // manning_friction_implicit (shallow_water_ext.c:1238-1317)
-            double si_x = 0.0, si_y = 0.0;
-            const double eta = F.friction[k];
-            const double mah = P.minimum_allowed_height;
-            if (eta > mah) {
-                const double uh = F.uh.c[k], vh = F.vh.c[k];
-                const double h = wc - F.zbar[k];
-                if (h >= mah) {
-                    double S;
-                    if (P.use_sloped_mannings) {
-                        double zx, zy;
-                        gradient3(M.vc[k6], M.vc[k6 + 1], M.vc[k6 + 2], M.vc[k6 + 3], M.vc[k6 + 4],
-                                  M.vc[k6 + 5], F.z.v[k3], F.z.v[k3 + 1], F.z.v[k3 + 2], zx, zy);
-                        const double zs = sqrt(1.0 + zx * zx + zy * zy);
-                        S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
-                    } else {
-                        S = -g * eta * eta * sqrt((uh * uh + vh * vh));
-                    }
-                    S /= pow(h, 7.0 / 3);
-                    si_x += S * uh;
-                    si_y += S * vh;
-                }
-            }
-            F.uh.si[k] = si_x;
-            F.vh.si[k] = si_y;
-        }
-        F.w.eu[k] = s0;
-        F.uh.eu[k] = s1;
-        F.vh.eu[k] = s2;
-        F.max_speed[k] = max_speed_total;
-    }
-    block_min_atomic<kBlock>(dtk, &st->flux_min_bits);
-}
-
-// gravity_wb as a separate operator (shallow_water_ext.c:1942-2037)
-__global__ __launch_bounds__(kBlock) void k_gravity_wb(DevMesh M, DevFields F, DevParams P) {
-    const int k = tri_index();
-    if (k >= M.N) return;
-    const int k3 = 3 * k, k6 = 6 * k;
-    const double g = P.g;
-    double wx, wy;
-    gradient3(M.vc[k6], M.vc[k6 + 1], M.vc[k6 + 2], M.vc[k6 + 3], M.vc[k6 + 4], M.vc[k6 + 5],
-              F.w.v[k3], F.w.v[k3 + 1], F.w.v[k3 + 2], wx, wy);
-    const double avg_h = F.w.c[k] - F.z.c[k];
-    double ex = F.uh.eu[k], ey = F.vh.eu[k];
-    ex += -g * wx * avg_h;
-    ey += -g * wy * avg_h;
-    double sidex = 0.0, sidey = 0.0;
-    for (int i = 0; i < 3; i++) {
-        const double hh = F.w.e[k3 + i] - F.z.e[k3 + i];
-        const double fact = -0.5 * g * hh * hh * M.len[k3 + i];
-        sidex = sidex + fact * M.normals[k6 + 2 * i];
-        sidey = sidey + fact * M.normals[k6 + 2 * i + 1];
-    }
-    const double area = M.areas[k];
-    ex += -sidex / area;
-    ey += -sidey / area;
-    F.uh.eu[k] = ex;
-    F.vh.eu[k] = ey;
-}
-
-// manning_friction_implicit (shallow_water_ext.c:1238-1317); accumulates into
-// the semi-implicit updates like the C.
-__global__ __launch_bounds__(kBlock) void k_manning(DevMesh M, DevFields F, DevParams P) {
-    const int k = tri_index();
-    if (k >= M.N) return;
-    const double eta = F.friction[k], eps = P.minimum_allowed_height, g = P.g;
-    if (!(eta > eps)) return;
-    const int k3 = 3 * k, k6 = 6 * k;
-    const double z0 = F.z.v[k3], z1 = F.z.v[k3 + 1], z2 = F.z.v[k3 + 2];
-    const double uh = F.uh.c[k], vh = F.vh.c[k];
-    double zs = 1.0;
-    if (P.use_sloped_mannings) {
-        double zx, zy;
-        gradient3(M.vc[k6], M.vc[k6 + 1], M.vc[k6 + 2], M.vc[k6 + 3], M.vc[k6 + 4], M.vc[k6 + 5],
-                  z0, z1, z2, zx, zy);
-        zs = sqrt(1.0 + zx * zx + zy * zy);
-    }
-    const double z = (z0 + z1 + z2) / 3.0;
-    const double h = F.w.c[k] - z;
-    if (h >= eps) {
-        double S;
-        if (P.use_sloped_mannings)
-            S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
-        else
-            S = -g * eta * eta * sqrt((uh * uh + vh * vh));
-        S /= pow(h, 7.0 / 3);
-        F.uh.si[k] += S * uh;
-        F.vh.si[k] += S * vh;
-    }
-}
+SW_DEV Q4 ldq(const Q4 *p) { return *p; }
+SW_DEV int nob_of(int flags) { return (flags >> 12) & 3; }
 
 // update_timestep (evolve.c:115-160 with the final-time clip as finaltime - time)
 SW_DEV void timestep_logic(DevState *st, const DevParams &P) {
@@ -365,8 +148,290 @@ SW_DEV void finish_step(DevState *st) {
     st->event = event;
 }
 
+// ---------------------------------------------------------------------------
+// compute_fluxes (shallow_water_ext.c:4519-4696).
+//
+// The C routine visits triangles in index order and computes every edge once,
+// from the lower-numbered side, adding the flux to the neighbour with the
+// opposite sign.  Here triangle k evaluates all three of its edges itself, but
+// an edge shared with a lower-numbered neighbour n is evaluated exactly as the
+// C evaluates it (left state from n, n's normal and length), and the sum is
+// formed in the C's order: contributions of lower neighbours in ascending n,
+// then k's own edges in local order (the order is precomputed per triangle),
+// then the 1/area scaling.  The CFL candidates radii[k]/max_speed of all
+// edges of k (gated by tri_full_flag of the evaluating triangle) give the same
+// minimum as the C's edge-once loop.
+//
+// FUSED adds: boundary values evaluated on the fly (reflective / Dirichlet /
+// transmissive), gravity_wb, Manning friction into the semi-implicit update,
+// and, in the last workgroup to finish, the timestep and time bookkeeping of
+// the device-side evolve loop.
+template <typename R, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st) {
+    if (FUSED && (st->event || st->error)) {
+        // the previous step ended at a yield / the final time / an error
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->halt = 1;
+        return;
+    }
+    const int N = M.N;
+    const int k = tri_index();
+    double dtk = P.evolve_max_timestep;
+    if (k < N) {
+        const int flags = ld_s(M.flags + k);
+        const double radius = ld_s(M.radii + k);
+        double ef[3][3];
+        bool have[3];
+        double hh[3];
+        double2 nrm[3];
+        double lens[3];
+        double max_speed_total = 0.0;
+        const double eps = P.epsilon;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ki = i * N + k;
+            const int e = ld_s(M.nb + ki);
+            const bool lower = (flags >> i) & 1;
+            const Q4 own = ldq(F.eq + ki);
+            const double2 nk = ld_s(M.normals + ki);
+            const double lk = ld_s(M.len + ki);
+            nrm[i] = nk;
+            lens[i] = lk;
+            hh[i] = own.w - own.z;
+            double n1 = nk.x, n2 = nk.y, len = lk;
+            double ql[3], qr[3], zl, zr;
+            if (e >= 0) {
+                const Q4 nbq = ldq(F.eq + e);
+                if (lower) {
+                    ql[0] = nbq.w; ql[1] = nbq.uh; ql[2] = nbq.vh; zl = nbq.z;
+                    qr[0] = own.w; qr[1] = own.uh; qr[2] = own.vh; zr = own.z;
+                    if (M.own_normals) {
+                        n1 = -n1;
+                        n2 = -n2;
+                    } else {
+                        const double2 nn = M.normals[e];
+                        n1 = nn.x;
+                        n2 = nn.y;
+                        len = M.len[e];
+                    }
+                } else {
+                    ql[0] = own.w; ql[1] = own.uh; ql[2] = own.vh; zl = own.z;
+                    qr[0] = nbq.w; qr[1] = nbq.uh; qr[2] = nbq.vh; zr = nbq.z;
+                }
+            } else {
+                const int m = -e - 1;
+                ql[0] = own.w; ql[1] = own.uh; ql[2] = own.vh; zl = own.z;
+                zr = zl;
+                if (FUSED) {
+                    const int kind = M.bkind[m];
+                    double b0 = ql[0], b1 = ql[1], b2 = ql[2];  // transmissive: inside edge state
+                    if (kind == 0) {  // reflective (evaluate_segment.cu:36-63)
+                        reflect_momentum(ql[1], ql[2], n1, n2, b1, b2);
+                    } else if (kind == 1) {  // Dirichlet constants
+                        b0 = M.bdir[3 * m];
+                        b1 = M.bdir[3 * m + 1];
+                        b2 = M.bdir[3 * m + 2];
+                    }
+                    qr[0] = b0; qr[1] = b1; qr[2] = b2;
+                } else {
+                    qr[0] = F.wb[m]; qr[1] = F.uhb[m]; qr[2] = F.vhb[m];
+                }
+            }
+            have[i] = false;
+            ef[i][0] = ef[i][1] = ef[i][2] = 0.0;
+            if (P.optimise_dry_cells && fabs(ql[0] - zl) < eps && fabs(qr[0] - zr) < eps) continue;
+            if (fabs(zl - zr) > 1.0e-10) set_error(st, kErrDiscontinuous);
+            R qlr[3] = {R(ql[0]), R(ql[1]), R(ql[2])};
+            R qrr[3] = {R(qr[0]), R(qr[1]), R(qr[2])};
+            R efr[3], msr;
+            flux_central<R>(qlr, qrr, R(zl), R(zr), R(n1), R(n2), R(eps), R(P.h0),
+                            R(P.limiting_threshold), R(P.g), efr, msr);
+            const double ms = double(msr);
+            ef[i][0] = double(efr[0]) * len;
+            ef[i][1] = double(efr[1]) * len;
+            ef[i][2] = double(efr[2]) * len;
+            have[i] = true;
+            if (((flags >> (3 + i)) & 1) && ms > eps) dtk = omin<double>(dtk, radius / ms);
+            if (!lower) max_speed_total = omax<double>(max_speed_total, ms);
+        }
+        // C summation order (lower neighbours by ascending index, then own
+        // edges), precomputed as three 2-bit edge indices
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int i = (flags >> (6 + 2 * j)) & 3;
+            const double f0 = (i == 0) ? ef[0][0] : ((i == 1) ? ef[1][0] : ef[2][0]);
+            const double f1 = (i == 0) ? ef[0][1] : ((i == 1) ? ef[1][1] : ef[2][1]);
+            const double f2 = (i == 0) ? ef[0][2] : ((i == 1) ? ef[1][2] : ef[2][2]);
+            const bool hv = (i == 0) ? have[0] : ((i == 1) ? have[1] : have[2]);
+            if (hv) {
+                if ((flags >> i) & 1) {
+                    s0 += f0;
+                    s1 += f1;
+                    s2 += f2;
+                } else {
+                    s0 -= f0;
+                    s1 -= f1;
+                    s2 -= f2;
+                }
+            }
+        }
+        const double area = ld_s(M.areas + k);
+        const double inv_area = 1.0 / area;
+        s0 *= inv_area;
+        s1 *= inv_area;
+        s2 *= inv_area;
+        if (FUSED) {
+            const Q4 c = ldq(F.cq + k);
+            const double g = P.g;
+            // gravity_wb (shallow_water_ext.c:1942-2037)
+            const double2 x0 = ld_s(M.vc + k), x1 = ld_s(M.vc + N + k), x2 = ld_s(M.vc + 2 * N + k);
+            double wx, wy;
+            gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, ld_s(F.wv + k), ld_s(F.wv + N + k),
+                      ld_s(F.wv + 2 * N + k), wx, wy);
+            const double avg_h = c.w - c.z;
+            s1 += -g * wx * avg_h;
+            s2 += -g * wy * avg_h;
+            double sidex = 0.0, sidey = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const double fact = -0.5 * g * hh[i] * hh[i] * lens[i];
+                sidex = sidex + fact * nrm[i].x;
+                sidey = sidey + fact * nrm[i].y;
+            }
+            s1 += -sidex / area;
+            s2 += -sidey / area;
+            // manning_friction_implicit (shallow_water_ext.c:1238-1317)
+            double si_x = 0.0, si_y = 0.0;
+            const double eta = ld_s(F.friction + k);
+            const double mah = P.minimum_allowed_height;
+            if (eta > mah) {
+                const double uh = c.uh, vh = c.vh;
+                const double h = c.w - ld_s(F.zbar + k);
+                if (h >= mah) {
+                    double S;
+                    if (P.use_sloped_mannings) {
+                        double zx, zy;
+                        gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, F.zv[k], F.zv[N + k], F.zv[2 * N + k],
+                                  zx, zy);
+                        const double zs = sqrt(1.0 + zx * zx + zy * zy);
+                        S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
+                    } else {
+                        S = -g * eta * eta * sqrt((uh * uh + vh * vh));
+                    }
+                    S /= pow(h, 7.0 / 3);
+                    si_x += S * uh;
+                    si_y += S * vh;
+                }
+            }
+            st_s(F.uhsi + k, si_x);
+            st_s(F.vhsi + k, si_y);
+        }
+        st_s(F.weu + k, s0);
+        st_s(F.uheu + k, s1);
+        st_s(F.vheu + k, s2);
+        st_s(F.max_speed + k, max_speed_total);
+    }
+    if (!FUSED) {
+        block_min_atomic<kBlock>(dtk, &st->flux_min_bits);
+        return;
+    }
+    // Fused: the workgroup minimum goes in with a returning atomic, so it has
+    // been performed before the same lane counts the workgroup as done; the
+    // workgroup that counts last takes the global minimum (and resets it) with
+    // one atomic exchange.  Only read-modify-write atomics touch the two
+    // words, so no cache writeback fence is needed per workgroup.
+    const unsigned long long wmin = block_min_bits<kBlock>(dtk);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        const unsigned long long old = __hip_atomic_fetch_min(&st->flux_min_bits, wmin, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+        // The branch consumes the returned value, so the count below is issued
+        // only after the minimum has been performed.  (The word always holds
+        // the bits of a positive double, never all ones.)
+        unsigned prev = 0xffffffffu;
+        if (old != ~0ull)
+            prev = __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        const unsigned long long b = __hip_atomic_exchange(&st->flux_min_bits, dbits(P.evolve_max_timestep),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_exchange(&st->blocks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        st->flux_timestep = bitsd(b);
+        timestep_logic(st, P);
+        if (!(st->error & kErrSmallsteps)) {
+            st->time = st->time + st->timestep;
+            finish_step(st);
+        }
+    }
+}
+
+// gravity_wb as a separate operator (shallow_water_ext.c:1942-2037)
+__global__ __launch_bounds__(kBlock) void k_gravity_wb(DevMesh M, DevFields F, DevParams P) {
+    const int N = M.N;
+    const int k = tri_index();
+    if (k >= N) return;
+    const double g = P.g;
+    const Q4 c = ldq(F.cq + k);
+    const double2 x0 = M.vc[k], x1 = M.vc[N + k], x2 = M.vc[2 * N + k];
+    double wx, wy;
+    gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, F.wv[k], F.wv[N + k], F.wv[2 * N + k], wx, wy);
+    const double avg_h = c.w - c.z;
+    double ex = F.uheu[k], ey = F.vheu[k];
+    ex += -g * wx * avg_h;
+    ey += -g * wy * avg_h;
+    double sidex = 0.0, sidey = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const Q4 eqi = ldq(F.eq + i * N + k);
+        const double hh = eqi.w - eqi.z;
+        const double fact = -0.5 * g * hh * hh * M.len[i * N + k];
+        const double2 n = M.normals[i * N + k];
+        sidex = sidex + fact * n.x;
+        sidey = sidey + fact * n.y;
+    }
+    const double area = M.areas[k];
+    ex += -sidex / area;
+    ey += -sidey / area;
+    F.uheu[k] = ex;
+    F.vheu[k] = ey;
+}
+
+// manning_friction_implicit (shallow_water_ext.c:1238-1317); accumulates into
+// the semi-implicit updates like the C.
+__global__ __launch_bounds__(kBlock) void k_manning(DevMesh M, DevFields F, DevParams P) {
+    const int N = M.N;
+    const int k = tri_index();
+    if (k >= N) return;
+    const double eta = F.friction[k], eps = P.minimum_allowed_height, g = P.g;
+    if (!(eta > eps)) return;
+    const double z0 = F.zv[k], z1 = F.zv[N + k], z2 = F.zv[2 * N + k];
+    const Q4 c = ldq(F.cq + k);
+    const double uh = c.uh, vh = c.vh;
+    double zs = 1.0;
+    if (P.use_sloped_mannings) {
+        const double2 x0 = M.vc[k], x1 = M.vc[N + k], x2 = M.vc[2 * N + k];
+        double zx, zy;
+        gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, z0, z1, z2, zx, zy);
+        zs = sqrt(1.0 + zx * zx + zy * zy);
+    }
+    const double z = (z0 + z1 + z2) / 3.0;
+    const double h = c.w - z;
+    if (h >= eps) {
+        double S;
+        if (P.use_sloped_mannings)
+            S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
+        else
+            S = -g * eta * eta * sqrt((uh * uh + vh * vh));
+        S /= pow(h, 7.0 / 3);
+        F.uhsi[k] += S * uh;
+        F.vhsi[k] += S * vh;
+    }
+}
+
 __global__ void k_reset_flux_min(DevState *st, double evolve_max_timestep) {
     st->flux_min_bits = dbits(evolve_max_timestep);
+    st->blocks_done = 0;
 }
 
 __global__ void k_flux_finalize(DevState *st, double evolve_max_timestep) {
@@ -375,135 +440,107 @@ __global__ void k_flux_finalize(DevState *st, double evolve_max_timestep) {
 }
 
 __global__ void k_update_timestep(DevState *st, DevParams P) { timestep_logic(st, P); }
-
-// Unfused loop: time = initial_time + timestep, then counters and events.
 __global__ void k_advance_time(DevState *st) { st->time = st->time + st->timestep; }
 __global__ void k_finish_step(DevState *st) { finish_step(st); }
-
-// Fused loop, one lane: flux minimum -> timestep -> time -> counters -> event.
-// halt_next stops the next step's flux launch, halt the update/distribute
-// launches of later steps; see sw_capi.hip for the launch sequence.
-__global__ void k_step_timestep(DevState *st, DevParams P) {
-    if (st->halt) return;
-    if (st->halt_next || st->error) {
-        st->halt = 1;
-        st->halt_next = 1;
-        return;
-    }
-    st->flux_timestep = bitsd(st->flux_min_bits);
-    st->flux_min_bits = dbits(P.evolve_max_timestep);
-    timestep_logic(st, P);
-    if (st->error) {
-        st->halt = 1;
-        st->halt_next = 1;
-        return;
-    }
-    const double initial_time = st->time;
-    st->time = initial_time + st->timestep;
-    finish_step(st);
-    if (st->event) st->halt_next = 1;
-}
 
 // update_conserved_quantities (quantity.c:769-820), optionally followed by
 // protect (the first stage of distribute_to_vertices_and_edges).
 template <bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st) {
-    if (FUSED && st->halt) return;
+    if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
     const int k = tri_index();
     if (k >= M.N) return;
     const double dt = st->timestep;
-    double w = F.w.c[k], uh = F.uh.c[k], vh = F.vh.c[k];
-    double siw = F.w.si[k], siu = F.uh.si[k], siv = F.vh.si[k];
-    bool ok = update_one(w, F.w.eu[k], siw, dt);
-    ok = update_one(uh, F.uh.eu[k], siu, dt) && ok;
-    ok = update_one(vh, F.vh.eu[k], siv, dt) && ok;
+    Q4 c = ldq(F.cq + k);
+    double siw = ld_s(F.wsi + k), siu = ld_s(F.uhsi + k), siv = ld_s(F.vhsi + k);
+    bool ok = update_one(c.w, ld_s(F.weu + k), siw, dt);
+    ok = update_one(c.uh, ld_s(F.uheu + k), siu, dt) && ok;
+    ok = update_one(c.vh, ld_s(F.vheu + k), siv, dt) && ok;
     if (!ok) set_error(st, kErrUpdate);
     if (FUSED) {
-        protect_one(w, F.z.c[k], uh, vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
+        protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
     } else {
-        F.w.si[k] = 0.0;
-        F.uh.si[k] = 0.0;
-        F.vh.si[k] = 0.0;
+        F.wsi[k] = 0.0;
+        F.uhsi[k] = 0.0;
+        F.vhsi[k] = 0.0;
     }
-    F.w.c[k] = w;
-    F.uh.c[k] = uh;
-    F.vh.c[k] = vh;
+    F.cq[k] = c;
 }
 
 __global__ __launch_bounds__(kBlock) void k_protect(DevMesh M, DevFields F, DevParams P) {
     const int k = tri_index();
     if (k >= M.N) return;
-    double w = F.w.c[k], uh = F.uh.c[k], vh = F.vh.c[k];
-    protect_one(w, F.z.c[k], uh, vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
-    F.w.c[k] = w;
-    F.uh.c[k] = uh;
-    F.vh.c[k] = vh;
+    Q4 c = ldq(F.cq + k);
+    protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
+    F.cq[k] = c;
 }
 
 // Centroid quantity as seen by the extrapolation: velocity when
 // extrapolate_velocity_second_order (shallow_water_ext.c:2862-2874).
-SW_DEV void centroid_wuv(const DevFields &F, const DevParams &P, int j, double &w, double &u,
-                         double &v) {
-    w = F.w.c[j];
-    u = F.uh.c[j];
-    v = F.vh.c[j];
+SW_DEV void centroid_wuv(const Q4 &c, const DevParams &P, double &w, double &u, double &v) {
+    w = c.w;
+    u = c.uh;
+    v = c.vh;
     if (P.extrapolate_velocity_second_order == 1) {
-        const double dk = omax<double>(w - F.z.c[j], P.minimum_allowed_height);
+        const double dk = omax<double>(c.w - c.z, P.minimum_allowed_height);
         u = u / dk;
         v = v / dk;
     }
 }
 
 // extrapolate_second_order_sw for triangle k (shallow_water_ext.c:2717-3324),
-// result in wv/uv/vv (momentum).  Returns false on the missing-neighbour error.
+// result in wv/uv/vv (momentum), zv = bed vertex values.  Returns false on the
+// missing-neighbour error.
 template <typename R>
-SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParams &P, int k,
-                            double wv[3], double uv[3], double vv[3]) {
-    const int k3 = 3 * k, k6 = 6 * k;
-    const int nob = M.nob[k];
+SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParams &P, int k, int nob,
+                            const Q4 &ck, const double zv[3], double wv[3], double uv[3], double vv[3]) {
+    const int N = M.N;
     double wc, uc, vc;
-    centroid_wuv(F, P, k, wc, uc, vc);
+    centroid_wuv(ck, P, wc, uc, vc);
     bool ok = true;
     bool constant = false;
     if (nob == 3) {
         constant = true;
     } else {
-        const double x = M.cc[2 * k], y = M.cc[2 * k + 1];
+        const double2 xy = M.cc[k];
+        const double x = xy.x, y = xy.y;
+        const double2 v0 = ld_s(M.vc + k), v1 = ld_s(M.vc + N + k), v2 = ld_s(M.vc + 2 * N + k);
         R dxv[3], dyv[3];
-        dxv[0] = R(M.vc[k6] - x);
-        dxv[1] = R(M.vc[k6 + 2] - x);
-        dxv[2] = R(M.vc[k6 + 4] - x);
-        dyv[0] = R(M.vc[k6 + 1] - y);
-        dyv[1] = R(M.vc[k6 + 3] - y);
-        dyv[2] = R(M.vc[k6 + 5] - y);
+        dxv[0] = R(v0.x - x);
+        dxv[1] = R(v1.x - x);
+        dxv[2] = R(v2.x - x);
+        dyv[0] = R(v0.y - y);
+        dyv[1] = R(v1.y - y);
+        dyv[2] = R(v2.y - y);
+        const int k0 = ld_s(M.sur + k), k1 = ld_s(M.sur + N + k), k2 = ld_s(M.sur + 2 * N + k);
         if (nob <= 1) {
-            const int k0 = M.sur[k3], k1 = M.sur[k3 + 1], k2 = M.sur[k3 + 2];
-            const double x0 = M.cc[2 * k0], y0 = M.cc[2 * k0 + 1];
-            const double x1 = M.cc[2 * k1], y1 = M.cc[2 * k1 + 1];
-            const double x2 = M.cc[2 * k2], y2 = M.cc[2 * k2 + 1];
-            const double dx1 = x1 - x0, dx2 = x2 - x0, dy1 = y1 - y0, dy2 = y2 - y0;
+            const double2 c0xy = M.cc[k0], c1xy = M.cc[k1], c2xy = M.cc[k2];
+            const double dx1 = c1xy.x - c0xy.x, dx2 = c2xy.x - c0xy.x;
+            const double dy1 = c1xy.y - c0xy.y, dy2 = c2xy.y - c0xy.y;
             const double area2 = dy2 * dx1 - dy1 * dx2;
             if (area2 <= 0) {
                 constant = true;
             } else {
-                double w0, u0, v0, w1, u1, v1, w2, u2, v2;
-                centroid_wuv(F, P, k0, w0, u0, v0);
-                centroid_wuv(F, P, k1, w1, u1, v1);
-                centroid_wuv(F, P, k2, w2, u2, v2);
-                const double hc = wc - F.z.c[k];
-                const double h0 = w0 - F.z.c[k0];
-                const double h1 = w1 - F.z.c[k1];
-                const double h2 = w2 - F.z.c[k2];
+                const Q4 c0 = ldq(F.cq + k0), c1 = ldq(F.cq + k1), c2 = ldq(F.cq + k2);
+                double w0, u0, v0_, w1, u1, v1_, w2, u2, v2_;
+                centroid_wuv(c0, P, w0, u0, v0_);
+                centroid_wuv(c1, P, w1, u1, v1_);
+                centroid_wuv(c2, P, w2, u2, v2_);
+                const double hc = wc - ck.z;
+                const double h0 = w0 - c0.z;
+                const double h1 = w1 - c1.z;
+                const double h2 = w2 - c2.z;
                 const double hmin = omin<double>(omin<double>(h0, omin<double>(h1, h2)), hc);
                 double hfactor = 0.0;
                 if (hmin > 0.001) hfactor = (hmin - 0.001) / (hmin + 0.004);
                 const double hmax = omax<double>(h0, omax<double>(h1, h2));
                 if (P.optimise_dry_cells && hmax < P.epsilon) {
                     // dry: vertex values keep their previous contents
+#pragma unroll
                     for (int i = 0; i < 3; i++) {
-                        wv[i] = F.w.v[k3 + i];
-                        uv[i] = F.uh.v[k3 + i];
-                        vv[i] = F.vh.v[k3 + i];
+                        wv[i] = F.wv[i * N + k];
+                        uv[i] = F.uhv[i * N + k];
+                        vv[i] = F.vhv[i * N + k];
                     }
                 } else {
                     const R inv_area2 = R(1.0) / R(area2);
@@ -511,54 +548,64 @@ SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParam
                     R beta = R(P.beta_w_dry + (P.beta_w - P.beta_w_dry) * hfactor);
                     extrap_aux<R>(R(wc), R(w0), R(w1), R(w2), R(dx1), R(dx2), R(dy1), R(dy2),
                                   inv_area2, dxv, dyv, beta, d);
-                    for (int i = 0; i < 3; i++) wv[i] = wc + double(d[i]);
+                    wv[0] = wc + double(d[0]);
+                    wv[1] = wc + double(d[1]);
+                    wv[2] = wc + double(d[2]);
                     beta = R(P.beta_uh_dry + (P.beta_uh - P.beta_uh_dry) * hfactor);
                     extrap_aux<R>(R(uc), R(u0), R(u1), R(u2), R(dx1), R(dx2), R(dy1), R(dy2),
                                   inv_area2, dxv, dyv, beta, d);
-                    for (int i = 0; i < 3; i++) uv[i] = uc + double(d[i]);
+                    uv[0] = uc + double(d[0]);
+                    uv[1] = uc + double(d[1]);
+                    uv[2] = uc + double(d[2]);
                     beta = R(P.beta_vh_dry + (P.beta_vh - P.beta_vh_dry) * hfactor);
-                    extrap_aux<R>(R(vc), R(v0), R(v1), R(v2), R(dx1), R(dx2), R(dy1), R(dy2),
+                    extrap_aux<R>(R(vc), R(v0_), R(v1_), R(v2_), R(dx1), R(dx2), R(dy1), R(dy2),
                                   inv_area2, dxv, dyv, beta, d);
-                    for (int i = 0; i < 3; i++) vv[i] = vc + double(d[i]);
+                    vv[0] = vc + double(d[0]);
+                    vv[1] = vc + double(d[1]);
+                    vv[2] = vc + double(d[2]);
                 }
             }
         } else {
-            int e = -1;
-            for (int i = 0; i < 3; i++)
-                if (e < 0 && M.sur[k3 + i] != k) e = i;
-            if (e < 0) {
+            const int kk = (k0 != k) ? k0 : ((k1 != k) ? k1 : k2);
+            if (kk == k) {
                 ok = false;
                 constant = true;
             } else {
-                const int k1 = M.sur[k3 + e];
-                const double dx1 = M.cc[2 * k1] - x, dy1 = M.cc[2 * k1 + 1] - y;
+                const double2 c1xy = M.cc[kk];
+                const double dx1 = c1xy.x - x, dy1 = c1xy.y - y;
                 const double area2 = dx1 * dx1 + dy1 * dy1;
                 double dx2 = 1.0 / area2;
                 const double dy2 = dx2 * dy1;
                 dx2 *= dx1;
-                double w1, u1, v1;
-                centroid_wuv(F, P, k1, w1, u1, v1);
+                const Q4 c1 = ldq(F.cq + kk);
+                double w1, u1, v1_;
+                centroid_wuv(c1, P, w1, u1, v1_);
                 R d[3];
                 const R bw = R(P.beta_w);
                 extrap_one<R>(R(wc), R(w1), R(dx2), R(dy2), dxv, dyv, bw, d);
-                for (int i = 0; i < 3; i++) wv[i] = wc + double(d[i]);
+                wv[0] = wc + double(d[0]);
+                wv[1] = wc + double(d[1]);
+                wv[2] = wc + double(d[2]);
                 extrap_one<R>(R(uc), R(u1), R(dx2), R(dy2), dxv, dyv, bw, d);
-                for (int i = 0; i < 3; i++) uv[i] = uc + double(d[i]);
-                extrap_one<R>(R(vc), R(v1), R(dx2), R(dy2), dxv, dyv, bw, d);
-                for (int i = 0; i < 3; i++) vv[i] = vc + double(d[i]);
+                uv[0] = uc + double(d[0]);
+                uv[1] = uc + double(d[1]);
+                uv[2] = uc + double(d[2]);
+                extrap_one<R>(R(vc), R(v1_), R(dx2), R(dy2), dxv, dyv, bw, d);
+                vv[0] = vc + double(d[0]);
+                vv[1] = vc + double(d[1]);
+                vv[2] = vc + double(d[2]);
             }
         }
     }
     if (constant) {
-        for (int i = 0; i < 3; i++) {
-            wv[i] = wc;
-            uv[i] = uc;
-            vv[i] = vc;
-        }
+        wv[0] = wv[1] = wv[2] = wc;
+        uv[0] = uv[1] = uv[2] = uc;
+        vv[0] = vv[1] = vv[2] = vc;
     }
     if (P.extrapolate_velocity_second_order == 1) {
+#pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double dv = omax<double>(wv[i] - F.z.v[k3 + i], 0.);
+            const double dv = omax<double>(wv[i] - zv[i], 0.);
             uv[i] = uv[i] * dv;
             vv[i] = vv[i] * dv;
         }
@@ -569,115 +616,160 @@ SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParam
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_extrapolate_second_order(DevMesh M, DevFields F, DevParams P,
                                                                      DevState *st) {
+    const int N = M.N;
     const int k = tri_index();
-    if (k >= M.N) return;
-    double wv[3], uv[3], vv[3];
-    if (!extrapolate_one<R>(M, F, P, k, wv, uv, vv)) set_error(st, kErrNeighbour);
+    if (k >= N) return;
+    double wv[3], uv[3], vv[3], zv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) zv[i] = F.zv[i * N + k];
+    if (!extrapolate_one<R>(M, F, P, k, nob_of(M.flags[k]), ldq(F.cq + k), zv, wv, uv, vv))
+        set_error(st, kErrNeighbour);
+#pragma unroll
     for (int i = 0; i < 3; i++) {
-        F.w.v[3 * k + i] = wv[i];
-        F.uh.v[3 * k + i] = uv[i];
-        F.vh.v[3 * k + i] = vv[i];
+        F.wv[i * N + k] = wv[i];
+        F.uhv[i * N + k] = uv[i];
+        F.vhv[i * N + k] = vv[i];
     }
 }
 
-// extrapolate_first_order (extrapolate_first_order.cu:3-33) for one quantity
-__global__ __launch_bounds__(kBlock) void k_first_order(int N, const double *c, double *e, double *v) {
+// extrapolate_first_order (extrapolate_first_order.cu:3-33) of the conserved quantities
+__global__ __launch_bounds__(kBlock) void k_first_order_conserved(DevMesh M, DevFields F) {
+    const int N = M.N;
     const int k = tri_index();
     if (k >= N) return;
-    const double q = c[k];
+    const Q4 c = ldq(F.cq + k);
+#pragma unroll
     for (int i = 0; i < 3; i++) {
-        e[3 * k + i] = q;
-        v[3 * k + i] = q;
+        Q4 e = ldq(F.eq + i * N + k);
+        e.w = c.w;
+        e.uh = c.uh;
+        e.vh = c.vh;
+        F.eq[i * N + k] = e;
+        F.wv[i * N + k] = c.w;
+        F.uhv[i * N + k] = c.uh;
+        F.vhv[i * N + k] = c.vh;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_balance(DevMesh M, DevFields F, DevParams P) {
+    const int N = M.N;
     const int k = tri_index();
-    if (k >= M.N) return;
-    const int k3 = 3 * k;
+    if (k >= N) return;
     double wv[3], zv[3], uv[3], vv[3];
+#pragma unroll
     for (int i = 0; i < 3; i++) {
-        wv[i] = F.w.v[k3 + i];
-        zv[i] = F.z.v[k3 + i];
-        uv[i] = F.uh.v[k3 + i];
-        vv[i] = F.vh.v[k3 + i];
+        wv[i] = F.wv[i * N + k];
+        zv[i] = F.zv[i * N + k];
+        uv[i] = F.uhv[i * N + k];
+        vv[i] = F.vhv[i * N + k];
     }
+    const Q4 c = ldq(F.cq + k);
     BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
-    balance_one(F.w.c[k], F.z.c[k], wv, zv, F.uh.c[k], F.vh.c[k], uv, vv, bp);
+    balance_one(c.w, c.z, wv, zv, c.uh, c.vh, uv, vv, bp);
+#pragma unroll
     for (int i = 0; i < 3; i++) {
-        F.w.v[k3 + i] = wv[i];
-        F.uh.v[k3 + i] = uv[i];
-        F.vh.v[k3 + i] = vv[i];
+        F.wv[i * N + k] = wv[i];
+        F.uhv[i * N + k] = uv[i];
+        F.vhv[i * N + k] = vv[i];
     }
 }
 
 // interpolate_from_vertices_to_edges for the three conserved quantities
 __global__ __launch_bounds__(kBlock) void k_interpolate(DevMesh M, DevFields F) {
+    const int N = M.N;
     const int k = tri_index();
-    if (k >= M.N) return;
-    const int k3 = 3 * k;
-    double v[3], e[3];
-    DevQ *qs[3] = {&F.w, &F.uh, &F.vh};
-    for (int q = 0; q < 3; q++) {
-        for (int i = 0; i < 3; i++) v[i] = qs[q]->v[k3 + i];
-        interpolate_one(v, e);
-        for (int i = 0; i < 3; i++) qs[q]->e[k3 + i] = e[i];
+    if (k >= N) return;
+    double w[3], u[3], v[3], ew[3], eu[3], ev[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        w[i] = F.wv[i * N + k];
+        u[i] = F.uhv[i * N + k];
+        v[i] = F.vhv[i * N + k];
+    }
+    interpolate_one(w, ew);
+    interpolate_one(u, eu);
+    interpolate_one(v, ev);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        Q4 e = ldq(F.eq + i * N + k);
+        e.w = ew[i];
+        e.uh = eu[i];
+        e.vh = ev[i];
+        F.eq[i * N + k] = e;
     }
 }
 
 // Fused distribute_to_vertices_and_edges after protect: reconstruction
 // (second order, or first order when the evolve loop fell back to it),
-// balance_deep_and_shallow and vertex->edge interpolation in registers.
+// balance_deep_and_shallow and vertex->edge interpolation in registers.  The
+// edge records are written whole; their bed value is recomputed from the bed
+// vertex values when the bed edges are known to be that interpolation
+// (st->bed_interp), else copied from the static bed edge array.
 template <typename R, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, DevParams P, DevState *st) {
-    if (FUSED && st->halt) return;
+    if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
+    const int N = M.N;
     const int k = tri_index();
-    if (k >= M.N) return;
-    const int k3 = 3 * k;
+    if (k >= N) return;
     double wv[3], uv[3], vv[3], zv[3];
-    const double wc = F.w.c[k], uc = F.uh.c[k], vc = F.vh.c[k];
+    const Q4 c = ldq(F.cq + k);
+#pragma unroll
+    for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + k);
     if (st->order == 1) {
-        for (int i = 0; i < 3; i++) {
-            wv[i] = wc;
-            uv[i] = uc;
-            vv[i] = vc;
-        }
+        wv[0] = wv[1] = wv[2] = c.w;
+        uv[0] = uv[1] = uv[2] = c.uh;
+        vv[0] = vv[1] = vv[2] = c.vh;
     } else {
-        if (!extrapolate_one<R>(M, F, P, k, wv, uv, vv)) set_error(st, kErrNeighbour);
+        if (!extrapolate_one<R>(M, F, P, k, nob_of(ld_s(M.flags + k)), c, zv, wv, uv, vv))
+            set_error(st, kErrNeighbour);
     }
-    for (int i = 0; i < 3; i++) zv[i] = F.z.v[k3 + i];
     BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
-    balance_one(wc, F.z.c[k], wv, zv, uc, vc, uv, vv, bp);
-    double e[3];
+    balance_one(c.w, c.z, wv, zv, c.uh, c.vh, uv, vv, bp);
+#pragma unroll
     for (int i = 0; i < 3; i++) {
-        F.w.v[k3 + i] = wv[i];
-        F.uh.v[k3 + i] = uv[i];
-        F.vh.v[k3 + i] = vv[i];
+        st_s(F.wv + i * N + k, wv[i]);
+        st_s(F.uhv + i * N + k, uv[i]);
+        st_s(F.vhv + i * N + k, vv[i]);
     }
-    interpolate_one(wv, e);
-    for (int i = 0; i < 3; i++) F.w.e[k3 + i] = e[i];
-    interpolate_one(uv, e);
-    for (int i = 0; i < 3; i++) F.uh.e[k3 + i] = e[i];
-    interpolate_one(vv, e);
-    for (int i = 0; i < 3; i++) F.vh.e[k3 + i] = e[i];
+    double ew[3], eu[3], ev[3], ez[3];
+    interpolate_one(wv, ew);
+    interpolate_one(uv, eu);
+    interpolate_one(vv, ev);
+    if (st->bed_interp) {
+        interpolate_one(zv, ez);
+    } else {
+        ez[0] = F.ze[k];
+        ez[1] = F.ze[N + k];
+        ez[2] = F.ze[2 * N + k];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        Q4 e;
+        e.w = ew[i];
+        e.uh = eu[i];
+        e.vh = ev[i];
+        e.z = ez[i];
+        F.eq[i * N + k] = e;
+    }
 }
 
 // update_boundary (evaluate_segment.cu:3-119, set_boundary.cu:1-18).  Each
-// branch only computes the six boundary values; the stores are done once,
+// branch only computes the boundary values; the stores are done once,
 // unconditionally and in a fixed order (no divergent stores to different
 // arrays for the compiler to merge across branches).
 __global__ __launch_bounds__(kBlock) void k_update_boundary(DevMesh M, DevFields F) {
     const int m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= M.Nb) return;
     const int vol = M.bcell[m], edge = M.bedge[m];
-    const int e = 3 * vol + edge;
+    const int e = edge * M.N + vol;
     const int kind = M.bkind[m];
-    const double we = F.w.e[e], uhe = F.uh.e[e], vhe = F.vh.e[e];
-    const double he = F.h.e[e], ue = F.u.e[e], ve = F.v.e[e];
-    const double n1 = M.normals[6 * vol + 2 * edge], n2 = M.normals[6 * vol + 2 * edge + 1];
-    double wb = we, uhb = uhe, vhb = vhe, hb = he, ub = ue, vb = ve;  // transmissive
+    const Q4 q = ldq(F.eq + e);
+    const double he = F.he[e], ue = F.ue[e], ve = F.ve[e];
+    const double2 n = M.normals[e];
+    const double n1 = n.x, n2 = n.y;
+    double wb = q.w, uhb = q.uh, vhb = q.vh, hb = he, ub = ue, vb = ve;  // transmissive
     if (kind == 0) {  // reflective
-        reflect_momentum(uhe, vhe, n1, n2, uhb, vhb);
+        reflect_momentum(q.uh, q.vh, n1, n2, uhb, vhb);
         const double r1 = ue * n1 + ve * n2;
         const double r2 = ue * n2 - ve * n1;
         ub = n1 * r1 - n2 * r2;
@@ -687,50 +779,92 @@ __global__ __launch_bounds__(kBlock) void k_update_boundary(DevMesh M, DevFields
         uhb = M.bdir[3 * m + 1];
         vhb = M.bdir[3 * m + 2];
     }
-    F.z.b[m] = F.z.e[e];
-    F.w.b[m] = wb;
-    F.uh.b[m] = uhb;
-    F.vh.b[m] = vhb;
-    F.h.b[m] = hb;
-    F.u.b[m] = ub;
-    F.v.b[m] = vb;
+    F.zb[m] = q.z;
+    F.wb[m] = wb;
+    F.uhb[m] = uhb;
+    F.vhb[m] = vhb;
+    F.hb[m] = hb;
+    F.ub[m] = ub;
+    F.vb[m] = vb;
 }
 
 // update_other_quantities (update_centroids_of_velocities_and_height.cu:1-52
 // + extrapolate_first_order of height / xvelocity / yvelocity)
 __global__ __launch_bounds__(kBlock) void k_other_quantities(DevMesh M, DevFields F) {
+    const int N = M.N;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     const double H0 = 1.0e-8;
-    if (k < M.N) {
-        double h = F.w.c[k] - F.z.c[k];
+    if (k < N) {
+        const Q4 c = ldq(F.cq + k);
+        double h = c.w - c.z;
         if (h < 0) h = 0;
         const double factor = h / (h * h + H0);
-        const double u = F.uh.c[k] * factor, v = F.vh.c[k] * factor;
-        F.h.c[k] = h;
-        F.u.c[k] = u;
-        F.v.c[k] = v;
+        const double u = c.uh * factor, v = c.vh * factor;
+        F.hc[k] = h;
+        F.uc[k] = u;
+        F.vc[k] = v;
+#pragma unroll
         for (int i = 0; i < 3; i++) {
-            F.h.e[3 * k + i] = h;
-            F.h.v[3 * k + i] = h;
-            F.u.e[3 * k + i] = u;
-            F.u.v[3 * k + i] = u;
-            F.v.e[3 * k + i] = v;
-            F.v.v[3 * k + i] = v;
+            F.he[i * N + k] = h;
+            F.hv[i * N + k] = h;
+            F.ue[i * N + k] = u;
+            F.uv[i * N + k] = u;
+            F.ve[i * N + k] = v;
+            F.vv[i * N + k] = v;
         }
     }
     if (k < M.Nb) {
-        double h = F.w.b[k] - F.z.b[k];
+        double h = F.wb[k] - F.zb[k];
         if (h < 0) h = 0;
         const double factor = h / (h * h + H0);
-        F.h.b[k] = h;
-        F.u.b[k] = F.uh.b[k] * factor;
-        F.v.b[k] = F.vh.b[k] * factor;
+        F.hb[k] = h;
+        F.ub[k] = F.uhb[k] * factor;
+        F.vb[k] = F.vhb[k] * factor;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double val) {
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = val;
+}
+
+// API transfers.  Host arrays are in the reference layout ([N] or [N][3]); the
+// device keeps [N] records or slot-major [3][N] arrays.  `slots` is 1 for
+// per-triangle arrays and 3 for per-edge / per-vertex arrays; `comp` selects a
+// component of {w, uh, vh, z} records (or -1 for a plain double array).
+__global__ __launch_bounds__(kBlock) void k_to_device(long long N, int slots, int comp, const double *src,
+                                                      double *dst) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= N * slots) return;
+    const long long k = t / slots, i = t - k * slots;  // src is [N][slots]
+    const long long d = i * N + k;
+    if (comp < 0)
+        dst[d] = src[t];
+    else
+        dst[4 * d + comp] = src[t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_host(long long N, int slots, int comp, const double *src,
+                                                    double *dst) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= N * slots) return;
+    const long long k = t / slots, i = t - k * slots;
+    const long long d = i * N + k;
+    dst[t] = (comp < 0) ? src[d] : src[4 * d + comp];
+}
+
+// mean bed per triangle (as the friction term forms it) and the check that the
+// bed edge values are the vertex interpolation (enables recomputing them).
+__global__ __launch_bounds__(kBlock) void k_bed_derived(int N, const double *zv, const double *ze, double *zbar,
+                                                        DevState *st) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= N) return;
+    const double v[3] = {zv[k], zv[N + k], zv[2 * N + k]};
+    zbar[k] = (v[0] + v[1] + v[2]) / 3.0;
+    double e[3];
+    interpolate_one(v, e);
+    if (dbits(e[0]) != dbits(ze[k]) || dbits(e[1]) != dbits(ze[N + k]) || dbits(e[2]) != dbits(ze[2 * N + k]))
+        atomicAnd((unsigned long long *)&st->bed_interp, 0ull);
 }
 
 }  // namespace sw

@@ -32,20 +32,20 @@ HBM_PEAK_GBS = 8000.0
 # indices, neighbour gathers as cache hits): geometry 196 + static quantities
 # 64 + centroid state 48 + vertex state 144 + edge exchange 144 + max_speed 8.
 STEP_BYTES_PER_TRIANGLE = 604
-# Per-launch algorithmic bytes per triangle of the fused kernels (arrays each
-# kernel reads or writes, counted once; gathers of neighbour data as hits).
+# Per-launch algorithmic bytes per triangle of the fused kernels: every array a
+# kernel reads or writes, counted once (neighbour gathers as cache hits).
 KERNEL_BYTES_PER_TRIANGLE = {
-    # edge w/uh/vh 72, bed edge 24, nbr+nbr_edge 24, normals 48, lengths 24,
-    # radii 8, area 8, full flag 4, stage vertex 24, vertex xy 48, centroid
-    # w/z/uh/vh 32, friction 8, mean bed 8 | writes: explicit 24, semi-impl 16, max_speed 8
-    "flux": 72 + 24 + 24 + 48 + 24 + 8 + 8 + 4 + 24 + 48 + 32 + 8 + 8 + 24 + 16 + 8,
-    # centroid w/uh/vh 24, explicit 24, semi-implicit 24, bed centroid 8 | write centroid 24
-    "update": 24 + 24 + 24 + 8 + 24,
-    # centroid w/uh/vh/z 32, centroid xy 16, surrogate 12, nob 4, vertex xy 48,
-    # bed vertex 24 | writes: vertex w/uh/vh 72, edge w/uh/vh 72
-    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 72,
+    # reads: edge records {w,uh,vh,z} x3 96, normals 48, lengths 24, neighbour
+    # word 16, radius 8, area 8, stage vertex 24, vertex xy 48, centroid record
+    # 32, mean bed 8, friction 8 | writes: explicit 24, semi-implicit 16, max_speed 8
+    "flux": 96 + 48 + 24 + 16 + 8 + 8 + 24 + 48 + 32 + 8 + 8 + 24 + 16 + 8,
+    # reads: centroid record 32, explicit 24, semi-implicit 24 | writes: centroid record 32
+    "update": 32 + 24 + 24 + 32,
+    # reads: centroid record 32, centroid xy 16, surrogate 12, nob 4, vertex xy
+    # 48, bed vertex 24 | writes: vertex w/uh/vh 72, edge records 96
+    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 96,
 }
-KERNELS = ("flux", "timestep", "update", "distribute")
+KERNELS = ("flux", "update", "distribute")
 
 
 def log(msg):
@@ -117,7 +117,7 @@ def main():
     h.call("sw_enable_timing", 1)
     h.evolve_steps(args.timing_steps)
     h.call("sw_enable_timing", 0)
-    ms, cnt = h.kernel_times(4)
+    ms, cnt = h.kernel_times(3)
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
     dominant = max(("flux", "update", "distribute"), key=lambda k: avg_us[k] or 0.0)
