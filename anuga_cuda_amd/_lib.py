@@ -261,7 +261,7 @@ class Handle:
         self.call("sw_evolve_one_step_unfused", C.byref(ev))
         return ev.value
 
-    def kernel_times(self, n=3):
+    def kernel_times(self, n=4):
         ms = np.zeros(n)
         cnt = np.zeros(n, dtype=np.int64)
         self.call("sw_kernel_times", f64p(ms), i64p(cnt), n)

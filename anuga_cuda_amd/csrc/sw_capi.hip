@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -24,7 +25,7 @@ namespace {
 
 constexpr int kGraphSteps = 16;
 constexpr int kBatchSteps = 8 * kGraphSteps;  // steps between host checks
-constexpr int kNumTimed = 3;                  // flux, update, distribute
+constexpr int kNumTimed = 4;                  // flux, timestep, update, distribute
 
 thread_local std::string g_create_error;
 
@@ -45,8 +46,8 @@ struct sw_handle {
     hipGraph_t graph = nullptr;
     bool timing = false;
     std::vector<hipEvent_t> ev;
-    double times_ms[kNumTimed] = {0, 0, 0};
-    int64_t launches[kNumTimed] = {0, 0, 0};
+    double times_ms[kNumTimed] = {};
+    int64_t launches[kNumTimed] = {};
     double *staging = nullptr;  // [3N] transfers of packed records
 };
 
@@ -257,13 +258,16 @@ int enqueue_fused_step(sw_handle *h, int slot) {
     else
         hipLaunchKernelGGL((k_flux<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     rec(1);
-    hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    hipLaunchKernelGGL((k_timestep<true>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->f.wave_min,
+                       (int)(g * (kBlock / 64)), h->st, h->dp);
     rec(2);
+    hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    rec(3);
     if (mixed)
         hipLaunchKernelGGL((k_distribute<float, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     else
         hipLaunchKernelGGL((k_distribute<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    rec(3);
+    rec(4);
     return launch_check(h, "fused step");
 }
 
@@ -351,7 +355,8 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     // Host-side validation, packing of the neighbour structure and transposition
     // of the per-edge / per-vertex geometry to the slot-major [3][N] layout.
     std::vector<int> nbv(3 * N), flagsv(N), surv(3 * N);
-    std::vector<double> nrm(6 * N), lenv(3 * N), vcv(6 * N), ccv(2 * N);
+    std::vector<EdgeRec> edgev(3 * N);
+    std::vector<double> vcv(6 * N), ccv(2 * N), rav(2 * N);
     bool own_normals = true;
     for (int64_t k = 0; k < N; k++) {
         int flags = 0;
@@ -393,9 +398,11 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
                 return nullptr;
             }
             surv[i * N + k] = (int)s;
-            nrm[2 * (i * N + k)] = m->normals[2 * ki];
-            nrm[2 * (i * N + k) + 1] = m->normals[2 * ki + 1];
-            lenv[i * N + k] = m->edgelengths[ki];
+            EdgeRec &er = edgev[i * N + k];
+            er.nx = m->normals[2 * ki];
+            er.ny = m->normals[2 * ki + 1];
+            er.len = m->edgelengths[ki];
+            er.nb = nbv[i * N + k];
             vcv[2 * (i * N + k)] = m->vertex_coordinates[6 * k + 2 * i];
             vcv[2 * (i * N + k) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
         }
@@ -411,7 +418,19 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
             return nullptr;
         }
         flags |= (int)nob << 12;
+        if (nob == 2) {
+            // the reconstruction of a triangle with two boundary edges uses its
+            // first surrogate neighbour that is not itself: keep it in slot 0
+            for (int i = 0; i < 3; i++)
+                if (surv[i * N + k] != k) {
+                    std::swap(surv[k], surv[i * N + k]);
+                    break;
+                }
+        }
         flagsv[k] = flags;
+        for (int i = 0; i < 3; i++) edgev[i * N + k].tflags = flags;
+        rav[2 * k] = m->radii[k];
+        rav[2 * k + 1] = m->areas[k];
         ccv[2 * k] = m->centroid_coordinates[2 * k];
         ccv[2 * k + 1] = m->centroid_coordinates[2 * k + 1];
     }
@@ -439,17 +458,17 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     M.N = (int)N;
     M.Nb = (int)Nb;
     M.own_normals = own_normals ? 1 : 0;
-    int *dnb, *dflags, *dsur, *bcell, *bedge, *bkind;
-    double *dnrm, *dlen, *dvc, *dcc, *radii, *areas, *bdir;
+    int *dflags, *dsur, *bcell, *bedge, *bkind;
+    EdgeRec *dedge;
+    double *dvc, *dcc, *dra, *bdir;
     auto put = [&](auto **dst, const auto &vec) -> int {
         if (dalloc(h, dst, vec.size())) return -1;
         if (hipMemcpy(*dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess)
             return fail(h, SW_ERR_HIP, "mesh upload failed");
         return 0;
     };
-    if (put(&dnb, nbv) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dnrm, nrm) || put(&dlen, lenv) ||
-        put(&dvc, vcv) || put(&dcc, ccv) ||
-        upload_f64(h, &radii, m->radii, N, "radii") || upload_f64(h, &areas, m->areas, N, "areas"))
+    if (put(&dedge, edgev) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dvc, vcv) || put(&dcc, ccv) ||
+        put(&dra, rav))
         return bail();
     if (Nb > 0) {
         if (upload_i32(h, &bcell, m->boundary_cells, Nb, 0, N - 1, "boundary_cells") ||
@@ -459,15 +478,12 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
         if (dalloc(h, &bcell, 1) || dalloc(h, &bedge, 1)) return bail();
     }
     if (dalloc(h, &bkind, Nb) || dalloc(h, &bdir, 3 * Nb)) return bail();
-    M.nb = dnb;
+    M.edge = dedge;
     M.flags = dflags;
     M.sur = dsur;
-    M.normals = reinterpret_cast<const double2 *>(dnrm);
-    M.len = dlen;
     M.vc = reinterpret_cast<const double2 *>(dvc);
     M.cc = reinterpret_cast<const double2 *>(dcc);
-    M.radii = radii;
-    M.areas = areas;
+    M.ra = reinterpret_cast<const double2 *>(dra);
     M.bcell = bcell;
     M.bedge = bedge;
     M.bkind = bkind;
@@ -484,12 +500,14 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
         if (dalloc(h, pp, 3 * N)) return bail();
     for (auto pp : perNb)
         if (dalloc(h, pp, Nb)) return bail();
-    if (dalloc(h, &F.cq, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &h->staging, 3 * N)) return bail();
+    if (dalloc(h, &F.cq, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &F.grav, N) ||
+        dalloc(h, &F.wave_min, (size_t)grid_for(N) * (kBlock / 64)) ||
+        dalloc(h, &h->staging, 3 * N))
+        return bail();
     if (dalloc(h, &h->st, 1)) return bail();
     DevState s0{};
     s0.order = p->default_order;
     s0.bed_interp = 1;
-    std::memcpy(&s0.flux_min_bits, &p->evolve_max_timestep, sizeof(double));
     if (hipMemcpy(h->st, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
         fail(h, SW_ERR_HIP, "state upload failed");
         return bail();
@@ -627,12 +645,12 @@ int sw_set_state(sw_handle *h, const sw_state *s) {
 int sw_compute_fluxes(sw_handle *h, double *flux_timestep) {
     if (!h) return SW_ERR_ARG;
     const unsigned g = grid_for(h->N);
-    hipLaunchKernelGGL(k_reset_flux_min, dim3(1), dim3(1), 0, h->stream, h->st, h->dp.evolve_max_timestep);
     if (h->params.precision == SW_PRECISION_MIXED)
         hipLaunchKernelGGL((k_flux<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     else
         hipLaunchKernelGGL((k_flux<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    hipLaunchKernelGGL(k_flux_finalize, dim3(1), dim3(1), 0, h->stream, h->st, h->dp.evolve_max_timestep);
+    hipLaunchKernelGGL((k_timestep<false>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->f.wave_min,
+                       (int)(g * (kBlock / 64)), h->st, h->dp);
     if (int rc = launch_check(h, "compute_fluxes")) return rc;
     DevState d;
     int rc = sync_state(h, &d);
@@ -758,8 +776,6 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
     d.event = 0;
     d.error = 0;
     d.halt = 0;
-    d.blocks_done = 0;
-    std::memcpy(&d.flux_min_bits, &h->params.evolve_max_timestep, sizeof(double));
     HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
     if (int rc = sw_update_boundary(h)) return rc;
@@ -774,6 +790,10 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         return fail(h, SW_ERR_STATE, "evolve_steps called while a yield is pending (call sw_advance_yield)");
     int64_t launched = 0;
     int rc = SW_OK;
+    if (max_steps > 0) {
+        hipLaunchKernelGGL(k_source_terms, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
+        rc = launch_check(h, "k_source_terms");
+    }
     // Launch in batches between host checks so a yield early in a long request
     // does not leave many idle steps queued.
     const int64_t batch = kBatchSteps;

@@ -30,6 +30,12 @@ SW_DEV double2 ld_s(const double2 *p) {
     const sw_f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const sw_f64x2 *>(p));
     return make_double2(v.x, v.y);
 }
+SW_DEV void st_s(double2 *p, double2 v) {
+    sw_f64x2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<sw_f64x2 *>(p));
+}
 
 // util_ext.h:42-56: comparisons, not fmax/fmin (NaN propagates the same way)
 template <typename T> SW_DEV T omax(T x, T y) { return (x > y) ? x : y; }
@@ -340,49 +346,16 @@ SW_DEV void reflect_momentum(double q1, double q2, double n1, double n2, double 
 SW_DEV unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 SW_DEV double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// Workgroup minimum of positive doubles as bit patterns: wavefront (64-lane)
-// shuffle reduction, then one LDS pass across the wavefronts.  The result is
-// valid in thread 0.
-template <int BLOCK>
-SW_DEV unsigned long long block_min_bits(double v) {
+// Wavefront (64-lane) minimum of positive doubles as bit patterns, by
+// cross-lane shuffles; the result is in every lane.
+SW_DEV unsigned long long wave_min_bits(double v) {
     unsigned long long b = dbits(v);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
+        const unsigned long long o = __shfl_xor(b, off, 64);
         b = (o < b) ? o : b;
     }
-    constexpr int NW = BLOCK / 64;
-    __shared__ unsigned long long wmin[NW];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wmin[wid] = b;
-    __syncthreads();
-    unsigned long long m = wmin[0];
-#pragma unroll
-    for (int w = 1; w < NW; w++) m = (wmin[w] < m) ? wmin[w] : m;
-    return m;
-}
-
-// Wavefront (64-lane) minimum with cross-lane shuffles, then one LDS pass
-// across the block's wavefronts; one atomic per block.
-template <int BLOCK>
-SW_DEV void block_min_atomic(double v, unsigned long long *target) {
-    unsigned long long b = dbits(v);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = (o < b) ? o : b;
-    }
-    constexpr int NW = BLOCK / 64;
-    __shared__ unsigned long long wmin[NW];
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wmin[wid] = b;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long m = wmin[0];
-#pragma unroll
-        for (int w = 1; w < NW; w++) m = (wmin[w] < m) ? wmin[w] : m;
-        atomicMin(target, m);
-    }
+    return b;
 }
 
 }  // namespace sw

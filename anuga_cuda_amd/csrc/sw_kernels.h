@@ -5,12 +5,14 @@
 //     each mirrors one reference kernel / C routine and its array contract);
 //   * fused step kernels used by evolve(); per Euler step
 //       flux       = compute_fluxes + boundary evaluation + gravity_wb +
-//                    Manning friction + CFL minimum (wavefront shuffles, one
-//                    atomic per workgroup) + update_timestep and the time
-//                    bookkeeping, done by the workgroup that finishes last;
+//                    CFL minimum (wavefront shuffles, one atomic per
+//                    workgroup) + update_timestep and the time bookkeeping,
+//                    done by the workgroup that finishes last;
 //       update     = update_conserved_quantities + protect;
 //       distribute = extrapolate_second_order_sw + balance_deep_and_shallow +
-//                    interpolate_from_vertices_to_edges;
+//                    interpolate_from_vertices_to_edges + the next step's
+//                    state-only source terms (stage-gradient gravity term,
+//                    Manning friction);
 //     the explicit / semi-implicit updates are the only intermediates that go
 //     through HBM and the host is not involved between steps.
 //
@@ -21,15 +23,14 @@
 //   * centroid state  cq[k]     = {w, uh, vh, z}  32-byte record
 //   * edge state      eq[i*N+k] = {w, uh, vh, z}  32-byte record; a
 //     neighbour's edge is a single 32-byte gather;
-//   * nb[i*N+k] = m*N+n, the slot of the neighbour's matching edge, or
-//     -(b+1) for boundary edge b;
+//   * edge[i*N+k] = {normal, length, m*N+n the slot of the neighbour's
+//     matching edge or -(b+1) for boundary edge b}, a 32-byte record;
 //   * flags[k]: bit i edge i is evaluated from the lower-numbered neighbour;
 //     bit 3+i the evaluating triangle is full; bits 6-11 the order in which
 //     the three edge fluxes are summed (2 bits per position); bits 12-13
 //     number_of_boundaries;
-//   * normals (double2), edge lengths, vertex coordinates (double2), vertex
-//     values, surrogate neighbours: [3][N]; centroids (double2), radii,
-//     areas, updates: [N].
+//   * vertex coordinates (double2), vertex values, surrogate neighbours:
+//     [3][N]; centroids (double2), {radius, area} (double2), updates: [N].
 // Workgroup indices are remapped so each of the 8 XCDs sweeps one contiguous
 // range of a locality-ordered mesh and neighbour gathers hit its own L2.
 #pragma once
@@ -52,17 +53,22 @@ struct DevParams {
     int use_centroid_velocities, use_sloped_mannings, max_smallsteps, default_order;
 };
 
+// Static geometry of edge i of triangle k, one 32-byte record at i*N + k.
+struct alignas(32) EdgeRec {
+    double nx, ny, len;  // outward normal and length
+    int nb;              // neighbour's matching edge slot m*N+n, or -(b+1) for boundary edge b
+    int tflags;          // the triangle's flags word (same in all three slots)
+};
+
 struct DevMesh {
     int N, Nb;
     int own_normals;  // shared-edge normals are exact negations, lengths equal
-    const int *nb;    // [3][N]
-    const int *flags; // [N]
-    const int *sur;   // [3][N]
-    const double2 *normals;  // [3][N]
-    const double *len;       // [3][N]
-    const double2 *vc;       // [3][N] vertex coordinates
-    const double2 *cc;       // [N]    centroid coordinates
-    const double *radii, *areas;  // [N]
+    const EdgeRec *edge;  // [3][N]
+    const int *flags;     // [N]
+    const int *sur;       // [3][N]
+    const double2 *vc;    // [3][N] vertex coordinates
+    const double2 *cc;    // [N]    centroid coordinates
+    const double2 *ra;    // [N]    {radius, area}
     const int *bcell, *bedge, *bkind;  // [Nb]
     const double *bdir;                // [Nb][3]
 };
@@ -79,6 +85,8 @@ struct DevFields {
     double *uc, *ue, *uv, *ub;            // xvelocity
     double *vc, *ve, *vv, *vb;            // yvelocity
     double *friction, *zbar, *max_speed;  // [N]
+    double2 *grav;                        // [N] -g*(wx, wy)*avg_h of the next step
+    unsigned long long *wave_min;         // [kBlock/64 * blocks] CFL minimum of each wavefront (bits)
 };
 
 // Scalar evolve state kept in device memory.  The first block of fields has
@@ -90,8 +98,6 @@ struct DevState {
     long long has_finaltime, event, error;
     // device-only
     long long halt, total_steps, bed_interp;
-    unsigned long long flux_min_bits;
-    unsigned int blocks_done, pad_;
 };
 
 enum : long long { kErrNone = 0, kErrUpdate = 1, kErrSmallsteps = 2, kErrNeighbour = 4, kErrDiscontinuous = 8 };
@@ -108,6 +114,18 @@ SW_DEV void set_error(DevState *st, long long code) {
 }
 
 SW_DEV Q4 ldq(const Q4 *p) { return *p; }
+SW_DEV EdgeRec ld_edge(const EdgeRec *p) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 *q = reinterpret_cast<const u32x4 *>(p);
+    const u32x4 a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1);
+    EdgeRec r;
+    r.nx = __hiloint2double((int)a.y, (int)a.x);
+    r.ny = __hiloint2double((int)a.w, (int)a.z);
+    r.len = __hiloint2double((int)b.y, (int)b.x);
+    r.nb = (int)b.z;
+    r.tflags = (int)b.w;
+    return r;
+}
 SW_DEV int nob_of(int flags) { return (flags >> 12) & 3; }
 
 // update_timestep (evolve.c:115-160 with the final-time clip as finaltime - time)
@@ -162,10 +180,11 @@ SW_DEV void finish_step(DevState *st) {
 // edges of k (gated by tri_full_flag of the evaluating triangle) give the same
 // minimum as the C's edge-once loop.
 //
+// Each wavefront stores the minimum of its triangles' CFL candidates to
+// wave_min (a plain store: no atomics, no fences); k_timestep reduces them.
+//
 // FUSED adds: boundary values evaluated on the fly (reflective / Dirichlet /
-// transmissive), gravity_wb, Manning friction into the semi-implicit update,
-// and, in the last workgroup to finish, the timestep and time bookkeeping of
-// the device-side evolve loop.
+// transmissive) and gravity_wb.
 template <typename R, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st) {
     if (FUSED && (st->event || st->error)) {
@@ -177,49 +196,69 @@ __global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevPara
     const int k = tri_index();
     double dtk = P.evolve_max_timestep;
     if (k < N) {
-        const int flags = ld_s(M.flags + k);
-        const double radius = ld_s(M.radii + k);
-        double ef[3][3];
-        bool have[3];
-        double hh[3];
-        double2 nrm[3];
-        double lens[3];
-        double max_speed_total = 0.0;
-        const double eps = P.epsilon;
+        // Load phase: every per-triangle and per-edge input is requested before
+        // any of it is used, then the three neighbour edge records are gathered
+        // together, so a wavefront waits for two memory round trips in all.
+        EdgeRec er[3];
+        Q4 own[3], nbq[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) er[i] = ld_edge(M.edge + i * N + k);
+#pragma unroll
+        for (int i = 0; i < 3; i++) own[i] = ldq(F.eq + i * N + k);
+        const double2 ra = ld_s(M.ra + k);
+        const double2 gr = FUSED ? ld_s(F.grav + k) : make_double2(0.0, 0.0);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const int ki = i * N + k;
-            const int e = ld_s(M.nb + ki);
+            const int e = er[i].nb;
+            nbq[i] = ldq(F.eq + (e >= 0 ? e : i * N + k));  // boundary edge: its own record
+        }
+        // (every loaded word is consumed, so no load result register is
+        // recycled while the load is still in flight)
+        const int flags = er[0].tflags | er[1].tflags | er[2].tflags;
+        const double eps = P.epsilon;
+        const double g = P.g;
+        // The edge summed last in the C's order; the first two in that order
+        // may be added as they come (0 + a + b is exact in either order).
+        const int last_edge = (flags >> 10) & 3;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;   // flux sum of the first two edges
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0;   // signed flux of the last edge
+        bool have_last = false;
+        double sidex = 0.0, sidey = 0.0;        // gravity_wb side term, edge order
+        double cfl_speed = 0.0;  // max over CFL-gated edges: min r/ms_i == r/max ms_i
+        double max_speed_total = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int e = er[i].nb;
             const bool lower = (flags >> i) & 1;
-            const Q4 own = ldq(F.eq + ki);
-            const double2 nk = ld_s(M.normals + ki);
-            const double lk = ld_s(M.len + ki);
-            nrm[i] = nk;
-            lens[i] = lk;
-            hh[i] = own.w - own.z;
-            double n1 = nk.x, n2 = nk.y, len = lk;
+            const double lk = er[i].len;
+            if (FUSED) {
+                const double hh = own[i].w - own[i].z;
+                const double fact = -0.5 * g * hh * hh * lk;
+                sidex = sidex + fact * er[i].nx;
+                sidey = sidey + fact * er[i].ny;
+            }
+            double n1 = er[i].nx, n2 = er[i].ny, len = lk;
             double ql[3], qr[3], zl, zr;
             if (e >= 0) {
-                const Q4 nbq = ldq(F.eq + e);
                 if (lower) {
-                    ql[0] = nbq.w; ql[1] = nbq.uh; ql[2] = nbq.vh; zl = nbq.z;
-                    qr[0] = own.w; qr[1] = own.uh; qr[2] = own.vh; zr = own.z;
+                    ql[0] = nbq[i].w; ql[1] = nbq[i].uh; ql[2] = nbq[i].vh; zl = nbq[i].z;
+                    qr[0] = own[i].w; qr[1] = own[i].uh; qr[2] = own[i].vh; zr = own[i].z;
                     if (M.own_normals) {
                         n1 = -n1;
                         n2 = -n2;
                     } else {
-                        const double2 nn = M.normals[e];
-                        n1 = nn.x;
-                        n2 = nn.y;
-                        len = M.len[e];
+                        const EdgeRec en = ld_edge(M.edge + e);
+                        n1 = en.nx;
+                        n2 = en.ny;
+                        len = en.len;
                     }
                 } else {
-                    ql[0] = own.w; ql[1] = own.uh; ql[2] = own.vh; zl = own.z;
-                    qr[0] = nbq.w; qr[1] = nbq.uh; qr[2] = nbq.vh; zr = nbq.z;
+                    ql[0] = own[i].w; ql[1] = own[i].uh; ql[2] = own[i].vh; zl = own[i].z;
+                    qr[0] = nbq[i].w; qr[1] = nbq[i].uh; qr[2] = nbq[i].vh; zr = nbq[i].z;
                 }
             } else {
                 const int m = -e - 1;
-                ql[0] = own.w; ql[1] = own.uh; ql[2] = own.vh; zl = own.z;
+                ql[0] = nbq[i].w; ql[1] = nbq[i].uh; ql[2] = nbq[i].vh; zl = nbq[i].z;
                 zr = zl;
                 if (FUSED) {
                     const int kind = M.bkind[m];
@@ -236,9 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevPara
                     qr[0] = F.wb[m]; qr[1] = F.uhb[m]; qr[2] = F.vhb[m];
                 }
             }
-            have[i] = false;
-            ef[i][0] = ef[i][1] = ef[i][2] = 0.0;
-            if (P.optimise_dry_cells && fabs(ql[0] - zl) < eps && fabs(qr[0] - zr) < eps) continue;
+        if (P.optimise_dry_cells && fabs(ql[0] - zl) < eps && fabs(qr[0] - zr) < eps) continue;
             if (fabs(zl - zr) > 1.0e-10) set_error(st, kErrDiscontinuous);
             R qlr[3] = {R(ql[0]), R(ql[1]), R(ql[2])};
             R qrr[3] = {R(qr[0]), R(qr[1]), R(qr[2])};
@@ -246,125 +283,93 @@ __global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevPara
             flux_central<R>(qlr, qrr, R(zl), R(zr), R(n1), R(n2), R(eps), R(P.h0),
                             R(P.limiting_threshold), R(P.g), efr, msr);
             const double ms = double(msr);
-            ef[i][0] = double(efr[0]) * len;
-            ef[i][1] = double(efr[1]) * len;
-            ef[i][2] = double(efr[2]) * len;
-            have[i] = true;
-            if (((flags >> (3 + i)) & 1) && ms > eps) dtk = omin<double>(dtk, radius / ms);
+            // the lower-numbered triangle of the edge adds the flux, the other subtracts
+            double f0 = double(efr[0]) * len, f1 = double(efr[1]) * len, f2 = double(efr[2]) * len;
+            if (!lower) {
+                f0 = -f0;
+                f1 = -f1;
+                f2 = -f2;
+            }
+            if (i == last_edge) {
+                l0 = f0;
+                l1 = f1;
+                l2 = f2;
+                have_last = true;
+            } else {
+                s0 = s0 + f0;
+                s1 = s1 + f1;
+                s2 = s2 + f2;
+            }
+            if (((flags >> (3 + i)) & 1) && ms > eps) cfl_speed = omax<double>(cfl_speed, ms);
             if (!lower) max_speed_total = omax<double>(max_speed_total, ms);
         }
-        // C summation order (lower neighbours by ascending index, then own
-        // edges), precomputed as three 2-bit edge indices
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int i = (flags >> (6 + 2 * j)) & 3;
-            const double f0 = (i == 0) ? ef[0][0] : ((i == 1) ? ef[1][0] : ef[2][0]);
-            const double f1 = (i == 0) ? ef[0][1] : ((i == 1) ? ef[1][1] : ef[2][1]);
-            const double f2 = (i == 0) ? ef[0][2] : ((i == 1) ? ef[1][2] : ef[2][2]);
-            const bool hv = (i == 0) ? have[0] : ((i == 1) ? have[1] : have[2]);
-            if (hv) {
-                if ((flags >> i) & 1) {
-                    s0 += f0;
-                    s1 += f1;
-                    s2 += f2;
-                } else {
-                    s0 -= f0;
-                    s1 -= f1;
-                    s2 -= f2;
-                }
-            }
+        if (have_last) {
+            s0 = s0 + l0;
+            s1 = s1 + l1;
+            s2 = s2 + l2;
         }
-        const double area = ld_s(M.areas + k);
+        if (cfl_speed > 0.0) dtk = omin<double>(dtk, ra.x / cfl_speed);
+        const double area = ra.y;
         const double inv_area = 1.0 / area;
         s0 *= inv_area;
         s1 *= inv_area;
         s2 *= inv_area;
         if (FUSED) {
-            const Q4 c = ldq(F.cq + k);
-            const double g = P.g;
-            // gravity_wb (shallow_water_ext.c:1942-2037)
-            const double2 x0 = ld_s(M.vc + k), x1 = ld_s(M.vc + N + k), x2 = ld_s(M.vc + 2 * N + k);
-            double wx, wy;
-            gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, ld_s(F.wv + k), ld_s(F.wv + N + k),
-                      ld_s(F.wv + 2 * N + k), wx, wy);
-            const double avg_h = c.w - c.z;
-            s1 += -g * wx * avg_h;
-            s2 += -g * wy * avg_h;
-            double sidex = 0.0, sidey = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const double fact = -0.5 * g * hh[i] * hh[i] * lens[i];
-                sidex = sidex + fact * nrm[i].x;
-                sidey = sidey + fact * nrm[i].y;
-            }
+            // gravity_wb (shallow_water_ext.c:1942-2037): the stage-gradient term
+            // was formed by the previous distribute launch from the same values.
+            s1 += gr.x;
+            s2 += gr.y;
             s1 += -sidex / area;
             s2 += -sidey / area;
-            // manning_friction_implicit (shallow_water_ext.c:1238-1317)
-            double si_x = 0.0, si_y = 0.0;
-            const double eta = ld_s(F.friction + k);
-            const double mah = P.minimum_allowed_height;
-            if (eta > mah) {
-                const double uh = c.uh, vh = c.vh;
-                const double h = c.w - ld_s(F.zbar + k);
-                if (h >= mah) {
-                    double S;
-                    if (P.use_sloped_mannings) {
-                        double zx, zy;
-                        gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, F.zv[k], F.zv[N + k], F.zv[2 * N + k],
-                                  zx, zy);
-                        const double zs = sqrt(1.0 + zx * zx + zy * zy);
-                        S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
-                    } else {
-                        S = -g * eta * eta * sqrt((uh * uh + vh * vh));
-                    }
-                    S /= pow(h, 7.0 / 3);
-                    si_x += S * uh;
-                    si_y += S * vh;
-                }
-            }
-            st_s(F.uhsi + k, si_x);
-            st_s(F.vhsi + k, si_y);
         }
         st_s(F.weu + k, s0);
         st_s(F.uheu + k, s1);
         st_s(F.vheu + k, s2);
         st_s(F.max_speed + k, max_speed_total);
     }
-    if (!FUSED) {
-        block_min_atomic<kBlock>(dtk, &st->flux_min_bits);
-        return;
+    const unsigned long long wm = wave_min_bits(dtk);
+    if ((threadIdx.x & 63) == 0) F.wave_min[k >> 6] = wm;
+}
+
+// CFL minimum over the wavefront minima of k_flux (one workgroup of
+// kReduceBlock threads).  Unfused: sets flux_timestep only.  Fused: also
+// update_timestep, the time advance and the end-of-step bookkeeping, unless
+// the loop is halted.
+constexpr int kReduceBlock = 1024;
+template <bool FUSED>
+__global__ __launch_bounds__(kReduceBlock) void k_timestep(const unsigned long long *wave_min, int nwaves,
+                                                           DevState *st, DevParams P) {
+    // state and partial minima are requested together: one memory round trip
+    DevState s = *st;
+    constexpr int U = 16;
+    unsigned long long b = dbits(P.evolve_max_timestep);
+    for (int base = 0; base < nwaves; base += U * kReduceBlock) {
+        unsigned long long v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = base + u * kReduceBlock + (int)threadIdx.x;
+            v[u] = (j < nwaves) ? wave_min[j] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) b = (v[u] < b) ? v[u] : b;
     }
-    // Fused: the workgroup minimum goes in with a returning atomic, so it has
-    // been performed before the same lane counts the workgroup as done; the
-    // workgroup that counts last takes the global minimum (and resets it) with
-    // one atomic exchange.  Only read-modify-write atomics touch the two
-    // words, so no cache writeback fence is needed per workgroup.
-    const unsigned long long wmin = block_min_bits<kBlock>(dtk);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        const unsigned long long old = __hip_atomic_fetch_min(&st->flux_min_bits, wmin, __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT);
-        // The branch consumes the returned value, so the count below is issued
-        // only after the minimum has been performed.  (The word always holds
-        // the bits of a positive double, never all ones.)
-        unsigned prev = 0xffffffffu;
-        if (old != ~0ull)
-            prev = __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-    }
+    b = wave_min_bits(bitsd(b));
+    __shared__ unsigned long long wmin[kReduceBlock / 64];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = b;
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        const unsigned long long b = __hip_atomic_exchange(&st->flux_min_bits, dbits(P.evolve_max_timestep),
-                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_exchange(&st->blocks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        st->flux_timestep = bitsd(b);
-        timestep_logic(st, P);
-        if (!(st->error & kErrSmallsteps)) {
-            st->time = st->time + st->timestep;
-            finish_step(st);
+    if (threadIdx.x != 0) return;
+    if (FUSED && s.halt) return;
+#pragma unroll
+    for (int w = 0; w < kReduceBlock / 64; w++) b = (wmin[w] < b) ? wmin[w] : b;
+    s.flux_timestep = bitsd(b);
+    if (FUSED) {
+        timestep_logic(&s, P);
+        if (!(s.error & kErrSmallsteps)) {
+            s.time = s.time + s.timestep;
+            finish_step(&s);
         }
     }
+    *st = s;
 }
 
 // gravity_wb as a separate operator (shallow_water_ext.c:1942-2037)
@@ -385,12 +390,12 @@ __global__ __launch_bounds__(kBlock) void k_gravity_wb(DevMesh M, DevFields F, D
     for (int i = 0; i < 3; i++) {
         const Q4 eqi = ldq(F.eq + i * N + k);
         const double hh = eqi.w - eqi.z;
-        const double fact = -0.5 * g * hh * hh * M.len[i * N + k];
-        const double2 n = M.normals[i * N + k];
-        sidex = sidex + fact * n.x;
-        sidey = sidey + fact * n.y;
+        const EdgeRec er = M.edge[i * N + k];
+        const double fact = -0.5 * g * hh * hh * er.len;
+        sidex = sidex + fact * er.nx;
+        sidey = sidey + fact * er.ny;
     }
-    const double area = M.areas[k];
+    const double area = M.ra[k].y;
     ex += -sidex / area;
     ey += -sidey / area;
     F.uheu[k] = ex;
@@ -429,19 +434,53 @@ __global__ __launch_bounds__(kBlock) void k_manning(DevMesh M, DevFields F, DevP
     }
 }
 
-__global__ void k_reset_flux_min(DevState *st, double evolve_max_timestep) {
-    st->flux_min_bits = dbits(evolve_max_timestep);
-    st->blocks_done = 0;
-}
-
-__global__ void k_flux_finalize(DevState *st, double evolve_max_timestep) {
-    st->flux_timestep = bitsd(st->flux_min_bits);
-    st->flux_min_bits = dbits(evolve_max_timestep);
-}
 
 __global__ void k_update_timestep(DevState *st, DevParams P) { timestep_logic(st, P); }
 __global__ void k_advance_time(DevState *st) { st->time = st->time + st->timestep; }
 __global__ void k_finish_step(DevState *st) { finish_step(st); }
+
+// Source terms of a step that depend only on the state the previous step left
+// behind, formed as soon as that state is known (bit-identical to forming them
+// at the start of the step: same inputs, same operations).
+//
+// Implicit Manning friction into the semi-implicit momentum updates
+// (shallow_water_ext.c:1238-1317), from the centroid values.
+SW_DEV void friction_si(const DevMesh &M, const DevFields &F, const DevParams &P, int k, const Q4 &c,
+                        double &si_x, double &si_y) {
+    si_x = 0.0;
+    si_y = 0.0;
+    const double g = P.g;
+    const double eta = ld_s(F.friction + k);
+    const double mah = P.minimum_allowed_height;
+    if (!(eta > mah)) return;
+    const double uh = c.uh, vh = c.vh;
+    const double h = c.w - ld_s(F.zbar + k);
+    if (h >= mah) {
+        double S;
+        if (P.use_sloped_mannings) {
+            const int N = M.N;
+            const double2 x0 = M.vc[k], x1 = M.vc[N + k], x2 = M.vc[2 * N + k];
+            double zx, zy;
+            gradient3(x0.x, x0.y, x1.x, x1.y, x2.x, x2.y, F.zv[k], F.zv[N + k], F.zv[2 * N + k], zx, zy);
+            const double zs = sqrt(1.0 + zx * zx + zy * zy);
+            S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
+        } else {
+            S = -g * eta * eta * sqrt((uh * uh + vh * vh));
+        }
+        S /= pow(h, 7.0 / 3);
+        si_x += S * uh;
+        si_y += S * vh;
+    }
+}
+
+// The stage-gradient part of gravity_wb (shallow_water_ext.c:1942-1990:
+// -g * w_x * avg_h, w_x from the vertex stage), from the vertex stage values.
+SW_DEV double2 gravity_gradient_term(const double2 v[3], const double wv[3], const Q4 &c, double g) {
+    double wx, wy;
+    gradient3(v[0].x, v[0].y, v[1].x, v[1].y, v[2].x, v[2].y, wv[0], wv[1], wv[2], wx, wy);
+    const double avg_h = c.w - c.z;
+    return make_double2(-g * wx * avg_h, -g * wy * avg_h);
+}
 
 // update_conserved_quantities (quantity.c:769-820), optionally followed by
 // protect (the first stage of distribute_to_vertices_and_edges).
@@ -459,6 +498,12 @@ __global__ __launch_bounds__(kBlock) void k_update(DevMesh M, DevFields F, DevPa
     if (!ok) set_error(st, kErrUpdate);
     if (FUSED) {
         protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
+        // The centroid values are final for this step (distribute leaves them
+        // alone), so the next step's Manning friction is formed here.
+        double fx, fy;
+        friction_si(M, F, P, k, c, fx, fy);
+        st_s(F.uhsi + k, fx);
+        st_s(F.vhsi + k, fy);
     } else {
         F.wsi[k] = 0.0;
         F.uhsi[k] = 0.0;
@@ -488,13 +533,46 @@ SW_DEV void centroid_wuv(const Q4 &c, const DevParams &P, double &w, double &u, 
     }
 }
 
-// extrapolate_second_order_sw for triangle k (shallow_water_ext.c:2717-3324),
-// result in wv/uv/vv (momentum), zv = bed vertex values.  Returns false on the
-// missing-neighbour error.
-template <typename R>
-SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParams &P, int k, int nob,
-                            const Q4 &ck, const double zv[3], double wv[3], double uv[3], double vv[3]) {
+// Inputs of the reconstruction of triangle k.  They are loaded in two batches
+// (own data, then the surrogate neighbours' records) before any is used, so a
+// wavefront waits for two memory round trips.
+struct RecIn {
+    Q4 c;           // own centroid record
+    double2 xy;     // own centroid
+    double2 v[3];   // vertex coordinates
+    int sk[3];      // surrogate neighbours
+    Q4 cs[3];       // their centroid records
+    double2 xs[3];  // their centroids
+    double zv[3];   // bed vertex values
+    int flags;
+};
+
+SW_DEV void load_rec(const DevMesh &M, const DevFields &F, int k, bool second_order, RecIn &r) {
     const int N = M.N;
+    r.c = ldq(F.cq + k);
+#pragma unroll
+    for (int i = 0; i < 3; i++) r.zv[i] = ld_s(F.zv + i * N + k);
+    r.flags = ld_s(M.flags + k);
+#pragma unroll
+    for (int i = 0; i < 3; i++) r.v[i] = ld_s(M.vc + i * N + k);
+    if (!second_order) return;
+    r.xy = M.cc[k];
+#pragma unroll
+    for (int i = 0; i < 3; i++) r.sk[i] = ld_s(M.sur + i * N + k);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        r.cs[i] = ldq(F.cq + r.sk[i]);
+        r.xs[i] = M.cc[r.sk[i]];
+    }
+}
+
+// extrapolate_second_order_sw for triangle k (shallow_water_ext.c:2717-3324),
+// result in wv/uv/vv (momentum).  Returns false on the missing-neighbour error.
+template <typename R>
+SW_DEV bool extrapolate_one(const DevFields &F, const DevParams &P, int N, int k, const RecIn &r, double wv[3],
+                            double uv[3], double vv[3]) {
+    const Q4 &ck = r.c;
+    const int nob = nob_of(r.flags);
     double wc, uc, vc;
     centroid_wuv(ck, P, wc, uc, vc);
     bool ok = true;
@@ -502,26 +580,22 @@ SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParam
     if (nob == 3) {
         constant = true;
     } else {
-        const double2 xy = M.cc[k];
-        const double x = xy.x, y = xy.y;
-        const double2 v0 = ld_s(M.vc + k), v1 = ld_s(M.vc + N + k), v2 = ld_s(M.vc + 2 * N + k);
+        const double x = r.xy.x, y = r.xy.y;
         R dxv[3], dyv[3];
-        dxv[0] = R(v0.x - x);
-        dxv[1] = R(v1.x - x);
-        dxv[2] = R(v2.x - x);
-        dyv[0] = R(v0.y - y);
-        dyv[1] = R(v1.y - y);
-        dyv[2] = R(v2.y - y);
-        const int k0 = ld_s(M.sur + k), k1 = ld_s(M.sur + N + k), k2 = ld_s(M.sur + 2 * N + k);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            dxv[i] = R(r.v[i].x - x);
+            dyv[i] = R(r.v[i].y - y);
+        }
         if (nob <= 1) {
-            const double2 c0xy = M.cc[k0], c1xy = M.cc[k1], c2xy = M.cc[k2];
+            const double2 c0xy = r.xs[0], c1xy = r.xs[1], c2xy = r.xs[2];
             const double dx1 = c1xy.x - c0xy.x, dx2 = c2xy.x - c0xy.x;
             const double dy1 = c1xy.y - c0xy.y, dy2 = c2xy.y - c0xy.y;
             const double area2 = dy2 * dx1 - dy1 * dx2;
             if (area2 <= 0) {
                 constant = true;
             } else {
-                const Q4 c0 = ldq(F.cq + k0), c1 = ldq(F.cq + k1), c2 = ldq(F.cq + k2);
+                const Q4 &c0 = r.cs[0], &c1 = r.cs[1], &c2 = r.cs[2];
                 double w0, u0, v0_, w1, u1, v1_, w2, u2, v2_;
                 centroid_wuv(c0, P, w0, u0, v0_);
                 centroid_wuv(c1, P, w1, u1, v1_);
@@ -566,18 +640,19 @@ SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParam
                 }
             }
         } else {
-            const int kk = (k0 != k) ? k0 : ((k1 != k) ? k1 : k2);
-            if (kk == k) {
+            // the first surrogate that is not k itself; the device copy of the
+            // surrogate list holds it in slot 0
+            if (r.sk[0] == k) {
                 ok = false;
                 constant = true;
             } else {
-                const double2 c1xy = M.cc[kk];
+                const double2 c1xy = r.xs[0];
+                const Q4 &c1 = r.cs[0];
                 const double dx1 = c1xy.x - x, dy1 = c1xy.y - y;
                 const double area2 = dx1 * dx1 + dy1 * dy1;
                 double dx2 = 1.0 / area2;
                 const double dy2 = dx2 * dy1;
                 dx2 *= dx1;
-                const Q4 c1 = ldq(F.cq + kk);
                 double w1, u1, v1_;
                 centroid_wuv(c1, P, w1, u1, v1_);
                 R d[3];
@@ -605,7 +680,7 @@ SW_DEV bool extrapolate_one(const DevMesh &M, const DevFields &F, const DevParam
     if (P.extrapolate_velocity_second_order == 1) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double dv = omax<double>(wv[i] - zv[i], 0.);
+            const double dv = omax<double>(wv[i] - r.zv[i], 0.);
             uv[i] = uv[i] * dv;
             vv[i] = vv[i] * dv;
         }
@@ -619,11 +694,10 @@ __global__ __launch_bounds__(kBlock) void k_extrapolate_second_order(DevMesh M, 
     const int N = M.N;
     const int k = tri_index();
     if (k >= N) return;
-    double wv[3], uv[3], vv[3], zv[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) zv[i] = F.zv[i * N + k];
-    if (!extrapolate_one<R>(M, F, P, k, nob_of(M.flags[k]), ldq(F.cq + k), zv, wv, uv, vv))
-        set_error(st, kErrNeighbour);
+    RecIn r;
+    load_rec(M, F, k, true, r);
+    double wv[3], uv[3], vv[3];
+    if (!extrapolate_one<R>(F, P, N, k, r, wv, uv, vv)) set_error(st, kErrNeighbour);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         F.wv[i * N + k] = wv[i];
@@ -699,6 +773,21 @@ __global__ __launch_bounds__(kBlock) void k_interpolate(DevMesh M, DevFields F) 
     }
 }
 
+// Source terms of the first fused step of an evolve call.
+__global__ __launch_bounds__(kBlock) void k_source_terms(DevMesh M, DevFields F, DevParams P) {
+    const int N = M.N;
+    const int k = tri_index();
+    if (k >= N) return;
+    const Q4 c = ldq(F.cq + k);
+    const double2 v[3] = {M.vc[k], M.vc[N + k], M.vc[2 * N + k]};
+    const double wv[3] = {F.wv[k], F.wv[N + k], F.wv[2 * N + k]};
+    F.grav[k] = gravity_gradient_term(v, wv, c, P.g);
+    double fx, fy;
+    friction_si(M, F, P, k, c, fx, fy);
+    F.uhsi[k] = fx;
+    F.vhsi[k] = fy;
+}
+
 // Fused distribute_to_vertices_and_edges after protect: reconstruction
 // (second order, or first order when the evolve loop fell back to it),
 // balance_deep_and_shallow and vertex->edge interpolation in registers.  The
@@ -711,32 +800,34 @@ __global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, D
     const int N = M.N;
     const int k = tri_index();
     if (k >= N) return;
-    double wv[3], uv[3], vv[3], zv[3];
-    const Q4 c = ldq(F.cq + k);
-#pragma unroll
-    for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + k);
-    if (st->order == 1) {
+    const bool second_order = st->order != 1;
+    RecIn r;
+    load_rec(M, F, k, second_order, r);
+    const Q4 &c = r.c;
+    double wv[3], uv[3], vv[3];
+    if (!second_order) {
         wv[0] = wv[1] = wv[2] = c.w;
         uv[0] = uv[1] = uv[2] = c.uh;
         vv[0] = vv[1] = vv[2] = c.vh;
     } else {
-        if (!extrapolate_one<R>(M, F, P, k, nob_of(ld_s(M.flags + k)), c, zv, wv, uv, vv))
-            set_error(st, kErrNeighbour);
+        if (!extrapolate_one<R>(F, P, N, k, r, wv, uv, vv)) set_error(st, kErrNeighbour);
     }
     BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
-    balance_one(c.w, c.z, wv, zv, c.uh, c.vh, uv, vv, bp);
+    balance_one(c.w, c.z, wv, r.zv, c.uh, c.vh, uv, vv, bp);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         st_s(F.wv + i * N + k, wv[i]);
         st_s(F.uhv + i * N + k, uv[i]);
         st_s(F.vhv + i * N + k, vv[i]);
     }
+    // next step's stage-gradient gravity term, from the final vertex stage
+    if (FUSED) st_s(F.grav + k, gravity_gradient_term(r.v, wv, c, P.g));
     double ew[3], eu[3], ev[3], ez[3];
     interpolate_one(wv, ew);
     interpolate_one(uv, eu);
     interpolate_one(vv, ev);
     if (st->bed_interp) {
-        interpolate_one(zv, ez);
+        interpolate_one(r.zv, ez);
     } else {
         ez[0] = F.ze[k];
         ez[1] = F.ze[N + k];
@@ -765,8 +856,8 @@ __global__ __launch_bounds__(kBlock) void k_update_boundary(DevMesh M, DevFields
     const int kind = M.bkind[m];
     const Q4 q = ldq(F.eq + e);
     const double he = F.he[e], ue = F.ue[e], ve = F.ve[e];
-    const double2 n = M.normals[e];
-    const double n1 = n.x, n2 = n.y;
+    const EdgeRec er = M.edge[e];
+    const double n1 = er.nx, n2 = er.ny;
     double wb = q.w, uhb = q.uh, vhb = q.vh, hb = he, ub = ue, vb = ve;  // transmissive
     if (kind == 0) {  // reflective
         reflect_momentum(q.uh, q.vh, n1, n2, uhb, vhb);

@@ -28,24 +28,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Unique HBM footprint of one Euler step per triangle (FP64 values, int32
-# indices, neighbour gathers as cache hits): geometry 196 + static quantities
-# 64 + centroid state 48 + vertex state 144 + edge exchange 144 + max_speed 8.
-STEP_BYTES_PER_TRIANGLE = 604
-# Per-launch algorithmic bytes per triangle of the fused kernels: every array a
-# kernel reads or writes, counted once (neighbour gathers as cache hits).
+# Algorithmic HBM bytes per triangle of each fused kernel launch: every array
+# the kernel reads or writes, counted once; neighbour gathers (edge records,
+# centroid records, centroid coordinates) counted as cache hits.
 KERNEL_BYTES_PER_TRIANGLE = {
-    # reads: edge records {w,uh,vh,z} x3 96, normals 48, lengths 24, neighbour
-    # word 16, radius 8, area 8, stage vertex 24, vertex xy 48, centroid record
-    # 32, mean bed 8, friction 8 | writes: explicit 24, semi-implicit 16, max_speed 8
-    "flux": 96 + 48 + 24 + 16 + 8 + 8 + 24 + 48 + 32 + 8 + 8 + 24 + 16 + 8,
+    # reads: edge geometry records 3x32 96, own edge state records 3x32 96,
+    # {radius, area} 16, gravity stage-gradient term 16 | writes: explicit
+    # updates 24, max_speed 8
+    "flux": 96 + 96 + 16 + 16 + 24 + 8,
     # reads: centroid record 32, explicit 24, semi-implicit 24 | writes: centroid record 32
     "update": 32 + 24 + 24 + 32,
-    # reads: centroid record 32, centroid xy 16, surrogate 12, nob 4, vertex xy
-    # 48, bed vertex 24 | writes: vertex w/uh/vh 72, edge records 96
-    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 96,
+    # reads: centroid record 32, centroid xy 16, surrogates 12, flags 4, vertex
+    # xy 48, bed vertex 24, friction 8, mean bed 8 | writes: vertex w/uh/vh 72,
+    # edge state records 96, gravity term 16, semi-implicit updates 16
+    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 8 + 8 + 72 + 96 + 16 + 16,
 }
-KERNELS = ("flux", "update", "distribute")
+# One Euler step = the three kernels above (the timestep reduction moves 8
+# bytes per 64 triangles).
+STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
+KERNELS = ("flux", "timestep", "update", "distribute")
 
 
 def log(msg):
@@ -117,7 +118,7 @@ def main():
     h.call("sw_enable_timing", 1)
     h.evolve_steps(args.timing_steps)
     h.call("sw_enable_timing", 0)
-    ms, cnt = h.kernel_times(3)
+    ms, cnt = h.kernel_times(4)
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
     dominant = max(("flux", "update", "distribute"), key=lambda k: avg_us[k] or 0.0)

@@ -171,7 +171,7 @@ int sw_synchronize(sw_handle *h);
 /* Kernel timing of the fused step: when enabled, sw_evolve_steps records HIP
  * events around every launch on the handle's stream; sw_kernel_times returns
  * accumulated milliseconds and launch counts per fused kernel (0 flux,
- * 1 update, 2 distribute) and resets them. */
+ * 1 timestep reduction, 2 update, 3 distribute) and resets them. */
 int sw_enable_timing(sw_handle *h, int32_t enable);
 int sw_kernel_times(sw_handle *h, double *ms, int64_t *launches, int32_t nkernels);
 
