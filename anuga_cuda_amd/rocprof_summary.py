@@ -1,0 +1,103 @@
+"""Per-kernel summary of rocprofv3 output (rocpd SQLite databases).
+
+Usage::
+
+    python -m anuga_cuda_amd.rocprof_summary --trace T.db [--fetch F.db --write W.db]
+        [--copy-bytes B] [--triangles N] [--json out.json]
+
+* ``--trace``: a ``rocprofv3 --kernel-trace`` database; gives calls and the
+  average / min / max duration per kernel.
+* ``--fetch`` / ``--write``: databases of ``rocprofv3 --pmc FETCH_SIZE`` and
+  ``--pmc WRITE_SIZE`` runs (KiB per dispatch).  Both counters are scaled by
+  what they read for the largest plain device copy in the same run
+  (``__amd_rocclr_copyBuffer`` of ``--copy-bytes`` bytes read and written):
+  scale = copy bytes / counter bytes.  On gfx950 FETCH_SIZE counts the 128-byte
+  read requests of the memory fabric at a smaller size, so its factor is not 1.
+* ``--json``: writes {kernel short name: {avg_us, fetch_bytes, write_bytes,
+  traffic_bytes}} for bench.py's roofline ``traffic`` field.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sqlite3
+import sys
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def short_name(name):
+    n = name.split("(")[0]
+    if n.startswith("void "):
+        n = n[5:]
+    return n.replace("sw::", "")
+
+
+def kernel_stats(db):
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select name, count(*), avg(duration), min(duration), max(duration) from kernels group by name").fetchall()
+    return {r[0]: (r[1], r[2] / 1e3, r[3] / 1e3, r[4] / 1e3) for r in rows}
+
+
+def counter(db, name):
+    """{kernel: (dispatches, mean value, max value)} of one PMC counter."""
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select name, count(*), avg(counter_value), max(counter_value) from pmc_events "
+        "where counter_name = ? group by name", (name,)).fetchall()
+    return {r[0]: (r[1], r[2], r[3]) for r in rows}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
+    ap.add_argument("--triangles", type=int, default=0, help="triangles per launch, for bytes per triangle")
+    ap.add_argument("--json")
+    a = ap.parse_args(argv)
+
+    ks = kernel_stats(a.trace)
+    out = {}
+    print("%-44s %6s %10s %10s %10s" % ("kernel", "calls", "avg_us", "min_us", "max_us"))
+    for name, (n, avg, mn, mx) in sorted(ks.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        print("%-44s %6d %10.2f %10.2f %10.2f" % (short_name(name)[:44], n, avg, mn, mx))
+        out[short_name(name)] = {"calls": n, "avg_us": avg}
+    if a.fetch and a.write:
+        f = counter(a.fetch, "FETCH_SIZE")
+        w = counter(a.write, "WRITE_SIZE")
+        if COPY_KERNEL not in f or COPY_KERNEL not in w:
+            sys.exit("no %s dispatch in the PMC runs: run the workload with a device copy" % COPY_KERNEL)
+        # the largest copy dispatch is the calibration copy
+        sf = a.copy_bytes / (f[COPY_KERNEL][2] * 1024.0)
+        sw = a.copy_bytes / (w[COPY_KERNEL][2] * 1024.0)
+        print()
+        print("calibration: copy of %.0f MiB reads FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB" % (
+            a.copy_bytes / 2**20, f[COPY_KERNEL][2], w[COPY_KERNEL][2]))
+        print("scale factors: FETCH_SIZE x %.4f, WRITE_SIZE x %.4f" % (sf, sw))
+        print()
+        hdr = "%-44s %10s %10s %10s" % ("kernel", "fetch_MB", "write_MB", "total_MB")
+        if a.triangles:
+            hdr += " %10s" % "B/triangle"
+        print(hdr)
+        for name in sorted(f):
+            if name not in w or name == COPY_KERNEL:
+                continue
+            fb = f[name][1] * 1024.0 * sf
+            wb = w[name][1] * 1024.0 * sw
+            line = "%-44s %10.2f %10.2f %10.2f" % (short_name(name)[:44], fb / 1e6, wb / 1e6, (fb + wb) / 1e6)
+            if a.triangles:
+                line += " %10.1f" % ((fb + wb) / a.triangles)
+            print(line)
+            d = out.setdefault(short_name(name), {})
+            d.update({"fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb})
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump({"fetch_scale": sf if a.fetch else None, "write_scale": sw if a.write else None,
+                       "kernels": out}, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -36,17 +36,29 @@ KERNEL_BYTES_PER_TRIANGLE = {
     # {radius, area} 16, gravity stage-gradient term 16 | writes: explicit
     # updates 24, max_speed 8
     "flux": 96 + 96 + 16 + 16 + 24 + 8,
-    # reads: centroid record 32, explicit 24, semi-implicit 24 | writes: centroid record 32
-    "update": 32 + 24 + 24 + 32,
+    # reads: centroid record 32, explicit 24, semi-implicit 24, Manning n 8,
+    # mean bed 8 | writes: centroid record 32, next step's semi-implicit
+    # momentum updates (friction) 16
+    "update": 32 + 24 + 24 + 8 + 8 + 32 + 16,
     # reads: centroid record 32, centroid xy 16, surrogates 12, flags 4, vertex
-    # xy 48, bed vertex 24, friction 8, mean bed 8 | writes: vertex w/uh/vh 72,
-    # edge state records 96, gravity term 16, semi-implicit updates 16
-    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 8 + 8 + 72 + 96 + 16 + 16,
+    # xy 48, bed vertex 24 | writes: vertex w/uh/vh 72, edge state records 96,
+    # next step's gravity term 16
+    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 96 + 16,
 }
 # One Euler step = the three kernels above (the timestep reduction moves 8
 # bytes per 64 triangles).
 STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
 KERNELS = ("flux", "timestep", "update", "distribute")
+
+
+def workload_tag(args):
+    return "fv_%s_%s_%dk_points" % (args.precision, args.order, args.points // 1000)
+
+
+def kernel_symbol(kernel, precision):
+    r = "double" if precision == "fp64" else "float"
+    return {"flux": "k_flux<%s, true>" % r, "timestep": "k_timestep<true>", "update": "k_update<true>",
+            "distribute": "k_distribute<%s, true>" % r}[kernel]
 
 
 def log(msg):
@@ -64,6 +76,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=64)
+    ap.add_argument("--traffic-json", default=None,
+                    help="rocprof traffic summary (default: profiles/<workload>.traffic.json)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -125,10 +139,15 @@ def main():
     bytes_launch = KERNEL_BYTES_PER_TRIANGLE[dominant] * N
     achieved = bytes_launch / (avg_us[dominant] * 1e-6) / 1e9
     step_gbs = STEP_BYTES_PER_TRIANGLE * N * args.steps / elapsed / 1e9
+    # HBM bytes per launch of the dominant kernel, measured by rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated; written by
+    # anuga_cuda_amd.rocprof_summary), if such a profile is present.
     traffic = None
-    tfile = os.environ.get("SW_TRAFFIC_JSON")
-    if tfile and os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(dominant)
+    tfile = args.traffic_json or os.path.join(ROOT, "profiles", "%s.traffic.json" % workload_tag(args))
+    if os.path.exists(tfile):
+        kname = kernel_symbol(dominant, args.precision)
+        rec = json.load(open(tfile)).get("kernels", {}).get(kname, {})
+        traffic = rec.get("traffic_bytes")
 
     cpu = None
     if cpu_snapshot is not None:
