@@ -18,6 +18,7 @@ SW_STAGE, SW_XMOM, SW_YMOM, SW_ELEVATION, SW_FRICTION, SW_HEIGHT, SW_XVELOCITY, 
 SW_CENTROID, SW_EDGE, SW_VERTEX, SW_BOUNDARY, SW_EXPLICIT_UPDATE, SW_SEMI_IMPLICIT_UPDATE = range(6)
 SW_BC_REFLECTIVE, SW_BC_DIRICHLET, SW_BC_TRANSMISSIVE = range(3)
 SW_PRECISION_FP64, SW_PRECISION_MIXED = 0, 1
+SW_ORDER_KEEP, SW_ORDER_HILBERT = 0, 1
 SW_EVENT_NONE, SW_EVENT_YIELD, SW_EVENT_FINAL = 0, 1, 2
 
 SW_ERR_UPDATE = -1
@@ -33,7 +34,7 @@ class SwParams(C.Structure):
         (n, C.c_int32) for n in (
             "optimise_dry_cells", "extrapolate_velocity_second_order",
             "tight_slope_limiters", "use_centroid_velocities", "use_sloped_mannings",
-            "max_smallsteps", "default_order", "precision")]
+            "max_smallsteps", "default_order", "precision", "locality_order", "reserved")]
 
 
 _I64P = C.POINTER(C.c_int64)

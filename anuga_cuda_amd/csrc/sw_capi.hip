@@ -49,6 +49,8 @@ struct sw_handle {
     double times_ms[kNumTimed] = {};
     int64_t launches[kNumTimed] = {};
     double *staging = nullptr;  // [3N] transfers of packed records
+    int *cal_of = nullptr;      // [N] caller index of device triangle d
+    int *dev_of = nullptr;      // [N] device index of caller triangle k
 };
 
 namespace {
@@ -146,6 +148,8 @@ int check_params(sw_handle *h, const sw_params *p) {
     if (!p) return fail(h, SW_ERR_ARG, "params is NULL");
     if (p->precision != SW_PRECISION_FP64 && p->precision != SW_PRECISION_MIXED)
         return fail(h, SW_ERR_ARG, "unknown precision %d", p->precision);
+    if (p->locality_order != SW_ORDER_KEEP && p->locality_order != SW_ORDER_HILBERT)
+        return fail(h, SW_ERR_ARG, "unknown locality order %d", p->locality_order);
     if (p->default_order != 1 && p->default_order != 2)
         return fail(h, SW_ERR_ARG, "default_order must be 1 or 2");
     if (!(p->H0 > 0) || !(p->evolve_max_timestep > 0))
@@ -188,6 +192,7 @@ struct Loc {
     int slots = 1;
     int comp = -1;
     int64_t count = 0;  // values in the host array
+    bool per_triangle = true;  // false: boundary array (caller's order)
 };
 
 bool locate(sw_handle *h, int q, int loc, Loc *out) {
@@ -211,7 +216,7 @@ bool locate(sw_handle *h, int q, int loc, Loc *out) {
                 case SW_CENTROID: set(F.cq, 1, comp, N); break;
                 case SW_EDGE: set(F.eq, 3, comp, 3 * N); break;
                 case SW_VERTEX: set(vert, 3, -1, 3 * N); break;
-                case SW_BOUNDARY: set(bnd, 1, -1, Nb); break;
+                case SW_BOUNDARY: set(bnd, 1, -1, Nb); r.per_triangle = false; break;
                 case SW_EXPLICIT_UPDATE: if (!eu) return false; set(eu, 1, -1, N); break;
                 case SW_SEMI_IMPLICIT_UPDATE: if (!si) return false; set(si, 1, -1, N); break;
                 default: return false;
@@ -227,7 +232,7 @@ bool locate(sw_handle *h, int q, int loc, Loc *out) {
                 case SW_CENTROID: set(c, 1, -1, N); break;
                 case SW_EDGE: set(e, 3, -1, 3 * N); break;
                 case SW_VERTEX: set(v, 3, -1, 3 * N); break;
-                case SW_BOUNDARY: set(b, 1, -1, Nb); break;
+                case SW_BOUNDARY: set(b, 1, -1, Nb); r.per_triangle = false; break;
                 default: return false;
             }
             break;
@@ -322,6 +327,49 @@ int epilogue(sw_handle *h) {
     return launch_check(h, "evolve epilogue");
 }
 
+// Position of cell (x, y) along the Hilbert curve filling a 2^16 x 2^16 grid
+// (the iterative xy -> d conversion with quadrant rotation).
+uint64_t hilbert_index(uint32_t x, uint32_t y) {
+    const uint32_t n = 1u << 16;
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// cal_of = the triangles sorted by the Hilbert index of their centroids
+// (ties by caller index), so neighbours are close in memory and in the
+// workgroup / XCD sweep.
+void hilbert_order(const double *centroids, int64_t N, std::vector<int> &cal_of) {
+    double x0 = centroids[0], x1 = x0, y0 = centroids[1], y1 = y0;
+    for (int64_t k = 1; k < N; k++) {
+        x0 = std::min(x0, centroids[2 * k]);
+        x1 = std::max(x1, centroids[2 * k]);
+        y0 = std::min(y0, centroids[2 * k + 1]);
+        y1 = std::max(y1, centroids[2 * k + 1]);
+    }
+    const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+    const double scale = 65535.0 / span;
+    std::vector<std::pair<uint64_t, int>> key(N);
+    for (int64_t k = 0; k < N; k++) {
+        const double fx = (centroids[2 * k] - x0) * scale, fy = (centroids[2 * k + 1] - y0) * scale;
+        const uint32_t ix = (uint32_t)std::min(std::max(fx, 0.0), 65535.0);
+        const uint32_t iy = (uint32_t)std::min(std::max(fy, 0.0), 65535.0);
+        key[k] = {hilbert_index(ix, iy), (int)k};
+    }
+    std::sort(key.begin(), key.end());
+    for (int64_t d = 0; d < N; d++) cal_of[d] = key[d].second;
+}
+
 }  // namespace
 
 extern "C" {
@@ -352,13 +400,23 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
         fail(nullptr, SW_ERR_ARG, "boundary arrays are NULL");
         return nullptr;
     }
+    // Device numbering: caller triangle k lives at device index dev_of[k].
+    std::vector<int> dev_of(N), cal_of(N);
+    if (p->locality_order == SW_ORDER_HILBERT)
+        hilbert_order(m->centroid_coordinates, N, cal_of);
+    else
+        for (int64_t d = 0; d < N; d++) cal_of[d] = (int)d;
+    for (int64_t d = 0; d < N; d++) dev_of[cal_of[d]] = (int)d;
     // Host-side validation, packing of the neighbour structure and transposition
-    // of the per-edge / per-vertex geometry to the slot-major [3][N] layout.
+    // of the per-edge / per-vertex geometry to the slot-major [3][N] layout in
+    // device numbering.  Flags (which side evaluates an edge, summation order,
+    // full flag of the evaluating triangle) come from the caller's numbering.
     std::vector<int> nbv(3 * N), flagsv(N), surv(3 * N);
     std::vector<EdgeRec> edgev(3 * N);
     std::vector<double> vcv(6 * N), ccv(2 * N), rav(2 * N);
     bool own_normals = true;
     for (int64_t k = 0; k < N; k++) {
+        const int64_t d = dev_of[k];
         int flags = 0;
         int64_t key[3];
         for (int i = 0; i < 3; i++) {
@@ -373,14 +431,14 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
                          (long long)ki, (long long)n, (long long)b);
                     return nullptr;
                 }
-                nbv[i * N + k] = (int)n;
+                nbv[i * N + d] = (int)n;
             } else {
                 const int64_t e = m->neighbour_edges[ki];
                 if (n >= N || e < 0 || e > 2 || m->neighbours[3 * n + e] != k) {
                     fail(nullptr, SW_ERR_ARG, "neighbour structure is not symmetric at edge %lld", (long long)ki);
                     return nullptr;
                 }
-                nbv[i * N + k] = (int)(e * N + n);
+                nbv[i * N + d] = (int)(e * N + dev_of[n]);
                 if (n < k) {
                     flags |= 1 << i;
                     computing = n;
@@ -397,14 +455,14 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
                 fail(nullptr, SW_ERR_ARG, "surrogate_neighbours[%lld] out of range", (long long)ki);
                 return nullptr;
             }
-            surv[i * N + k] = (int)s;
-            EdgeRec &er = edgev[i * N + k];
+            surv[i * N + d] = dev_of[s];
+            EdgeRec &er = edgev[i * N + d];
             er.nx = m->normals[2 * ki];
             er.ny = m->normals[2 * ki + 1];
             er.len = m->edgelengths[ki];
-            er.nb = nbv[i * N + k];
-            vcv[2 * (i * N + k)] = m->vertex_coordinates[6 * k + 2 * i];
-            vcv[2 * (i * N + k) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
+            er.nb = nbv[i * N + d];
+            vcv[2 * (i * N + d)] = m->vertex_coordinates[6 * k + 2 * i];
+            vcv[2 * (i * N + d) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
         }
         // summation order of the C edge-once loop: lower neighbours ascending, then own edges
         int ord[3] = {0, 1, 2};
@@ -422,17 +480,17 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
             // the reconstruction of a triangle with two boundary edges uses its
             // first surrogate neighbour that is not itself: keep it in slot 0
             for (int i = 0; i < 3; i++)
-                if (surv[i * N + k] != k) {
-                    std::swap(surv[k], surv[i * N + k]);
+                if (surv[i * N + d] != d) {
+                    std::swap(surv[d], surv[i * N + d]);
                     break;
                 }
         }
-        flagsv[k] = flags;
-        for (int i = 0; i < 3; i++) edgev[i * N + k].tflags = flags;
-        rav[2 * k] = m->radii[k];
-        rav[2 * k + 1] = m->areas[k];
-        ccv[2 * k] = m->centroid_coordinates[2 * k];
-        ccv[2 * k + 1] = m->centroid_coordinates[2 * k + 1];
+        flagsv[d] = flags;
+        for (int i = 0; i < 3; i++) edgev[i * N + d].tflags = flags;
+        rav[2 * d] = m->radii[k];
+        rav[2 * d + 1] = m->areas[k];
+        ccv[2 * d] = m->centroid_coordinates[2 * k];
+        ccv[2 * d + 1] = m->centroid_coordinates[2 * k + 1];
     }
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
@@ -468,11 +526,19 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
         return 0;
     };
     if (put(&dedge, edgev) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dvc, vcv) || put(&dcc, ccv) ||
-        put(&dra, rav))
+        put(&dra, rav) || put(&h->cal_of, cal_of) || put(&h->dev_of, dev_of))
         return bail();
     if (Nb > 0) {
-        if (upload_i32(h, &bcell, m->boundary_cells, Nb, 0, N - 1, "boundary_cells") ||
-            upload_i32(h, &bedge, m->boundary_edges, Nb, 0, 2, "boundary_edges"))
+        std::vector<int> bcv(Nb);
+        for (int64_t b = 0; b < Nb; b++) {
+            const int64_t c = m->boundary_cells[b];
+            if (c < 0 || c >= N) {
+                fail(h, SW_ERR_ARG, "boundary_cells[%lld] out of range", (long long)b);
+                return bail();
+            }
+            bcv[b] = dev_of[c];
+        }
+        if (put(&bcell, bcv) || upload_i32(h, &bedge, m->boundary_edges, Nb, 0, 2, "boundary_edges"))
             return bail();
     } else {
         if (dalloc(h, &bcell, 1) || dalloc(h, &bedge, 1)) return bail();
@@ -570,17 +636,17 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
         return fail(h, SW_ERR_ARG, "upload of %lld values into an array of %lld", (long long)count, (long long)L.count);
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     if (L.count == 0) return SW_OK;
-    if (L.slots == 1 && L.comp < 0) {
+    if (!L.per_triangle) {
         HIPCHECK(h, hipMemcpy(L.ptr, host, L.count * sizeof(double), hipMemcpyHostToDevice));
     } else {
         const long long n = L.count / L.slots;
         HIPCHECK(h, hipMemcpy(h->staging, host, L.count * sizeof(double), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, n, L.slots, L.comp,
-                           h->staging, L.ptr);
+                           h->cal_of, h->staging, L.ptr);
         if (int rc = launch_check(h, "k_to_device")) return rc;
         if (quantity == SW_ELEVATION && location == SW_EDGE) {
             hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, n, 3, -1,
-                               h->staging, h->f.ze);
+                               h->cal_of, h->staging, h->f.ze);
             if (int rc = launch_check(h, "k_to_device")) return rc;
         }
     }
@@ -603,11 +669,11 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
         return fail(h, SW_ERR_ARG, "download of %lld values from an array of %lld", (long long)count, (long long)L.count);
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     if (L.count == 0) return SW_OK;
-    if (L.slots == 1 && L.comp < 0) {
+    if (!L.per_triangle) {
         HIPCHECK(h, hipMemcpy(host, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)(L.count / L.slots),
-                           L.slots, L.comp, L.ptr, h->staging);
+                           L.slots, L.comp, h->dev_of, L.ptr, h->staging);
         if (int rc = launch_check(h, "k_to_host")) return rc;
         HIPCHECK(h, hipStreamSynchronize(h->stream));
         HIPCHECK(h, hipMemcpy(host, h->staging, L.count * sizeof(double), hipMemcpyDeviceToHost));
@@ -617,8 +683,11 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
 
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count) {
     if (!h || !host || count != h->N) return SW_ERR_ARG;
+    hipLaunchKernelGGL(k_to_host, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, (long long)h->N, 1, -1,
+                       h->dev_of, h->f.max_speed, h->staging);
+    if (int rc = launch_check(h, "k_to_host")) return rc;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy(host, h->f.max_speed, h->N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(host, h->staging, h->N * sizeof(double), hipMemcpyDeviceToHost));
     return SW_OK;
 }
 

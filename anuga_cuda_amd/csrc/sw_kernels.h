@@ -923,29 +923,40 @@ __global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double 
 // device keeps [N] records or slot-major [3][N] arrays.  `slots` is 1 for
 // per-triangle arrays and 3 for per-edge / per-vertex arrays; `comp` selects a
 // component of {w, uh, vh, z} records (or -1 for a plain double array).
-__global__ __launch_bounds__(kBlock) void k_to_device(long long N, int slots, int comp, const double *src,
-                                                      double *dst) {
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= N * slots) return;
-    const long long k = t / slots, i = t - k * slots;  // src is [N][slots]
-    const long long d = i * N + k;
+// Host-order staging [n][slots] <-> device array.  A per-triangle array
+// (perm != nullptr) is slot-major [slots][n] in device numbering: cal_of[d] is
+// the caller's index of device triangle d, dev_of its inverse.  Boundary
+// arrays (perm == nullptr) keep the caller's order.  comp >= 0 addresses one
+// component of the 32-byte {w, uh, vh, z} records.
+__global__ __launch_bounds__(kBlock) void k_to_device(long long n, int slots, int comp, const int *cal_of,
+                                                      const double *src, double *dst) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;  // device position
+    if (t >= n * slots) return;
+    long long s;
+    if (cal_of) {
+        const long long i = t / n, d = t - i * n;
+        s = (long long)cal_of[d] * slots + i;
+    } else {
+        s = t;
+    }
     if (comp < 0)
-        dst[d] = src[t];
+        dst[t] = src[s];
     else
-        dst[4 * d + comp] = src[t];
+        dst[4 * t + comp] = src[s];
 }
-
-__global__ __launch_bounds__(kBlock) void k_to_host(long long N, int slots, int comp, const double *src,
-                                                    double *dst) {
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= N * slots) return;
-    const long long k = t / slots, i = t - k * slots;
-    const long long d = i * N + k;
+__global__ __launch_bounds__(kBlock) void k_to_host(long long n, int slots, int comp, const int *dev_of,
+                                                    const double *src, double *dst) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;  // host position
+    if (t >= n * slots) return;
+    long long d;
+    if (dev_of) {
+        const long long k = t / slots, i = t - k * slots;
+        d = i * n + dev_of[k];
+    } else {
+        d = t;
+    }
     dst[t] = (comp < 0) ? src[d] : src[4 * d + comp];
 }
-
-// mean bed per triangle (as the friction term forms it) and the check that the
-// bed edge values are the vertex interpolation (enables recomputing them).
 __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, const double *zv, const double *ze, double *zbar,
                                                         DevState *st) {
     const int k = blockIdx.x * kBlock + threadIdx.x;

@@ -241,10 +241,14 @@ class Domain:
     ``Domain(mesh=Mesh(...))``.  Keyword parameters override
     ``DEFAULT_PARAMETERS``; ``precision='mixed'`` selects FP32 flux and
     reconstruction with FP64 accumulation, timestep and wet/dry tests.
+    ``locality_order='hilbert'`` (default) lets the library renumber the
+    triangles on the device along a Hilbert curve of their centroids;
+    ``'keep'`` uses the caller's numbering.  Results and the arrays seen here
+    do not depend on it.
     """
 
     def __init__(self, coordinates=None, vertices=None, boundary=None, mesh=None,
-                 precision="fp64", device=0, **parameters):
+                 precision="fp64", device=0, locality_order="hilbert", **parameters):
         if mesh is None:
             if coordinates is None or vertices is None:
                 raise ValueError("need coordinates and vertices, or mesh=")
@@ -261,6 +265,9 @@ class Domain:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
+        if locality_order not in ("hilbert", "keep"):
+            raise ValueError("locality_order must be 'hilbert' or 'keep'")
+        self.locality_order = locality_order
         self.device = device
         self.quantities = {name: Quantity(self, name) for name in _QID}
         self.conserved_quantities = list(CONSERVED)
@@ -413,6 +420,7 @@ class Domain:
         for k, v in self._params.items():
             setattr(p, k, v)
         p.precision = _lib.SW_PRECISION_MIXED if self.precision == "mixed" else _lib.SW_PRECISION_FP64
+        p.locality_order = _lib.SW_ORDER_HILBERT if self.locality_order == "hilbert" else _lib.SW_ORDER_KEEP
         return p
 
     def _ensure_device(self):

@@ -52,7 +52,8 @@ KERNELS = ("flux", "timestep", "update", "distribute")
 
 
 def workload_tag(args):
-    return "fv_%s_%s_%dk_points" % (args.precision, args.order, args.points // 1000)
+    tag = "fv_%s_%s_%dk_points" % (args.precision, args.order, args.points // 1000)
+    return tag if args.locality == "hilbert" else tag + "_keep_order"
 
 
 def kernel_symbol(kernel, precision):
@@ -73,6 +74,8 @@ def main():
     ap.add_argument("--points", type=int, default=500000)
     ap.add_argument("--order", default="morton", choices=["morton", "random", "none"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--locality", default="hilbert", choices=["hilbert", "keep"],
+                    help="device numbering of the triangles (library renumbering or the input order)")
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=64)
@@ -101,7 +104,8 @@ def main():
         del a, b
 
     t0 = time.time()
-    d = scenarios.delaunay_terrain(npoints=args.points, order=args.order, precision=args.precision)
+    d = scenarios.delaunay_terrain(npoints=args.points, order=args.order, precision=args.precision,
+                                   locality_order=args.locality)
     N = d.number_of_elements
     log("mesh: %d triangles, %d boundary edges, built in %.1f s" % (N, d.number_of_boundary_elements,
                                                                   time.time() - t0))
@@ -187,7 +191,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain FV, seeded Delaunay %d points in 10 km x 10 km, %s order, fBm terrain "
                                "20 m, wet/dry mound, Manning 0.025, transmissive" % (args.points, args.order),
-                   "triangles": N, "precision": args.precision, "order": 2, "timestepping": "euler"},
+                   "triangles": N, "precision": args.precision, "order": 2, "timestepping": "euler",
+                   "device_numbering": args.locality},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_triangle": KERNEL_BYTES_PER_TRIANGLE[dominant],

@@ -50,6 +50,10 @@ enum sw_location {
 };
 enum sw_boundary_kind { SW_BC_REFLECTIVE = 0, SW_BC_DIRICHLET = 1, SW_BC_TRANSMISSIVE = 2 };
 enum sw_precision { SW_PRECISION_FP64 = 0, SW_PRECISION_MIXED = 1 };
+/* Device numbering of the triangles (read by sw_create only).  The arithmetic
+ * never depends on it: which side evaluates a shared edge and the order in
+ * which edge fluxes are summed follow the caller's numbering, as in the C. */
+enum sw_locality_order { SW_ORDER_KEEP = 0, SW_ORDER_HILBERT = 1 };
 enum sw_event { SW_EVENT_NONE = 0, SW_EVENT_YIELD = 1, SW_EVENT_FINAL = 2 };
 
 /* Domain parameters; names follow struct domain (src/anuga_HMPP/sw_domain.h:16-60)
@@ -61,6 +65,8 @@ typedef struct sw_params {
     int32_t optimise_dry_cells, extrapolate_velocity_second_order;
     int32_t tight_slope_limiters, use_centroid_velocities, use_sloped_mannings;
     int32_t max_smallsteps, default_order, precision;
+    int32_t locality_order;  /* SW_ORDER_*; arrays at the ABI stay in the caller's order */
+    int32_t reserved;
 } sw_params;
 
 /* Mesh arrays (host, borrowed during sw_create). */

@@ -4,6 +4,7 @@ builders."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -30,11 +31,26 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == set(_lib.exported_symbols())
 
 
-def test_struct_layouts_match_header():
-    # sizes of the three ABI structs as laid out by the C compiler
-    assert ctypes.sizeof(_lib.SwParams) == 15 * 8 + 8 * 4
-    assert ctypes.sizeof(_lib.SwMesh) == 2 * 8 + 13 * 8
-    assert ctypes.sizeof(_lib.SwState) == 9 * 8 + 7 * 8
+def test_struct_layouts_match_header(tmp_path):
+    """Field offsets and sizes of the ctypes structs equal those the C compiler
+    gives the structs of include/anuga_sw.h."""
+    structs = {"sw_params": _lib.SwParams, "sw_mesh": _lib.SwMesh, "sw_state": _lib.SwState}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "anuga_sw.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for cname, cls in structs.items():
+        assert int(got["%s size" % cname]) == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got["%s.%s" % (cname, fname)]) == getattr(cls, fname).offset, (cname, fname)
 
 
 def test_rectangular_cross_geometry():

@@ -259,3 +259,28 @@ def test_update_error_path_forced_from_host():
     with pytest.raises(_lib.SwError) as exc:
         h.call("sw_update_conserved_quantities")
     assert exc.value.code == _lib.SW_ERR_UPDATE
+
+
+def test_device_renumbering_is_a_relabelling():
+    """The library's Hilbert renumbering of the triangles changes only where they
+    live on the device: every array seen through the ABI, and the timestep
+    sequence, are bitwise those of a handle that keeps the caller's numbering."""
+    runs = {}
+    for order in ("keep", "hilbert"):
+        d = scenarios.delaunay_terrain(npoints=3000, order="random", extent=2000.0, mound_radius=200.0,
+                                       seed=7, terrain_seed=11, locality_order=order)
+        dts = []
+        for _ in d.evolve(yieldstep=0.5, finaltime=2.0):
+            dts.append(d.timestep)
+        runs[order] = (dts, {q: (d.quantities[q].centroid_values.copy(), d.quantities[q].edge_values.copy(),
+                                 d.quantities[q].vertex_values.copy())
+                             for q in ("stage", "xmomentum", "ymomentum", "height", "xvelocity")},
+                       d.max_speed.copy(), d.number_of_steps)
+        d.close()
+    a, b = runs["keep"], runs["hilbert"]
+    assert a[3] == b[3] and a[3] > 10
+    assert a[0] == b[0]
+    for q in a[1]:
+        for x, y in zip(a[1][q], b[1][q]):
+            assert np.array_equal(x, y), q
+    assert np.array_equal(a[2], b[2])
