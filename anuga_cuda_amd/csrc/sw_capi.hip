@@ -259,7 +259,7 @@ int enqueue_fused_step(sw_handle *h, int slot) {
     };
     rec(0);
     if (mixed)
-        hipLaunchKernelGGL((k_flux<float, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+        hipLaunchKernelGGL(k_flux_mixed, dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     else
         hipLaunchKernelGGL((k_flux<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     rec(1);
@@ -413,6 +413,7 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     // full flag of the evaluating triangle) come from the caller's numbering.
     std::vector<int> nbv(3 * N), flagsv(N), surv(3 * N);
     std::vector<EdgeRec> edgev(3 * N);
+    std::vector<EdgeRecF> edgefv(3 * N);
     std::vector<double> vcv(6 * N), ccv(2 * N), rav(2 * N);
     bool own_normals = true;
     for (int64_t k = 0; k < N; k++) {
@@ -461,6 +462,7 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
             er.ny = m->normals[2 * ki + 1];
             er.len = m->edgelengths[ki];
             er.nb = nbv[i * N + d];
+            edgefv[i * N + d] = EdgeRecF{(float)er.nx, (float)er.ny, (float)er.len, er.nb};
             vcv[2 * (i * N + d)] = m->vertex_coordinates[6 * k + 2 * i];
             vcv[2 * (i * N + d) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
         }
@@ -525,7 +527,8 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
             return fail(h, SW_ERR_HIP, "mesh upload failed");
         return 0;
     };
-    if (put(&dedge, edgev) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dvc, vcv) || put(&dcc, ccv) ||
+    EdgeRecF *dedgef;
+    if (put(&dedge, edgev) || put(&dedgef, edgefv) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dvc, vcv) || put(&dcc, ccv) ||
         put(&dra, rav) || put(&h->cal_of, cal_of) || put(&h->dev_of, dev_of))
         return bail();
     if (Nb > 0) {
@@ -545,6 +548,7 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     }
     if (dalloc(h, &bkind, Nb) || dalloc(h, &bdir, 3 * Nb)) return bail();
     M.edge = dedge;
+    M.edgef = dedgef;
     M.flags = dflags;
     M.sur = dsur;
     M.vc = reinterpret_cast<const double2 *>(dvc);
@@ -567,7 +571,7 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     for (auto pp : perNb)
         if (dalloc(h, pp, Nb)) return bail();
     if (dalloc(h, &F.cq, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &F.grav, N) ||
-        dalloc(h, &F.wave_min, (size_t)grid_for(N) * (kBlock / 64)) ||
+        dalloc(h, &F.wave_min, (size_t)grid_for(N) * (kBlock / 64)) || dalloc(h, &F.eqf, 3 * N) ||
         dalloc(h, &h->staging, 3 * N))
         return bail();
     if (dalloc(h, &h->st, 1)) return bail();
@@ -859,7 +863,13 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         return fail(h, SW_ERR_STATE, "evolve_steps called while a yield is pending (call sw_advance_yield)");
     int64_t launched = 0;
     int rc = SW_OK;
+    const bool mixed = h->params.precision == SW_PRECISION_MIXED;
     if (max_steps > 0) {
+        if (mixed) {
+            const unsigned g3 = grid_for(3 * h->N);
+            hipLaunchKernelGGL(k_check_bed, dim3(g3), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->st);
+            hipLaunchKernelGGL(k_pack_edges, dim3(g3), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
+        }
         hipLaunchKernelGGL(k_source_terms, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
         rc = launch_check(h, "k_source_terms");
     }
@@ -885,6 +895,10 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         if (rc) break;
         rc = sync_state(h, &d);
         if (rc || d.event != SW_EVENT_NONE || d.halt) break;
+    }
+    if (mixed && launched > 0) {
+        // the mixed loop keeps FP32 edge state: rebuild the FP64 edge records
+        hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     }
     int rc2 = epilogue(h);
     int rc3 = sync_state(h, &d);

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define SW_DEV __device__ __forceinline__
 
 namespace sw {
@@ -129,6 +131,86 @@ SW_DEV void flux_central(const R ql_in[3], const R qr_in[3], R z_left, R z_right
     }
 }
 
+// Mixed-precision flux.  The same KNP flux as flux_central with the velocity,
+// wave-speed and flux arithmetic in FP32; the depth tests stay FP64: each side
+// comes with its depth class, decided on the FP64 depth h = w - z,
+//   0: |h| < eps (dry: the edge is skipped when both sides are dry, and
+//      _compute_speed zeroes h and u), 1: h <= -eps (zeroed), 2: eps <= h <
+//   limiting_threshold (damped velocity), 3: deeper.
+SW_DEV int depth_class(double h, double eps, double limiting_threshold) {
+    if (fabs(h) < eps) return 0;
+    if (h < eps) return 1;
+    if (h < limiting_threshold) return 2;
+    return 3;
+}
+
+// _compute_speed (shallow_water_ext.c:214-240) with the branch given by the class.
+SW_DEV float speed_f(float uh, float h, int cls, float h0, float &uh_out, float &h_out) {
+    float u;
+    if (cls <= 1) {
+        h_out = 0.0f;
+        u = 0.0f;
+        uh_out = 0.0f;
+    } else if (cls == 2) {
+        h_out = h;
+        u = uh / (h + h0 / h);
+        uh_out = u * h;
+    } else {
+        h_out = h;
+        u = uh / h;
+        uh_out = uh;
+    }
+    return u;
+}
+
+// hl, hr: FP32 depths; dw: the stage jump w_r - w_l; (uhl, vhl), (uhr, vhr):
+// momenta in the x/y frame.  Result in the x/y frame, FP64.
+SW_DEV void flux_mixed_core(float hl, float hr, int cl, int cr, float dw, float uhl, float vhl, float uhr,
+                            float vhr, float n1, float n2, float g, float h0, double epsilon, double ef[3],
+                            double &max_speed) {
+    const float ql1 = n1 * uhl + n2 * vhl, ql2 = -n2 * uhl + n1 * vhl;
+    const float qr1 = n1 * uhr + n2 * vhr, qr2 = -n2 * uhr + n1 * vhr;
+    float h_left, h_right, uh_left, uh_right, vh_left, vh_right, tmp;
+    const float u_left = speed_f(ql1, hl, cl, h0, uh_left, h_left);
+    const float u_right = speed_f(qr1, hr, cr, h0, uh_right, h_right);
+    speed_f(ql2, hl, cl, h0, vh_left, tmp);
+    speed_f(qr2, hr, cr, h0, vh_right, tmp);
+    const float c_left = sqrtf(g * h_left), c_right = sqrtf(g * h_right);
+    float s_max = omax<float>(u_left + c_left, u_right + c_right);
+    if (s_max < 0.0f) s_max = 0.0f;
+    float s_min = omin<float>(u_left - c_left, u_right - c_right);
+    if (s_min > 0.0f) s_min = 0.0f;
+    const float fl0 = u_left * h_left, fl1 = u_left * uh_left + 0.5f * g * h_left * h_left,
+                fl2 = u_left * vh_left;
+    const float fr0 = u_right * h_right, fr1 = u_right * uh_right + 0.5f * g * h_right * h_right,
+                fr2 = u_right * vh_right;
+    const float denom = s_max - s_min;
+    if (double(denom) < epsilon) {
+        ef[0] = ef[1] = ef[2] = 0.0;
+        max_speed = 0.0;
+        return;
+    }
+    const float inv = 1.0f / denom, smm = s_max * s_min;
+    const float e0 = (s_max * fl0 - s_min * fr0 + smm * dw) * inv;
+    const float e1 = (s_max * fl1 - s_min * fr1 + smm * (qr1 - ql1)) * inv;
+    const float e2 = (s_max * fl2 - s_min * fr2 + smm * (qr2 - ql2)) * inv;
+    max_speed = double(omax<float>(fabsf(s_max), fabsf(s_min)));
+    ef[0] = double(e0);
+    ef[1] = double(n1 * e1 - n2 * e2);
+    ef[2] = double(n2 * e1 + n1 * e2);
+}
+
+// Mixed flux from FP64 edge states (w, uh, vh, z of each side).
+SW_DEV void flux_central_mixed(const double ql[3], const double qr[3], double z_left, double z_right, double n1,
+                               double n2, double epsilon, double h0, double limiting_threshold, double g,
+                               double ef[3], double &max_speed) {
+    const double z = 0.5 * (z_left + z_right);
+    const double hl = ql[0] - z, hr = qr[0] - z;
+    flux_mixed_core(float(hl), float(hr), depth_class(hl, epsilon, limiting_threshold),
+                    depth_class(hr, epsilon, limiting_threshold), float(qr[0] - ql[0]), float(ql[1]), float(ql[2]),
+                    float(qr[1]), float(qr[2]), float(n1), float(n2), float(g), float(h0), epsilon, ef, max_speed);
+}
+
 // util_ext.h:66-109 (_gradient)
 SW_DEV void gradient3(double x0, double y0, double x1, double y1, double x2, double y2,
                       double q0, double q1, double q2, double &a, double &b) {
@@ -188,12 +270,12 @@ SW_DEV void limit_gradient(R dqv[3], R qmin, R qmax, R beta) {
 
 // Auxiliary-triangle reconstruction of one quantity (shallow_water_ext.c:3023-3062).
 // Returns the three limited jumps from the centroid.
+// The differences dq0 = q0 - qk, dq1 = q1 - q0, dq2 = q2 - q0 are formed by the
+// caller in FP64 (exactly the C's operations), then rounded to R: in the mixed
+// path this keeps the cancellation of nearly equal stages out of FP32.
 template <typename R>
-SW_DEV void extrap_aux(R qk, R q0, R q1, R q2, R dx1, R dx2, R dy1, R dy2, R inv_area2,
+SW_DEV void extrap_aux(R dq0, R dq1, R dq2, R dx1, R dx2, R dy1, R dy2, R inv_area2,
                        const R dxv[3], const R dyv[3], R beta, R dqv[3]) {
-    R dq0 = q0 - qk;
-    R dq1 = q1 - q0;
-    R dq2 = q2 - q0;
     R a = dy2 * dq1 - dy1 * dq2;
     a *= inv_area2;
     R b = dx1 * dq2 - dx2 * dq1;
@@ -206,10 +288,9 @@ SW_DEV void extrap_aux(R qk, R q0, R q1, R q2, R dx1, R dx2, R dy1, R dy2, R inv
 }
 
 // Reconstruction toward the single interior neighbour (shallow_water_ext.c:3174-3213).
+// dq1 = q1 - qk formed by the caller in FP64.
 template <typename R>
-SW_DEV void extrap_one(R qk, R q1, R dx2, R dy2, const R dxv[3], const R dyv[3], R beta,
-                       R dqv[3]) {
-    R dq1 = q1 - qk;
+SW_DEV void extrap_one(R dq1, R dx2, R dy2, const R dxv[3], const R dyv[3], R beta, R dqv[3]) {
     R a = dq1 * dx2;
     R b = dq1 * dy2;
     dqv[0] = a * dxv[0] + b * dyv[0];

@@ -60,10 +60,25 @@ struct alignas(32) EdgeRec {
     int tflags;          // the triangle's flags word (same in all three slots)
 };
 
+// FP32 edge geometry of the mixed-precision flux: normal, length, neighbour slot.
+struct alignas(16) EdgeRecF {
+    float nx, ny, len;
+    int nb;
+};
+
+// Edge state of the mixed-precision loop, written by k_distribute and read by
+// k_flux_mixed: FP32 depth (w - z formed in FP64) and momenta, and the depth
+// class (depth_class) that carries the FP64 depth tests.
+struct alignas(16) EdgeF {
+    float h, uh, vh;
+    int cls;
+};
+
 struct DevMesh {
     int N, Nb;
     int own_normals;  // shared-edge normals are exact negations, lengths equal
     const EdgeRec *edge;  // [3][N]
+    const EdgeRecF *edgef;  // [3][N] the same in FP32 (mixed precision)
     const int *flags;     // [N]
     const int *sur;       // [3][N]
     const double2 *vc;    // [3][N] vertex coordinates
@@ -87,6 +102,7 @@ struct DevFields {
     double *friction, *zbar, *max_speed;  // [N]
     double2 *grav;                        // [N] -g*(wx, wy)*avg_h of the next step
     unsigned long long *wave_min;         // [kBlock/64 * blocks] CFL minimum of each wavefront (bits)
+    EdgeF *eqf;                           // [3][N] mixed-precision edge state
 };
 
 // Scalar evolve state kept in device memory.  The first block of fields has
@@ -114,6 +130,11 @@ SW_DEV void set_error(DevState *st, long long code) {
 }
 
 SW_DEV Q4 ldq(const Q4 *p) { return *p; }
+SW_DEV EdgeRecF ld_edgef(const EdgeRecF *p) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    return EdgeRecF{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), (int)a.w};
+}
 SW_DEV EdgeRec ld_edge(const EdgeRec *p) {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     const u32x4 *q = reinterpret_cast<const u32x4 *>(p);
@@ -277,14 +298,14 @@ __global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevPara
             }
         if (P.optimise_dry_cells && fabs(ql[0] - zl) < eps && fabs(qr[0] - zr) < eps) continue;
             if (fabs(zl - zr) > 1.0e-10) set_error(st, kErrDiscontinuous);
-            R qlr[3] = {R(ql[0]), R(ql[1]), R(ql[2])};
-            R qrr[3] = {R(qr[0]), R(qr[1]), R(qr[2])};
-            R efr[3], msr;
-            flux_central<R>(qlr, qrr, R(zl), R(zr), R(n1), R(n2), R(eps), R(P.h0),
-                            R(P.limiting_threshold), R(P.g), efr, msr);
-            const double ms = double(msr);
+            double efd[3], ms;
+            if constexpr (std::is_same<R, double>::value) {
+                flux_central<double>(ql, qr, zl, zr, n1, n2, eps, P.h0, P.limiting_threshold, P.g, efd, ms);
+            } else {
+                flux_central_mixed(ql, qr, zl, zr, n1, n2, eps, P.h0, P.limiting_threshold, P.g, efd, ms);
+            }
             // the lower-numbered triangle of the edge adds the flux, the other subtracts
-            double f0 = double(efr[0]) * len, f1 = double(efr[1]) * len, f2 = double(efr[2]) * len;
+            double f0 = efd[0] * len, f1 = efd[1] * len, f2 = efd[2] * len;
             if (!lower) {
                 f0 = -f0;
                 f1 = -f1;
@@ -370,6 +391,156 @@ __global__ __launch_bounds__(kReduceBlock) void k_timestep(const unsigned long l
         }
     }
     *st = s;
+}
+
+// Fused flux of the mixed-precision loop: the k_flux<.., true> step with FP32
+// edge state and geometry (16-byte records) and FP32 flux arithmetic; the
+// depth tests, the explicit-update sums, gravity and the CFL candidates stay
+// FP64.  The bed must be continuous across edges (k_check_bed), so a side's
+// depth is its own edge depth.
+__global__ __launch_bounds__(kBlock) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st) {
+    if (st->event || st->error) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->halt = 1;
+        return;
+    }
+    const int N = M.N;
+    const int k = tri_index();
+    double dtk = P.evolve_max_timestep;
+    if (k < N) {
+        EdgeRecF er[3];
+        EdgeF own[3], nbe[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) er[i] = ld_edgef(M.edgef + i * N + k);
+#pragma unroll
+        for (int i = 0; i < 3; i++) own[i] = F.eqf[i * N + k];
+        const int flags = ld_s(M.flags + k);
+        const double2 ra = ld_s(M.ra + k);
+        const double2 gr = ld_s(F.grav + k);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int e = er[i].nb;
+            nbe[i] = F.eqf[e >= 0 ? e : i * N + k];  // boundary edge: its own record
+        }
+        const double eps = P.epsilon, g = P.g;
+        const float gf = float(P.g), h0f = float(P.h0);
+        const int last_edge = (flags >> 10) & 3;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0;
+        bool have_last = false;
+        double sidex = 0.0, sidey = 0.0, cfl_speed = 0.0, max_speed_total = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int e = er[i].nb;
+            const bool lower = (flags >> i) & 1;
+            {
+                const double hh = double(own[i].h);
+                const double fact = -0.5 * g * hh * hh * double(er[i].len);
+                sidex = sidex + fact * double(er[i].nx);
+                sidey = sidey + fact * double(er[i].ny);
+            }
+            float n1 = er[i].nx, n2 = er[i].ny, len = er[i].len;
+            EdgeF L, R;
+            if (e >= 0) {
+                if (lower) {
+                    L = nbe[i];
+                    R = own[i];
+                    if (M.own_normals) {
+                        n1 = -n1;
+                        n2 = -n2;
+                    } else {
+                        const EdgeRecF en = M.edgef[e];
+                        n1 = en.nx;
+                        n2 = en.ny;
+                        len = en.len;
+                    }
+                } else {
+                    L = own[i];
+                    R = nbe[i];
+                }
+            } else {
+                const int m = -e - 1;
+                L = nbe[i];
+                R = L;  // transmissive: inside edge state
+                const int kind = M.bkind[m];
+                if (kind == 0) {  // reflective (evaluate_segment.cu:36-63)
+                    double b1, b2;
+                    reflect_momentum(double(L.uh), double(L.vh), double(n1), double(n2), b1, b2);
+                    R.uh = float(b1);
+                    R.vh = float(b2);
+                } else if (kind == 1) {  // Dirichlet constants
+                    const double hb = M.bdir[3 * m] - F.ze[i * N + k];
+                    R.h = float(hb);
+                    R.cls = depth_class(hb, eps, P.limiting_threshold);
+                    R.uh = float(M.bdir[3 * m + 1]);
+                    R.vh = float(M.bdir[3 * m + 2]);
+                }
+            }
+            if (P.optimise_dry_cells && L.cls == 0 && R.cls == 0) continue;
+            double efd[3], ms;
+            flux_mixed_core(L.h, R.h, L.cls, R.cls, R.h - L.h, L.uh, L.vh, R.uh, R.vh, n1, n2, gf, h0f, eps, efd,
+                            ms);
+            const double lend = double(len);
+            double f0 = efd[0] * lend, f1 = efd[1] * lend, f2 = efd[2] * lend;
+            if (!lower) {
+                f0 = -f0;
+                f1 = -f1;
+                f2 = -f2;
+            }
+            if (i == last_edge) {
+                l0 = f0;
+                l1 = f1;
+                l2 = f2;
+                have_last = true;
+            } else {
+                s0 = s0 + f0;
+                s1 = s1 + f1;
+                s2 = s2 + f2;
+            }
+            if (((flags >> (3 + i)) & 1) && ms > eps) cfl_speed = omax<double>(cfl_speed, ms);
+            if (!lower) max_speed_total = omax<double>(max_speed_total, ms);
+        }
+        if (have_last) {
+            s0 = s0 + l0;
+            s1 = s1 + l1;
+            s2 = s2 + l2;
+        }
+        if (cfl_speed > 0.0) dtk = omin<double>(dtk, ra.x / cfl_speed);
+        const double area = ra.y;
+        const double inv_area = 1.0 / area;
+        s0 *= inv_area;
+        s1 *= inv_area;
+        s2 *= inv_area;
+        s1 += gr.x;
+        s2 += gr.y;
+        s1 += -sidex / area;
+        s2 += -sidey / area;
+        st_s(F.weu + k, s0);
+        st_s(F.uheu + k, s1);
+        st_s(F.vheu + k, s2);
+        st_s(F.max_speed + k, max_speed_total);
+    }
+    const unsigned long long wm = wave_min_bits(dtk);
+    if ((threadIdx.x & 63) == 0) F.wave_min[k >> 6] = wm;
+}
+
+// Mixed-precision edge state from the FP64 edge records (start of a mixed
+// evolve call).
+__global__ __launch_bounds__(kBlock) void k_pack_edges(DevMesh M, DevFields F, DevParams P) {
+    const int N = M.N;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 3 * N) return;
+    const Q4 e = F.eq[t];
+    const double h = e.w - e.z;
+    F.eqf[t] = EdgeF{float(h), float(e.uh), float(e.vh), depth_class(h, P.epsilon, P.limiting_threshold)};
+}
+
+// Bed continuity across shared edges (the mixed flux uses each side's own
+// edge depth): flags the discontinuous-elevation error as k_flux does.
+__global__ __launch_bounds__(kBlock) void k_check_bed(DevMesh M, DevFields F, DevState *st) {
+    const int N = M.N;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 3 * N) return;
+    const int e = M.edge[t].nb;
+    if (e >= 0 && fabs(F.ze[t] - F.ze[e]) > 1.0e-10) set_error(st, kErrDiscontinuous);
 }
 
 // gravity_wb as a separate operator (shallow_water_ext.c:1942-2037)
@@ -620,19 +791,19 @@ SW_DEV bool extrapolate_one(const DevFields &F, const DevParams &P, int N, int k
                     const R inv_area2 = R(1.0) / R(area2);
                     R d[3];
                     R beta = R(P.beta_w_dry + (P.beta_w - P.beta_w_dry) * hfactor);
-                    extrap_aux<R>(R(wc), R(w0), R(w1), R(w2), R(dx1), R(dx2), R(dy1), R(dy2),
+                    extrap_aux<R>(R(w0 - wc), R(w1 - w0), R(w2 - w0), R(dx1), R(dx2), R(dy1), R(dy2),
                                   inv_area2, dxv, dyv, beta, d);
                     wv[0] = wc + double(d[0]);
                     wv[1] = wc + double(d[1]);
                     wv[2] = wc + double(d[2]);
                     beta = R(P.beta_uh_dry + (P.beta_uh - P.beta_uh_dry) * hfactor);
-                    extrap_aux<R>(R(uc), R(u0), R(u1), R(u2), R(dx1), R(dx2), R(dy1), R(dy2),
+                    extrap_aux<R>(R(u0 - uc), R(u1 - u0), R(u2 - u0), R(dx1), R(dx2), R(dy1), R(dy2),
                                   inv_area2, dxv, dyv, beta, d);
                     uv[0] = uc + double(d[0]);
                     uv[1] = uc + double(d[1]);
                     uv[2] = uc + double(d[2]);
                     beta = R(P.beta_vh_dry + (P.beta_vh - P.beta_vh_dry) * hfactor);
-                    extrap_aux<R>(R(vc), R(v0_), R(v1_), R(v2_), R(dx1), R(dx2), R(dy1), R(dy2),
+                    extrap_aux<R>(R(v0_ - vc), R(v1_ - v0_), R(v2_ - v0_), R(dx1), R(dx2), R(dy1), R(dy2),
                                   inv_area2, dxv, dyv, beta, d);
                     vv[0] = vc + double(d[0]);
                     vv[1] = vc + double(d[1]);
@@ -657,15 +828,15 @@ SW_DEV bool extrapolate_one(const DevFields &F, const DevParams &P, int N, int k
                 centroid_wuv(c1, P, w1, u1, v1_);
                 R d[3];
                 const R bw = R(P.beta_w);
-                extrap_one<R>(R(wc), R(w1), R(dx2), R(dy2), dxv, dyv, bw, d);
+                extrap_one<R>(R(w1 - wc), R(dx2), R(dy2), dxv, dyv, bw, d);
                 wv[0] = wc + double(d[0]);
                 wv[1] = wc + double(d[1]);
                 wv[2] = wc + double(d[2]);
-                extrap_one<R>(R(uc), R(u1), R(dx2), R(dy2), dxv, dyv, bw, d);
+                extrap_one<R>(R(u1 - uc), R(dx2), R(dy2), dxv, dyv, bw, d);
                 uv[0] = uc + double(d[0]);
                 uv[1] = uc + double(d[1]);
                 uv[2] = uc + double(d[2]);
-                extrap_one<R>(R(vc), R(v1_), R(dx2), R(dy2), dxv, dyv, bw, d);
+                extrap_one<R>(R(v1_ - vc), R(dx2), R(dy2), dxv, dyv, bw, d);
                 vv[0] = vc + double(d[0]);
                 vv[1] = vc + double(d[1]);
                 vv[2] = vc + double(d[2]);
@@ -832,6 +1003,17 @@ __global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, D
         ez[0] = F.ze[k];
         ez[1] = F.ze[N + k];
         ez[2] = F.ze[2 * N + k];
+    }
+    if (FUSED && !std::is_same<R, double>::value) {
+        // mixed-precision loop: compact FP32 edge state for k_flux_mixed; the
+        // FP64 edge records are rebuilt from the vertex values at the end of the call
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double h = ew[i] - ez[i];
+            F.eqf[i * N + k] = EdgeF{float(h), float(eu[i]), float(ev[i]),
+                                     depth_class(h, P.epsilon, P.limiting_threshold)};
+        }
+        return;
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
