@@ -6,6 +6,7 @@ C ABI (include/anuga_sw.h) whose every stage is a hand-written HIP kernel.
 """
 from .domain import (DEFAULT_PARAMETERS, Dirichlet_boundary, Domain, Quantity,
                      Reflective_boundary, Transmissive_boundary)
+from .ensemble import Ensemble
 from .mesh import Mesh, delaunay_random, read_tsh, rectangular_cross
 
 GPU_domain = Domain
@@ -13,5 +14,5 @@ GPU_domain = Domain
 __all__ = [
     "Domain", "GPU_domain", "Quantity", "Mesh", "Reflective_boundary", "Dirichlet_boundary",
     "Transmissive_boundary", "rectangular_cross", "read_tsh", "delaunay_random",
-    "DEFAULT_PARAMETERS",
+    "DEFAULT_PARAMETERS", "Ensemble",
 ]

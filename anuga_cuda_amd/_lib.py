@@ -83,6 +83,8 @@ _lib = None
 # (name, argtypes) of every exported entry point
 _SIGNATURES = {
     "sw_create": ([C.POINTER(SwMesh), C.POINTER(SwParams), C.c_int], C.c_void_p),
+    "sw_create_ensemble": ([C.POINTER(SwMesh), C.c_int32, C.POINTER(SwParams), C.c_int], C.c_void_p),
+    "sw_select_instance": ([C.c_void_p, C.c_int32], C.c_int),
     "sw_destroy": ([C.c_void_p], C.c_int),
     "sw_last_error": ([C.c_void_p], C.c_char_p),
     "sw_set_params": ([C.c_void_p, C.POINTER(SwParams)], C.c_int),
@@ -146,49 +148,81 @@ def i64p(a):
     return a.ctypes.data_as(_I64P)
 
 
+def _sw_mesh(mesh, keep):
+    """ctypes sw_mesh of a Mesh; the int64 / float64 copies stay alive in ``keep``."""
+    m = SwMesh()
+    m.number_of_elements = mesh.number_of_elements
+    m.number_of_boundary_elements = mesh.number_of_boundary_elements
+
+    def ia(name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.int64)
+        if a.size == 0:
+            a = np.zeros(1, dtype=np.int64)
+        keep.append(a)
+        setattr(m, name, i64p(a))
+
+    def fa(name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        keep.append(a)
+        setattr(m, name, f64p(a))
+
+    ia("neighbours", mesh.neighbours)
+    ia("neighbour_edges", mesh.neighbour_edges)
+    ia("surrogate_neighbours", mesh.surrogate_neighbours)
+    ia("number_of_boundaries", mesh.number_of_boundaries)
+    ia("tri_full_flag", mesh.tri_full_flag)
+    fa("normals", mesh.normals)
+    fa("edgelengths", mesh.edgelengths)
+    fa("radii", mesh.radii)
+    fa("areas", mesh.areas)
+    fa("centroid_coordinates", mesh.centroid_coordinates)
+    fa("vertex_coordinates", mesh.vertex_coordinates)
+    ia("boundary_cells", mesh.boundary_cells)
+    ia("boundary_edges", mesh.boundary_edges)
+    return m
+
+
 class Handle:
-    """Owner of one ``sw_handle``: device buffers of one domain."""
+    """Owner of one ``sw_handle``: device buffers of one domain, or of an
+    ensemble of domains (``Handle.ensemble``).  ``view(i)`` gives a non-owning
+    handle whose calls address instance ``i`` of an ensemble."""
 
-    def __init__(self, mesh, params: SwParams, device=0):
+    def __init__(self, mesh, params: SwParams, device=0, _meshes=None):
         self.lib = load()
-        self._keep = {}
-        m = SwMesh()
-        m.number_of_elements = mesh.number_of_elements
-        m.number_of_boundary_elements = mesh.number_of_boundary_elements
-
-        def ia(name, arr):
-            a = np.ascontiguousarray(arr, dtype=np.int64)
-            if a.size == 0:
-                a = np.zeros(1, dtype=np.int64)
-            self._keep[name] = a
-            setattr(m, name, i64p(a))
-
-        def fa(name, arr):
-            a = np.ascontiguousarray(arr, dtype=np.float64)
-            self._keep[name] = a
-            setattr(m, name, f64p(a))
-
-        ia("neighbours", mesh.neighbours)
-        ia("neighbour_edges", mesh.neighbour_edges)
-        ia("surrogate_neighbours", mesh.surrogate_neighbours)
-        ia("number_of_boundaries", mesh.number_of_boundaries)
-        ia("tri_full_flag", mesh.tri_full_flag)
-        fa("normals", mesh.normals)
-        fa("edgelengths", mesh.edgelengths)
-        fa("radii", mesh.radii)
-        fa("areas", mesh.areas)
-        fa("centroid_coordinates", mesh.centroid_coordinates)
-        fa("vertex_coordinates", mesh.vertex_coordinates)
-        ia("boundary_cells", mesh.boundary_cells)
-        ia("boundary_edges", mesh.boundary_edges)
+        keep = []
         self.params = params
-        h = self.lib.sw_create(C.byref(m), C.byref(params), int(device))
-        self._keep.clear()
+        self.inst = None
+        self._owner = True
+        if _meshes is None:
+            m = _sw_mesh(mesh, keep)
+            h = self.lib.sw_create(C.byref(m), C.byref(params), int(device))
+            self.N = mesh.number_of_elements
+            self.Nb = mesh.number_of_boundary_elements
+            self.n_instances = 1
+        else:
+            arr = (SwMesh * len(_meshes))(*[_sw_mesh(x, keep) for x in _meshes])
+            h = self.lib.sw_create_ensemble(arr, len(_meshes), C.byref(params), int(device))
+            self.N = sum(x.number_of_elements for x in _meshes)
+            self.Nb = sum(x.number_of_boundary_elements for x in _meshes)
+            self.n_instances = len(_meshes)
+            self._sizes = [(x.number_of_elements, x.number_of_boundary_elements) for x in _meshes]
+        del keep
         if not h:
             raise SwError(-10, self.lib.sw_last_error(None).decode())
         self.h = C.c_void_p(h)
-        self.N = mesh.number_of_elements
-        self.Nb = mesh.number_of_boundary_elements
+
+    @classmethod
+    def ensemble(cls, meshes, params, device=0):
+        return cls(None, params, device, _meshes=list(meshes))
+
+    def view(self, i):
+        """Non-owning handle on instance i (its per-instance calls select it first)."""
+        v = Handle.__new__(Handle)
+        v.__dict__.update(self.__dict__)
+        v._owner = False
+        v.inst = int(i)
+        v.N, v.Nb = self._sizes[i] if self.n_instances > 1 else (self.N, self.Nb)
+        return v
 
     # ------------------------------------------------------------------
     def check(self, rc):
@@ -196,11 +230,16 @@ class Handle:
             raise SwError(rc, self.lib.sw_last_error(self.h).decode())
 
     def call(self, name, *args):
+        if self.inst is not None:
+            self.check(self.lib.sw_select_instance(self.h, self.inst))
+        elif self.n_instances > 1:
+            self.check(self.lib.sw_select_instance(self.h, -1))
         self.check(getattr(self.lib, name)(self.h, *args))
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.sw_destroy(self.h)
+            if self._owner:
+                self.lib.sw_destroy(self.h)
             self.h = None
 
     def __del__(self):

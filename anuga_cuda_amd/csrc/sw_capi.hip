@@ -1,18 +1,19 @@
 // C ABI of the shallow-water library (declarations: include/anuga_sw.h).
 //
-// A handle owns every device buffer of one domain and one HIP stream (default
-// priority).  Operator entry points launch the operator kernels of
+// A handle owns every device buffer of one domain -- or of an ensemble of
+// independent domains sharing one set of arrays -- and one HIP stream
+// (default priority).  Operator entry points launch the operator kernels of
 // sw_kernels.h; sw_evolve_steps runs the fused step (flux -> timestep ->
-// update -> distribute) with no host synchronisation between steps, replayed
-// from a hipGraph of kGraphSteps steps, and stops at yields through device-side
-// halt flags.
+// update -> distribute) for every instance at once, with no host
+// synchronisation between steps, replayed from a hipGraph of kGraphSteps
+// steps, and stops at yields through device-side halt flags.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
-#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -29,17 +30,27 @@ constexpr int kNumTimed = 4;                  // flux, timestep, update, distrib
 
 thread_local std::string g_create_error;
 
+// One instance of a handle: its triangles are device triangles
+// [off, off + n), its boundary edges [boff, boff + nb); its caller -> device
+// permutation starts at dev_of[poff].
+struct Inst {
+    int64_t n = 0, off = 0, nb = 0, boff = 0, poff = 0;
+};
+
 }  // namespace
 
 struct sw_handle {
     int device = 0;
     hipStream_t stream = nullptr;
-    int64_t N = 0, Nb = 0;
+    int64_t N = 0, Nb = 0;  // device triangles (instances and their padding), boundary edges
+    int n_inst = 1;
+    int cur = -1;  // selected instance, -1 = all
+    std::vector<Inst> inst;
     sw_params params{};
     DevParams dp{};
     DevMesh mesh{};
     DevFields f{};
-    DevState *st = nullptr;
+    DevState *st = nullptr;  // [n_inst]
     std::vector<void *> allocations;
     std::string err;
     hipGraphExec_t graph_exec = nullptr;
@@ -48,9 +59,9 @@ struct sw_handle {
     std::vector<hipEvent_t> ev;
     double times_ms[kNumTimed] = {};
     int64_t launches[kNumTimed] = {};
-    double *staging = nullptr;  // [3N] transfers of packed records
-    int *cal_of = nullptr;      // [N] caller index of device triangle d
-    int *dev_of = nullptr;      // [N] device index of caller triangle k
+    double *staging = nullptr;  // [3 * largest instance] transfers
+    int *cal_of = nullptr;      // [N] caller index (within its instance) of device triangle d
+    int *dev_of = nullptr;      // [sum n] device index (within its instance) of caller triangle k
 };
 
 namespace {
@@ -88,30 +99,6 @@ int dalloc(sw_handle *h, T **p, size_t count) {
     *p = static_cast<T *>(q);
     e = hipMemset(q, 0, count * sizeof(T));
     if (e != hipSuccess) return fail(h, SW_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
-    return SW_OK;
-}
-
-int upload_i32(sw_handle *h, int **dst, const int64_t *src, size_t count, int64_t lo, int64_t hi,
-               const char *name) {
-    if (!src) return fail(h, SW_ERR_ARG, "mesh array %s is NULL", name);
-    std::vector<int> tmp(count ? count : 1);
-    for (size_t i = 0; i < count; i++) {
-        if (src[i] < lo || src[i] > hi)
-            return fail(h, SW_ERR_ARG, "%s[%zu]=%lld outside [%lld, %lld]", name, i,
-                        (long long)src[i], (long long)lo, (long long)hi);
-        tmp[i] = (int)src[i];
-    }
-    int rc = dalloc(h, dst, count);
-    if (rc) return rc;
-    HIPCHECK(h, hipMemcpy(*dst, tmp.data(), count * sizeof(int), hipMemcpyHostToDevice));
-    return SW_OK;
-}
-
-int upload_f64(sw_handle *h, double **dst, const double *src, size_t count, const char *name) {
-    if (!src) return fail(h, SW_ERR_ARG, "mesh array %s is NULL", name);
-    int rc = dalloc(h, dst, count);
-    if (rc) return rc;
-    HIPCHECK(h, hipMemcpy(*dst, src, count * sizeof(double), hipMemcpyHostToDevice));
     return SW_OK;
 }
 
@@ -165,10 +152,8 @@ int launch_check(sw_handle *h, const char *what) {
     return SW_OK;
 }
 
-// Reads the device state and turns latched device errors into a return code.
-int sync_state(sw_handle *h, DevState *out) {
-    HIPCHECK(h, hipMemcpyAsync(out, h->st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
+// Turns the latched device errors of one instance's state into a return code.
+int state_error(sw_handle *h, const DevState *out) {
     if (out->error) {
         if (out->error & kErrUpdate)
             return fail(h, SW_ERR_UPDATE, "update_conserved_quantities: semi-implicit denominator <= 0");
@@ -184,6 +169,45 @@ int sync_state(sw_handle *h, DevState *out) {
     return SW_OK;
 }
 
+// Reads the device state of instance i and reports its latched errors.
+int sync_state(sw_handle *h, int i, DevState *out) {
+    HIPCHECK(h, hipMemcpyAsync(out, h->st + i, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return state_error(h, out);
+}
+
+// Reads the states of all instances; reports the first instance's error.
+int sync_all(sw_handle *h, std::vector<DevState> &out) {
+    out.resize(h->n_inst);
+    HIPCHECK(h, hipMemcpyAsync(out.data(), h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < h->n_inst; i++)
+        if (out[i].error) {
+            int rc = state_error(h, &out[i]);
+            if (h->n_inst > 1) h->err = "instance " + std::to_string(i) + ": " + h->err;
+            return rc;
+        }
+    return SW_OK;
+}
+
+int put_state(sw_handle *h, int i, const DevState &d) {
+    HIPCHECK(h, hipMemcpy(h->st + i, &d, sizeof d, hipMemcpyHostToDevice));
+    return SW_OK;
+}
+
+// The instance that per-instance calls address: the selected one, or the only one.
+int selected(sw_handle *h) {
+    if (h->cur >= 0) return h->cur;
+    if (h->n_inst == 1) return 0;
+    return fail(h, SW_ERR_STATE, "ensemble handle: select an instance first (sw_select_instance)");
+}
+
+// Operator-by-operator entry points work on single-domain handles only.
+int single(sw_handle *h, const char *what) {
+    if (h->n_inst == 1) return SW_OK;
+    return fail(h, SW_ERR_STATE, "%s: operator calls need a single-domain handle (this one holds an ensemble)", what);
+}
+
 // Where one (quantity, location) array of the API lives on the device: a
 // per-triangle [N] array or a slot-major [3][N] array (slots = 1 or 3), either
 // plain doubles or component `comp` of the {w, uh, vh, z} records.
@@ -195,9 +219,11 @@ struct Loc {
     bool per_triangle = true;  // false: boundary array (caller's order)
 };
 
-bool locate(sw_handle *h, int q, int loc, Loc *out) {
+// Array of instance `inst`: counts are the instance's; a boundary array's
+// pointer is moved to the instance's first boundary edge.
+bool locate(sw_handle *h, int inst, int q, int loc, Loc *out) {
     DevFields &F = h->f;
-    const int64_t N = h->N, Nb = h->Nb;
+    const int64_t N = h->inst[inst].n, Nb = h->inst[inst].nb;
     Loc r;
     auto set = [&](void *p, int slots, int comp, int64_t count) {
         r.ptr = static_cast<double *>(p);
@@ -243,6 +269,7 @@ bool locate(sw_handle *h, int q, int loc, Loc *out) {
             break;
         default: return false;
     }
+    if (!r.per_triangle) r.ptr += h->inst[inst].boff;
     *out = r;
     return true;
 }
@@ -263,8 +290,8 @@ int enqueue_fused_step(sw_handle *h, int slot) {
     else
         hipLaunchKernelGGL((k_flux<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     rec(1);
-    hipLaunchKernelGGL((k_timestep<true>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->f.wave_min,
-                       (int)(g * (kBlock / 64)), h->st, h->dp);
+    hipLaunchKernelGGL((k_timestep<true>), dim3(h->n_inst), dim3(kReduceBlock), 0, h->stream, h->mesh,
+                       h->f.wave_min, h->st, h->dp);
     rec(2);
     hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     rec(3);
@@ -376,50 +403,51 @@ extern "C" {
 
 const char *sw_last_error(const sw_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
-sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
-    g_create_error.clear();
-    if (!m) {
-        fail(nullptr, SW_ERR_ARG, "mesh is NULL");
-        return nullptr;
-    }
-    if (check_params(nullptr, p)) return nullptr;
-    const int64_t N = m->number_of_elements, Nb = m->number_of_boundary_elements;
-    if (N <= 0 || N > (int64_t)0x2AAAAAAA || Nb < 0 || Nb > 3 * N) {
-        fail(nullptr, SW_ERR_ARG, "bad sizes N=%lld Nb=%lld", (long long)N, (long long)Nb);
-        return nullptr;
-    }
+}  // extern "C"
+
+namespace {
+
+// Validates instance `m` and packs it into the device arrays at triangle offset
+// I.off and boundary offset I.boff (device numbering of the whole handle).
+bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t Ntot, int inst_index,
+                   std::vector<EdgeRec> &edgev, std::vector<EdgeRecF> &edgefv, std::vector<int> &flagsv,
+                   std::vector<int> &surv, std::vector<double> &vcv, std::vector<double> &ccv,
+                   std::vector<double> &rav, std::vector<int> &cal_of, std::vector<int> &dev_of,
+                   std::vector<int> &bcellv, std::vector<int> &bedgev, bool &own_normals) {
+    const int64_t N = I.n, Nb = I.nb;
+    auto bad = [&](const char *fmt, long long a, long long b = 0, long long c = 0) {
+        char buf[256];
+        snprintf(buf, sizeof buf, fmt, a, b, c);
+        fail(nullptr, SW_ERR_ARG, "mesh %d: %s", inst_index, buf);
+        return false;
+    };
     const int64_t *arrays_i[] = {m->neighbours, m->neighbour_edges, m->surrogate_neighbours,
                                  m->number_of_boundaries, m->tri_full_flag};
     const double *arrays_f[] = {m->normals, m->edgelengths, m->radii, m->areas,
                                 m->centroid_coordinates, m->vertex_coordinates};
-    for (auto a : arrays_i)
-        if (!a) { fail(nullptr, SW_ERR_ARG, "a mesh index array is NULL"); return nullptr; }
-    for (auto a : arrays_f)
-        if (!a) { fail(nullptr, SW_ERR_ARG, "a mesh geometry array is NULL"); return nullptr; }
-    if (Nb > 0 && (!m->boundary_cells || !m->boundary_edges)) {
-        fail(nullptr, SW_ERR_ARG, "boundary arrays are NULL");
-        return nullptr;
+    for (auto q : arrays_i)
+        if (!q) return bad("a mesh index array is NULL", 0);
+    for (auto q : arrays_f)
+        if (!q) return bad("a mesh geometry array is NULL", 0);
+    if (Nb > 0 && (!m->boundary_cells || !m->boundary_edges)) return bad("boundary arrays are NULL", 0);
+    // device numbering within the instance: caller triangle k at I.off + loc[k]
+    int *cal = cal_of.data() + I.off;
+    int *loc = dev_of.data() + I.poff;
+    if (p->locality_order == SW_ORDER_HILBERT) {
+        std::vector<int> order(N);
+        hilbert_order(m->centroid_coordinates, N, order);
+        for (int64_t d = 0; d < N; d++) cal[d] = order[d];
+    } else {
+        for (int64_t d = 0; d < N; d++) cal[d] = (int)d;
     }
-    // Device numbering: caller triangle k lives at device index dev_of[k].
-    std::vector<int> dev_of(N), cal_of(N);
-    if (p->locality_order == SW_ORDER_HILBERT)
-        hilbert_order(m->centroid_coordinates, N, cal_of);
-    else
-        for (int64_t d = 0; d < N; d++) cal_of[d] = (int)d;
-    for (int64_t d = 0; d < N; d++) dev_of[cal_of[d]] = (int)d;
-    // Host-side validation, packing of the neighbour structure and transposition
-    // of the per-edge / per-vertex geometry to the slot-major [3][N] layout in
-    // device numbering.  Flags (which side evaluates an edge, summation order,
-    // full flag of the evaluating triangle) come from the caller's numbering.
-    std::vector<int> nbv(3 * N), flagsv(N), surv(3 * N);
-    std::vector<EdgeRec> edgev(3 * N);
-    std::vector<EdgeRecF> edgefv(3 * N);
-    std::vector<double> vcv(6 * N), ccv(2 * N), rav(2 * N);
-    bool own_normals = true;
+    for (int64_t d = 0; d < N; d++) loc[cal[d]] = (int)d;
+    // Flags (which side evaluates an edge, summation order, full flag of the
+    // evaluating triangle) come from the caller's numbering.
     for (int64_t k = 0; k < N; k++) {
-        const int64_t d = dev_of[k];
+        const int64_t d = I.off + loc[k];
         int flags = 0;
         int64_t key[3];
+        int nbslot[3];
         for (int i = 0; i < 3; i++) {
             const int64_t ki = 3 * k + i;
             const int64_t n = m->neighbours[ki];
@@ -427,19 +455,14 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
             key[i] = (int64_t)1 << 62 | i;  // own edges after lower neighbours, in local order
             if (n < 0) {
                 const int64_t b = -n - 1;
-                if (b >= Nb || m->boundary_cells[b] != k || m->boundary_edges[b] != i) {
-                    fail(nullptr, SW_ERR_ARG, "neighbours[%lld] = %lld does not match boundary entry %lld",
-                         (long long)ki, (long long)n, (long long)b);
-                    return nullptr;
-                }
-                nbv[i * N + d] = (int)n;
+                if (b >= Nb || m->boundary_cells[b] != k || m->boundary_edges[b] != i)
+                    return bad("neighbours[%lld] = %lld does not match boundary entry %lld", ki, n, b);
+                nbslot[i] = (int)(-(I.boff + b) - 1);
             } else {
                 const int64_t e = m->neighbour_edges[ki];
-                if (n >= N || e < 0 || e > 2 || m->neighbours[3 * n + e] != k) {
-                    fail(nullptr, SW_ERR_ARG, "neighbour structure is not symmetric at edge %lld", (long long)ki);
-                    return nullptr;
-                }
-                nbv[i * N + d] = (int)(e * N + dev_of[n]);
+                if (n >= N || e < 0 || e > 2 || m->neighbours[3 * n + e] != k)
+                    return bad("neighbour structure is not symmetric at edge %lld", ki);
+                nbslot[i] = (int)(e * Ntot + I.off + loc[n]);
                 if (n < k) {
                     flags |= 1 << i;
                     computing = n;
@@ -451,20 +474,17 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
                     own_normals = false;
             }
             if (m->tri_full_flag[computing] == 1) flags |= 1 << (3 + i);
-            const int64_t s = m->surrogate_neighbours[ki];
-            if (s < 0 || s >= N) {
-                fail(nullptr, SW_ERR_ARG, "surrogate_neighbours[%lld] out of range", (long long)ki);
-                return nullptr;
-            }
-            surv[i * N + d] = dev_of[s];
-            EdgeRec &er = edgev[i * N + d];
+            const int64_t sn = m->surrogate_neighbours[ki];
+            if (sn < 0 || sn >= N) return bad("surrogate_neighbours[%lld] out of range", ki);
+            surv[i * Ntot + d] = (int)(I.off + loc[sn]);
+            EdgeRec &er = edgev[i * Ntot + d];
             er.nx = m->normals[2 * ki];
             er.ny = m->normals[2 * ki + 1];
             er.len = m->edgelengths[ki];
-            er.nb = nbv[i * N + d];
-            edgefv[i * N + d] = EdgeRecF{(float)er.nx, (float)er.ny, (float)er.len, er.nb};
-            vcv[2 * (i * N + d)] = m->vertex_coordinates[6 * k + 2 * i];
-            vcv[2 * (i * N + d) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
+            er.nb = nbslot[i];
+            edgefv[i * Ntot + d] = EdgeRecF{(float)er.nx, (float)er.ny, (float)er.len, er.nb};
+            vcv[2 * (i * Ntot + d)] = m->vertex_coordinates[6 * k + 2 * i];
+            vcv[2 * (i * Ntot + d) + 1] = m->vertex_coordinates[6 * k + 2 * i + 1];
         }
         // summation order of the C edge-once loop: lower neighbours ascending, then own edges
         int ord[3] = {0, 1, 2};
@@ -473,27 +493,83 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
                 if (key[ord[b]] < key[ord[a]]) std::swap(ord[a], ord[b]);
         flags |= (ord[0] | ord[1] << 2 | ord[2] << 4) << 6;
         const int64_t nob = m->number_of_boundaries[k];
-        if (nob < 0 || nob > 3) {
-            fail(nullptr, SW_ERR_ARG, "number_of_boundaries[%lld] = %lld", (long long)k, (long long)nob);
-            return nullptr;
-        }
+        if (nob < 0 || nob > 3) return bad("number_of_boundaries[%lld] = %lld", k, nob);
         flags |= (int)nob << 12;
         if (nob == 2) {
             // the reconstruction of a triangle with two boundary edges uses its
             // first surrogate neighbour that is not itself: keep it in slot 0
             for (int i = 0; i < 3; i++)
-                if (surv[i * N + d] != d) {
-                    std::swap(surv[d], surv[i * N + d]);
+                if (surv[i * Ntot + d] != d) {
+                    std::swap(surv[d], surv[i * Ntot + d]);
                     break;
                 }
         }
         flagsv[d] = flags;
-        for (int i = 0; i < 3; i++) edgev[i * N + d].tflags = flags;
+        for (int i = 0; i < 3; i++) edgev[i * Ntot + d].tflags = flags;
         rav[2 * d] = m->radii[k];
         rav[2 * d + 1] = m->areas[k];
         ccv[2 * d] = m->centroid_coordinates[2 * k];
         ccv[2 * d + 1] = m->centroid_coordinates[2 * k + 1];
     }
+    for (int64_t b = 0; b < Nb; b++) {
+        const int64_t c = m->boundary_cells[b], e = m->boundary_edges[b];
+        if (c < 0 || c >= N || e < 0 || e > 2) return bad("boundary entry %lld out of range", b);
+        bcellv[I.boff + b] = (int)(I.off + loc[c]);
+        bedgev[I.boff + b] = (int)e;
+    }
+    return true;
+}
+
+sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, int device) {
+    g_create_error.clear();
+    if (!meshes || n_inst < 1) {
+        fail(nullptr, SW_ERR_ARG, "no meshes");
+        return nullptr;
+    }
+    if (check_params(nullptr, p)) return nullptr;
+    // instance layout: each instance starts on a multiple of kInstAlign triangles
+    std::vector<Inst> inst(n_inst);
+    int64_t Ntot = 0, Nbtot = 0, Nsum = 0, Nmax = 0;
+    for (int i = 0; i < n_inst; i++) {
+        const int64_t n = meshes[i].number_of_elements, nb = meshes[i].number_of_boundary_elements;
+        if (n <= 0 || nb < 0 || nb > 3 * n) {
+            fail(nullptr, SW_ERR_ARG, "mesh %d: bad sizes N=%lld Nb=%lld", i, (long long)n, (long long)nb);
+            return nullptr;
+        }
+        inst[i].off = (Ntot + kInstAlign - 1) / kInstAlign * kInstAlign;
+        inst[i].n = n;
+        inst[i].boff = Nbtot;
+        inst[i].nb = nb;
+        inst[i].poff = Nsum;
+        Ntot = inst[i].off + n;
+        Nbtot += nb;
+        Nsum += n;
+        Nmax = std::max(Nmax, n);
+    }
+    if (Ntot > (int64_t)0x2AAAA000) {
+        fail(nullptr, SW_ERR_ARG, "too many triangles (%lld) for 32-bit edge slots", (long long)Ntot);
+        return nullptr;
+    }
+    // padding triangles: no neighbours (nb = -1 marks them out of every edge test)
+    std::vector<EdgeRec> edgev(3 * Ntot, EdgeRec{0.0, 0.0, 0.0, -1, 0});
+    std::vector<EdgeRecF> edgefv(3 * Ntot, EdgeRecF{0.f, 0.f, 0.f, -1});
+    std::vector<int> flagsv(Ntot, 0), surv(3 * Ntot, 0), cal_of(Ntot, 0), dev_of(Nsum, 0);
+    std::vector<double> vcv(6 * Ntot, 0.0), ccv(2 * Ntot, 0.0), rav(2 * Ntot, 0.0);
+    std::vector<int> bcellv(std::max<int64_t>(Nbtot, 1), 0), bedgev(std::max<int64_t>(Nbtot, 1), 0);
+    bool own_normals = true;
+    for (int i = 0; i < n_inst; i++)
+        if (!pack_instance(&meshes[i], p, inst[i], Ntot, i, edgev, edgefv, flagsv, surv, vcv, ccv, rav, cal_of,
+                           dev_of, bcellv, bedgev, own_normals))
+            return nullptr;
+    const int64_t nwaves = (Ntot + kInstAlign - 1) / kInstAlign * (kInstAlign / 64);
+    std::vector<int> wave_inst(nwaves, n_inst - 1), inst_off(n_inst), inst_end(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        inst_off[i] = (int)inst[i].off;
+        inst_end[i] = (int)(inst[i].off + inst[i].n);
+        const int64_t w1 = (i + 1 < n_inst) ? inst[i + 1].off / 64 : nwaves;
+        for (int64_t w = inst[i].off / 64; w < w1; w++) wave_inst[w] = i;
+    }
+
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
         fail(nullptr, SW_ERR_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
@@ -501,8 +577,10 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     }
     sw_handle *h = new sw_handle();
     h->device = device;
-    h->N = N;
-    h->Nb = Nb;
+    h->N = Ntot;
+    h->Nb = Nbtot;
+    h->n_inst = n_inst;
+    h->inst = inst;
     h->params = *p;
     h->dp = to_dev(*p);
     auto bail = [&](void) -> sw_handle * {
@@ -514,39 +592,27 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
         fail(h, SW_ERR_HIP, "hipStreamCreate failed");
         return bail();
     }
-    DevMesh &M = h->mesh;
-    M.N = (int)N;
-    M.Nb = (int)Nb;
-    M.own_normals = own_normals ? 1 : 0;
-    int *dflags, *dsur, *bcell, *bedge, *bkind;
-    EdgeRec *dedge;
-    double *dvc, *dcc, *dra, *bdir;
     auto put = [&](auto **dst, const auto &vec) -> int {
         if (dalloc(h, dst, vec.size())) return -1;
         if (hipMemcpy(*dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess)
             return fail(h, SW_ERR_HIP, "mesh upload failed");
         return 0;
     };
+    DevMesh &M = h->mesh;
+    M.N = (int)Ntot;
+    M.Nb = (int)Nbtot;
+    M.own_normals = own_normals ? 1 : 0;
+    M.n_inst = n_inst;
+    EdgeRec *dedge;
     EdgeRecF *dedgef;
-    if (put(&dedge, edgev) || put(&dedgef, edgefv) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dvc, vcv) || put(&dcc, ccv) ||
-        put(&dra, rav) || put(&h->cal_of, cal_of) || put(&h->dev_of, dev_of))
+    int *dflags, *dsur, *bcell, *bedge, *bkind, *dwave, *doff, *dend;
+    double *dvc, *dcc, *dra, *bdir;
+    if (put(&dedge, edgev) || put(&dedgef, edgefv) || put(&dflags, flagsv) || put(&dsur, surv) || put(&dvc, vcv) ||
+        put(&dcc, ccv) || put(&dra, rav) || put(&h->cal_of, cal_of) || put(&h->dev_of, dev_of) ||
+        put(&bcell, bcellv) || put(&bedge, bedgev) || put(&dwave, wave_inst) || put(&doff, inst_off) ||
+        put(&dend, inst_end))
         return bail();
-    if (Nb > 0) {
-        std::vector<int> bcv(Nb);
-        for (int64_t b = 0; b < Nb; b++) {
-            const int64_t c = m->boundary_cells[b];
-            if (c < 0 || c >= N) {
-                fail(h, SW_ERR_ARG, "boundary_cells[%lld] out of range", (long long)b);
-                return bail();
-            }
-            bcv[b] = dev_of[c];
-        }
-        if (put(&bcell, bcv) || upload_i32(h, &bedge, m->boundary_edges, Nb, 0, 2, "boundary_edges"))
-            return bail();
-    } else {
-        if (dalloc(h, &bcell, 1) || dalloc(h, &bedge, 1)) return bail();
-    }
-    if (dalloc(h, &bkind, Nb) || dalloc(h, &bdir, 3 * Nb)) return bail();
+    if (dalloc(h, &bkind, Nbtot) || dalloc(h, &bdir, 3 * Nbtot)) return bail();
     M.edge = dedge;
     M.edgef = dedgef;
     M.flags = dflags;
@@ -558,8 +624,12 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     M.bedge = bedge;
     M.bkind = bkind;
     M.bdir = bdir;
+    M.inst_off = doff;
+    M.inst_end = dend;
+    M.wave_inst = dwave;
 
     DevFields &F = h->f;
+    const int64_t N = Ntot, Nb = Nbtot;
     double **perN[] = {&F.weu, &F.uheu, &F.vheu, &F.wsi, &F.uhsi, &F.vhsi, &F.hc, &F.uc, &F.vc,
                        &F.friction, &F.zbar, &F.max_speed};
     double **per3N[] = {&F.wv, &F.uhv, &F.vhv, &F.zv, &F.ze, &F.he, &F.hv, &F.ue, &F.uv, &F.ve, &F.vv};
@@ -571,18 +641,44 @@ sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
     for (auto pp : perNb)
         if (dalloc(h, pp, Nb)) return bail();
     if (dalloc(h, &F.cq, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &F.grav, N) ||
-        dalloc(h, &F.wave_min, (size_t)grid_for(N) * (kBlock / 64)) || dalloc(h, &F.eqf, 3 * N) ||
-        dalloc(h, &h->staging, 3 * N))
+        dalloc(h, &F.wave_min, (size_t)nwaves) || dalloc(h, &F.eqf, 3 * N) || dalloc(h, &h->staging, 3 * Nmax))
         return bail();
-    if (dalloc(h, &h->st, 1)) return bail();
-    DevState s0{};
-    s0.order = p->default_order;
-    s0.bed_interp = 1;
-    if (hipMemcpy(h->st, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) {
+    if (dalloc(h, &h->st, n_inst)) return bail();
+    std::vector<DevState> s0(n_inst);
+    for (auto &d : s0) {
+        d = DevState{};
+        d.order = p->default_order;
+        d.bed_interp = 1;
+    }
+    if (hipMemcpy(h->st, s0.data(), n_inst * sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) {
         fail(h, SW_ERR_HIP, "state upload failed");
         return bail();
     }
     return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+sw_handle *sw_create(const sw_mesh *m, const sw_params *p, int device) {
+    if (!m) {
+        g_create_error = "mesh is NULL";
+        return nullptr;
+    }
+    return create_impl(m, 1, p, device);
+}
+
+sw_handle *sw_create_ensemble(const sw_mesh *meshes, int32_t n_instances, const sw_params *p, int device) {
+    return create_impl(meshes, n_instances, p, device);
+}
+
+int sw_select_instance(sw_handle *h, int32_t instance) {
+    if (!h) return SW_ERR_ARG;
+    if (instance < -1 || instance >= h->n_inst)
+        return fail(h, SW_ERR_ARG, "instance %d outside [-1, %d)", instance, h->n_inst);
+    h->cur = instance;
+    return SW_OK;
 }
 
 int sw_destroy(sw_handle *h) {
@@ -615,26 +711,33 @@ int sw_get_params(const sw_handle *h, sw_params *p) {
 
 int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet) {
     if (!h) return SW_ERR_ARG;
-    if (h->Nb == 0) return SW_OK;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
+    if (I.nb == 0) return SW_OK;
     if (!kind) return fail(h, SW_ERR_ARG, "boundary kind array is NULL");
-    std::vector<int> k(h->Nb);
-    for (int64_t i = 0; i < h->Nb; i++) {
-        if (kind[i] < 0 || kind[i] > 2) return fail(h, SW_ERR_ARG, "boundary kind[%lld]=%d", (long long)i, kind[i]);
-        if (kind[i] == SW_BC_DIRICHLET && !dirichlet)
+    std::vector<int> k(I.nb);
+    for (int64_t b = 0; b < I.nb; b++) {
+        if (kind[b] < 0 || kind[b] > 2) return fail(h, SW_ERR_ARG, "boundary kind[%lld]=%d", (long long)b, kind[b]);
+        if (kind[b] == SW_BC_DIRICHLET && !dirichlet)
             return fail(h, SW_ERR_ARG, "Dirichlet boundary without values");
-        k[i] = kind[i];
+        k[b] = kind[b];
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy((void *)h->mesh.bkind, k.data(), h->Nb * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy((void *)(h->mesh.bkind + I.boff), k.data(), I.nb * sizeof(int), hipMemcpyHostToDevice));
     if (dirichlet)
-        HIPCHECK(h, hipMemcpy((void *)h->mesh.bdir, dirichlet, 3 * h->Nb * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(h, hipMemcpy((void *)(h->mesh.bdir + 3 * I.boff), dirichlet, 3 * I.nb * sizeof(double),
+                              hipMemcpyHostToDevice));
     return SW_OK;
 }
 
 int sw_upload(sw_handle *h, int quantity, int location, const double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
     Loc L;
-    if (!locate(h, quantity, location, &L))
+    if (!locate(h, i, quantity, location, &L))
         return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
     if (count != L.count)
         return fail(h, SW_ERR_ARG, "upload of %lld values into an array of %lld", (long long)count, (long long)L.count);
@@ -643,21 +746,20 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
     if (!L.per_triangle) {
         HIPCHECK(h, hipMemcpy(L.ptr, host, L.count * sizeof(double), hipMemcpyHostToDevice));
     } else {
-        const long long n = L.count / L.slots;
         HIPCHECK(h, hipMemcpy(h->staging, host, L.count * sizeof(double), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, n, L.slots, L.comp,
-                           h->cal_of, h->staging, L.ptr);
+        hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                           (long long)h->N, (long long)I.off, L.slots, L.comp, h->cal_of + I.off, h->staging, L.ptr);
         if (int rc = launch_check(h, "k_to_device")) return rc;
         if (quantity == SW_ELEVATION && location == SW_EDGE) {
-            hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, n, 3, -1,
-                               h->cal_of, h->staging, h->f.ze);
+            hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                               (long long)h->N, (long long)I.off, 3, -1, h->cal_of + I.off, h->staging, h->f.ze);
             if (int rc = launch_check(h, "k_to_device")) return rc;
         }
     }
     if (quantity == SW_ELEVATION && (location == SW_VERTEX || location == SW_EDGE)) {
-        hipLaunchKernelGGL(k_set_bed_interp, dim3(1), dim3(1), 0, h->stream, h->st, 1ll);
-        hipLaunchKernelGGL(k_bed_derived, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, (int)h->N, h->f.zv,
-                           h->f.ze, h->f.zbar, h->st);
+        hipLaunchKernelGGL(k_set_bed_interp, dim3(1), dim3(1), 0, h->stream, h->st + i, 1ll);
+        hipLaunchKernelGGL(k_bed_derived, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (int)h->N, (int)I.off,
+                           (int)I.n, h->f.zv, h->f.ze, h->f.zbar, h->st + i);
         if (int rc = launch_check(h, "k_bed_derived")) return rc;
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -666,8 +768,11 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
 
 int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
     Loc L;
-    if (!locate(h, quantity, location, &L))
+    if (!locate(h, i, quantity, location, &L))
         return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
     if (count != L.count)
         return fail(h, SW_ERR_ARG, "download of %lld values from an array of %lld", (long long)count, (long long)L.count);
@@ -676,8 +781,8 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
     if (!L.per_triangle) {
         HIPCHECK(h, hipMemcpy(host, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToHost));
     } else {
-        hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)(L.count / L.slots),
-                           L.slots, L.comp, h->dev_of, L.ptr, h->staging);
+        hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                           (long long)h->N, (long long)I.off, L.slots, L.comp, h->dev_of + I.poff, L.ptr, h->staging);
         if (int rc = launch_check(h, "k_to_host")) return rc;
         HIPCHECK(h, hipStreamSynchronize(h->stream));
         HIPCHECK(h, hipMemcpy(host, h->staging, L.count * sizeof(double), hipMemcpyDeviceToHost));
@@ -686,19 +791,25 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
 }
 
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count) {
-    if (!h || !host || count != h->N) return SW_ERR_ARG;
-    hipLaunchKernelGGL(k_to_host, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, (long long)h->N, 1, -1,
-                       h->dev_of, h->f.max_speed, h->staging);
+    if (!h || !host) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
+    if (count != I.n) return fail(h, SW_ERR_ARG, "max_speed has %lld values", (long long)I.n);
+    hipLaunchKernelGGL(k_to_host, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (long long)I.n, (long long)h->N,
+                       (long long)I.off, 1, -1, h->dev_of + I.poff, h->f.max_speed, h->staging);
     if (int rc = launch_check(h, "k_to_host")) return rc;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy(host, h->staging, h->N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(host, h->staging, I.n * sizeof(double), hipMemcpyDeviceToHost));
     return SW_OK;
 }
 
 int sw_get_state(sw_handle *h, sw_state *s) {
     if (!h || !s) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
     DevState d;
-    int rc = sync_state(h, &d);
+    int rc = sync_state(h, i, &d);
     static_assert(sizeof(sw_state) <= sizeof(DevState), "layout");
     std::memcpy(s, &d, sizeof(sw_state));
     return rc;
@@ -706,103 +817,115 @@ int sw_get_state(sw_handle *h, sw_state *s) {
 
 int sw_set_state(sw_handle *h, const sw_state *s) {
     if (!h || !s) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
     DevState d;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy(&d, h->st, sizeof d, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(&d, h->st + i, sizeof d, hipMemcpyDeviceToHost));
     std::memcpy(&d, s, sizeof(sw_state));
     d.halt = (s->event != SW_EVENT_NONE) ? 1 : 0;
-    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
-    return SW_OK;
+    return put_state(h, i, d);
 }
 
 int sw_compute_fluxes(sw_handle *h, double *flux_timestep) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "compute_fluxes")) return rc;
     const unsigned g = grid_for(h->N);
     if (h->params.precision == SW_PRECISION_MIXED)
         hipLaunchKernelGGL((k_flux<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     else
         hipLaunchKernelGGL((k_flux<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    hipLaunchKernelGGL((k_timestep<false>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->f.wave_min,
-                       (int)(g * (kBlock / 64)), h->st, h->dp);
+    hipLaunchKernelGGL((k_timestep<false>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->mesh, h->f.wave_min,
+                       h->st, h->dp);
     if (int rc = launch_check(h, "compute_fluxes")) return rc;
     DevState d;
-    int rc = sync_state(h, &d);
+    int rc = sync_state(h, 0, &d);
     if (flux_timestep) *flux_timestep = d.flux_timestep;
     return rc;
 }
 
 int sw_gravity_wb(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "gravity_wb")) return rc;
     hipLaunchKernelGGL(k_gravity_wb, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "gravity_wb");
 }
 
 int sw_manning_friction(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "manning_friction")) return rc;
     hipLaunchKernelGGL(k_manning, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "manning_friction");
 }
 
 int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "update_timestep")) return rc;
     (void)yieldstep;
     DevState d;
-    if (int rc = sync_state(h, &d)) return rc;
+    if (int rc = sync_state(h, 0, &d)) return rc;
     d.finaltime = finaltime;
     d.has_finaltime = has_finaltime;
     HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_update_timestep, dim3(1), dim3(1), 0, h->stream, h->st, h->dp);
     if (int rc = launch_check(h, "update_timestep")) return rc;
-    return sync_state(h, &d);
+    return sync_state(h, 0, &d);
 }
 
 int sw_update_conserved_quantities(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "update_conserved_quantities")) return rc;
     hipLaunchKernelGGL((k_update<false>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     if (int rc = launch_check(h, "update")) return rc;
     DevState d;
-    return sync_state(h, &d);
+    return sync_state(h, 0, &d);
 }
 
 int sw_protect(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "protect")) return rc;
     hipLaunchKernelGGL(k_protect, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "protect");
 }
 
 int sw_extrapolate_second_order_sw(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "extrapolate_second_order_sw")) return rc;
     if (h->params.precision == SW_PRECISION_MIXED)
         hipLaunchKernelGGL((k_extrapolate_second_order<float>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     else
         hipLaunchKernelGGL((k_extrapolate_second_order<double>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
     if (int rc = launch_check(h, "extrapolate_second_order_sw")) return rc;
     DevState d;
-    return sync_state(h, &d);
+    return sync_state(h, 0, &d);
 }
 
 int sw_extrapolate_first_order(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "extrapolate_first_order")) return rc;
     hipLaunchKernelGGL(k_first_order_conserved, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "extrapolate_first_order");
 }
 
 int sw_balance_deep_and_shallow(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "balance_deep_and_shallow")) return rc;
     hipLaunchKernelGGL(k_balance, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "balance_deep_and_shallow");
 }
 
 int sw_interpolate_from_vertices_to_edges(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "interpolate_from_vertices_to_edges")) return rc;
     hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "interpolate_from_vertices_to_edges");
 }
 
 int sw_distribute_to_vertices_and_edges(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "distribute_to_vertices_and_edges")) return rc;
     DevState d;
-    if (int rc = sync_state(h, &d)) return rc;
+    if (int rc = sync_state(h, 0, &d)) return rc;
     if (int rc = sw_protect(h)) return rc;
     if (d.order == 1) {
         if (int rc = sw_extrapolate_first_order(h)) return rc;
@@ -827,40 +950,65 @@ int sw_update_other_quantities(sw_handle *h) {
     return launch_check(h, "update_other_quantities");
 }
 
+// distribute_to_vertices_and_edges of every instance (protect, then the
+// reconstruction kernel with each instance's order), used by the evolve
+// preamble.
+int distribute_all(sw_handle *h) {
+    const unsigned g = grid_for(h->N);
+    hipLaunchKernelGGL(k_protect, dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
+    if (h->params.precision == SW_PRECISION_MIXED)
+        hipLaunchKernelGGL((k_distribute<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp,
+                           h->st);
+    else
+        hipLaunchKernelGGL((k_distribute<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp,
+                           h->st);
+    if (int rc = launch_check(h, "distribute")) return rc;
+    std::vector<DevState> d;
+    return sync_all(h, d);
+}
+
 int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
     if (!h) return SW_ERR_ARG;
-    DevState d;
-    if (int rc = sync_state(h, &d)) return rc;
-    d.order = h->params.default_order;
-    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
-    if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
-    if (int rc = sync_state(h, &d)) return rc;
-    d.time = d.starttime;
-    d.yieldstep = yieldstep;
-    d.order = h->params.default_order;
-    d.has_finaltime = has_finaltime;
-    d.finaltime = finaltime;
-    d.yieldtime = d.time + yieldstep;
-    d.recorded_min_timestep = h->params.evolve_max_timestep;
-    d.recorded_max_timestep = h->params.evolve_min_timestep;
-    d.number_of_steps = 0;
-    d.number_of_first_order_steps = 0;
-    d.smallsteps = 0;
-    d.event = 0;
-    d.error = 0;
-    d.halt = 0;
-    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
-    if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
-    if (int rc = sw_update_boundary(h)) return rc;
-    return sync_state(h, &d);
+    if (h->n_inst > 1 && h->cur >= 0)
+        return fail(h, SW_ERR_STATE, "an ensemble is prepared as a whole (select instance -1)");
+    std::vector<DevState> d;
+    if (int rc = sync_all(h, d)) return rc;
+    for (auto &s : d) s.order = h->params.default_order;
+    HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    if (int rc = distribute_all(h)) return rc;
+    if (int rc = sync_all(h, d)) return rc;
+    for (auto &s : d) {
+        s.time = s.starttime;
+        s.yieldstep = yieldstep;
+        s.order = h->params.default_order;
+        s.has_finaltime = has_finaltime;
+        s.finaltime = finaltime;
+        s.yieldtime = s.time + yieldstep;
+        s.recorded_min_timestep = h->params.evolve_max_timestep;
+        s.recorded_max_timestep = h->params.evolve_min_timestep;
+        s.number_of_steps = 0;
+        s.number_of_first_order_steps = 0;
+        s.smallsteps = 0;
+        s.event = 0;
+        s.error = 0;
+        s.halt = 0;
+    }
+    HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    if (int rc = distribute_all(h)) return rc;
+    if (h->Nb > 0) {
+        hipLaunchKernelGGL(k_update_boundary, dim3(grid_for(h->Nb)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
+        if (int rc = launch_check(h, "update_boundary")) return rc;
+    }
+    return sync_all(h, d);
 }
 
 int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_t *event) {
     if (!h || max_steps < 0) return SW_ERR_ARG;
-    DevState d0, d;
-    if (int rc = sync_state(h, &d0)) return rc;
-    if (d0.event != SW_EVENT_NONE || d0.halt)
-        return fail(h, SW_ERR_STATE, "evolve_steps called while a yield is pending (call sw_advance_yield)");
+    std::vector<DevState> d0, d;
+    if (int rc = sync_all(h, d0)) return rc;
+    for (const auto &s : d0)
+        if (s.event != SW_EVENT_NONE || s.halt)
+            return fail(h, SW_ERR_STATE, "evolve_steps called while a yield is pending (call sw_advance_yield)");
     int64_t launched = 0;
     int rc = SW_OK;
     const bool mixed = h->params.precision == SW_PRECISION_MIXED;
@@ -893,17 +1041,27 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         }
         launched += done_here;
         if (rc) break;
-        rc = sync_state(h, &d);
-        if (rc || d.event != SW_EVENT_NONE || d.halt) break;
+        rc = sync_all(h, d);
+        if (rc) break;
+        // stop once every instance has reached a yield, its final time or an error
+        bool all_stopped = true;
+        for (const auto &s : d) all_stopped = all_stopped && (s.event != SW_EVENT_NONE || s.halt);
+        if (all_stopped) break;
     }
     if (mixed && launched > 0) {
         // the mixed loop keeps FP32 edge state: rebuild the FP64 edge records
         hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     }
     int rc2 = epilogue(h);
-    int rc3 = sync_state(h, &d);
-    if (steps_done) *steps_done = d.total_steps - d0.total_steps;
-    if (event) *event = (int32_t)d.event;
+    int rc3 = sync_all(h, d);
+    int64_t steps = 0;
+    int32_t ev = SW_EVENT_NONE;
+    for (int i = 0; i < h->n_inst && i < (int)d.size(); i++) {
+        steps = std::max<int64_t>(steps, d[i].total_steps - d0[i].total_steps);
+        ev = std::max<int32_t>(ev, (int32_t)d[i].event);
+    }
+    if (steps_done) *steps_done = steps;
+    if (event) *event = ev;
     if (rc) return rc;
     if (rc2) return rc2;
     return rc3;
@@ -911,23 +1069,28 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
 
 int sw_advance_yield(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
-    DevState d;
-    if (int rc = sync_state(h, &d)) return rc;
-    d.yieldtime += d.yieldstep;
-    d.recorded_min_timestep = h->params.evolve_max_timestep;
-    d.recorded_max_timestep = h->params.evolve_min_timestep;
-    d.number_of_steps = 0;
-    d.number_of_first_order_steps = 0;
-    d.event = 0;
-    d.halt = 0;
-    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
+    std::vector<DevState> d;
+    if (int rc = sync_all(h, d)) return rc;
+    for (int i = 0; i < h->n_inst; i++) {
+        if (h->cur >= 0 && i != h->cur) continue;
+        DevState &s = d[i];
+        s.yieldtime += s.yieldstep;
+        s.recorded_min_timestep = h->params.evolve_max_timestep;
+        s.recorded_max_timestep = h->params.evolve_min_timestep;
+        s.number_of_steps = 0;
+        s.number_of_first_order_steps = 0;
+        s.event = 0;
+        s.halt = 0;
+    }
+    HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
     return SW_OK;
 }
 
 int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "evolve_one_step_unfused")) return rc;
     DevState d;
-    if (int rc = sync_state(h, &d)) return rc;
+    if (int rc = sync_state(h, 0, &d)) return rc;
     if (d.event != SW_EVENT_NONE) return fail(h, SW_ERR_STATE, "a yield is pending (call sw_advance_yield)");
     double ft;
     if (int rc = sw_compute_fluxes(h, &ft)) return rc;
@@ -941,10 +1104,10 @@ int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event) {
     if (int rc = sw_update_other_quantities(h)) return rc;
     hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(1), 0, h->stream, h->st);
     if (int rc = launch_check(h, "step bookkeeping")) return rc;
-    int rc = sync_state(h, &d);
+    int rc = sync_state(h, 0, &d);
     if (d.event != SW_EVENT_NONE) {
         d.halt = 1;
-        HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
+        if (int rc2 = put_state(h, 0, d)) return rc2;
     }
     if (event) *event = (int32_t)d.event;
     return rc;

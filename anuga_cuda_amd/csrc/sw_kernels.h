@@ -86,6 +86,12 @@ struct DevMesh {
     const double2 *ra;    // [N]    {radius, area}
     const int *bcell, *bedge, *bkind;  // [Nb]
     const double *bdir;                // [Nb][3]
+    // Ensembles: instance i owns device triangles [inst_off[i], inst_end[i]);
+    // every inst_off is a multiple of kInstAlign, so no workgroup straddles
+    // two instances.  wave_inst[w] is the instance of triangles 64w .. 64w+63.
+    int n_inst;
+    const int *inst_off, *inst_end;  // [n_inst]
+    const int *wave_inst;            // [device triangles / 64, rounded up]
 };
 
 struct DevFields {
@@ -124,6 +130,24 @@ SW_DEV int xcd_remap(int b, int nb) {
 }
 
 SW_DEV int tri_index() { return xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; }
+
+// Instance ranges start on multiples of this many triangles (the largest
+// workgroup of any fused kernel).
+constexpr int kInstAlign = 512;
+
+// The triangle of this lane in the fused kernels, its instance and whether it
+// is a real triangle of that instance (not padding).
+struct Lane {
+    int k, inst;
+    bool valid;
+};
+SW_DEV Lane lane_of(const DevMesh &M) {
+    const int k0 = xcd_remap(blockIdx.x, gridDim.x) * kBlock;
+    const int inst = (M.n_inst == 1) ? 0 : M.wave_inst[k0 >> 6];
+    const int end = (M.n_inst == 1) ? M.N : M.inst_end[inst];
+    const int k = k0 + (int)threadIdx.x;
+    return Lane{k, inst, k < end};
+}
 
 SW_DEV void set_error(DevState *st, long long code) {
     atomicOr((unsigned long long *)&st->error, (unsigned long long)code);
@@ -207,16 +231,18 @@ SW_DEV void finish_step(DevState *st) {
 // FUSED adds: boundary values evaluated on the fly (reflective / Dirichlet /
 // transmissive) and gravity_wb.
 template <typename R, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of(M);
+    DevState *st = st_all + ln.inst;
     if (FUSED && (st->event || st->error)) {
-        // the previous step ended at a yield / the final time / an error
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->halt = 1;
+        // the previous step of this instance ended at a yield / the final time / an error
+        if (threadIdx.x == 0) st->halt = 1;
         return;
     }
     const int N = M.N;
-    const int k = tri_index();
+    const int k = ln.k;
     double dtk = P.evolve_max_timestep;
-    if (k < N) {
+    if (ln.valid) {
         // Load phase: every per-triangle and per-edge input is requested before
         // any of it is used, then the three neighbour edge records are gathered
         // together, so a wavefront waits for two memory round trips in all.
@@ -352,24 +378,27 @@ __global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevPara
     if ((threadIdx.x & 63) == 0) F.wave_min[k >> 6] = wm;
 }
 
-// CFL minimum over the wavefront minima of k_flux (one workgroup of
-// kReduceBlock threads).  Unfused: sets flux_timestep only.  Fused: also
-// update_timestep, the time advance and the end-of-step bookkeeping, unless
-// the loop is halted.
+// CFL minimum over the wavefront minima of k_flux: one workgroup of
+// kReduceBlock threads per instance, over that instance's wavefronts.
+// Unfused: sets flux_timestep only.  Fused: also update_timestep, the time
+// advance and the end-of-step bookkeeping, unless the instance is halted.
 constexpr int kReduceBlock = 1024;
 template <bool FUSED>
-__global__ __launch_bounds__(kReduceBlock) void k_timestep(const unsigned long long *wave_min, int nwaves,
-                                                           DevState *st, DevParams P) {
+__global__ __launch_bounds__(kReduceBlock) void k_timestep(DevMesh M, const unsigned long long *wave_min,
+                                                           DevState *st_all, DevParams P) {
+    const int inst = blockIdx.x;
+    DevState *st = st_all + inst;
     // state and partial minima are requested together: one memory round trip
     DevState s = *st;
+    const int w0 = M.inst_off[inst] >> 6, w1 = (M.inst_end[inst] + 63) >> 6;
     constexpr int U = 16;
     unsigned long long b = dbits(P.evolve_max_timestep);
-    for (int base = 0; base < nwaves; base += U * kReduceBlock) {
+    for (int base = w0; base < w1; base += U * kReduceBlock) {
         unsigned long long v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int j = base + u * kReduceBlock + (int)threadIdx.x;
-            v[u] = (j < nwaves) ? wave_min[j] : ~0ull;
+            v[u] = (j < w1) ? wave_min[j] : ~0ull;
         }
 #pragma unroll
         for (int u = 0; u < U; u++) b = (v[u] < b) ? v[u] : b;
@@ -398,15 +427,17 @@ __global__ __launch_bounds__(kReduceBlock) void k_timestep(const unsigned long l
 // depth tests, the explicit-update sums, gravity and the CFL candidates stay
 // FP64.  The bed must be continuous across edges (k_check_bed), so a side's
 // depth is its own edge depth.
-__global__ __launch_bounds__(kBlock) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of(M);
+    DevState *st = st_all + ln.inst;
     if (st->event || st->error) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->halt = 1;
+        if (threadIdx.x == 0) st->halt = 1;
         return;
     }
     const int N = M.N;
-    const int k = tri_index();
+    const int k = ln.k;
     double dtk = P.evolve_max_timestep;
-    if (k < N) {
+    if (ln.valid) {
         EdgeRecF er[3];
         EdgeF own[3], nbe[3];
 #pragma unroll
@@ -535,12 +566,15 @@ __global__ __launch_bounds__(kBlock) void k_pack_edges(DevMesh M, DevFields F, D
 
 // Bed continuity across shared edges (the mixed flux uses each side's own
 // edge depth): flags the discontinuous-elevation error as k_flux does.
-__global__ __launch_bounds__(kBlock) void k_check_bed(DevMesh M, DevFields F, DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_check_bed(DevMesh M, DevFields F, DevState *st_all) {
     const int N = M.N;
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= 3 * N) return;
     const int e = M.edge[t].nb;
-    if (e >= 0 && fabs(F.ze[t] - F.ze[e]) > 1.0e-10) set_error(st, kErrDiscontinuous);
+    if (e >= 0 && fabs(F.ze[t] - F.ze[e]) > 1.0e-10) {
+        const int k = t % N;
+        set_error(st_all + ((M.n_inst == 1) ? 0 : M.wave_inst[k >> 6]), kErrDiscontinuous);
+    }
 }
 
 // gravity_wb as a separate operator (shallow_water_ext.c:1942-2037)
@@ -656,10 +690,12 @@ SW_DEV double2 gravity_gradient_term(const double2 v[3], const double wv[3], con
 // update_conserved_quantities (quantity.c:769-820), optionally followed by
 // protect (the first stage of distribute_to_vertices_and_edges).
 template <bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of(M);
+    DevState *st = st_all + ln.inst;
     if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
-    const int k = tri_index();
-    if (k >= M.N) return;
+    const int k = ln.k;
+    if (!ln.valid) return;
     const double dt = st->timestep;
     Q4 c = ldq(F.cq + k);
     double siw = ld_s(F.wsi + k), siu = ld_s(F.uhsi + k), siv = ld_s(F.vhsi + k);
@@ -947,8 +983,9 @@ __global__ __launch_bounds__(kBlock) void k_interpolate(DevMesh M, DevFields F) 
 // Source terms of the first fused step of an evolve call.
 __global__ __launch_bounds__(kBlock) void k_source_terms(DevMesh M, DevFields F, DevParams P) {
     const int N = M.N;
-    const int k = tri_index();
-    if (k >= N) return;
+    const Lane ln = lane_of(M);
+    const int k = ln.k;
+    if (!ln.valid) return;
     const Q4 c = ldq(F.cq + k);
     const double2 v[3] = {M.vc[k], M.vc[N + k], M.vc[2 * N + k]};
     const double wv[3] = {F.wv[k], F.wv[N + k], F.wv[2 * N + k]};
@@ -966,11 +1003,13 @@ __global__ __launch_bounds__(kBlock) void k_source_terms(DevMesh M, DevFields F,
 // vertex values when the bed edges are known to be that interpolation
 // (st->bed_interp), else copied from the static bed edge array.
 template <typename R, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, DevParams P, DevState *st) {
+__global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of(M);
+    DevState *st = st_all + ln.inst;
     if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
     const int N = M.N;
-    const int k = tri_index();
-    if (k >= N) return;
+    const int k = ln.k;
+    if (!ln.valid) return;
     const bool second_order = st->order != 1;
     RecIn r;
     load_rec(M, F, k, second_order, r);
@@ -1105,50 +1144,50 @@ __global__ __launch_bounds__(kBlock) void k_fill(double *p, long long n, double 
 // device keeps [N] records or slot-major [3][N] arrays.  `slots` is 1 for
 // per-triangle arrays and 3 for per-edge / per-vertex arrays; `comp` selects a
 // component of {w, uh, vh, z} records (or -1 for a plain double array).
-// Host-order staging [n][slots] <-> device array.  A per-triangle array
-// (perm != nullptr) is slot-major [slots][n] in device numbering: cal_of[d] is
-// the caller's index of device triangle d, dev_of its inverse.  Boundary
-// arrays (perm == nullptr) keep the caller's order.  comp >= 0 addresses one
-// component of the 32-byte {w, uh, vh, z} records.
-__global__ __launch_bounds__(kBlock) void k_to_device(long long n, int slots, int comp, const int *cal_of,
-                                                      const double *src, double *dst) {
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;  // device position
+// Host-order staging [n][slots] <-> device array of one instance.  A
+// per-triangle array is slot-major with stride `stride` (the handle's device
+// triangle count); the instance's triangles start at device index `off`.
+// cal_of[d] is the caller's index of the instance's device triangle off + d,
+// dev_of its inverse.  comp >= 0 addresses one component of the 32-byte
+// {w, uh, vh, z} records.
+__global__ __launch_bounds__(kBlock) void k_to_device(long long n, long long stride, long long off, int slots,
+                                                      int comp, const int *cal_of, const double *src,
+                                                      double *dst) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (t >= n * slots) return;
-    long long s;
-    if (cal_of) {
-        const long long i = t / n, d = t - i * n;
-        s = (long long)cal_of[d] * slots + i;
-    } else {
-        s = t;
-    }
+    const long long i = t / n, d = t - i * n;
+    const long long pos = i * stride + off + d;
+    const double v = src[(long long)cal_of[d] * slots + i];
     if (comp < 0)
-        dst[t] = src[s];
+        dst[pos] = v;
     else
-        dst[4 * t + comp] = src[s];
+        dst[4 * pos + comp] = v;
 }
-__global__ __launch_bounds__(kBlock) void k_to_host(long long n, int slots, int comp, const int *dev_of,
-                                                    const double *src, double *dst) {
+__global__ __launch_bounds__(kBlock) void k_to_host(long long n, long long stride, long long off, int slots,
+                                                    int comp, const int *dev_of, const double *src, double *dst) {
     const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;  // host position
     if (t >= n * slots) return;
-    long long d;
-    if (dev_of) {
-        const long long k = t / slots, i = t - k * slots;
-        d = i * n + dev_of[k];
-    } else {
-        d = t;
-    }
-    dst[t] = (comp < 0) ? src[d] : src[4 * d + comp];
+    const long long k = t / slots, i = t - k * slots;
+    const long long pos = i * stride + off + dev_of[k];
+    dst[t] = (comp < 0) ? src[pos] : src[4 * pos + comp];
 }
-__global__ __launch_bounds__(kBlock) void k_bed_derived(int N, const double *zv, const double *ze, double *zbar,
-                                                        DevState *st) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= N) return;
-    const double v[3] = {zv[k], zv[N + k], zv[2 * N + k]};
-    zbar[k] = (v[0] + v[1] + v[2]) / 3.0;
-    double e[3];
-    interpolate_one(v, e);
-    if (dbits(e[0]) != dbits(ze[k]) || dbits(e[1]) != dbits(ze[N + k]) || dbits(e[2]) != dbits(ze[2 * N + k]))
-        atomicAnd((unsigned long long *)&st->bed_interp, 0ull);
+
+// Mean bed and the check that the bed edge values are the interpolation of the
+// bed vertex values, over device triangles [off, off + n) of one instance.
+__global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, const double *zv, const double *ze,
+                                                        double *zbar, DevState *st) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    bool bad = false;
+    if (t < n) {
+        const int k = off + t;
+        const double v[3] = {zv[k], zv[N + k], zv[2 * N + k]};
+        zbar[k] = (v[0] + v[1] + v[2]) / 3.0;
+        double e[3];
+        interpolate_one(v, e);
+        bad = dbits(e[0]) != dbits(ze[k]) || dbits(e[1]) != dbits(ze[N + k]) || dbits(e[2]) != dbits(ze[2 * N + k]);
+    }
+    // one atomic per wavefront that found a mismatch
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicAnd((unsigned long long *)&st->bed_interp, 0ull);
 }
 
 }  // namespace sw

@@ -423,17 +423,30 @@ class Domain:
         p.locality_order = _lib.SW_ORDER_HILBERT if self.locality_order == "hilbert" else _lib.SW_ORDER_KEEP
         return p
 
+    def _attach(self, handle):
+        """Use ``handle`` (an instance view of an ensemble handle) as this
+        domain's device storage and send it the host state."""
+        if self._handle is not None:
+            raise RuntimeError("domain already has device storage")
+        self._handle = handle
+        self._in_ensemble = True
+        self._init_device_state()
+        self._push()
+
     def _ensure_device(self):
         if self._handle is None:
             self._handle = _lib.Handle(self.mesh, self._sw_params(), self.device)
-            self._handle.set_boundary(self._kinds, self._dirichlet)
-            s = self._handle.get_state()
-            s.time = self._time
-            s.starttime = self.starttime
-            s.order = self._order_
-            self._handle.set_state(s)
-            self._all_dirty = True
+            self._init_device_state()
         return self._handle
+
+    def _init_device_state(self):
+        self._handle.set_boundary(self._kinds, self._dirichlet)
+        s = self._handle.get_state()
+        s.time = self._time
+        s.starttime = self.starttime
+        s.order = self._order_
+        self._handle.set_state(s)
+        self._all_dirty = True
 
     def _each_array(self):
         for name, q in self.quantities.items():
@@ -566,6 +579,8 @@ class Domain:
         """Generator yielding the model time at every yieldstep and at the end
         (gpu_domain_advanced.py:2296-2698).  ``fused=False`` runs the
         one-launch-per-operator step instead of the fused device loop."""
+        if self.__dict__.get("_in_ensemble"):
+            raise RuntimeError("this domain belongs to an Ensemble: evolve the ensemble")
         h = self._dev()
         if yieldstep is None:
             yieldstep = self.evolve_max_timestep

@@ -177,6 +177,31 @@ def delaunay_random(npoints, extent=(10000.0, 10000.0), seed=12345):
     return pts, tri
 
 
+def delaunay_jittered(npoints, extent=(1000.0, 1000.0), seed=12345, jitter=0.35):
+    """Delaunay triangulation of about ``npoints`` seeded points: a square grid
+    of spacing ``s`` (``npoints`` cells) whose interior points are moved by
+    up to ``jitter * s`` in x and y, plus the sides sampled at spacing ``s``.
+    Unstructured, and without the slivers of uniform random points.  Returns
+    ``(points, triangles)``."""
+    from scipy.spatial import Delaunay  # setup-time only
+
+    rng = np.random.default_rng(seed)
+    lx, ly = extent
+    s = np.sqrt(lx * ly / max(int(npoints), 1))
+    nx = max(int(round(lx / s)), 2)
+    ny = max(int(round(ly / s)), 2)
+    gx, gy = np.meshgrid(np.linspace(0.0, lx, nx + 1), np.linspace(0.0, ly, ny + 1), indexing="ij")
+    hx, hy = lx / nx, ly / ny
+    px = gx.copy()
+    py = gy.copy()
+    inner = (slice(1, -1), slice(1, -1))
+    px[inner] += rng.uniform(-jitter, jitter, px[inner].shape) * hx
+    py[inner] += rng.uniform(-jitter, jitter, py[inner].shape) * hy
+    pts = np.stack([px.ravel(), py.ravel()], axis=1)
+    tri = Delaunay(pts).simplices.astype(np.int64)
+    return pts, tri
+
+
 def morton_order(cx, cy, bits=16):
     """Permutation that sorts points by the Z-order (Morton) code of their
     coordinates quantised to ``2**bits`` cells per axis."""

@@ -20,8 +20,8 @@ import os
 
 import numpy as np
 
-from .domain import Domain, Reflective_boundary, Transmissive_boundary
-from .mesh import Mesh, delaunay_random, read_tsh, rectangular_cross
+from .domain import Dirichlet_boundary, Domain, Reflective_boundary, Transmissive_boundary
+from .mesh import Mesh, delaunay_jittered, delaunay_random, read_tsh, rectangular_cross
 
 __all__ = ["dam_break_channel", "delaunay_terrain", "fbm", "merimbula", "MERIMBULA_TSH"]
 
@@ -103,3 +103,63 @@ def merimbula(path=MERIMBULA_TSH, x0=756000.0, x1=756500.0, h=2.0, **params):
     d.set_quantity("stage", lambda x, y: h * ((x > x0) & (x < x1)) + 1.0)
     d.set_boundary({tag: Reflective_boundary(d) for tag in mesh.get_tag_names()})
     return d
+
+
+def tag_square_sides(mesh, lx, ly):
+    """Tag the boundary edges of a mesh of the rectangle [0, lx] x [0, ly] as
+    'left', 'right', 'bottom' or 'top' by the side their midpoint lies on."""
+    t, p = mesh.triangles, mesh.points
+    k, e = mesh.boundary_cells, mesh.boundary_edges
+    a = p[t[k, (e + 1) % 3]]
+    b = p[t[k, (e + 2) % 3]]
+    mx, my = 0.5 * (a[:, 0] + b[:, 0]), 0.5 * (a[:, 1] + b[:, 1])
+    dist = np.stack([mx, lx - mx, my, ly - my], axis=1)
+    names = np.array(["left", "right", "bottom", "top"], dtype=object)
+    tags = names[np.argmin(dist, axis=1)]
+    mesh.boundary_tags = tags
+    mesh.boundary = {(int(kk), int(ee)): tag for kk, ee, tag in zip(k, e, tags)}
+    return mesh
+
+
+def ensemble_member(triangles, seed, manning, dam_height, inflow, extent=1000.0, precision="fp64", **params):
+    """One scenario of the batched ensemble: a seeded Delaunay mesh (jittered
+    grid points) of about ``triangles`` triangles on a square of side
+    ``extent``, bed sloping 1:1000
+    with a Gaussian bump, a dam break of ``dam_height`` m against 0.5 m, Manning
+    n ``manning``, an inflow boundary on the left (Dirichlet: stage dam_height,
+    x-momentum ``inflow`` m^2/s), transmissive on the right, reflective walls
+    top and bottom."""
+    points, tris = delaunay_jittered(max(int(triangles) // 2, 50), extent=(extent, extent), seed=seed)
+    mesh = tag_square_sides(Mesh(points, tris), extent, extent)
+    d = Domain(mesh=mesh, precision=precision, **params)
+    c = extent / 2.0
+    sigma = extent / 10.0
+
+    def bed(x, y):
+        r2 = (x - c) ** 2 + (y - c) ** 2
+        return x / 1000.0 + 0.5 * np.exp(-r2 / (2.0 * sigma * sigma))
+
+    def stage(x, y):
+        return np.maximum(np.where(x < c, dam_height, 0.5), bed(x, y))
+
+    d.set_quantity("elevation", bed)
+    d.set_quantity("stage", stage)
+    d.set_quantity("friction", manning, location="centroids")
+    d.set_boundary({"left": Dirichlet_boundary([dam_height, inflow, 0.0]), "right": Transmissive_boundary(d),
+                    "bottom": Reflective_boundary(d), "top": Reflective_boundary(d)})
+    return d
+
+
+def ensemble_scenarios(count=64, seed=4242, triangle_range=(20000, 500000), precision="fp64", **params):
+    """The batched-ensemble workload: ``count`` members with triangle counts
+    log-uniform in ``triangle_range``, Manning n ~ U[0.01, 0.06], dam height
+    ~ U[1, 4] m and inflow ~ U[0, 1] m^2/s (all seeded).  Returns the list of
+    domains and the list of their parameters."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(triangle_range[0]), np.log(triangle_range[1])
+    specs = []
+    for i in range(count):
+        specs.append(dict(triangles=int(np.exp(rng.uniform(lo, hi))), seed=int(seed * 1000 + i),
+                          manning=float(rng.uniform(0.01, 0.06)), dam_height=float(rng.uniform(1.0, 4.0)),
+                          inflow=float(rng.uniform(0.0, 1.0))))
+    return [ensemble_member(precision=precision, **s, **params) for s in specs], specs

@@ -102,6 +102,19 @@ typedef struct sw_state {
  * (src/anuga_HMPP/sw_domain_fun.h:57-329).  Returns NULL on failure
  * (message via sw_last_error(NULL)). */
 sw_handle *sw_create(const sw_mesh *mesh, const sw_params *params, int device);
+/* Ensemble of independent domains in one handle: every fused kernel launch
+ * covers all instances (each with its own timestep, time, yields and order)
+ * and one hipGraph replays the step for the whole ensemble.  The reference
+ * runs one domain per driver object (gpu_domain_advanced.py:47-180); this has
+ * no counterpart there.  Per-instance calls (sw_upload, sw_download,
+ * sw_get_state, sw_set_state, sw_set_boundary, sw_download_max_speed) address
+ * the instance chosen with sw_select_instance; sw_evolve_prepare,
+ * sw_evolve_steps and sw_advance_yield act on every instance (or, for
+ * sw_advance_yield, on the selected one).  The operator-by-operator entry
+ * points need a single-domain handle (SW_ERR_STATE otherwise). */
+sw_handle *sw_create_ensemble(const sw_mesh *meshes, int32_t n_instances, const sw_params *params, int device);
+/* instance in [0, n_instances), or -1 for all */
+int sw_select_instance(sw_handle *h, int32_t instance);
 int sw_destroy(sw_handle *h);
 const char *sw_last_error(const sw_handle *h);
 int sw_set_params(sw_handle *h, const sw_params *params);

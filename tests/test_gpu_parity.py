@@ -9,7 +9,7 @@ device math library; everything else is expected bit-identical.
 import numpy as np
 import pytest
 
-from anuga_cuda_amd import _lib, scenarios
+from anuga_cuda_amd import Ensemble, _lib, scenarios
 from anuga_cuda_amd.domain import Dirichlet_boundary, Domain, Reflective_boundary, Transmissive_boundary
 from anuga_cuda_amd.mesh import Mesh, rectangular_cross
 from oracle.oracle import OracleDomain
@@ -344,3 +344,44 @@ def test_config1_dam_break_to_10s_matches_oracle():
     assert next(gen_o, None) is None
     assert abs(d.get_time() - 10.0) < 1e-12 and yields == 11
     print("config 1: %d steps to t = 10 s over %d yields" % (total, yields))
+
+
+def test_ensemble_instances_match_separate_runs_and_oracle():
+    """Three members of different size, Manning n, dam height and inflow in one
+    ensemble handle: each instance's timestep sequence and state equal those
+    of the same domain evolved alone, bit for bit, and its step counts equal
+    the FP64 oracle's, with the state within 1e-12 per step taken (the last
+    bit of Manning's pow() may differ each step and is not re-synchronised
+    over the run)."""
+    specs = [dict(triangles=900, seed=11, manning=0.012, dam_height=1.5, inflow=0.2),
+             dict(triangles=2500, seed=12, manning=0.05, dam_height=3.7, inflow=0.9),
+             dict(triangles=1400, seed=13, manning=0.03, dam_height=2.2, inflow=0.0)]
+    members = [scenarios.ensemble_member(extent=300.0, **sp) for sp in specs]
+    alone = [scenarios.ensemble_member(extent=300.0, **sp) for sp in specs]
+    oracles = [OracleDomain.from_domain(d) for d in alone]
+    ens = Ensemble(members)
+    ens_steps = []
+    for t in ens.evolve(yieldstep=2.0, finaltime=6.0):
+        ens_steps.append([d.number_of_steps for d in members])
+    assert len(ens_steps) == 4
+    for i, (d, a, od) in enumerate(zip(members, alone, oracles)):
+        steps = [s[i] for s in ens_steps]
+        steps_alone = []
+        for t in a.evolve(yieldstep=2.0, finaltime=6.0):
+            steps_alone.append(a.number_of_steps)
+        steps_o = []
+        for t in od.evolve(2.0, 6.0):
+            steps_o.append(od.d.number_of_steps)
+        assert steps == steps_alone == steps_o, (i, steps, steps_alone, steps_o)
+        print("member %d: %d triangles, steps per yield %s" % (i, d.number_of_elements, steps))
+        assert abs(d.get_time() - 6.0) < 1e-12
+        for q in ("stage", "xmomentum", "ymomentum"):
+            for loc in ("centroid_values", "vertex_values", "edge_values"):
+                x = getattr(d.quantities[q], loc)
+                y = getattr(a.quantities[q], loc)
+                assert np.array_equal(x, y), (i, q, loc)
+            compare(d.quantities[q].centroid_values, od.q(q, "centroid_values"), "ens%d.%s" % (i, q),
+                    TOL * sum(steps))
+        assert np.array_equal(d.max_speed, a.max_speed)
+        a.close()
+    ens.close()
