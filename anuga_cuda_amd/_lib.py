@@ -113,6 +113,7 @@ _SIGNATURES = {
     "sw_advance_yield": ([C.c_void_p], C.c_int),
     "sw_evolve_one_step_unfused": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
     "sw_synchronize": ([C.c_void_p], C.c_int),
+    "sw_autotune": ([C.c_void_p, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "sw_enable_timing": ([C.c_void_p, C.c_int32], C.c_int),
     "sw_kernel_times": ([C.c_void_p, _F64P, _I64P, C.c_int32], C.c_int),
 }
@@ -300,6 +301,13 @@ class Handle:
         ev = C.c_int32()
         self.call("sw_evolve_one_step_unfused", C.byref(ev))
         return ev.value
+
+    def autotune(self, steps=3):
+        """Workgroup sizes (flux, update, distribute) chosen by sw_autotune
+        after ``steps`` real steps per candidate."""
+        chosen = (C.c_int32 * 3)()
+        self.call("sw_autotune", int(steps), chosen)
+        return tuple(chosen)
 
     def kernel_times(self, n=4):
         ms = np.zeros(n)

@@ -62,6 +62,7 @@ struct sw_handle {
     double *staging = nullptr;  // [3 * largest instance] transfers
     int *cal_of = nullptr;      // [N] caller index (within its instance) of device triangle d
     int *dev_of = nullptr;      // [sum n] device index (within its instance) of caller triangle k
+    int blk[3] = {kBlock, kBlock, kBlock};  // workgroup size of the fused flux, update, distribute kernels
 };
 
 namespace {
@@ -276,29 +277,60 @@ bool locate(sw_handle *h, int inst, int q, int loc, Loc *out) {
 
 __global__ void k_set_bed_interp(DevState *st, long long v) { st->bed_interp = v; }
 
+// Fused-kernel launches at one of the workgroup sizes the autotuner chooses
+// from (128, 256, 512 lanes).
+template <int B>
+void launch_flux_b(sw_handle *h, bool mixed) {
+    const unsigned g = (unsigned)((h->N + B - 1) / B);
+    if (mixed)
+        hipLaunchKernelGGL((k_flux_mixed<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    else
+        hipLaunchKernelGGL((k_flux<double, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+}
+template <int B>
+void launch_update_b(sw_handle *h) {
+    const unsigned g = (unsigned)((h->N + B - 1) / B);
+    hipLaunchKernelGGL((k_update<true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+}
+template <int B>
+void launch_distribute_b(sw_handle *h, bool mixed) {
+    const unsigned g = (unsigned)((h->N + B - 1) / B);
+    if (mixed)
+        hipLaunchKernelGGL((k_distribute<float, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp,
+                           h->st);
+    else
+        hipLaunchKernelGGL((k_distribute<double, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp,
+                           h->st);
+}
+
 // Enqueues one fused step.  slot >= 0 records HIP events around every kernel
 // into event group `slot` (timing mode).
 int enqueue_fused_step(sw_handle *h, int slot) {
-    const unsigned g = grid_for(h->N);
     const bool mixed = h->params.precision == SW_PRECISION_MIXED;
     auto rec = [&](int idx) {
         if (slot >= 0) (void)hipEventRecord(h->ev[slot * (kNumTimed + 1) + idx], h->stream);
     };
     rec(0);
-    if (mixed)
-        hipLaunchKernelGGL(k_flux_mixed, dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    else
-        hipLaunchKernelGGL((k_flux<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    switch (h->blk[0]) {
+        case 128: launch_flux_b<128>(h, mixed); break;
+        case 512: launch_flux_b<512>(h, mixed); break;
+        default: launch_flux_b<256>(h, mixed); break;
+    }
     rec(1);
     hipLaunchKernelGGL((k_timestep<true>), dim3(h->n_inst), dim3(kReduceBlock), 0, h->stream, h->mesh,
                        h->f.wave_min, h->st, h->dp);
     rec(2);
-    hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    switch (h->blk[1]) {
+        case 128: launch_update_b<128>(h); break;
+        case 512: launch_update_b<512>(h); break;
+        default: launch_update_b<256>(h); break;
+    }
     rec(3);
-    if (mixed)
-        hipLaunchKernelGGL((k_distribute<float, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
-    else
-        hipLaunchKernelGGL((k_distribute<double, true>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    switch (h->blk[2]) {
+        case 128: launch_distribute_b<128>(h, mixed); break;
+        case 512: launch_distribute_b<512>(h, mixed); break;
+        default: launch_distribute_b<256>(h, mixed); break;
+    }
     rec(4);
     return launch_check(h, "fused step");
 }
@@ -1110,6 +1142,39 @@ int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event) {
         if (int rc2 = put_state(h, 0, d)) return rc2;
     }
     if (event) *event = (int32_t)d.event;
+    return rc;
+}
+
+int sw_autotune(sw_handle *h, int32_t steps, int32_t *chosen) {
+    if (!h || steps < 1) return SW_ERR_ARG;
+    static const int cand[3] = {128, 256, 512};
+    const int timed_slot[3] = {0, 2, 3};  // flux, update, distribute in the timing table
+    double best_t[3] = {1e300, 1e300, 1e300};
+    int best_b[3] = {h->blk[0], h->blk[1], h->blk[2]};
+    const bool was_timing = h->timing;
+    if (int rc = sw_enable_timing(h, 1)) return rc;
+    int rc = SW_OK;
+    for (int c = 0; c < 3 && rc == SW_OK; c++) {
+        for (int k = 0; k < 3; k++) h->blk[k] = cand[c];
+        for (int i = 0; i < kNumTimed; i++) h->times_ms[i] = 0, h->launches[i] = 0;
+        int64_t done = 0;
+        int32_t ev = SW_EVENT_NONE;
+        rc = sw_evolve_steps(h, steps, &done, &ev);
+        if (rc) break;
+        for (int k = 0; k < 3; k++) {
+            const int i = timed_slot[k];
+            if (h->launches[i] == 0) continue;
+            const double t = h->times_ms[i] / h->launches[i];
+            if (t < best_t[k]) best_t[k] = t, best_b[k] = cand[c];
+        }
+        if (ev != SW_EVENT_NONE || done < steps) break;  // the run reached a yield: keep what was measured
+    }
+    for (int k = 0; k < 3; k++) h->blk[k] = best_b[k];
+    for (int i = 0; i < kNumTimed; i++) h->times_ms[i] = 0, h->launches[i] = 0;
+    drop_graph(h);  // launch shapes changed
+    h->timing = was_timing;
+    if (chosen)
+        for (int k = 0; k < 3; k++) chosen[k] = h->blk[k];
     return rc;
 }
 

@@ -141,8 +141,9 @@ struct Lane {
     int k, inst;
     bool valid;
 };
+template <int BLOCK = kBlock>
 SW_DEV Lane lane_of(const DevMesh &M) {
-    const int k0 = xcd_remap(blockIdx.x, gridDim.x) * kBlock;
+    const int k0 = xcd_remap(blockIdx.x, gridDim.x) * BLOCK;
     const int inst = (M.n_inst == 1) ? 0 : M.wave_inst[k0 >> 6];
     const int end = (M.n_inst == 1) ? M.N : M.inst_end[inst];
     const int k = k0 + (int)threadIdx.x;
@@ -230,9 +231,9 @@ SW_DEV void finish_step(DevState *st) {
 //
 // FUSED adds: boundary values evaluated on the fly (reflective / Dirichlet /
 // transmissive) and gravity_wb.
-template <typename R, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
-    const Lane ln = lane_of(M);
+template <typename R, bool FUSED, int BLOCK = kBlock>
+__global__ __launch_bounds__(BLOCK) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
     if (FUSED && (st->event || st->error)) {
         // the previous step of this instance ended at a yield / the final time / an error
@@ -427,8 +428,9 @@ __global__ __launch_bounds__(kReduceBlock) void k_timestep(DevMesh M, const unsi
 // depth tests, the explicit-update sums, gravity and the CFL candidates stay
 // FP64.  The bed must be continuous across edges (k_check_bed), so a side's
 // depth is its own edge depth.
-__global__ __launch_bounds__(kBlock) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
-    const Lane ln = lane_of(M);
+template <int BLOCK = kBlock>
+__global__ __launch_bounds__(BLOCK) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
     if (st->event || st->error) {
         if (threadIdx.x == 0) st->halt = 1;
@@ -689,9 +691,9 @@ SW_DEV double2 gravity_gradient_term(const double2 v[3], const double wv[3], con
 
 // update_conserved_quantities (quantity.c:769-820), optionally followed by
 // protect (the first stage of distribute_to_vertices_and_edges).
-template <bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
-    const Lane ln = lane_of(M);
+template <bool FUSED, int BLOCK = kBlock>
+__global__ __launch_bounds__(BLOCK) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
     if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
     const int k = ln.k;
@@ -1002,9 +1004,9 @@ __global__ __launch_bounds__(kBlock) void k_source_terms(DevMesh M, DevFields F,
 // edge records are written whole; their bed value is recomputed from the bed
 // vertex values when the bed edges are known to be that interpolation
 // (st->bed_interp), else copied from the static bed edge array.
-template <typename R, bool FUSED>
-__global__ __launch_bounds__(kBlock) void k_distribute(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
-    const Lane ln = lane_of(M);
+template <typename R, bool FUSED, int BLOCK = kBlock>
+__global__ __launch_bounds__(BLOCK) void k_distribute(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+    const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
     if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
     const int N = M.N;

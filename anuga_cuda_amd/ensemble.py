@@ -72,6 +72,15 @@ class Ensemble:
         self._sync()
         return done, event
 
+    def autotune(self, steps=3):
+        """Choose the fused kernels' workgroup sizes by timing ``steps`` real
+        steps per candidate (sw_autotune).  Call after ``prepare``; the tuning
+        steps advance every member."""
+        self._push_all()
+        chosen = self.handle.autotune(steps)
+        self._sync()
+        return chosen
+
     def evolve(self, yieldstep, finaltime):
         """Generator over the common yield times: every instance reaches each
         yield (with its own timestep sequence) before the ensemble yields."""

@@ -126,8 +126,9 @@ def main():
     h.call("sw_evolve_prepare", 1e9, 0.0, 0)
     log("device prepared")
     done, ev = h.evolve_steps(args.warmup)
+    launch = h.autotune(3)
     h.call("sw_synchronize")
-    log("warmup %d steps done" % done)
+    log("warmup %d steps done; autotuned workgroup sizes (flux, update, distribute): %s" % (done, launch))
 
     t_start = time.perf_counter()
     done, ev = h.evolve_steps(args.steps)
@@ -200,7 +201,7 @@ def main():
         "config": {"workload": "plain FV, seeded Delaunay %d points in 10 km x 10 km, %s order, fBm terrain "
                                "20 m, wet/dry mound, Manning 0.025, transmissive" % (args.points, args.order),
                    "triangles": N, "precision": args.precision, "order": 2, "timestepping": "euler",
-                   "device_numbering": args.locality},
+                   "device_numbering": args.locality, "workgroup_sizes": list(launch)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_triangle": KERNEL_BYTES_PER_TRIANGLE[dominant],
@@ -230,6 +231,8 @@ def bench_ensemble(args):
         time.time() - t0))
     ens.prepare(1e9, None)
     ens.evolve_steps(args.warmup)
+    launch = ens.autotune(3)
+    log("autotuned workgroup sizes (flux, update, distribute): %s" % (launch,))
     h = ens.handle
     h.call("sw_synchronize")
     t_start = time.perf_counter()
@@ -267,7 +270,8 @@ def bench_ensemble(args):
         "config": {"workload": "batched ensemble: %d seeded Delaunay dam-break scenarios, triangles log-uniform "
                                "in [20k, 500k], Manning n ~ U[0.01, 0.06], dam height ~ U[1, 4] m, inflow "
                                "~ U[0, 1] m^2/s, one handle, per-member adaptive timesteps" % len(domains),
-                   "triangles": N, "members": len(domains), "precision": args.precision},
+                   "triangles": N, "members": len(domains), "precision": args.precision,
+                   "workgroup_sizes": list(launch)},
         "step_roofline": ({"bytes_per_triangle_step": step_bytes,
                            "achieved": step_bytes * N * args.steps / elapsed / 1e9,
                            "frac": step_bytes * N * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,

@@ -185,6 +185,15 @@ int sw_advance_yield(sw_handle *h);
  * driver's evolve_one_euler_step + distribute + update_boundary). */
 int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event);
 
+/* Autotuner of the fused step's launch shapes (no counterpart in the
+ * reference): runs `steps` real Euler steps with each candidate workgroup
+ * size (128, 256, 512 lanes) for the flux, update and distribute kernels,
+ * timing every launch with HIP events, and keeps the fastest size per
+ * kernel.  The tuning steps advance the simulation like any other steps (the
+ * results do not depend on the workgroup size).  chosen[3] (may be NULL)
+ * receives the sizes kept for flux, update, distribute. */
+int sw_autotune(sw_handle *h, int32_t steps, int32_t *chosen);
+
 /* Synchronise the handle's stream. */
 int sw_synchronize(sw_handle *h);
 /* Kernel timing of the fused step: when enabled, sw_evolve_steps records HIP

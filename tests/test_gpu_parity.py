@@ -385,3 +385,24 @@ def test_ensemble_instances_match_separate_runs_and_oracle():
         assert np.array_equal(d.max_speed, a.max_speed)
         a.close()
     ens.close()
+
+
+def test_autotuned_launch_shapes_do_not_change_results():
+    """sw_autotune runs real steps with workgroups of 128, 256 and 512 lanes
+    and keeps the fastest per kernel; the state after it and after further
+    steps is bitwise that of a run that was never tuned."""
+    a = _delaunay("random")
+    b = _delaunay("random")
+    ha, hb = a._dev(), b._dev()
+    for h in (ha, hb):
+        h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    chosen = ha.autotune(2)  # 3 candidates x 2 steps
+    assert all(c in (128, 256, 512) for c in chosen)
+    hb.evolve_steps(6)
+    ha.evolve_steps(10)
+    hb.evolve_steps(10)
+    a._host_stale = b._host_stale = True
+    for q in ("stage", "xmomentum", "ymomentum"):
+        for loc in ("centroid_values", "vertex_values", "edge_values"):
+            assert np.array_equal(getattr(a.quantities[q], loc), getattr(b.quantities[q], loc)), (q, loc)
+    assert a._handle.get_state().time == b._handle.get_state().time
