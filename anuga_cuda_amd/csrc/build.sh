@@ -1,5 +1,7 @@
 #!/bin/bash
-# Builds libanuga_sw.so (gfx950) in-tree from sw_capi.hip.
+# Builds libanuga_sw.so (the shallow-water step, from sw_capi.hip) and
+# libanuga_gnn.so (the learned edge-flux surrogate, from gnn_kernels.hip) for
+# gfx950, in-tree.
 # -ffp-contract=off: no FMA contraction, so FP64 results follow the C oracle's
 #                    operation order bit for bit.
 # -packed-fp32-ops disabled: hipcc must not emit packed FP32 instructions.
@@ -14,3 +16,10 @@ ARCH="${SW_OFFLOAD_ARCH:-gfx950}"
     -o "${OUT}.tmp" "${HERE}/sw_capi.hip"
 mv -f "${OUT}.tmp" "${OUT}"
 echo "built ${OUT}"
+GOUT="${HERE}/../libanuga_gnn.so"
+/opt/rocm/bin/hipcc --offload-arch="${ARCH}" -O3 -std=c++17 -fPIC -shared \
+    -Xclang -target-feature -Xclang -packed-fp32-ops \
+    -Wall -Wno-unused-function \
+    -o "${GOUT}.tmp" "${HERE}/gnn_kernels.hip"
+mv -f "${GOUT}.tmp" "${GOUT}"
+echo "built ${GOUT}"

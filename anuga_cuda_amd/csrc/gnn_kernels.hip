@@ -1,0 +1,550 @@
+// Kernels of the learned edge-flux surrogate (declarations: include/anuga_gnn.h).
+//
+// Dense layers run on the gfx950 matrix cores: v_mfma_f32_16x16x32_bf16, one
+// 16 x 16 output block per instruction with K = 32, BF16 operands and FP32
+// accumulation.  Operand layout of that instruction (lane l, r = l % 16,
+// g = l / 16): A[m = r][k = 8g + j], B[k = 8g + j][n = r], D[m = 4g + j][n = r],
+// j = 0..7 for A and B, 0..3 for D.  One wavefront computes 16 rows of a layer.
+// Activations and gradients are BF16 in HBM; weight gradients are reduced in
+// FP32 (per-workgroup partials, then a deterministic sum); the state update is
+// FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "../../include/anuga_gnn.h"
+
+namespace gnn {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kB = 256;      // threads per workgroup
+constexpr int kParts = 256;  // workgroups of the weight-gradient reduction
+constexpr int kFeat = 32;    // edge input features (padded)
+constexpr int kOut = 32;     // head outputs (padded; 5 used)
+
+#define GNN_DEV __device__ __forceinline__
+
+GNN_DEV float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+GNN_DEV float silu(float z) { return z * sigmoid(z); }
+GNN_DEV float silu_grad(float z) {
+    const float s = sigmoid(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+GNN_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+GNN_DEV bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+GNN_DEV void st8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
+
+// ---------------------------------------------------------------------------
+// Edge features of the current state, in the edge's normal frame.
+__global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                                 const float *nx, const float *ny, const float *len,
+                                                 const uint8_t *closed, const double *w, const double *uh,
+                                                 const double *vh, const double *z, float hs, float qs, float ls,
+                                                 bf16 *X) {
+    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (e >= Epad) return;
+    float f[kFeat];
+#pragma unroll
+    for (int i = 0; i < kFeat; i++) f[i] = 0.0f;
+    if (e < E) {
+        const int a = ea[e], b = eb[e];
+        const double n1 = nx[e], n2 = ny[e];
+        const bool wall = closed[e] != 0;
+        const double ha = w[a] - z[a];
+        const double qna = uh[a] * n1 + vh[a] * n2, qta = -uh[a] * n2 + vh[a] * n1;
+        double hb = ha, qnb = wall ? -qna : qna, qtb = qta, dw = 0.0, dz = 0.0;
+        if (b >= 0) {
+            hb = w[b] - z[b];
+            qnb = uh[b] * n1 + vh[b] * n2;
+            qtb = -uh[b] * n2 + vh[b] * n1;
+            dw = w[b] - w[a];
+            dz = z[b] - z[a];
+        }
+        f[0] = float(ha / hs);
+        f[1] = float(hb / hs);
+        f[2] = float(qna / qs);
+        f[3] = float(qnb / qs);
+        f[4] = float(qta / qs);
+        f[5] = float(qtb / qs);
+        f[6] = float(dw / hs);
+        f[7] = float(dz / hs);
+        f[8] = b < 0 ? 1.0f : 0.0f;
+        f[9] = wall ? 1.0f : 0.0f;
+        f[10] = len[e] / ls;
+        f[11] = sqrtf(float(fmax(ha, 0.0) / hs));
+        f[12] = sqrtf(float(fmax(hb, 0.0) / hs));
+        f[13] = ha > 1.0e-3 ? 1.0f : 0.0f;
+        f[14] = hb > 1.0e-3 ? 1.0f : 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < kFeat / 8; c++) {
+        bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = (bf16)f[c * 8 + j];
+        st8(X + e * kFeat + c * 8, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Y = act(X W + b) (+ R): one wavefront per 16 rows; the 16 rows' A
+// fragments stay in registers across the Nout/16 output blocks.
+template <int K, int NOUT, bool OUTF32>
+__global__ __launch_bounds__(kB) void k_linear_fwd(int64_t M, const bf16 *X, const bf16 *WT, const float *bias,
+                                                   const bf16 *R, bf16 *Zs, void *Yv, int act) {
+    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int64_t m0 = wave * 16;
+    if (m0 >= M) return;
+    bf16x8 a[K / 32];
+#pragma unroll
+    for (int kk = 0; kk < K / 32; kk++) a[kk] = ld8(X + (m0 + r) * K + kk * 32 + g * 8);
+#pragma unroll
+    for (int nb = 0; nb < NOUT / 16; nb++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < K / 32; kk++) acc = mfma(a[kk], ld8(WT + (nb * 16 + r) * K + kk * 32 + g * 8), acc);
+        const int n = nb * 16 + r;
+        const float bn = bias[n];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t idx = (m0 + 4 * g + j) * NOUT + n;
+            const float zz = acc[j] + bn;
+            if (Zs) Zs[idx] = (bf16)zz;
+            float y = act ? silu(zz) : zz;
+            if (R) y += (float)R[idx];
+            if (OUTF32)
+                static_cast<float *>(Yv)[idx] = y;
+            else
+                static_cast<bf16 *>(Yv)[idx] = (bf16)y;
+        }
+    }
+}
+
+// dX = dZ W^T (W is [K][NOUT]).
+template <int K, int NOUT>
+__global__ __launch_bounds__(kB) void k_linear_bwd_x(int64_t M, const bf16 *dZ, const bf16 *W, bf16 *dX) {
+    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int64_t m0 = wave * 16;
+    if (m0 >= M) return;
+    bf16x8 a[NOUT / 32];
+#pragma unroll
+    for (int nn = 0; nn < NOUT / 32; nn++) a[nn] = ld8(dZ + (m0 + r) * NOUT + nn * 32 + g * 8);
+#pragma unroll
+    for (int kb = 0; kb < K / 16; kb++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int nn = 0; nn < NOUT / 32; nn++) acc = mfma(a[nn], ld8(W + (kb * 16 + r) * NOUT + nn * 32 + g * 8), acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) dX[(m0 + 4 * g + j) * K + kb * 16 + r] = (bf16)acc[j];
+    }
+}
+
+// Partial dW = X^T dZ and db = colsum(dZ) of one workgroup's chunks of 32 rows
+// (grid-strided), staged through LDS so both operands are read transposed.
+template <int K, int NOUT>
+__global__ __launch_bounds__(kB) void k_linear_bwd_w(int64_t M, const bf16 *X, const bf16 *dZ, float *partial) {
+    constexpr int NBLK = (K / 16) * (NOUT / 16);
+    constexpr int PER = (NBLK + 3) / 4;  // output blocks per wavefront
+    __shared__ bf16 xs[32][K + 8];
+    __shared__ bf16 ds[32][NOUT + 8];
+    const int t = threadIdx.x, wv = t >> 6, lane = t & 63, r = lane & 15, g = lane >> 4;
+    f32x4 acc[PER];
+#pragma unroll
+    for (int p = 0; p < PER; p++) acc[p] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    const int64_t nchunks = M / 32;
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const int64_t m0 = c * 32;
+        for (int i = t; i < 32 * K / 8; i += kB) {
+            const int row = i / (K / 8), col = (i % (K / 8)) * 8;
+            const bf16x8 v = ld8(X + (m0 + row) * K + col);
+#pragma unroll
+            for (int j = 0; j < 8; j++) xs[row][col + j] = v[j];
+        }
+        for (int i = t; i < 32 * NOUT / 8; i += kB) {
+            const int row = i / (NOUT / 8), col = (i % (NOUT / 8)) * 8;
+            const bf16x8 v = ld8(dZ + (m0 + row) * NOUT + col);
+#pragma unroll
+            for (int j = 0; j < 8; j++) ds[row][col + j] = v[j];
+        }
+        __syncthreads();
+        if (t < NOUT) {
+            for (int row = 0; row < 32; row++) bsum += (float)ds[row][t];
+        }
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int blk = wv + 4 * p;
+            if (blk < NBLK) {
+                const int kb = blk / (NOUT / 16), nb = blk % (NOUT / 16);
+                bf16x8 av, bv;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    av[j] = xs[g * 8 + j][kb * 16 + r];  // A[k][m] = X[m][k]
+                    bv[j] = ds[g * 8 + j][nb * 16 + r];  // B[m][n] = dZ[m][n]
+                }
+                acc[p] = mfma(av, bv, acc[p]);
+            }
+        }
+        __syncthreads();
+    }
+    float *out = partial + (int64_t)blockIdx.x * (K + 1) * NOUT;
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        const int blk = wv + 4 * p;
+        if (blk < NBLK) {
+            const int kb = blk / (NOUT / 16), nb = blk % (NOUT / 16);
+#pragma unroll
+            for (int j = 0; j < 4; j++) out[(kb * 16 + 4 * g + j) * NOUT + nb * 16 + r] = acc[p][j];
+        }
+    }
+    if (t < NOUT) out[K * NOUT + t] = bsum;
+}
+
+// Sum of the per-workgroup partials (fixed order: deterministic).
+__global__ __launch_bounds__(kB) void k_reduce_partials(int n, int nparts, const float *partial, float *dW, float *db,
+                                                        int nW) {
+    const int i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int p = 0; p < nparts; p++) s += partial[(int64_t)p * n + i];
+    if (i < nW)
+        dW[i] = s;
+    else
+        db[i - nW] = s;
+}
+
+__global__ __launch_bounds__(kB) void k_silu_bwd(int64_t n, const bf16 *dY, const bf16 *Z, bf16 *dZ) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    dZ[i] = (bf16)((float)dY[i] * silu_grad((float)Z[i]));
+}
+
+// ---------------------------------------------------------------------------
+// Message passing: node sums over a triangle's edges, edge concatenation.
+__global__ __launch_bounds__(kB) void k_node_sum(int64_t N, const int32_t *tri_edges, const bf16 *H, bf16 *S) {
+    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (t >= N * 8) return;
+    const int64_t i = t >> 3;
+    const int c = (int)(t & 7) * 8;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int e = tri_edges[i * 3 + q];
+        if (e < 0) continue;
+        const bf16x8 v = ld8(H + (int64_t)e * 64 + c);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += (float)v[j];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = (bf16)s[j];
+    st8(S + i * 64 + c, o);
+}
+
+__global__ __launch_bounds__(kB) void k_concat(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                               const bf16 *H, const bf16 *S, bf16 *C) {
+    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (t >= Epad * 16) return;
+    const int64_t e = t >> 4;
+    const int c = (int)(t & 15) * 8;
+    bf16x8 o;
+    if (e >= E) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) o[j] = (bf16)0.0f;
+    } else if (c < 64) {
+        o = ld8(H + e * 64 + c);
+    } else {
+        const int a = ea[e], b = eb[e];
+        const bf16x8 sa = ld8(S + (int64_t)a * 64 + c - 64);
+        if (b >= 0) {
+            const bf16x8 sb = ld8(S + (int64_t)b * 64 + c - 64);
+#pragma unroll
+            for (int j = 0; j < 8; j++) o[j] = (bf16)((float)sa[j] + (float)sb[j]);
+        } else {
+            o = sa;
+        }
+    }
+    st8(C + e * 128 + c, o);
+}
+
+__global__ __launch_bounds__(kB) void k_node_grad(int64_t N, const int32_t *tri_edges, const bf16 *dC, bf16 *dS) {
+    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (t >= N * 8) return;
+    const int64_t i = t >> 3;
+    const int c = (int)(t & 7) * 8;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int e = tri_edges[i * 3 + q];
+        if (e < 0) continue;
+        const bf16x8 v = ld8(dC + (int64_t)e * 128 + 64 + c);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += (float)v[j];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = (bf16)s[j];
+    st8(dS + i * 64 + c, o);
+}
+
+__global__ __launch_bounds__(kB) void k_edge_grad(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                                  const bf16 *dHn, const bf16 *dC, const bf16 *dS, bf16 *dH) {
+    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (t >= Epad * 8) return;
+    const int64_t e = t >> 3;
+    const int c = (int)(t & 7) * 8;
+    bf16x8 o;
+    if (e >= E) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) o[j] = (bf16)0.0f;
+    } else {
+        const int a = ea[e], b = eb[e];
+        const bf16x8 x0 = ld8(dHn + e * 64 + c), x1 = ld8(dC + e * 128 + c), x2 = ld8(dS + (int64_t)a * 64 + c);
+        float s[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] = (float)x0[j] + (float)x1[j] + (float)x2[j];
+        if (b >= 0) {
+            const bf16x8 x3 = ld8(dS + (int64_t)b * 64 + c);
+#pragma unroll
+            for (int j = 0; j < 8; j++) s[j] += (float)x3[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) o[j] = (bf16)s[j];
+    }
+    st8(dH + e * 64 + c, o);
+}
+
+// ---------------------------------------------------------------------------
+// State update from the head's edge outputs (FP64 accumulation), loss and its
+// gradient with respect to the state change.
+__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx,
+                                              const float *ny, const float *len, const uint8_t *closed,
+                                              const double *area, const double *delta, const int32_t *graph_of,
+                                              const float *F, float fs, const double *w0, const double *uh0,
+                                              const double *vh0, double *w1, double *uh1, double *vh1,
+                                              const double *dw_t, const double *duh_t, const double *dvh_t,
+                                              const float *sigma, float *gq, float *loss_terms) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= N) return;
+    double sh = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int e = tri_edges[i * 3 + q];
+        if (e < 0) continue;
+        const bool side_a = ea[e] == (int)i;
+        const double L = (double)len[e] * (double)fs;
+        const double n1 = nx[e], n2 = ny[e];
+        const float *f = F + (int64_t)e * kOut;
+        // the mass flux of an edge: one value, added to one side, subtracted from the other
+        const double c = closed[e] ? 0.0 : (double)f[0] * L;
+        sh += side_a ? -c : c;
+        const double mn = side_a ? f[1] : f[3], mt = side_a ? f[2] : f[4];
+        sx += (mn * n1 - mt * n2) * L;
+        sy += (mn * n2 + mt * n1) * L;
+    }
+    const double s = delta[graph_of[i]] / area[i];
+    const double dh = s * sh, dx = s * sx, dy = s * sy;
+    w1[i] = w0[i] + dh;
+    uh1[i] = uh0[i] + dx;
+    vh1[i] = vh0[i] + dy;
+    if (dw_t) {
+        const double ex = (dh - dw_t[i]) / sigma[0], ey = (dx - duh_t[i]) / sigma[1], ez = (dy - dvh_t[i]) / sigma[2];
+        loss_terms[i] = (float)(ex * ex + ey * ey + ez * ez);
+        const double k = 2.0 / (double)N;
+        gq[i * 3 + 0] = (float)(k * ex / sigma[0]);
+        gq[i * 3 + 1] = (float)(k * ey / sigma[1]);
+        gq[i * 3 + 2] = (float)(k * ez / sigma[2]);
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                                  const float *nx, const float *ny, const float *len,
+                                                  const uint8_t *closed, const double *area, const double *delta,
+                                                  const int32_t *graph_of, float fs, const float *gq, bf16 *dF) {
+    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (e >= Epad) return;
+    float d[kOut];
+#pragma unroll
+    for (int i = 0; i < kOut; i++) d[i] = 0.0f;
+    if (e < E) {
+        const int a = ea[e], b = eb[e];
+        const double L = (double)len[e] * (double)fs;
+        const double n1 = nx[e], n2 = ny[e];
+        const double sa = delta[graph_of[a]] / area[a] * L;
+        const double ga0 = gq[a * 3], ga1 = gq[a * 3 + 1], ga2 = gq[a * 3 + 2];
+        double d0 = -ga0 * sa;
+        d[1] = (float)(sa * (ga1 * n1 + ga2 * n2));
+        d[2] = (float)(sa * (-ga1 * n2 + ga2 * n1));
+        if (b >= 0) {
+            const double sb = delta[graph_of[b]] / area[b] * L;
+            const double gb0 = gq[b * 3], gb1 = gq[b * 3 + 1], gb2 = gq[b * 3 + 2];
+            d0 += gb0 * sb;
+            d[3] = (float)(sb * (gb1 * n1 + gb2 * n2));
+            d[4] = (float)(sb * (-gb1 * n2 + gb2 * n1));
+        }
+        d[0] = closed[e] ? 0.0f : (float)d0;
+    }
+#pragma unroll
+    for (int c = 0; c < kOut / 8; c++) {
+        bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = (bf16)d[c * 8 + j];
+        st8(dF + e * kOut + c * 8, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kB) void k_adam(int64_t n, float *p, const float *g, float *m, float *v, float lr,
+                                             float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ __launch_bounds__(kB) void k_cast_weights(int K, int NOUT, const float *W, bf16 *Wb, bf16 *WTb) {
+    const int i = blockIdx.x * kB + threadIdx.x;
+    if (i >= K * NOUT) return;
+    const int k = i / NOUT, n = i % NOUT;
+    const bf16 x = (bf16)W[i];
+    Wb[i] = x;
+    if (WTb) WTb[n * K + k] = x;
+}
+
+inline unsigned blocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
+inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace gnn
+
+using namespace gnn;
+
+extern "C" {
+
+int gnn_features(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
+                 const float *ny, const float *len, const uint8_t *closed, const double *w, const double *uh,
+                 const double *vh, const double *z, float depth_scale, float momentum_scale, float length_scale,
+                 void *X0) {
+    hipLaunchKernelGGL(k_features, dim3(blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb, nx, ny, len, closed,
+                       w, uh, vh, z, depth_scale, momentum_scale, length_scale, static_cast<bf16 *>(X0));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, const void *WT, const float *b,
+                   const void *R, void *Zsave, void *Y, int act, int out_f32) {
+    if (M % 16) return -1;
+    const dim3 grid(blocks(M / 16 * 64)), blk(kB);
+    const bf16 *x = static_cast<const bf16 *>(X), *wt = static_cast<const bf16 *>(WT), *r = static_cast<const bf16 *>(R);
+    bf16 *zs = static_cast<bf16 *>(Zsave);
+#define GNN_FWD(KK, NN, F32)                                                                                    \
+    if (K == KK && Nout == NN && (out_f32 != 0) == F32) {                                                      \
+        hipLaunchKernelGGL((k_linear_fwd<KK, NN, F32>), grid, blk, 0, S(stream), M, x, wt, b, r, zs, Y, act); \
+        return hipGetLastError() == hipSuccess ? 0 : -1;                                                       \
+    }
+    GNN_FWD(32, 64, false)
+    GNN_FWD(128, 64, false)
+    GNN_FWD(64, 32, true)
+    GNN_FWD(64, 64, false)
+#undef GNN_FWD
+    return -1;
+}
+
+int gnn_linear_bwd_x(void *stream, int64_t M, int K, int Nout, const void *dZ, const void *W, void *dX) {
+    if (M % 16) return -1;
+    const dim3 grid(blocks(M / 16 * 64)), blk(kB);
+    const bf16 *dz = static_cast<const bf16 *>(dZ), *w = static_cast<const bf16 *>(W);
+    bf16 *dx = static_cast<bf16 *>(dX);
+    if (K == 128 && Nout == 64) hipLaunchKernelGGL((k_linear_bwd_x<128, 64>), grid, blk, 0, S(stream), M, dz, w, dx);
+    else if (K == 64 && Nout == 32) hipLaunchKernelGGL((k_linear_bwd_x<64, 32>), grid, blk, 0, S(stream), M, dz, w, dx);
+    else if (K == 32 && Nout == 64) hipLaunchKernelGGL((k_linear_bwd_x<32, 64>), grid, blk, 0, S(stream), M, dz, w, dx);
+    else return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_linear_bwd_w(void *stream, int64_t M, int K, int Nout, const void *X, const void *dZ, float *partial,
+                     float *dW, float *db) {
+    if (M % 32) return -1;
+    const bf16 *x = static_cast<const bf16 *>(X), *dz = static_cast<const bf16 *>(dZ);
+    if (K == 128 && Nout == 64) hipLaunchKernelGGL((k_linear_bwd_w<128, 64>), dim3(kParts), dim3(kB), 0, S(stream), M, x, dz, partial);
+    else if (K == 64 && Nout == 32) hipLaunchKernelGGL((k_linear_bwd_w<64, 32>), dim3(kParts), dim3(kB), 0, S(stream), M, x, dz, partial);
+    else if (K == 32 && Nout == 64) hipLaunchKernelGGL((k_linear_bwd_w<32, 64>), dim3(kParts), dim3(kB), 0, S(stream), M, x, dz, partial);
+    else return -1;
+    const int n = (K + 1) * Nout;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(n)), dim3(kB), 0, S(stream), n, kParts, partial, dW, db, K * Nout);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_silu_bwd(void *stream, int64_t n, const void *dY, const void *Z, void *dZ) {
+    hipLaunchKernelGGL(k_silu_bwd, dim3(blocks(n)), dim3(kB), 0, S(stream), n, static_cast<const bf16 *>(dY),
+                       static_cast<const bf16 *>(Z), static_cast<bf16 *>(dZ));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_node_sum(void *stream, int64_t N, const int32_t *tri_edges, const void *H, void *Sv) {
+    hipLaunchKernelGGL(k_node_sum, dim3(blocks(N * 8)), dim3(kB), 0, S(stream), N, tri_edges,
+                       static_cast<const bf16 *>(H), static_cast<bf16 *>(Sv));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_concat(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
+               const void *Sv, void *C) {
+    hipLaunchKernelGGL(k_concat, dim3(blocks(Epad * 16)), dim3(kB), 0, S(stream), E, Epad, ea, eb,
+                       static_cast<const bf16 *>(H), static_cast<const bf16 *>(Sv), static_cast<bf16 *>(C));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_node_grad(void *stream, int64_t N, const int32_t *tri_edges, const void *dC, void *dS) {
+    hipLaunchKernelGGL(k_node_grad, dim3(blocks(N * 8)), dim3(kB), 0, S(stream), N, tri_edges,
+                       static_cast<const bf16 *>(dC), static_cast<bf16 *>(dS));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *dHnext,
+                  const void *dC, const void *dS, void *dH) {
+    hipLaunchKernelGGL(k_edge_grad, dim3(blocks(Epad * 8)), dim3(kB), 0, S(stream), E, Epad, ea, eb,
+                       static_cast<const bf16 *>(dHnext), static_cast<const bf16 *>(dC), static_cast<const bf16 *>(dS),
+                       static_cast<bf16 *>(dH));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx, const float *ny,
+              const float *len, const uint8_t *closed, const double *area, const double *delta, const int32_t *graph_of,
+              const float *F, float flux_scale, const double *w0, const double *uh0, const double *vh0, double *w1,
+              double *uh1, double *vh1, const double *dw_t, const double *duh_t, const double *dvh_t,
+              const float *sigma, float *gq, float *loss_terms) {
+    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, nx, ny, len, closed, area,
+                       delta, graph_of, F, flux_scale, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, gq,
+                       loss_terms);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
+                  const float *ny, const float *len, const uint8_t *closed, const double *area, const double *delta,
+                  const int32_t *graph_of, float flux_scale, const float *gq, void *dF) {
+    hipLaunchKernelGGL(k_apply_bwd, dim3(blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb, nx, ny, len, closed,
+                       area, delta, graph_of, flux_scale, gq, static_cast<bf16 *>(dF));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_adam(void *stream, int64_t n, float *param, const float *grad, float *m, float *v, float lr, float beta1,
+             float beta2, float eps, int step) {
+    const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(n)), dim3(kB), 0, S(stream), n, param, grad, m, v, lr, beta1, beta2, eps,
+                       c1, c2);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_cast_weights(void *stream, int K, int Nout, const float *W, void *Wb, void *WTb) {
+    hipLaunchKernelGGL(k_cast_weights, dim3(blocks((int64_t)K * Nout)), dim3(kB), 0, S(stream), K, Nout, W,
+                       static_cast<bf16 *>(Wb), static_cast<bf16 *>(WTb));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+/* anuga_gnn.h -- C ABI of the learned edge-flux surrogate's kernels (gfx950).
+ *
+ * A small edge graph network advances a shallow-water state by a time step
+ * Delta spanning several CFL steps of the finite-volume solver: per edge it
+ * predicts a mass flux, applied with opposite signs to the two triangles of
+ * the edge (so the total mass changes only through open boundary edges, by
+ * construction), and momentum contributions to each side.  Layers are dense
+ * BF16 matrices on MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation;
+ * the state update is accumulated in FP64.  These entry points have no
+ * counterpart in the reference (its solver has no learned component); they
+ * serve the learned rollout and its training (anuga_cuda_amd/learned.py).
+ *
+ * All pointers are device pointers; every launch goes to `stream` (a
+ * hipStream_t, 0 for the default stream).  Matrices are row-major.  Row
+ * counts M are multiples of 16 (callers pad), K is 32, 64 or 128, Nout is 32
+ * or 64.  Returns 0, or -1 for an unsupported shape.
+ */
+#ifndef ANUGA_GNN_H
+#define ANUGA_GNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Edge features X0[E][32] (BF16) of the current state.  Edge e joins
+ * triangles ea[e] and eb[e] (eb < 0: boundary edge, the outside state is the
+ * inside one); (nx, ny) is its unit normal from ea to eb, len its length.
+ * closed[e] != 0 marks a wall (reflected outside momentum).  The scales
+ * normalise depths, momenta and lengths.  Rows E .. Epad-1 are zero. */
+int gnn_features(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
+                 const float *ny, const float *len, const uint8_t *closed, const double *w, const double *uh,
+                 const double *vh, const double *z, float depth_scale, float momentum_scale, float length_scale,
+                 void *X0);
+
+/* Y[M][Nout] = act(X[M][K] W + b) (+ R).  WT = W transposed [Nout][K]
+ * (BF16), b FP32.  act: 0 identity, 1 SiLU.  Zsave (may be NULL) receives the
+ * pre-activation (BF16).  out_f32: Y is FP32 instead of BF16.  R (may be NULL)
+ * is a BF16 residual added after the activation. */
+int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, const void *WT, const float *b,
+                   const void *R, void *Zsave, void *Y, int act, int out_f32);
+
+/* dX[M][K] = dZ[M][Nout] W^T, with W [K][Nout] (BF16). */
+int gnn_linear_bwd_x(void *stream, int64_t M, int K, int Nout, const void *dZ, const void *W, void *dX);
+
+/* dW[K][Nout] (FP32) = X^T dZ and db[Nout] = column sums of dZ, reduced over
+ * the M rows (partial is scratch of nparts*(K+1)*Nout floats, nparts = 256). */
+int gnn_linear_bwd_w(void *stream, int64_t M, int K, int Nout, const void *X, const void *dZ, float *partial,
+                     float *dW, float *db);
+
+/* dZ = dY * SiLU'(Z), elementwise over n values (BF16). */
+int gnn_silu_bwd(void *stream, int64_t n, const void *dY, const void *Z, void *dZ);
+
+/* Node sums S[i][64] = sum over the (up to 3) edges of triangle i of H[e][64]
+ * (tri_edges[i][3], -1 for none), then C[e][128] = [H[e], S[ea] + S[eb]]. */
+int gnn_node_sum(void *stream, int64_t N, const int32_t *tri_edges, const void *H, void *S);
+int gnn_concat(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
+               const void *S, void *C);
+
+/* Backward of gnn_concat + gnn_node_sum for one layer:
+ * dS[i] = sum over edges of i of dC[e][64:128];
+ * dH[e] = dHnext[e] + dC[e][0:64] + dS[ea] + dS[eb]. */
+int gnn_node_grad(void *stream, int64_t N, const int32_t *tri_edges, const void *dC, void *dS);
+int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *dHnext,
+                  const void *dC, const void *dS, void *dH);
+
+/* Apply the head's outputs F[E][32] (FP32; columns 0: mass flux, 1-2: side-a
+ * momentum (normal, tangential), 3-4: side-b momentum) over a step delta:
+ * dq[i] = delta / area[i] * sum over edges of side-signed, rotated
+ * contributions * len * flux_scale; closed[e] != 0 zeroes the mass flux of a
+ * wall.  Writes the new state (w, uh, vh; may alias the inputs).  With
+ * targets (non-NULL dw_t, duh_t, dvh_t: the true changes), also writes the
+ * loss gradient gq[i][3] = 2 (dq - dq_true) / (sigma^2 * Ntri) and per-triangle
+ * loss terms. */
+int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx, const float *ny,
+              const float *len, const uint8_t *closed, const double *area, const double *delta, const int32_t *graph_of,
+              const float *F, float flux_scale, const double *w0, const double *uh0, const double *vh0, double *w1,
+              double *uh1, double *vh1, const double *dw_t, const double *duh_t, const double *dvh_t,
+              const float *sigma, float *gq, float *loss_terms);
+
+/* dF[Epad][32] (BF16) from gq: the transpose of gnn_apply's linear map. */
+int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
+                  const float *ny, const float *len, const uint8_t *closed, const double *area, const double *delta,
+                  const int32_t *graph_of, float flux_scale, const float *gq, void *dF);
+
+/* Adam step on n FP32 parameters, then BF16 copies in both layouts:
+ * W [K][Nout] -> Wb (same layout) and WTb ([Nout][K]); for biases pass K = 1
+ * and WTb = NULL. */
+int gnn_adam(void *stream, int64_t n, float *param, const float *grad, float *m, float *v, float lr, float beta1,
+             float beta2, float eps, int step);
+int gnn_cast_weights(void *stream, int K, int Nout, const float *W, void *Wb, void *WTb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

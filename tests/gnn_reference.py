@@ -1,0 +1,146 @@
+"""CPU (numpy, FP64) restatement of the learned edge-flux surrogate: forward
+pass, state update, loss and gradients, written independently of the HIP
+kernels so their BF16 MFMA results can be checked against it.  Test
+infrastructure only."""
+import numpy as np
+
+FEAT, HID, CAT, OUT = 32, 64, 128, 32
+
+
+def bf16(x):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    f = np.asarray(x, dtype=np.float32)
+    b = f.view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _sigmoid(z):
+    return 0.5 * (1.0 + np.tanh(0.5 * z))
+
+
+def silu(z):
+    return z * _sigmoid(z)
+
+
+def silu_grad(z):
+    s = _sigmoid(z)
+    return s * (1.0 + z * (1.0 - s))
+
+
+def features(g, w, uh, vh, z, hs, qs, ls):
+    """Edge features (as k_features), BF16-rounded."""
+    E = len(g["ea"])
+    X = np.zeros((E, FEAT))
+    for e in range(E):
+        a, b = g["ea"][e], g["eb"][e]
+        n1, n2 = float(g["nx"][e]), float(g["ny"][e])
+        wall = g["closed"][e] != 0
+        ha = w[a] - z[a]
+        qna, qta = uh[a] * n1 + vh[a] * n2, -uh[a] * n2 + vh[a] * n1
+        hb, qnb, qtb, dw, dz = ha, (-qna if wall else qna), qta, 0.0, 0.0
+        if b >= 0:
+            hb = w[b] - z[b]
+            qnb, qtb = uh[b] * n1 + vh[b] * n2, -uh[b] * n2 + vh[b] * n1
+            dw, dz = w[b] - w[a], z[b] - z[a]
+        X[e, :15] = [ha / hs, hb / hs, qna / qs, qnb / qs, qta / qs, qtb / qs, dw / hs, dz / hs,
+                     1.0 if b < 0 else 0.0, 1.0 if wall else 0.0, g["len"][e] / ls,
+                     np.sqrt(max(ha, 0.0) / hs), np.sqrt(max(hb, 0.0) / hs),
+                     1.0 if ha > 1e-3 else 0.0, 1.0 if hb > 1e-3 else 0.0]
+    return bf16(X)
+
+
+def forward(params, g, X0):
+    """Returns F and the saved activations."""
+    W = {k: bf16(v) for k, v in params["W"].items()}
+    b = params["b"]
+    Z0 = X0 @ W["enc"] + b["enc"]
+    H = {1: silu(Z0)}
+    Z = {0: Z0}
+    Cs = {}
+    for l in (1, 2, 3):
+        S = node_sum(g, H[l])
+        Cs[l] = np.concatenate([H[l], concat_nodes(g, S)], axis=1)
+        Z[l] = Cs[l] @ W["mp%d" % l] + b["mp%d" % l]
+        H[l + 1] = H[l] + silu(Z[l])
+    F = H[4] @ W["head"] + b["head"]
+    return F, dict(W=W, Z=Z, H=H, C=Cs, X0=X0)
+
+
+def node_sum(g, H):
+    S = np.zeros((len(g["tri_edges"]), H.shape[1]))
+    for i, es in enumerate(g["tri_edges"]):
+        for e in es:
+            if e >= 0:
+                S[i] += H[e]
+    return S
+
+
+def concat_nodes(g, S):
+    out = np.zeros((len(g["ea"]), S.shape[1]))
+    for e, (a, b) in enumerate(zip(g["ea"], g["eb"])):
+        out[e] = S[a] + (S[b] if b >= 0 else 0.0)
+    return out
+
+
+def apply_update(g, F, delta, fs):
+    """State change dq[i] = (dh, duh, dvh) of the learned step."""
+    N = len(g["tri_edges"])
+    dq = np.zeros((N, 3))
+    for i, es in enumerate(g["tri_edges"]):
+        for e in es:
+            if e < 0:
+                continue
+            side_a = g["ea"][e] == i
+            L = float(g["len"][e]) * fs
+            n1, n2 = float(g["nx"][e]), float(g["ny"][e])
+            c = 0.0 if g["closed"][e] else F[e, 0] * L
+            dq[i, 0] += -c if side_a else c
+            mn, mt = (F[e, 1], F[e, 2]) if side_a else (F[e, 3], F[e, 4])
+            dq[i, 1] += (mn * n1 - mt * n2) * L
+            dq[i, 2] += (mn * n2 + mt * n1) * L
+        dq[i] *= delta / g["area"][i]
+    return dq
+
+
+def loss_and_grads(params, g, X0, delta, fs, target, sigma):
+    """Loss = sum_i sum_c ((dq - target)/sigma)^2 / N and its weight gradients."""
+    F, saved = forward(params, g, X0)
+    N = len(g["tri_edges"])
+    dq = apply_update(g, F, delta, fs)
+    err = (dq - target) / sigma
+    loss = np.sum(err ** 2) / N
+    gq = 2.0 * err / sigma / N
+    # d loss / d F through the (linear) update
+    dF = np.zeros_like(F)
+    for e in range(len(g["ea"])):
+        a, b = g["ea"][e], g["eb"][e]
+        L = float(g["len"][e]) * fs
+        n1, n2 = float(g["nx"][e]), float(g["ny"][e])
+        sa = delta / g["area"][a] * L
+        d0 = -gq[a, 0] * sa
+        dF[e, 1] = sa * (gq[a, 1] * n1 + gq[a, 2] * n2)
+        dF[e, 2] = sa * (-gq[a, 1] * n2 + gq[a, 2] * n1)
+        if b >= 0:
+            sb = delta / g["area"][b] * L
+            d0 += gq[b, 0] * sb
+            dF[e, 3] = sb * (gq[b, 1] * n1 + gq[b, 2] * n2)
+            dF[e, 4] = sb * (-gq[b, 1] * n2 + gq[b, 2] * n1)
+        dF[e, 0] = 0.0 if g["closed"][e] else d0
+    W, Z, H, Cs = saved["W"], saved["Z"], saved["H"], saved["C"]
+    gW, gb = {}, {}
+    gW["head"] = H[4].T @ dF
+    gb["head"] = dF.sum(0)
+    dH = dF @ W["head"].T
+    for l in (3, 2, 1):
+        name = "mp%d" % l
+        dZ = dH * silu_grad(Z[l])
+        gW[name] = Cs[l].T @ dZ
+        gb[name] = dZ.sum(0)
+        dC = dZ @ W[name].T
+        dS = node_sum(g, dC[:, HID:])
+        dH = dH + dC[:, :HID] + concat_nodes(g, dS)
+    dZ = dH * silu_grad(Z[0])
+    gW["enc"] = X0.T @ dZ
+    gb["enc"] = dZ.sum(0)
+    return loss, F, dq, gW, gb
