@@ -92,6 +92,8 @@ _SIGNATURES = {
     "sw_set_boundary": ([C.c_void_p, C.POINTER(C.c_int32), _F64P], C.c_int),
     "sw_upload": ([C.c_void_p, C.c_int, C.c_int, _F64P, C.c_int64], C.c_int),
     "sw_download": ([C.c_void_p, C.c_int, C.c_int, _F64P, C.c_int64], C.c_int),
+    "sw_upload_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64], C.c_int),
+    "sw_download_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64], C.c_int),
     "sw_get_state": ([C.c_void_p, C.POINTER(SwState)], C.c_int),
     "sw_set_state": ([C.c_void_p, C.POINTER(SwState)], C.c_int),
     "sw_download_max_speed": ([C.c_void_p, _F64P, C.c_int64], C.c_int),
@@ -230,11 +232,15 @@ class Handle:
         if rc != 0:
             raise SwError(rc, self.lib.sw_last_error(self.h).decode())
 
-    def call(self, name, *args):
+    def select(self):
+        """Selects this view's instance (or all, for an ensemble's owner)."""
         if self.inst is not None:
             self.check(self.lib.sw_select_instance(self.h, self.inst))
         elif self.n_instances > 1:
             self.check(self.lib.sw_select_instance(self.h, -1))
+
+    def call(self, name, *args):
+        self.select()
         self.check(getattr(self.lib, name)(self.h, *args))
 
     def close(self):
@@ -273,9 +279,24 @@ class Handle:
         self.call("sw_download", int(q), int(loc), f64p(out), out.size)
         return out
 
-    def get_state(self):
+    def upload_device(self, q, loc, ptr, count):
+        """Upload from a device buffer (address ``ptr``, ``count`` doubles)."""
+        self.call("sw_upload_device", int(q), int(loc), C.c_void_p(int(ptr)), int(count))
+
+    def download_device(self, q, loc, ptr, count):
+        """Download into a device buffer (address ``ptr``, ``count`` doubles)."""
+        self.call("sw_download_device", int(q), int(loc), C.c_void_p(int(ptr)), int(count))
+
+    def get_state(self, check=True):
+        """The evolve state of the selected instance.  With ``check`` a latched
+        device error raises SwError; without, the state (its ``error`` field
+        holding the latched error bits) is returned regardless."""
         s = SwState()
-        self.call("sw_get_state", C.byref(s))
+        if check:
+            self.call("sw_get_state", C.byref(s))
+        else:
+            self.select()
+            self.lib.sw_get_state(self.h, C.byref(s))  # rc ignored: the bits are in s.error
         return s
 
     def set_state(self, s):

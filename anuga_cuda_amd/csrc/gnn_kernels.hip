@@ -78,6 +78,32 @@ __global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const 
         f[12] = sqrtf(float(fmax(hb, 0.0) / hs));
         f[13] = ha > 1.0e-3 ? 1.0f : 0.0f;
         f[14] = hb > 1.0e-3 ? 1.0f : 0.0f;
+        // First-order physics prior: the central-upwind flux of the two
+        // centroid states after hydrostatic reconstruction (depths above the
+        // higher bed), and the reconstruction's pressure corrections.
+        const double zm = b >= 0 ? fmax(z[a], z[b]) : z[a];
+        const double wa = w[a], wb = b >= 0 ? w[b] : w[a];
+        const float sa = float(fmax(0.0, wa - zm)), sb = float(fmax(0.0, wb - zm));
+        const float g = 9.8f, h0 = 1.0e-6f;
+        const float ua = float(qna) * sa / (sa * sa + h0), ub = float(qnb) * sb / (sb * sb + h0);
+        const float va = float(qta) * sa / (sa * sa + h0), vb = float(qtb) * sb / (sb * sb + h0);
+        const float ca = sqrtf(g * sa), cb = sqrtf(g * sb);
+        const float smax = fmaxf(fmaxf(ua + ca, ub + cb), 0.0f), smin = fminf(fminf(ua - ca, ub - cb), 0.0f);
+        float k0 = 0.0f, k1 = 0.0f, k2 = 0.0f;
+        if (smax - smin > 1.0e-6f) {
+            const float inv = 1.0f / (smax - smin), sm = smax * smin;
+            const float fa0 = sa * ua, fa1 = sa * ua * ua + 0.5f * g * sa * sa, fa2 = sa * ua * va;
+            const float fb0 = sb * ub, fb1 = sb * ub * ub + 0.5f * g * sb * sb, fb2 = sb * ub * vb;
+            k0 = (smax * fa0 - smin * fb0 + sm * (sb - sa)) * inv;
+            k1 = (smax * fa1 - smin * fb1 + sm * (sb * ub - sa * ua)) * inv;
+            k2 = (smax * fa2 - smin * fb2 + sm * (sb * vb - sa * va)) * inv;
+        }
+        const float ps = 0.5f * g * hs * hs;
+        f[15] = k0 / qs;
+        f[16] = k1 / ps;
+        f[17] = k2 / qs;
+        f[18] = 0.5f * g * (float(ha * ha) - sa * sa) / ps;
+        f[19] = 0.5f * g * (float(hb * hb) - sb * sb) / ps;
     }
 #pragma unroll
     for (int c = 0; c < kFeat / 8; c++) {
@@ -327,7 +353,7 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
                                               const float *F, float fs, const double *w0, const double *uh0,
                                               const double *vh0, double *w1, double *uh1, double *vh1,
                                               const double *dw_t, const double *duh_t, const double *dvh_t,
-                                              const float *sigma, float *gq, float *loss_terms) {
+                                              const float *sigma, double area_ref, float *gq, float *loss_terms) {
     const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (i >= N) return;
     double sh = 0.0, sx = 0.0, sy = 0.0;
@@ -352,12 +378,22 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
     uh1[i] = uh0[i] + dx;
     vh1[i] = vh0[i] + dy;
     if (dw_t) {
-        const double ex = (dh - dw_t[i]) / sigma[0], ey = (dx - duh_t[i]) / sigma[1], ez = (dy - dvh_t[i]) / sigma[2];
-        loss_terms[i] = (float)(ex * ex + ey * ey + ez * ez);
-        const double k = 2.0 / (double)N;
-        gq[i * 3 + 0] = (float)(k * ex / sigma[0]);
-        gq[i * 3 + 1] = (float)(k * ey / sigma[1]);
-        gq[i * 3 + 2] = (float)(k * ez / sigma[2]);
+        // Area-weighted (the error of the volume and momentum a triangle
+        // gains) Huber loss of the normalised error: quadratic up to kHuber
+        // standard deviations, linear beyond, so the few violently changing
+        // triangles of a trajectory do not dominate.
+        constexpr double kHuber = 3.0;
+        const double wgt = area[i] / area_ref;
+        const double e[3] = {(dh - dw_t[i]) / sigma[0], (dx - duh_t[i]) / sigma[1], (dy - dvh_t[i]) / sigma[2]};
+        double l = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double ae = fabs(e[c]);
+            l += ae <= kHuber ? e[c] * e[c] : kHuber * (2.0 * ae - kHuber);
+            const double d = ae <= kHuber ? 2.0 * e[c] : 2.0 * kHuber * (e[c] > 0 ? 1.0 : -1.0);
+            gq[i * 3 + c] = (float)(wgt * d / ((double)N * sigma[c]));
+        }
+        loss_terms[i] = (float)(wgt * l);
     }
 }
 
@@ -518,10 +554,10 @@ int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *
               const float *len, const uint8_t *closed, const double *area, const double *delta, const int32_t *graph_of,
               const float *F, float flux_scale, const double *w0, const double *uh0, const double *vh0, double *w1,
               double *uh1, double *vh1, const double *dw_t, const double *duh_t, const double *dvh_t,
-              const float *sigma, float *gq, float *loss_terms) {
+              const float *sigma, double area_ref, float *gq, float *loss_terms) {
     hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, nx, ny, len, closed, area,
-                       delta, graph_of, F, flux_scale, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, gq,
-                       loss_terms);
+                       delta, graph_of, F, flux_scale, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, area_ref,
+                       gq, loss_terms);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

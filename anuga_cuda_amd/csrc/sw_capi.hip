@@ -822,6 +822,54 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
     return SW_OK;
 }
 
+// Device-memory forms of sw_upload / sw_download: the caller's array (same
+// layout and order as for the host forms) lives in device memory.
+int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsrc, int64_t count) {
+    if (!h || !dsrc) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
+    Loc L;
+    if (!locate(h, i, quantity, location, &L))
+        return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
+    if (count != L.count)
+        return fail(h, SW_ERR_ARG, "upload of %lld values into an array of %lld", (long long)count, (long long)L.count);
+    if (quantity == SW_ELEVATION)
+        return fail(h, SW_ERR_ARG, "elevation is uploaded from host memory (sw_upload)");
+    if (L.count == 0) return SW_OK;
+    if (!L.per_triangle) {
+        HIPCHECK(h, hipMemcpyAsync(L.ptr, dsrc, L.count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                           (long long)h->N, (long long)I.off, L.slots, L.comp, h->cal_of + I.off, dsrc, L.ptr);
+        if (int rc = launch_check(h, "k_to_device")) return rc;
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return SW_OK;
+}
+
+int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, int64_t count) {
+    if (!h || !ddst) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
+    Loc L;
+    if (!locate(h, i, quantity, location, &L))
+        return fail(h, SW_ERR_ARG, "no array for quantity %d location %d", quantity, location);
+    if (count != L.count)
+        return fail(h, SW_ERR_ARG, "download of %lld values from an array of %lld", (long long)count, (long long)L.count);
+    if (L.count == 0) return SW_OK;
+    if (!L.per_triangle) {
+        HIPCHECK(h, hipMemcpyAsync(ddst, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                           (long long)h->N, (long long)I.off, L.slots, L.comp, h->dev_of + I.poff, L.ptr, ddst);
+        if (int rc = launch_check(h, "k_to_host")) return rc;
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return SW_OK;
+}
+
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
     const int i = selected(h);

@@ -1,7 +1,8 @@
 """Learned edge-flux surrogate of the shallow-water step.
 
-An edge graph network advances a shallow-water state by ``stride`` CFL steps
-of the finite-volume (FV) solver at once.  Per edge it predicts a mass flux,
+An edge graph network advances a shallow-water state by a time step delta
+that spans many CFL steps of the finite-volume (FV) solver (whose global
+timestep is set by the mesh's worst triangle).  Per edge it predicts a mass flux,
 added to one side of the edge and subtracted from the other (so mass changes
 only through open boundary edges, by construction), and momentum
 contributions to each side.  Architecture (per edge, widths as in the
@@ -9,7 +10,10 @@ configuration of the learned workload):
 
     X0  = features(state)                  32   (depths, normal / tangential
                                                  momenta of both sides, stage
-                                                 and bed jumps, flags, length)
+                                                 and bed jumps, flags, length,
+                                                 first-order central-upwind
+                                                 flux after hydrostatic
+                                                 reconstruction)
     H1  = SiLU(X0 W0 + b0)                 64
     for l = 1, 2, 3 (message passing):
         S_i  = sum of H_l over the edges of triangle i
@@ -59,7 +63,7 @@ def _load():
         _gnn.gnn_concat.argtypes = [p, i64, i64, p, p, p, p, p]
         _gnn.gnn_node_grad.argtypes = [p, i64, p, p, p]
         _gnn.gnn_edge_grad.argtypes = [p, i64, i64, p, p, p, p, p, p]
-        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 9 + [p, f] + [p] * 12
+        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 9 + [p, f] + [p] * 10 + [C.c_double, p, p]
         _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 9 + [f, p, p]
         _gnn.gnn_adam.argtypes = [p, i64, p, p, p, p, f, f, f, f, i32]
         _gnn.gnn_cast_weights.argtypes = [p, i32, i32, p, p, p]
@@ -133,6 +137,7 @@ class EdgeGraph:
         self.area = torch.tensor(np.tile(mesh.areas, B), dtype=torch.float64, device=dev)
         self.graph_of = torch.tensor(np.repeat(np.arange(B), N), dtype=torch.int32, device=dev)
         self.mean_len = float(np.mean(ln))
+        self.mean_area = float(np.mean(mesh.areas))
 
 
 # ---------------------------------------------------------------------------
@@ -233,22 +238,23 @@ class Workspace:
         self.gb = {n: torch.empty(Nn, dtype=torch.float32, device="cuda") for n, K, Nn in LAYERS}
 
 
-def forward(net, graph, ws, w, uh, vh, z):
-    """Edge outputs F (FP32 [Epad][32]) of the state (FP64 device tensors)."""
+def forward(net, graph, ws, w, uh, vh, z, save=True):
+    """Edge outputs F (FP32 [Epad][32]) of the state (FP64 device tensors).
+    save=False (inference) skips the pre-activations the backward pass needs."""
     L, s = net.lib, net.stream()
     g = graph
     _ck(L.gnn_features(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.len), _ptr(g.closed),
                        _ptr(w), _ptr(uh), _ptr(vh), _ptr(z), net.scales["depth"], net.scales["momentum"],
                        net.scales["length"], _ptr(ws.X0)), "features")
     _ck(L.gnn_linear_fwd(s, g.Epad, FEAT, HID, _ptr(ws.X0), _ptr(net.WTb["enc"]), _ptr(net.b["enc"]), None,
-                         _ptr(ws.Z[0]), _ptr(ws.H[1]), 1, 0), "encoder")
+                         _ptr(ws.Z[0]) if save else None, _ptr(ws.H[1]), 1, 0), "encoder")
     for l in (1, 2, 3):
         name = "mp%d" % l
         _ck(L.gnn_node_sum(s, g.N, _ptr(g.tri_edges), _ptr(ws.H[l]), _ptr(ws.S)), "node sum")
         _ck(L.gnn_concat(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(ws.H[l]), _ptr(ws.S), _ptr(ws.Cc[l])),
             "concat")
         _ck(L.gnn_linear_fwd(s, g.Epad, CAT, HID, _ptr(ws.Cc[l]), _ptr(net.WTb[name]), _ptr(net.b[name]),
-                             _ptr(ws.H[l]), _ptr(ws.Z[l]), _ptr(ws.H[l + 1]), 1, 0), name)
+                             _ptr(ws.H[l]), _ptr(ws.Z[l]) if save else None, _ptr(ws.H[l + 1]), 1, 0), name)
     _ck(L.gnn_linear_fwd(s, g.Epad, HID, OUT, _ptr(ws.H[4]), _ptr(net.WTb["head"]), _ptr(net.b["head"]), None, None,
                          _ptr(ws.F), 0, 1), "head")
     return ws.F
@@ -262,7 +268,7 @@ def apply(net, graph, ws, delta, w0, uh0, vh0, w1, uh1, vh1, targets=None):
     _ck(L.gnn_apply(s, g.N, _ptr(g.tri_edges), _ptr(g.ea), _ptr(g.nx), _ptr(g.ny), _ptr(g.len), _ptr(g.closed),
                     _ptr(g.area), _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(w0), _ptr(uh0),
                     _ptr(vh0), _ptr(w1), _ptr(uh1), _ptr(vh1), _ptr(dw), _ptr(duh), _ptr(dvh), _ptr(net.sigma),
-                    _ptr(ws.gq), _ptr(ws.loss_terms)), "apply")
+                    graph.mean_area, _ptr(ws.gq), _ptr(ws.loss_terms)), "apply")
 
 
 def backward(net, graph, ws, delta):
@@ -302,3 +308,389 @@ def adam_step(net, ws, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
             _ck(L.gnn_adam(s, p.numel(), _ptr(p), _ptr(gr), _ptr(net.m[key]), _ptr(net.v[key]), lr, beta1, beta2,
                            eps, net.step_count), "adam")
     net.cast()
+
+
+# ---------------------------------------------------------------------------
+# FV trajectories, training, rollout
+def training_domain(mesh, seed, extent, base_stage=5.0, friction=0.025):
+    """A configuration-2-style scenario on ``mesh`` (a window of side
+    ``extent`` with the workload's point density): seeded fBm terrain of
+    amplitude 20 m, stage max(z, base_stage) plus a seeded circular mound
+    (height 1..4 m, radius 0.1..0.35 extent), Manning n, transmissive
+    boundaries."""
+    from .domain import Domain, Transmissive_boundary
+    from .scenarios import fbm
+
+    rng = np.random.default_rng(seed)
+    cx, cy = rng.uniform(0.3, 0.7, 2) * extent
+    rad = rng.uniform(0.1, 0.35) * extent
+    height = rng.uniform(1.0, 4.0)
+    d = Domain(mesh=mesh)
+
+    def bed(x, y):
+        return fbm(x, y, seed=int(seed), base_wavelength=5000.0)
+
+    def stage(x, y):
+        r2 = (x - cx) ** 2 + (y - cy) ** 2
+        return np.maximum(bed(x, y), base_stage) + np.where(r2 < rad * rad, height, 0.0)
+
+    d.set_quantity("elevation", bed)
+    d.set_quantity("stage", stage)
+    d.set_quantity("friction", friction, location="centroids")
+    d.set_boundary({tag: Transmissive_boundary(d) for tag in mesh.get_tag_names()})
+    return d
+
+
+def fv_trajectories(meshes, per_mesh, samples, delta, warmup=2, extent=1400.0, seed=0, log=None):
+    """FV trajectories for training: for each mesh, ``per_mesh`` scenarios run
+    together as one ensemble with yields every ``delta`` seconds; after
+    ``warmup`` yields each records ``samples + 1`` states, one per yield.
+    Trajectories with non-finite or implausible values (depth > 100 m,
+    |momentum| > 200 m^2/s) are dropped.  Returns one dict per mesh: z [T, N],
+    states [T, samples + 1, 3, N] (stage, xmomentum, ymomentum; FP64, on the
+    device), delta [T]."""
+    torch = _torch()
+    from . import _lib
+    from .ensemble import Ensemble
+
+    data = []
+    for mi, mesh in enumerate(meshes):
+        N = mesh.number_of_elements
+        doms = [training_domain(mesh, seed * 100000 + mi * 1000 + t, extent) for t in range(per_mesh)]
+        ens = Ensemble(doms)
+        z = torch.empty(per_mesh, N, dtype=torch.float64, device="cuda")
+        states = torch.empty(per_mesh, samples + 1, 3, N, dtype=torch.float64, device="cuda")
+        ok = True
+        try:
+            ens.prepare(delta, None)
+            for _ in range(warmup):
+                ens.evolve_steps(1 << 30)
+                ens.handle.call("sw_advance_yield")
+            torch.cuda.synchronize()
+            for t, d in enumerate(doms):
+                d._handle.download_device(_lib.SW_ELEVATION, _lib.SW_CENTROID, z[t].data_ptr(), N)
+            for s in range(samples + 1):
+                for t, d in enumerate(doms):
+                    for c, q in enumerate((_lib.SW_STAGE, _lib.SW_XMOM, _lib.SW_YMOM)):
+                        d._handle.download_device(q, _lib.SW_CENTROID, states[t, s, c].data_ptr(), N)
+                if s == samples:
+                    break
+                ens.evolve_steps(1 << 30)
+                ens.handle.call("sw_advance_yield")
+        except _lib.SwError:
+            ok = False  # an unstable member stops its ensemble: this mesh's batch is dropped
+        ens.close()
+        if not ok:
+            if log:
+                log("mesh %d: dropped (a member became unstable)" % mi)
+            continue
+        h = states[:, :, 0, :] - z[:, None, :]
+        good = (torch.isfinite(states).flatten(1).all(1) & (h.flatten(1).amax(1) < 100.0)
+                & (states[:, :, 1:, :].abs().flatten(1).amax(1) < 200.0))
+        keep = torch.nonzero(good).flatten()
+        if keep.numel() == 0:
+            continue
+        data.append(dict(z=z[keep].contiguous(), states=states[keep].contiguous(),
+                         delta=torch.full((keep.numel(),), float(delta), dtype=torch.float64, device="cuda"),
+                         mesh=mesh))
+        if log:
+            log("mesh %d: %d triangles, %d of %d trajectories kept x %d samples" % (
+                mi, N, keep.numel(), per_mesh, samples))
+    return data
+
+
+def fit_scales(net, data):
+    """Feature scales, flux scale and loss normalisation from the data."""
+    torch = _torch()
+    h, q, dq = [], [], []
+    for d in data:
+        st = d["states"]
+        h.append((st[:, :, 0, :] - d["z"][:, None, :]).clamp(min=0).flatten())
+        q.append(st[:, :, 1:, :].flatten())
+        dq.append((st[:, 1:] - st[:, :-1]).transpose(0, 2).reshape(3, -1))
+    h = torch.cat(h)
+    q = torch.cat(q)
+    dq = torch.cat(dq, dim=1)
+    hs = float(torch.sqrt(torch.mean(h * h)))
+    qs = float(torch.sqrt(torch.mean(q * q)))
+    net.scales.update(depth=max(hs, 1e-3), momentum=max(qs, 1e-3), length=float(np.mean(
+        [float(np.mean(d["mesh"].edgelengths)) for d in data])), flux=max(qs, 1e-3))
+    sig = torch.sqrt(torch.mean(dq * dq, dim=1)).float().clamp(min=1e-12)
+    net.sigma = sig.contiguous()
+
+
+class Trainer:
+    """Adam on minibatches of B samples of one mesh (its graph replicated B times)."""
+
+    def __init__(self, net, data, batch=16, lr=1e-3, seed=0):
+        torch = _torch()
+        self.net, self.data, self.B, self.lr = net, data, batch, lr
+        self.rng = np.random.default_rng(seed)
+        self.graphs = [EdgeGraph(d["mesh"], copies=batch, closed_tags=()) for d in data]
+        self.ws = [Workspace(g) for g in self.graphs]
+        self.buf = [dict((k, torch.empty(g.N, dtype=torch.float64, device="cuda"))
+                         for k in ("w", "uh", "vh", "z", "dw", "duh", "dvh", "w1", "uh1", "vh1"))
+                    for g in self.graphs]
+        self.delta = [torch.empty(batch, dtype=torch.float64, device="cuda") for _ in self.graphs]
+
+    def _fill(self, m, pick):
+        d, b = self.data[m], self.buf[m]
+        T, S1 = d["states"].shape[:2]
+        tr = torch_index(pick[:, 0])
+        sm = torch_index(pick[:, 1])
+        s0 = d["states"][tr, sm]          # [B, 3, N]
+        s1 = d["states"][tr, sm + 1]
+        b["w"].copy_(s0[:, 0].reshape(-1))
+        b["uh"].copy_(s0[:, 1].reshape(-1))
+        b["vh"].copy_(s0[:, 2].reshape(-1))
+        b["z"].copy_(d["z"][tr].reshape(-1))
+        b["dw"].copy_((s1[:, 0] - s0[:, 0]).reshape(-1))
+        b["duh"].copy_((s1[:, 1] - s0[:, 1]).reshape(-1))
+        b["dvh"].copy_((s1[:, 2] - s0[:, 2]).reshape(-1))
+        self.delta[m].copy_(d["delta"][tr])
+
+    def loss(self, m, pick, train):
+        g, ws, b = self.graphs[m], self.ws[m], self.buf[m]
+        self._fill(m, pick)
+        forward(self.net, g, ws, b["w"], b["uh"], b["vh"], b["z"])
+        apply(self.net, g, ws, self.delta[m], b["w"], b["uh"], b["vh"], b["w1"], b["uh1"], b["vh1"],
+              targets=(b["dw"], b["duh"], b["dvh"]))
+        if train:
+            backward(self.net, g, ws, self.delta[m])
+            adam_step(self.net, ws, lr=self.lr)
+        return float(ws.loss_terms.sum()) / g.N
+
+    def sample(self, m, holdout=False):
+        d = self.data[m]
+        T, S1 = d["states"].shape[:2]
+        S = S1 - 1
+        # the last sample of every trajectory is held out for evaluation
+        if holdout:
+            tr = self.rng.integers(0, T, self.B)
+            return np.stack([tr, np.full(self.B, S - 1)], 1)
+        tr = self.rng.integers(0, T, self.B)
+        sm = self.rng.integers(0, max(S - 1, 1), self.B)
+        return np.stack([tr, sm], 1)
+
+    def train(self, iters, log=None, every=100, cosine=True):
+        """``iters`` Adam steps; the learning rate decays from lr to lr / 10
+        along a half cosine."""
+        hist = []
+        lr0 = self.lr
+        for it in range(iters):
+            if cosine:
+                self.lr = lr0 * (0.1 + 0.9 * 0.5 * (1.0 + np.cos(np.pi * it / max(iters, 1))))
+            m = int(self.rng.integers(0, len(self.data)))
+            l = self.loss(m, self.sample(m), train=True)
+            hist.append(l)
+            if log and (it % every == 0 or it == iters - 1):
+                log("iter %d loss %.4g (mean of last %d: %.4g)" % (it, l, min(it + 1, every),
+                                                                   float(np.mean(hist[-every:]))))
+        self.lr = lr0
+        return hist
+
+    def evaluate(self, batches=8):
+        """Mean held-out loss (1.0 = no better than leaving the state unchanged
+        for unit-variance changes)."""
+        ls = []
+        for _ in range(batches):
+            m = int(self.rng.integers(0, len(self.data)))
+            ls.append(self.loss(m, self.sample(m, holdout=True), train=False))
+        return float(np.mean(ls))
+
+
+def torch_index(a):
+    import torch
+
+    return torch.as_tensor(np.asarray(a, dtype=np.int64), device="cuda")
+
+
+def learned_step(net, graph, ws, delta, w, uh, vh, z):
+    """One learned step in place (inference)."""
+    forward(net, graph, ws, w, uh, vh, z, save=False)
+    apply(net, graph, ws, delta, w, uh, vh, w, uh, vh)
+
+
+def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, log=None):
+    """Learned rollout against the FP64 FV path on ``domain`` (an FV Domain,
+    initial state set), in steps of ``delta`` seconds.
+
+    1. Reference: the FV loop with yields every delta, ``steps`` yields; the
+       stage at every yield is kept (device copies) and the FV loop is timed.
+    2. Learned: from the same initial state, ``steps`` learned steps.  At the
+       end of every window of ``judge_every`` steps the FV loop advances the
+       state of the window's last step from the same starting point (the
+       judge) and its result is kept.  If the learned step's stage L2 error
+       against it (relative to the depth norm) exceeds ``threshold``, or the
+       judge fails, the whole window is recomputed with the FV loop from the
+       window's start and the FV loop also runs the next window (fallback).
+    Returns timings, per-step stage errors against the reference, the mass
+    change against the reference and the judge's record."""
+    import time
+
+    torch = _torch()
+    from . import _lib
+
+    N = domain.number_of_elements
+    h = domain._dev()
+    h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    dev = lambda: torch.empty(N, dtype=torch.float64, device="cuda")
+    w0, uh0, vh0, z = dev(), dev(), dev(), dev()
+    for q, buf in ((_lib.SW_STAGE, w0), (_lib.SW_XMOM, uh0), (_lib.SW_YMOM, vh0), (_lib.SW_ELEVATION, z)):
+        h.download_device(q, _lib.SW_CENTROID, buf.data_ptr(), N)
+    t0 = h.get_state().time
+    area = torch.tensor(domain.mesh.areas, dtype=torch.float64, device="cuda")
+
+    def wl2(x):
+        """Area-weighted L2 norm over the domain (sqrt of the integral of x^2)."""
+        return float(torch.sqrt(torch.sum(area * x * x)))
+
+    def set_state(w, uh, vh, t):
+        torch.cuda.synchronize()
+        for q, buf in ((_lib.SW_STAGE, w), (_lib.SW_XMOM, uh), (_lib.SW_YMOM, vh)):
+            h.upload_device(q, _lib.SW_CENTROID, buf.data_ptr(), N)
+        s = h.get_state(check=False)  # a failed judge leaves its error latched
+        s.time = t
+        s.starttime = t
+        s.error = 0
+        s.event = 0
+        h.set_state(s)
+        h.call("sw_evolve_prepare", delta, 0.0, 0)
+
+    def fv_advance(w, uh, vh, t, nyields):
+        """FV over nyields yields of delta from (w, uh, vh) at time t; result in place."""
+        set_state(w, uh, vh, t)
+        total = 0
+        for k in range(nyields):
+            done, ev = h.evolve_steps(1 << 30)
+            total += done
+            if k + 1 < nyields:
+                h.call("sw_advance_yield")
+        for q, buf in ((_lib.SW_STAGE, w), (_lib.SW_XMOM, uh), (_lib.SW_YMOM, vh)):
+            h.download_device(q, _lib.SW_CENTROID, buf.data_ptr(), N)
+        return total
+
+    # 1. reference
+    ref_w = torch.empty(steps, N, dtype=torch.float64, device="cuda")
+    set_state(w0, uh0, vh0, t0)
+    torch.cuda.synchronize()
+    tic = time.perf_counter()
+    fv_steps = 0
+    for k in range(steps):
+        done, ev = h.evolve_steps(1 << 30)
+        fv_steps += done
+        h.download_device(_lib.SW_STAGE, _lib.SW_CENTROID, ref_w[k].data_ptr(), N)
+        if k + 1 < steps:
+            h.call("sw_advance_yield")
+    h.call("sw_synchronize")
+    fv_time = time.perf_counter() - tic
+    ref_mass = float(torch.sum(area * (ref_w[-1] - z)))
+    mass0 = float(torch.sum(area * (w0 - z)))
+    if log:
+        log("FV reference: %d yields of %.4g s = %d FV steps in %.3f s (%.3e triangle-steps/s)" % (
+            steps, delta, fv_steps, fv_time, N * fv_steps / fv_time))
+
+    # 2. learned rollout with the FV judge
+    graph = EdgeGraph(domain.mesh, copies=1, closed_tags=())
+    ws = Workspace(graph)
+    dl = torch.tensor([delta], dtype=torch.float64, device="cuda")
+    w, uh, vh = w0.clone(), uh0.clone(), vh0.clone()
+    errors, judged = [], []
+    fv_windows = 0
+    fv_in_rollout = 0
+    fallback = False
+    torch.cuda.synchronize()
+    tic = time.perf_counter()
+    k = 0
+    while k < steps:
+        n = min(judge_every, steps - k)
+        start = (w.clone(), uh.clone(), vh.clone())
+        tw = t0 + k * delta
+        if fallback:
+            fv_in_rollout += fv_advance(w, uh, vh, tw, n)
+            fv_windows += 1
+            fallback = False
+        else:
+            for j in range(n - 1):
+                learned_step(net, graph, ws, dl, w, uh, vh, z)
+            last = (w.clone(), uh.clone(), vh.clone())
+            learned_step(net, graph, ws, dl, w, uh, vh, z)
+            try:
+                fv_in_rollout += fv_advance(last[0], last[1], last[2], tw + (n - 1) * delta, 1)
+                err = wl2(w - last[0]) / wl2(last[0] - z)
+            except _lib.SwError:
+                err = float("inf")
+            judged.append((k + n - 1, err))
+            if err > threshold:
+                w, uh, vh = start
+                fv_in_rollout += fv_advance(w, uh, vh, tw, n)
+                fv_windows += 1
+                fallback = True
+            else:
+                w, uh, vh = last
+        k += n
+        errors.append((k - 1, w - ref_w[k - 1]))
+    torch.cuda.synchronize()
+    learned_time = time.perf_counter() - tic
+    err_l2 = [(i, wl2(e) / wl2(ref_w[i] - z)) for i, e in errors]
+    mass_final = float(torch.sum(area * (w - z)))
+    out = dict(triangles=N, steps=steps, delta=delta, fv_steps=fv_steps, fv_seconds=fv_time,
+               fv_rate=N * fv_steps / fv_time, learned_seconds=learned_time,
+               learned_rate=N * fv_steps / learned_time, speedup=fv_time / learned_time,
+               stage_rel_l2=err_l2, max_stage_rel_l2=max(e for _, e in err_l2), final_stage_rel_l2=err_l2[-1][1],
+               mass_change_vs_fv=abs(mass_final - ref_mass) / abs(mass0), judged=judged,
+               fallback_windows=fv_windows, fv_steps_in_rollout=fv_in_rollout)
+    if log:
+        log("learned: %d steps in %.3f s (%.3e FV-equivalent triangle-steps/s, %.2fx FV); stage rel L2 at window "
+            "ends max %.3g final %.3g; %d windows judged, %d recomputed by FV" % (
+                steps, learned_time, out["learned_rate"], out["speedup"], out["max_stage_rel_l2"],
+                out["final_stage_rel_l2"], len(judged), fv_windows))
+    return out
+
+
+def training_meshes(count, extent=1400.0, density=5000.0, seed=7):
+    """Seeded windows of the configuration-2 mesh family: Delaunay meshes of
+    uniform random points (``density`` points per km^2, as 500k points in
+    10 km x 10 km) on squares of side ``extent``."""
+    from .mesh import Mesh, delaunay_random
+
+    npts = int(density * (extent / 1000.0) ** 2)
+    return [Mesh(*delaunay_random(npts, extent=(extent, extent), seed=seed * 1000 + i)) for i in range(count)]
+
+
+def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.05, iters=3000, batch=16, lr=1e-3,
+                         eval_points=500000, eval_steps=100, judge_every=8, threshold=0.01, extent=1400.0, log=None):
+    """Train on FV trajectories of configuration-2-style windows, then roll the
+    network out on the configuration-2 workload (1M triangles) against the
+    FP64 FV path.  Returns a dict of the results."""
+    import time
+
+    from . import scenarios
+
+    t0 = time.time()
+    meshes = training_meshes(train_meshes, extent=extent)
+    data = fv_trajectories(meshes, per_mesh=per_mesh, samples=samples, delta=delta, extent=extent, log=None)
+    n_traj = sum(d["states"].shape[0] for d in data)
+    t_data = time.time() - t0
+    if log:
+        log("training data: %d trajectories x %d samples on %d meshes of %d triangles (%.1f s)" % (
+            n_traj, samples, len(data), meshes[0].number_of_elements, t_data))
+    net = EdgeGNN(seed=1)
+    fit_scales(net, data)
+    tr = Trainer(net, data, batch=batch, lr=lr)
+    untrained = tr.evaluate(16)
+    t0 = time.time()
+    hist = tr.train(iters, log=log, every=max(iters // 6, 1))
+    _torch().cuda.synchronize()
+    t_train = time.time() - t0
+    trained = tr.evaluate(16)
+    if log:
+        log("held-out one-step loss: untrained %.4f, trained %.4f (%d iterations, %.1f s)" % (
+            untrained, trained, iters, t_train))
+    dom = scenarios.delaunay_terrain(npoints=eval_points)
+    res = evaluate_rollout(net, dom, eval_steps, delta, judge_every=judge_every, threshold=threshold, log=log)
+    dom.close()
+    res.update(train_trajectories=n_traj, train_samples=samples, train_iters=iters, train_seconds=t_train,
+               data_seconds=t_data, holdout_loss_untrained=untrained, holdout_loss_trained=trained,
+               train_loss_last=float(np.mean(hist[-50:])))
+    return res

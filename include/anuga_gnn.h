@@ -71,13 +71,14 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
  * contributions * len * flux_scale; closed[e] != 0 zeroes the mass flux of a
  * wall.  Writes the new state (w, uh, vh; may alias the inputs).  With
  * targets (non-NULL dw_t, duh_t, dvh_t: the true changes), also writes the
- * loss gradient gq[i][3] = 2 (dq - dq_true) / (sigma^2 * Ntri) and per-triangle
- * loss terms. */
+ * area-weighted loss terms (area[i] / area_ref) * sum over components of
+ * huber((dq - dq_true) / sigma) (quadratic up to 3, linear beyond) and their
+ * gradient gq[i][3] with respect to dq, for the loss sum(terms) / Ntri. */
 int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx, const float *ny,
               const float *len, const uint8_t *closed, const double *area, const double *delta, const int32_t *graph_of,
               const float *F, float flux_scale, const double *w0, const double *uh0, const double *vh0, double *w1,
               double *uh1, double *vh1, const double *dw_t, const double *duh_t, const double *dvh_t,
-              const float *sigma, float *gq, float *loss_terms);
+              const float *sigma, double area_ref, float *gq, float *loss_terms);
 
 /* dF[Epad][32] (BF16) from gq: the transpose of gnn_apply's linear map. */
 int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,

@@ -130,6 +130,12 @@ int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet);
  * (src/utilities/utility.py:134-243). */
 int sw_upload(sw_handle *h, int quantity, int location, const double *host, int64_t count);
 int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t count);
+/* The same transfers with the caller's array in device memory (same layout
+ * and order; elevation only through sw_upload).  For callers that keep their
+ * own device copies of the state, such as the learned surrogate's rollout;
+ * no counterpart in the reference. */
+int sw_upload_device(sw_handle *h, int quantity, int location, const double *device_src, int64_t count);
+int sw_download_device(sw_handle *h, int quantity, int location, double *device_dst, int64_t count);
 int sw_get_state(sw_handle *h, sw_state *state);
 int sw_set_state(sw_handle *h, const sw_state *state);
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count);

@@ -47,6 +47,24 @@ def features(g, w, uh, vh, z, hs, qs, ls):
                      1.0 if b < 0 else 0.0, 1.0 if wall else 0.0, g["len"][e] / ls,
                      np.sqrt(max(ha, 0.0) / hs), np.sqrt(max(hb, 0.0) / hs),
                      1.0 if ha > 1e-3 else 0.0, 1.0 if hb > 1e-3 else 0.0]
+        # physics prior: first-order central-upwind flux after hydrostatic reconstruction
+        zm = max(z[a], z[b]) if b >= 0 else z[a]
+        wb = w[b] if b >= 0 else w[a]
+        sa, sb = max(0.0, w[a] - zm), max(0.0, wb - zm)
+        G, h0 = 9.8, 1e-6
+        ua, ub = qna * sa / (sa * sa + h0), qnb * sb / (sb * sb + h0)
+        va, vb = qta * sa / (sa * sa + h0), qtb * sb / (sb * sb + h0)
+        ca, cb = np.sqrt(G * sa), np.sqrt(G * sb)
+        smax, smin = max(ua + ca, ub + cb, 0.0), min(ua - ca, ub - cb, 0.0)
+        k = [0.0, 0.0, 0.0]
+        if smax - smin > 1e-6:
+            fa = [sa * ua, sa * ua * ua + 0.5 * G * sa * sa, sa * ua * va]
+            fb = [sb * ub, sb * ub * ub + 0.5 * G * sb * sb, sb * ub * vb]
+            du = [sb - sa, sb * ub - sa * ua, sb * vb - sa * va]
+            k = [(smax * fa[i] - smin * fb[i] + smax * smin * du[i]) / (smax - smin) for i in range(3)]
+        ps = 0.5 * G * hs * hs
+        X[e, 15:20] = [k[0] / qs, k[1] / ps, k[2] / qs, 0.5 * G * (ha * ha - sa * sa) / ps,
+                       0.5 * G * (hb * hb - sb * sb) / ps]
     return bf16(X)
 
 
@@ -104,13 +122,17 @@ def apply_update(g, F, delta, fs):
 
 
 def loss_and_grads(params, g, X0, delta, fs, target, sigma):
-    """Loss = sum_i sum_c ((dq - target)/sigma)^2 / N and its weight gradients."""
+    """Loss = sum_i (A_i / mean A) sum_c huber((dq - target)/sigma) / N (Huber
+    with threshold 3) and its weight gradients."""
     F, saved = forward(params, g, X0)
     N = len(g["tri_edges"])
     dq = apply_update(g, F, delta, fs)
+    wgt = (g["area"] / np.mean(g["area"]))[:, None]
     err = (dq - target) / sigma
-    loss = np.sum(err ** 2) / N
-    gq = 2.0 * err / sigma / N
+    k = 3.0
+    ae = np.abs(err)
+    loss = np.sum(wgt * np.where(ae <= k, err ** 2, k * (2 * ae - k))) / N
+    gq = wgt * np.where(ae <= k, 2.0 * err, 2.0 * k * np.sign(err)) / sigma / N
     # d loss / d F through the (linear) update
     dF = np.zeros_like(F)
     for e in range(len(g["ea"])):

@@ -53,6 +53,7 @@ def test_gnn_forward_and_gradients_match_cpu_reference():
     tw, tuh, tvh, tz = dev(w), dev(uh), dev(vh), dev(z)
     delta = dev([0.7])
     target = rng.normal(0, 0.05, (graph.N, 3))
+    target[::7] *= 40.0  # some targets beyond the Huber threshold
     tt = [dev(target[:, c]) for c in range(3)]
     ws = L.Workspace(graph)
     L.forward(net, graph, ws, tw, tuh, tvh, tz)
@@ -65,9 +66,11 @@ def test_gnn_forward_and_gradients_match_cpu_reference():
     params = dict(W={n: net.W[n].cpu().numpy().astype(np.float64) for n, _, _ in L.LAYERS},
                   b={n: net.b[n].cpu().numpy().astype(np.float64) for n, _, _ in L.LAYERS})
     X0 = ref.features(g, w, uh, vh, z, 2.0, 1.0, 10.0)
-    X0_gpu = ws.X0[:graph.E].float().cpu().numpy()
-    assert np.array_equal(X0_gpu, X0), "features differ"
-    loss, F, dq, gW, gb = ref.loss_and_grads(params, g, X0, 0.7, 0.5, target, np.array([0.05, 0.05, 0.05]))
+    X0_gpu = ws.X0[:graph.E].float().cpu().numpy().astype(np.float64)
+    # FP32 arithmetic of the flux prior may round to the neighbouring BF16 value
+    assert np.max(np.abs(X0_gpu - X0) / np.maximum(np.abs(X0), 1e-3)) < 1e-2, "features differ"
+    # the reference network runs on the device's features
+    loss, F, dq, gW, gb = ref.loss_and_grads(params, g, X0_gpu, 0.7, 0.5, target, np.array([0.05, 0.05, 0.05]))
 
     def rel(a, b):
         return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
@@ -112,3 +115,50 @@ def test_gnn_rollout_conserves_mass_in_a_closed_basin():
     print("mass %.17g -> %.17g, relative change %.2e; mass moved %.3e" % (m0, m1, abs(m1 - m0) / m0, moved))
     assert moved > 1e-3 * m0  # the network does move water around
     assert abs(m1 - m0) <= 1e-12 * m0
+
+
+def test_training_on_fv_trajectories_beats_the_untrained_network():
+    """Trained on FV trajectories (two seeded configuration-2-style windows,
+    8 scenarios each, 6 steps of 0.05 s) for 400 Adam steps by the HIP
+    kernels, the network's held-out one-step loss (normalised state change
+    error) is below half that of the untrained network, which leaves the state
+    unchanged (loss of doing nothing)."""
+    from anuga_cuda_amd import learned as L
+
+    meshes = L.training_meshes(2, extent=400.0, seed=3)
+    data = L.fv_trajectories(meshes, per_mesh=8, samples=6, delta=0.05, extent=400.0, seed=1)
+    assert len(data) == 2
+    net = L.EdgeGNN(seed=1)
+    L.fit_scales(net, data)
+    tr = L.Trainer(net, data, batch=8, lr=1e-3, seed=0)
+    before = tr.evaluate(8)
+    tr.train(400)
+    after = tr.evaluate(8)
+    print("held-out loss: untrained %.4f trained %.4f" % (before, after))
+    assert after < 0.5 * before
+
+
+def test_rollout_judge_and_fallback():
+    """The judged rollout on a small configuration-2-style domain: with a
+    threshold of 0 every window is recomputed by the FV loop, so the rollout
+    follows the uninterrupted FV reference up to the effect of restarting the
+    FV loop from centroid values (fresh extrapolation, stale vertex values of
+    dry cells, a new timestep sequence): area-weighted relative stage error
+    below 1e-6; with an infinite threshold the network runs every step and its
+    error is reported."""
+    from anuga_cuda_amd import learned as L, scenarios
+
+    meshes = L.training_meshes(1, extent=400.0, seed=4)
+    data = L.fv_trajectories(meshes, per_mesh=8, samples=4, delta=0.05, extent=400.0, seed=2)
+    net = L.EdgeGNN(seed=2)
+    L.fit_scales(net, data)
+    L.Trainer(net, data, batch=8, lr=1e-3).train(100)
+    dom = scenarios.delaunay_terrain(npoints=3000, extent=2000.0, mound_radius=200.0, seed=7, terrain_seed=11)
+    fv_only = L.evaluate_rollout(net, dom, 8, 0.05, judge_every=4, threshold=0.0)
+    assert fv_only["fallback_windows"] == 2
+    assert fv_only["final_stage_rel_l2"] < 1e-6, fv_only["final_stage_rel_l2"]
+    dom2 = scenarios.delaunay_terrain(npoints=3000, extent=2000.0, mound_radius=200.0, seed=7, terrain_seed=11)
+    free = L.evaluate_rollout(net, dom2, 8, 0.05, judge_every=4, threshold=float("inf"))
+    assert free["fallback_windows"] == 0 and len(free["judged"]) == 2
+    assert np.isfinite(free["final_stage_rel_l2"])
+    print("free rollout stage rel L2 %.3g, speed %.2fx FV" % (free["final_stage_rel_l2"], free["speedup"]))
