@@ -521,9 +521,12 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
        end of every window of ``judge_every`` steps the FV loop advances the
        state of the window's last step from the same starting point (the
        judge) and its result is kept.  If the learned step's stage L2 error
-       against it (relative to the depth norm) exceeds ``threshold``, or the
-       judge fails, the whole window is recomputed with the FV loop from the
-       window's start and the FV loop also runs the next window (fallback).
+       against it (area-weighted, relative to the depth norm) exceeds
+       ``threshold``, the judge fails or needs more than twice the reference's
+       mean FV steps per step (a damaged state: spurious speeds), the whole
+       window is recomputed with the FV loop from the window's start and the
+       FV loop also runs the next window (fallback).  Should the FV loop fail
+       from a window start, it restarts from the window start before.
     Returns timings, per-step stage errors against the reference, the mass
     change against the reference and the judge's record."""
     import time
@@ -540,6 +543,14 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
         h.download_device(q, _lib.SW_CENTROID, buf.data_ptr(), N)
     t0 = h.get_state().time
     area = torch.tensor(domain.mesh.areas, dtype=torch.float64, device="cuda")
+    # Vertex values of the initial state.  The reconstruction keeps the vertex
+    # values of cells in dry neighbourhoods from the previous step, so every
+    # restart puts these back first: a failed FV run must not leave its
+    # values there, and cells that stay dry then match the reference run.
+    vert0 = {}
+    for q in (_lib.SW_STAGE, _lib.SW_XMOM, _lib.SW_YMOM):
+        vert0[q] = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+        h.download_device(q, _lib.SW_VERTEX, vert0[q].data_ptr(), 3 * N)
 
     def wl2(x):
         """Area-weighted L2 norm over the domain (sqrt of the integral of x^2)."""
@@ -547,6 +558,8 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
 
     def set_state(w, uh, vh, t):
         torch.cuda.synchronize()
+        for q, buf in vert0.items():
+            h.upload_device(q, _lib.SW_VERTEX, buf.data_ptr(), 3 * N)
         for q, buf in ((_lib.SW_STAGE, w), (_lib.SW_XMOM, uh), (_lib.SW_YMOM, vh)):
             h.upload_device(q, _lib.SW_CENTROID, buf.data_ptr(), N)
         s = h.get_state(check=False)  # a failed judge leaves its error latched
@@ -599,15 +612,34 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
     fv_windows = 0
     fv_in_rollout = 0
     fallback = False
+    health_steps = 2.0 * fv_steps / steps  # a judge needing more FV steps than this saw a damaged state
+    history = [(0, (w0, uh0, vh0))]  # window starts, oldest first
     torch.cuda.synchronize()
     tic = time.perf_counter()
     k = 0
+
+    def recompute(k_end):
+        """FV from the latest window start that the FV loop can advance to
+        k_end (older starts are tried if a newer one fails); returns its
+        state at k_end and the FV steps taken."""
+        nonlocal history
+        while history:
+            k0, st = history[-1]
+            cur = (st[0].clone(), st[1].clone(), st[2].clone())
+            try:
+                return cur, fv_advance(cur[0], cur[1], cur[2], t0 + k0 * delta, k_end - k0)
+            except _lib.SwError:
+                if len(history) == 1:
+                    raise
+                history = history[:-1]
+        raise RuntimeError("no window start")
+
     while k < steps:
         n = min(judge_every, steps - k)
-        start = (w.clone(), uh.clone(), vh.clone())
         tw = t0 + k * delta
         if fallback:
-            fv_in_rollout += fv_advance(w, uh, vh, tw, n)
+            (w, uh, vh), done = recompute(k + n)
+            fv_in_rollout += done
             fv_windows += 1
             fallback = False
         else:
@@ -616,19 +648,26 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
             last = (w.clone(), uh.clone(), vh.clone())
             learned_step(net, graph, ws, dl, w, uh, vh, z)
             try:
-                fv_in_rollout += fv_advance(last[0], last[1], last[2], tw + (n - 1) * delta, 1)
+                done = fv_advance(last[0], last[1], last[2], tw + (n - 1) * delta, 1)
+                fv_in_rollout += done
                 err = wl2(w - last[0]) / wl2(last[0] - z)
+                if done > health_steps:
+                    err = float("inf")
             except _lib.SwError:
-                err = float("inf")
+                done, err = -1, float("inf")
             judged.append((k + n - 1, err))
+            if log:
+                log("window ending at step %d: judge error %.3g (%d FV steps) -> %s" % (
+                    k + n - 1, err, done, "recomputed by FV" if err > threshold else "accepted"))
             if err > threshold:
-                w, uh, vh = start
-                fv_in_rollout += fv_advance(w, uh, vh, tw, n)
+                (w, uh, vh), done = recompute(k + n)
+                fv_in_rollout += done
                 fv_windows += 1
                 fallback = True
             else:
                 w, uh, vh = last
         k += n
+        history.append((k, (w.clone(), uh.clone(), vh.clone())))
         errors.append((k - 1, w - ref_w[k - 1]))
     torch.cuda.synchronize()
     learned_time = time.perf_counter() - tic
