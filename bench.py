@@ -45,9 +45,23 @@ KERNEL_BYTES_PER_TRIANGLE = {
     # next step's gravity term 16
     "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 96 + 16,
 }
+# The mixed-precision loop (FP32 edge state and geometry, FP64 centroids).
+KERNEL_BYTES_PER_TRIANGLE_MIXED = {
+    # reads: FP32 edge geometry records 3x16 48, FP32 edge state records 3x16
+    # 48, {radius, area} 16, gravity term 16 | writes: explicit updates 24,
+    # max_speed 8
+    "flux": 48 + 48 + 16 + 16 + 24 + 8,
+    "update": KERNEL_BYTES_PER_TRIANGLE["update"],
+    # as FP64, but the edge state written is the FP32 records 3x16 48
+    "distribute": 32 + 16 + 12 + 4 + 48 + 24 + 72 + 48 + 16,
+}
 # One Euler step = the three kernels above (the timestep reduction moves 8
 # bytes per 64 triangles).
 STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
+
+
+def kernel_bytes(precision):
+    return KERNEL_BYTES_PER_TRIANGLE if precision == "fp64" else KERNEL_BYTES_PER_TRIANGLE_MIXED
 KERNELS = ("flux", "timestep", "update", "distribute")
 
 
@@ -56,10 +70,14 @@ def workload_tag(args):
     return tag if args.locality == "hilbert" else tag + "_keep_order"
 
 
-def kernel_symbol(kernel, precision):
+def kernel_symbol(kernel, precision, launch):
+    """Name of a fused kernel as rocprofv3 reports it; ``launch`` = workgroup
+    sizes of (flux, update, distribute)."""
     r = "double" if precision == "fp64" else "float"
-    return {"flux": "k_flux<%s, true>" % r, "timestep": "k_timestep<true>", "update": "k_update<true>",
-            "distribute": "k_distribute<%s, true>" % r}[kernel]
+    fb, ub, db = launch
+    flux = "k_flux<%s, true, %d>" % (r, fb) if precision == "fp64" else "k_flux_mixed<%d>" % fb
+    return {"flux": flux, "timestep": "k_timestep<true>", "update": "k_update<true, %d>" % ub,
+            "distribute": "k_distribute<%s, true, %d>" % (r, db)}[kernel]
 
 
 def log(msg):
@@ -155,16 +173,18 @@ def main():
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
     dominant = max(("flux", "update", "distribute"), key=lambda k: avg_us[k] or 0.0)
-    bytes_launch = KERNEL_BYTES_PER_TRIANGLE[dominant] * N
+    kb = kernel_bytes(args.precision)
+    step_bytes = sum(kb.values())
+    bytes_launch = kb[dominant] * N
     achieved = bytes_launch / (avg_us[dominant] * 1e-6) / 1e9
-    step_gbs = STEP_BYTES_PER_TRIANGLE * N * args.steps / elapsed / 1e9
+    step_gbs = step_bytes * N * args.steps / elapsed / 1e9
     # HBM bytes per launch of the dominant kernel, measured by rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated; written by
     # anuga_cuda_amd.rocprof_summary), if such a profile is present.
     traffic = None
     tfile = args.traffic_json or os.path.join(ROOT, "profiles", "%s.traffic.json" % workload_tag(args))
     if os.path.exists(tfile):
-        kname = kernel_symbol(dominant, args.precision)
+        kname = kernel_symbol(dominant, args.precision, launch)
         rec = json.load(open(tfile)).get("kernels", {}).get(kname, {})
         traffic = rec.get("traffic_bytes")
 
@@ -210,9 +230,9 @@ def main():
                    "device_numbering": args.locality, "workgroup_sizes": list(launch)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_triangle": KERNEL_BYTES_PER_TRIANGLE[dominant],
+                     "bytes_per_triangle": kb[dominant],
                      "avg_launch_us": avg_us[dominant]},
-        "step_roofline": {"bytes_per_triangle_step": STEP_BYTES_PER_TRIANGLE, "achieved": step_gbs,
+        "step_roofline": {"bytes_per_triangle_step": step_bytes, "achieved": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS, "kernel_avg_us": avg_us},
         "cpu_baseline": cpu,
     }
@@ -259,7 +279,7 @@ def bench_ensemble(args):
     ms, cnt = h.kernel_times(4)
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
-    step_bytes = STEP_BYTES_PER_TRIANGLE if args.precision == "fp64" else None
+    step_bytes = sum(kernel_bytes(args.precision).values())
     out = {
         "metric": "triangle-steps/s at 1M tris, FV vs learned at stated error; % HBM peak",
         "value": value,
