@@ -347,32 +347,66 @@ __global__ __launch_bounds__(kB) void k_edge_grad(int64_t E, int64_t Epad, const
 // ---------------------------------------------------------------------------
 // State update from the head's edge outputs (FP64 accumulation), loss and its
 // gradient with respect to the state change.
-__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx,
-                                              const float *ny, const float *len, const uint8_t *closed,
-                                              const double *area, const double *delta, const int32_t *graph_of,
-                                              const float *F, float fs, const double *w0, const double *uh0,
+// Contributions of the three edges of triangle m to its integrated change:
+// mass (side-signed edge flux), x and y momentum (rotated side outputs).
+// With a group, the mass flux of an edge whose other side is in the group is
+// left out (it would cancel in the group's sum).
+// theta (may be NULL) scales the mass flux of each edge by the limiter factor
+// of the cell it drains (k_limit).
+__device__ __forceinline__ void edge_sums(int m, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
+                                          const float *nx, const float *ny, const float *len, const uint8_t *closed,
+                                          const float *F, float fs, const double *theta, const int *mem, double &sh,
+                                          double &sx, double &sy) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int e = tri_edges[(int64_t)m * 3 + q];
+        if (e < 0) continue;
+        const bool side_a = ea[e] == m;
+        const double L = (double)len[e] * (double)fs;
+        const double n1 = nx[e], n2 = ny[e];
+        const float *f = F + (int64_t)e * kOut;
+        // the mass flux of an edge: one value, added to one side, subtracted from the other
+        const int other = side_a ? eb[e] : ea[e];
+        const bool internal = mem && other >= 0 &&
+                              (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3]);
+        double c = (closed[e] || internal) ? 0.0 : (double)f[0] * L;
+        if (theta) {
+            const int donor = c > 0.0 ? ea[e] : eb[e];
+            if (donor >= 0) c *= theta[donor];
+        }
+        sh += side_a ? -c : c;
+        const double mn = side_a ? f[1] : f[3], mt = side_a ? f[2] : f[4];
+        sx += (mn * n1 - mt * n2) * L;
+        sy += (mn * n2 + mt * n1) * L;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
+                                              const float *nx, const float *ny, const float *len,
+                                              const uint8_t *closed, const double *area, const int32_t *group,
+                                              const double *group_area, const double *delta, const int32_t *graph_of,
+                                              const float *F, float fs, const double *theta, const double *w0,
+                                              const double *uh0,
                                               const double *vh0, double *w1, double *uh1, double *vh1,
                                               const double *dw_t, const double *duh_t, const double *dvh_t,
                                               const float *sigma, double area_ref, float *gq, float *loss_terms) {
     const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (i >= N) return;
     double sh = 0.0, sx = 0.0, sy = 0.0;
+    double A = area[i];
+    if (group && group[i * 4 + 1] >= 0) {
+        // a merged group of small cells: every member takes the group's
+        // change, summed over the members in the same order
+        const int mem[4] = {group[i * 4], group[i * 4 + 1], group[i * 4 + 2], group[i * 4 + 3]};
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int e = tri_edges[i * 3 + q];
-        if (e < 0) continue;
-        const bool side_a = ea[e] == (int)i;
-        const double L = (double)len[e] * (double)fs;
-        const double n1 = nx[e], n2 = ny[e];
-        const float *f = F + (int64_t)e * kOut;
-        // the mass flux of an edge: one value, added to one side, subtracted from the other
-        const double c = closed[e] ? 0.0 : (double)f[0] * L;
-        sh += side_a ? -c : c;
-        const double mn = side_a ? f[1] : f[3], mt = side_a ? f[2] : f[4];
-        sx += (mn * n1 - mt * n2) * L;
-        sy += (mn * n2 + mt * n1) * L;
+        for (int j = 0; j < 4; j++)
+            if (mem[j] >= 0)
+                edge_sums(mem[j], tri_edges, ea, eb, nx, ny, len, closed, F, fs, theta, mem, sh, sx, sy);
+        A = group_area[i];
+    } else {
+        edge_sums((int)i, tri_edges, ea, eb, nx, ny, len, closed, F, fs, theta, nullptr, sh, sx, sy);
     }
-    const double s = delta[graph_of[i]] / area[i];
+    const double s = delta[graph_of[i]] / A;
     const double dh = s * sh, dx = s * sx, dy = s * sy;
     w1[i] = w0[i] + dh;
     uh1[i] = uh0[i] + dx;
@@ -397,9 +431,80 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
     }
 }
 
+// Positivity limiter of the mass fluxes (the draining-time idea of
+// Bollermann, Chen, Kurganov and Noelle, J. Sci. Comput. 56, 2013): the water
+// a cell (or merged group) can give over the step is its volume; if the
+// outgoing mass fluxes would take more, all of them are scaled by
+// theta = volume / outflow.  Each edge takes the factor of the cell it drains,
+// on both sides, so mass stays conserved and no cell's volume goes below 0.
+__global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edges, const int32_t *ea,
+                                              const int32_t *eb, const float *len, const uint8_t *closed,
+                                              const double *area, const int32_t *group, const double *delta,
+                                              const int32_t *graph_of, const float *F, float fs, const double *w,
+                                              const double *z, double *theta) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= N) return;
+    int mem[4] = {(int)i, -1, -1, -1};
+    if (group && group[i * 4 + 1] >= 0)
+        for (int j = 0; j < 4; j++) mem[j] = group[i * 4 + j];
+    double vol = 0.0, out = 0.0;
+    for (int j = 0; j < 4; j++) {
+        const int m = mem[j];
+        if (m < 0) continue;
+        vol += area[m] * fmax(w[m] - z[m], 0.0);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int e = tri_edges[(int64_t)m * 3 + q];
+            if (e < 0 || closed[e]) continue;
+            const bool side_a = ea[e] == m;
+            const int other = side_a ? eb[e] : ea[e];
+            if (other >= 0 && (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3])) continue;
+            const double c = (double)F[(int64_t)e * kOut] * (double)len[e] * (double)fs;
+            out += side_a ? fmax(c, 0.0) : fmax(-c, 0.0);
+        }
+    }
+    out *= delta[graph_of[i]];
+    theta[i] = out > vol ? vol / out : 1.0;
+}
+
+// Momentum of cells shallower than h_dry set to zero (as the solver's protect).
+__global__ __launch_bounds__(kB) void k_protect_dry(int64_t N, const double *w, const double *z, double *uh,
+                                                    double *vh, double h_dry) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= N) return;
+    if (w[i] - z[i] < h_dry) {
+        uh[i] = 0.0;
+        vh[i] = 0.0;
+    }
+}
+
+// Gradient with respect to the integrated change of triangle a (scaled by
+// 1 / area of its group): the sum of the group members' gq.
+__device__ __forceinline__ void group_grad(int a, const int32_t *group, const double *group_area, const double *area,
+                                           const float *gq, double &g0, double &g1, double &g2, double &A) {
+    if (group && group[(int64_t)a * 4 + 1] >= 0) {
+        g0 = g1 = g2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int m = group[(int64_t)a * 4 + j];
+            if (m < 0) continue;
+            g0 += gq[(int64_t)m * 3];
+            g1 += gq[(int64_t)m * 3 + 1];
+            g2 += gq[(int64_t)m * 3 + 2];
+        }
+        A = group_area[a];
+    } else {
+        g0 = gq[(int64_t)a * 3];
+        g1 = gq[(int64_t)a * 3 + 1];
+        g2 = gq[(int64_t)a * 3 + 2];
+        A = area[a];
+    }
+}
+
 __global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
                                                   const float *nx, const float *ny, const float *len,
-                                                  const uint8_t *closed, const double *area, const double *delta,
+                                                  const uint8_t *closed, const double *area, const int32_t *group,
+                                                  const double *group_area, const double *delta,
                                                   const int32_t *graph_of, float fs, const float *gq, bf16 *dF) {
     const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (e >= Epad) return;
@@ -410,18 +515,23 @@ __global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const
         const int a = ea[e], b = eb[e];
         const double L = (double)len[e] * (double)fs;
         const double n1 = nx[e], n2 = ny[e];
-        const double sa = delta[graph_of[a]] / area[a] * L;
-        const double ga0 = gq[a * 3], ga1 = gq[a * 3 + 1], ga2 = gq[a * 3 + 2];
+        double ga0, ga1, ga2, Aa;
+        group_grad(a, group, group_area, area, gq, ga0, ga1, ga2, Aa);
+        const double sa = delta[graph_of[a]] / Aa * L;
         double d0 = -ga0 * sa;
         d[1] = (float)(sa * (ga1 * n1 + ga2 * n2));
         d[2] = (float)(sa * (-ga1 * n2 + ga2 * n1));
+        bool internal = false;
         if (b >= 0) {
-            const double sb = delta[graph_of[b]] / area[b] * L;
-            const double gb0 = gq[b * 3], gb1 = gq[b * 3 + 1], gb2 = gq[b * 3 + 2];
+            double gb0, gb1, gb2, Ab;
+            group_grad(b, group, group_area, area, gq, gb0, gb1, gb2, Ab);
+            internal = group && group[(int64_t)a * 4] == group[(int64_t)b * 4];
+            const double sb = delta[graph_of[b]] / Ab * L;
             d0 += gb0 * sb;
             d[3] = (float)(sb * (gb1 * n1 + gb2 * n2));
             d[4] = (float)(sb * (-gb1 * n2 + gb2 * n1));
         }
+        if (internal) d0 = 0.0;
         d[0] = closed[e] ? 0.0f : (float)d0;
     }
 #pragma unroll
@@ -550,22 +660,40 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx, const float *ny,
-              const float *len, const uint8_t *closed, const double *area, const double *delta, const int32_t *graph_of,
-              const float *F, float flux_scale, const double *w0, const double *uh0, const double *vh0, double *w1,
-              double *uh1, double *vh1, const double *dw_t, const double *duh_t, const double *dvh_t,
-              const float *sigma, double area_ref, float *gq, float *loss_terms) {
-    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, nx, ny, len, closed, area,
-                       delta, graph_of, F, flux_scale, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, area_ref,
-                       gq, loss_terms);
+int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
+              const float *nx, const float *ny, const float *len, const uint8_t *closed, const double *area,
+              const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
+              const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
+              const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
+              const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms) {
+    if (group && !group_area) return -1;
+    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, eb, nx, ny, len, closed,
+                       area, group, group_area, delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1,
+                       dw_t, duh_t, dvh_t, sigma, area_ref, gq, loss_terms);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
+              const float *len, const uint8_t *closed, const double *area, const int32_t *group, const double *delta,
+              const int32_t *graph_of, const float *F, float flux_scale, const double *w, const double *z,
+              double *theta) {
+    hipLaunchKernelGGL(k_limit, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, eb, len, closed, area,
+                       group, delta, graph_of, F, flux_scale, w, z, theta);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_protect(void *stream, int64_t N, const double *w, const double *z, double *uh, double *vh, double h_dry) {
+    hipLaunchKernelGGL(k_protect_dry, dim3(blocks(N)), dim3(kB), 0, S(stream), N, w, z, uh, vh, h_dry);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
-                  const float *ny, const float *len, const uint8_t *closed, const double *area, const double *delta,
-                  const int32_t *graph_of, float flux_scale, const float *gq, void *dF) {
+                  const float *ny, const float *len, const uint8_t *closed, const double *area, const int32_t *group,
+                  const double *group_area, const double *delta, const int32_t *graph_of, float flux_scale,
+                  const float *gq, void *dF) {
+    if (group && !group_area) return -1;
     hipLaunchKernelGGL(k_apply_bwd, dim3(blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb, nx, ny, len, closed,
-                       area, delta, graph_of, flux_scale, gq, static_cast<bf16 *>(dF));
+                       area, group, group_area, delta, graph_of, flux_scale, gq, static_cast<bf16 *>(dF));
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

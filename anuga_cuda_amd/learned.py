@@ -51,7 +51,7 @@ def _load():
         _gnn = C.CDLL(GNN_LIB_PATH)
         for name in ("gnn_features", "gnn_linear_fwd", "gnn_linear_bwd_x", "gnn_linear_bwd_w", "gnn_silu_bwd",
                      "gnn_node_sum", "gnn_concat", "gnn_node_grad", "gnn_edge_grad", "gnn_apply", "gnn_apply_bwd",
-                     "gnn_adam", "gnn_cast_weights"):
+                     "gnn_limit", "gnn_protect", "gnn_adam", "gnn_cast_weights"):
             getattr(_gnn, name).restype = C.c_int
         f, i64, i32, p = C.c_float, C.c_int64, C.c_int, C.c_void_p
         _gnn.gnn_features.argtypes = [p, i64, i64] + [p] * 10 + [f, f, f, p]
@@ -63,8 +63,10 @@ def _load():
         _gnn.gnn_concat.argtypes = [p, i64, i64, p, p, p, p, p]
         _gnn.gnn_node_grad.argtypes = [p, i64, p, p, p]
         _gnn.gnn_edge_grad.argtypes = [p, i64, i64, p, p, p, p, p, p]
-        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 9 + [p, f] + [p] * 10 + [C.c_double, p, p]
-        _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 9 + [f, p, p]
+        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 12 + [p, f] + [p] * 11 + [C.c_double, p, p]
+        _gnn.gnn_limit.argtypes = [p, i64] + [p] * 10 + [f, p, p, p]
+        _gnn.gnn_protect.argtypes = [p, i64, p, p, p, p, C.c_double]
+        _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 11 + [f, p, p]
         _gnn.gnn_adam.argtypes = [p, i64, p, p, p, p, f, f, f, f, i32]
         _gnn.gnn_cast_weights.argtypes = [p, i32, i32, p, p, p]
     return _gnn
@@ -88,11 +90,82 @@ def _ck(rc, what):
 
 
 # ---------------------------------------------------------------------------
+def small_cell_groups(mesh, merge_radius, max_size=4):
+    """Agglomerates small cells for the learned update: a triangle whose
+    inscribed-circle radius 2A/P is below ``merge_radius`` joins the group
+    across its longest edge, and a group whose radius 2A/P (outer perimeter)
+    is still below it joins the group across its longest outer edge, up to
+    ``max_size`` members.  Sliver triangles of unstructured meshes would
+    otherwise see a change many times their content over one long step.
+    Returns (members [N, 4] int32, sorted, -1 padded; group area [N]) or
+    (None, None) if no triangle is merged."""
+    N = mesh.number_of_elements
+    A = np.asarray(mesh.areas, dtype=np.float64)
+    Lg = np.asarray(mesh.edgelengths, dtype=np.float64)
+    nbr = np.asarray(mesh.neighbours)
+    P = Lg.sum(1)
+    small = np.nonzero(2.0 * A / P < merge_radius)[0]
+    if merge_radius <= 0.0 or small.size == 0:
+        return None, None
+    small = small[np.argsort(2.0 * A[small] / P[small])]
+    root = np.arange(N)
+    members = {}
+    gA, gP = A.copy(), P.copy()
+
+    def find(i):
+        while root[i] != i:
+            root[i] = root[root[i]]
+            i = root[i]
+        return i
+
+    for _ in range(3):
+        changed = False
+        for i in small:
+            gi = find(i)
+            if 2.0 * gA[gi] / gP[gi] >= merge_radius:
+                continue
+            mi = members.get(gi, [gi])
+            best, best_len = -1, -1.0
+            for m in mi:
+                for q in range(3):
+                    j = nbr[m, q]
+                    if j < 0:
+                        continue
+                    gj = find(j)
+                    if gj == gi or len(mi) + len(members.get(gj, [gj])) > max_size:
+                        continue
+                    if Lg[m, q] > best_len:
+                        best, best_len = gj, Lg[m, q]
+            if best < 0:
+                continue
+            shared = sum(Lg[m, q] for m in mi for q in range(3) if nbr[m, q] >= 0 and find(nbr[m, q]) == best)
+            mj = members.pop(best, [best])
+            members.pop(gi, None)
+            r, o = (gi, best) if gi < best else (best, gi)
+            root[o] = r
+            gA[r] = gA[gi] + gA[best]
+            gP[r] = gP[gi] + gP[best] - 2.0 * shared
+            members[r] = sorted(mi + mj)
+            changed = True
+        if not changed:
+            break
+    grp = np.tile(np.arange(N, dtype=np.int32)[:, None], (1, 4))
+    grp[:, 1:] = -1
+    garea = A.copy()
+    for r, mem in members.items():
+        row = np.full(4, -1, dtype=np.int32)
+        row[:len(mem)] = mem
+        grp[mem] = row
+        garea[mem] = gA[r]
+    return grp, garea
+
+
 class EdgeGraph:
     """Edges (interior once, boundary edges too) of ``copies`` copies of a mesh,
-    and each triangle's edges, on the device."""
+    and each triangle's edges, on the device.  ``merge_radius`` > 0 merges
+    small cells for the update (small_cell_groups)."""
 
-    def __init__(self, mesh, copies=1, closed_tags=("exterior",)):
+    def __init__(self, mesh, copies=1, closed_tags=("exterior",), merge_radius=0.0):
         torch = _torch()
         N = mesh.number_of_elements
         nbr = mesh.neighbours
@@ -138,6 +211,14 @@ class EdgeGraph:
         self.graph_of = torch.tensor(np.repeat(np.arange(B), N), dtype=torch.int32, device=dev)
         self.mean_len = float(np.mean(ln))
         self.mean_area = float(np.mean(mesh.areas))
+        grp, garea = small_cell_groups(mesh, merge_radius)
+        self.group = self.group_area = None
+        self.merged = 0
+        if grp is not None:
+            self.merged = int(np.sum(grp[:, 1] >= 0))
+            g_all = np.where(grp[None, :, :] >= 0, grp[None, :, :] + offs_n[:, :, None], -1).reshape(-1, 4)
+            self.group = torch.tensor(g_all, dtype=torch.int32, device=dev)
+            self.group_area = torch.tensor(np.tile(garea, B), dtype=torch.float64, device=dev)
 
 
 # ---------------------------------------------------------------------------
@@ -145,12 +226,14 @@ class EdgeGNN:
     """The network (FP32 master weights, BF16 copies) and its scales."""
 
     def __init__(self, seed=0, depth_scale=5.0, momentum_scale=5.0, length_scale=10.0, flux_scale=1.0,
-                 sigma=(1e-2, 1e-2, 1e-2)):
+                 sigma=(1e-2, 1e-2, 1e-2), merge_radius=0.0):
         torch = _torch()
         self.lib = _load()
         g = np.random.default_rng(seed)
+        # merge_radius: the small-cell agglomeration radius of the graphs the
+        # network is trained and run on (small_cell_groups)
         self.scales = dict(depth=float(depth_scale), momentum=float(momentum_scale), length=float(length_scale),
-                           flux=float(flux_scale))
+                           flux=float(flux_scale), merge_radius=float(merge_radius))
         self.sigma = torch.tensor(np.asarray(sigma, dtype=np.float32), device="cuda")
         self.W, self.b = {}, {}
         for name, K, Nn in LAYERS:
@@ -234,6 +317,7 @@ class Workspace:
         self.partial = torch.empty(256 * (CAT + 1) * HID, dtype=torch.float32, device="cuda")
         self.gq = torch.empty(N, 3, dtype=torch.float32, device="cuda")
         self.loss_terms = torch.empty(N, dtype=torch.float32, device="cuda")
+        self.theta = torch.empty(N, dtype=torch.float64, device="cuda")
         self.gW = {n: torch.empty(K, Nn, dtype=torch.float32, device="cuda") for n, K, Nn in LAYERS}
         self.gb = {n: torch.empty(Nn, dtype=torch.float32, device="cuda") for n, K, Nn in LAYERS}
 
@@ -260,13 +344,15 @@ def forward(net, graph, ws, w, uh, vh, z, save=True):
     return ws.F
 
 
-def apply(net, graph, ws, delta, w0, uh0, vh0, w1, uh1, vh1, targets=None):
+def apply(net, graph, ws, delta, w0, uh0, vh0, w1, uh1, vh1, targets=None, theta=None):
     """State after one learned step (FP64); with targets (true changes), also
-    the loss gradient in ws.gq and loss terms in ws.loss_terms."""
+    the loss gradient in ws.gq and loss terms in ws.loss_terms.  ``theta``:
+    the positivity limiter's factors (limit())."""
     L, s, g = net.lib, net.stream(), graph
     dw, duh, dvh = targets if targets is not None else (None, None, None)
-    _ck(L.gnn_apply(s, g.N, _ptr(g.tri_edges), _ptr(g.ea), _ptr(g.nx), _ptr(g.ny), _ptr(g.len), _ptr(g.closed),
-                    _ptr(g.area), _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(w0), _ptr(uh0),
+    _ck(L.gnn_apply(s, g.N, _ptr(g.tri_edges), _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.len),
+                    _ptr(g.closed), _ptr(g.area), _ptr(g.group), _ptr(g.group_area), _ptr(delta), _ptr(g.graph_of),
+                    _ptr(ws.F), net.scales["flux"], _ptr(theta), _ptr(w0), _ptr(uh0),
                     _ptr(vh0), _ptr(w1), _ptr(uh1), _ptr(vh1), _ptr(dw), _ptr(duh), _ptr(dvh), _ptr(net.sigma),
                     graph.mean_area, _ptr(ws.gq), _ptr(ws.loss_terms)), "apply")
 
@@ -275,7 +361,8 @@ def backward(net, graph, ws, delta):
     """Weight gradients (ws.gW, ws.gb) from ws.gq."""
     L, s, g = net.lib, net.stream(), graph
     _ck(L.gnn_apply_bwd(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.len), _ptr(g.closed),
-                        _ptr(g.area), _ptr(delta), _ptr(g.graph_of), net.scales["flux"], _ptr(ws.gq), _ptr(ws.dF)),
+                        _ptr(g.area), _ptr(g.group), _ptr(g.group_area), _ptr(delta), _ptr(g.graph_of),
+                        net.scales["flux"], _ptr(ws.gq), _ptr(ws.dF)),
         "apply bwd")
     _ck(L.gnn_linear_bwd_w(s, g.Epad, HID, OUT, _ptr(ws.H[4]), _ptr(ws.dF), _ptr(ws.partial), _ptr(ws.gW["head"]),
                            _ptr(ws.gb["head"])), "head bwd w")
@@ -426,7 +513,8 @@ class Trainer:
         torch = _torch()
         self.net, self.data, self.B, self.lr = net, data, batch, lr
         self.rng = np.random.default_rng(seed)
-        self.graphs = [EdgeGraph(d["mesh"], copies=batch, closed_tags=()) for d in data]
+        self.graphs = [EdgeGraph(d["mesh"], copies=batch, closed_tags=(), merge_radius=net.scales["merge_radius"])
+                       for d in data]
         self.ws = [Workspace(g) for g in self.graphs]
         self.buf = [dict((k, torch.empty(g.N, dtype=torch.float64, device="cuda"))
                          for k in ("w", "uh", "vh", "z", "dw", "duh", "dvh", "w1", "uh1", "vh1"))
@@ -505,10 +593,24 @@ def torch_index(a):
     return torch.as_tensor(np.asarray(a, dtype=np.int64), device="cuda")
 
 
-def learned_step(net, graph, ws, delta, w, uh, vh, z):
-    """One learned step in place (inference)."""
+def limit(net, graph, ws, delta, w, z):
+    """Positivity limiter factors of the current edge outputs (ws.theta)."""
+    g = graph
+    _ck(net.lib.gnn_limit(net.stream(), g.N, _ptr(g.tri_edges), _ptr(g.ea), _ptr(g.eb), _ptr(g.len),
+                          _ptr(g.closed), _ptr(g.area), _ptr(g.group), _ptr(delta), _ptr(g.graph_of), _ptr(ws.F),
+                          net.scales["flux"], _ptr(w), _ptr(z), _ptr(ws.theta)), "limit")
+    return ws.theta
+
+
+def learned_step(net, graph, ws, delta, w, uh, vh, z, h_dry=1e-3):
+    """One learned step in place (inference): edge outputs, positivity
+    limiter, update, then zero momentum where the depth is below h_dry."""
     forward(net, graph, ws, w, uh, vh, z, save=False)
-    apply(net, graph, ws, delta, w, uh, vh, w, uh, vh)
+    theta = limit(net, graph, ws, delta, w, z)
+    apply(net, graph, ws, delta, w, uh, vh, w, uh, vh, theta=theta)
+    if h_dry > 0.0:
+        _ck(net.lib.gnn_protect(net.stream(), graph.N, _ptr(w), _ptr(z), _ptr(uh), _ptr(vh), float(h_dry)),
+            "protect")
 
 
 def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, log=None):
@@ -604,7 +706,7 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
             steps, delta, fv_steps, fv_time, N * fv_steps / fv_time))
 
     # 2. learned rollout with the FV judge
-    graph = EdgeGraph(domain.mesh, copies=1, closed_tags=())
+    graph = EdgeGraph(domain.mesh, copies=1, closed_tags=(), merge_radius=net.scales["merge_radius"])
     ws = Workspace(graph)
     dl = torch.tensor([delta], dtype=torch.float64, device="cuda")
     w, uh, vh = w0.clone(), uh0.clone(), vh0.clone()
@@ -698,7 +800,8 @@ def training_meshes(count, extent=1400.0, density=5000.0, seed=7):
 
 
 def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.05, iters=3000, batch=16, lr=1e-3,
-                         eval_points=500000, eval_steps=100, judge_every=8, threshold=0.01, extent=1400.0, log=None):
+                         eval_points=500000, eval_steps=100, judge_every=8, threshold=0.01, extent=1400.0,
+                         merge_radius=1.0, log=None):
     """Train on FV trajectories of configuration-2-style windows, then roll the
     network out on the configuration-2 workload (1M triangles) against the
     FP64 FV path.  Returns a dict of the results."""
@@ -714,7 +817,7 @@ def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.05, it
     if log:
         log("training data: %d trajectories x %d samples on %d meshes of %d triangles (%.1f s)" % (
             n_traj, samples, len(data), meshes[0].number_of_elements, t_data))
-    net = EdgeGNN(seed=1)
+    net = EdgeGNN(seed=1, merge_radius=merge_radius)
     fit_scales(net, data)
     tr = Trainer(net, data, batch=batch, lr=lr)
     untrained = tr.evaluate(16)

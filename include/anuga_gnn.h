@@ -69,21 +69,44 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
  * momentum (normal, tangential), 3-4: side-b momentum) over a step delta:
  * dq[i] = delta / area[i] * sum over edges of side-signed, rotated
  * contributions * len * flux_scale; closed[e] != 0 zeroes the mass flux of a
- * wall.  Writes the new state (w, uh, vh; may alias the inputs).  With
- * targets (non-NULL dw_t, duh_t, dvh_t: the true changes), also writes the
- * area-weighted loss terms (area[i] / area_ref) * sum over components of
- * huber((dq - dq_true) / sigma) (quadratic up to 3, linear beyond) and their
- * gradient gq[i][3] with respect to dq, for the loss sum(terms) / Ntri. */
-int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const float *nx, const float *ny,
-              const float *len, const uint8_t *closed, const double *area, const double *delta, const int32_t *graph_of,
-              const float *F, float flux_scale, const double *w0, const double *uh0, const double *vh0, double *w1,
-              double *uh1, double *vh1, const double *dw_t, const double *duh_t, const double *dvh_t,
-              const float *sigma, double area_ref, float *gq, float *loss_terms);
+ * wall.  group (may be NULL) merges small cells: group[i][4] lists the
+ * members of triangle i's group (-1 padded; a lone triangle lists itself
+ * only), group_area[i] the group's area; every member then takes
+ * delta / group_area * (sum of the members' contributions, the mass flux of
+ * edges inside the group left out).  theta (may be NULL) scales each edge's
+ * mass flux by the factor of the cell it drains (gnn_limit).  Writes the new
+ * state (w, uh, vh; may alias the inputs).  With targets (non-NULL dw_t, duh_t, dvh_t: the true
+ * changes), also writes the area-weighted loss terms (area[i] / area_ref) *
+ * sum over components of huber((dq - dq_true) / sigma) (quadratic up to 3,
+ * linear beyond) and their gradient gq[i][3] with respect to dq, for the loss
+ * sum(terms) / Ntri. */
+int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
+              const float *nx, const float *ny, const float *len, const uint8_t *closed, const double *area,
+              const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
+              const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
+              const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
+              const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms);
 
-/* dF[Epad][32] (BF16) from gq: the transpose of gnn_apply's linear map. */
+/* Positivity limiter of the learned mass fluxes (draining time, Bollermann,
+ * Chen, Kurganov and Noelle 2013): theta[i] = min(1, volume / outflow) of
+ * triangle i's group (or of i), volume = sum of area * max(w - z, 0),
+ * outflow = delta * the outgoing mass fluxes through the group's outer,
+ * open edges.  Used at inference; training treats theta as 1. */
+int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
+              const float *len, const uint8_t *closed, const double *area, const int32_t *group, const double *delta,
+              const int32_t *graph_of, const float *F, float flux_scale, const double *w, const double *z,
+              double *theta);
+
+/* uh = vh = 0 where w - z < h_dry (the solver's protect, without its mass
+ * change). */
+int gnn_protect(void *stream, int64_t N, const double *w, const double *z, double *uh, double *vh, double h_dry);
+
+/* dF[Epad][32] (BF16) from gq: the transpose of gnn_apply's linear map (same
+ * group arguments). */
 int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
-                  const float *ny, const float *len, const uint8_t *closed, const double *area, const double *delta,
-                  const int32_t *graph_of, float flux_scale, const float *gq, void *dF);
+                  const float *ny, const float *len, const uint8_t *closed, const double *area, const int32_t *group,
+                  const double *group_area, const double *delta, const int32_t *graph_of, float flux_scale,
+                  const float *gq, void *dF);
 
 /* Adam step on n FP32 parameters, then BF16 copies in both layouts:
  * W [K][Nout] -> Wb (same layout) and WTb ([Nout][K]); for biases pass K = 1

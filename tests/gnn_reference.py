@@ -102,23 +102,48 @@ def concat_nodes(g, S):
 
 
 def apply_update(g, F, delta, fs):
-    """State change dq[i] = (dh, duh, dvh) of the learned step."""
+    """State change dq[i] = (dh, duh, dvh) of the learned step.  With
+    g["group"] (merged small cells), each triangle's integrated change is
+    formed first and every member of a group takes the group's total divided
+    by the group's area; the mass flux between two members of one group is
+    left out."""
     N = len(g["tri_edges"])
-    dq = np.zeros((N, 3))
+    group = g.get("group")
+    members = [set(int(m) for m in group[i] if m >= 0) for i in range(N)] if group is not None else None
+    M = np.zeros((N, 3))
     for i, es in enumerate(g["tri_edges"]):
         for e in es:
             if e < 0:
                 continue
             side_a = g["ea"][e] == i
+            other = g["eb"][e] if side_a else g["ea"][e]
+            inside = members is not None and other >= 0 and other in members[i] and len(members[i]) > 1
             L = float(g["len"][e]) * fs
             n1, n2 = float(g["nx"][e]), float(g["ny"][e])
-            c = 0.0 if g["closed"][e] else F[e, 0] * L
-            dq[i, 0] += -c if side_a else c
+            c = 0.0 if (g["closed"][e] or inside) else F[e, 0] * L
+            M[i, 0] += -c if side_a else c
             mn, mt = (F[e, 1], F[e, 2]) if side_a else (F[e, 3], F[e, 4])
-            dq[i, 1] += (mn * n1 - mt * n2) * L
-            dq[i, 2] += (mn * n2 + mt * n1) * L
-        dq[i] *= delta / g["area"][i]
+            M[i, 1] += (mn * n1 - mt * n2) * L
+            M[i, 2] += (mn * n2 + mt * n1) * L
+    dq = np.zeros((N, 3))
+    for i in range(N):
+        if members is not None and len(members[i]) > 1:
+            dq[i] = delta / g["group_area"][i] * sum(M[m] for m in sorted(members[i]))
+        else:
+            dq[i] = delta / g["area"][i] * M[i]
     return dq
+
+
+def apply_transpose(g, E, delta, fs, gq):
+    """dF = J^T gq for the linear map F -> dq of apply_update, J taken column
+    by column from apply_update itself (a unit output of one edge at a time)."""
+    dF = np.zeros((E, OUT))
+    for e in range(E):
+        for c in range(5):
+            unit = np.zeros((E, OUT))
+            unit[e, c] = 1.0
+            dF[e, c] = np.sum(apply_update(g, unit, delta, fs) * gq)
+    return dF
 
 
 def loss_and_grads(params, g, X0, delta, fs, target, sigma):
@@ -134,21 +159,7 @@ def loss_and_grads(params, g, X0, delta, fs, target, sigma):
     loss = np.sum(wgt * np.where(ae <= k, err ** 2, k * (2 * ae - k))) / N
     gq = wgt * np.where(ae <= k, 2.0 * err, 2.0 * k * np.sign(err)) / sigma / N
     # d loss / d F through the (linear) update
-    dF = np.zeros_like(F)
-    for e in range(len(g["ea"])):
-        a, b = g["ea"][e], g["eb"][e]
-        L = float(g["len"][e]) * fs
-        n1, n2 = float(g["nx"][e]), float(g["ny"][e])
-        sa = delta / g["area"][a] * L
-        d0 = -gq[a, 0] * sa
-        dF[e, 1] = sa * (gq[a, 1] * n1 + gq[a, 2] * n2)
-        dF[e, 2] = sa * (-gq[a, 1] * n2 + gq[a, 2] * n1)
-        if b >= 0:
-            sb = delta / g["area"][b] * L
-            d0 += gq[b, 0] * sb
-            dF[e, 3] = sb * (gq[b, 1] * n1 + gq[b, 2] * n2)
-            dF[e, 4] = sb * (-gq[b, 1] * n2 + gq[b, 2] * n1)
-        dF[e, 0] = 0.0 if g["closed"][e] else d0
+    dF = apply_transpose(g, len(g["ea"]), delta, fs, gq)
     W, Z, H, Cs = saved["W"], saved["Z"], saved["H"], saved["C"]
     gW, gb = {}, {}
     gW["head"] = H[4].T @ dF
@@ -166,3 +177,22 @@ def loss_and_grads(params, g, X0, delta, fs, target, sigma):
     gW["enc"] = X0.T @ dZ
     gb["enc"] = dZ.sum(0)
     return loss, F, dq, gW, gb
+
+
+def limiter(g, F, delta, fs, w, z):
+    """Positivity limiter factors: per cell (lone cells only), volume over the
+    mass it would send out through its open edges over the step, capped at 1."""
+    N = len(g["tri_edges"])
+    theta = np.ones(N)
+    for i, es in enumerate(g["tri_edges"]):
+        vol = g["area"][i] * max(w[i] - z[i], 0.0)
+        out = 0.0
+        for e in es:
+            if e < 0 or g["closed"][e]:
+                continue
+            flux = F[e, 0] * float(g["len"][e]) * fs  # positive: from ea to eb
+            out += max(flux, 0.0) if g["ea"][e] == i else max(-flux, 0.0)
+        out *= delta
+        if out > vol:
+            theta[i] = vol / out
+    return theta

@@ -27,20 +27,42 @@ def _state(mesh, seed):
 
 def _graph_np(graph):
     cp = lambda t: t.detach().cpu().numpy()
-    return dict(ea=cp(graph.ea)[:graph.E], eb=cp(graph.eb)[:graph.E], nx=cp(graph.nx), ny=cp(graph.ny),
-                len=cp(graph.len), closed=cp(graph.closed), tri_edges=cp(graph.tri_edges), area=cp(graph.area))
+    g = dict(ea=cp(graph.ea)[:graph.E], eb=cp(graph.eb)[:graph.E], nx=cp(graph.nx), ny=cp(graph.ny),
+             len=cp(graph.len), closed=cp(graph.closed), tri_edges=cp(graph.tri_edges), area=cp(graph.area))
+    if graph.group is not None:
+        g.update(group=cp(graph.group), group_area=cp(graph.group_area))
+    return g
 
 
-def test_gnn_forward_and_gradients_match_cpu_reference():
+def _sliver_mesh():
+    """A small Delaunay mesh of random points (slivers included)."""
+    from anuga_cuda_amd.mesh import delaunay_random
+
+    pts, tris = delaunay_random(40, extent=(40.0, 30.0), seed=8)
+    return Mesh(pts, tris)
+
+
+@pytest.mark.parametrize("case", ["lone_cells", "merged_small_cells"])
+def test_gnn_forward_and_gradients_match_cpu_reference(case):
     """Edge outputs, state change and every layer's weight / bias gradient of
     the HIP kernels against the FP64 numpy restatement (same BF16-rounded
-    weights and features).  Bound: relative Frobenius error below 3e-2 (BF16
-    activations, FP32 accumulation)."""
+    weights and features), whose update gradient is formed from its forward
+    map alone (J^T gq, J column by column).  The second case merges the cells
+    below the 40th percentile of the inscribed radius into groups.  Bound:
+    relative Frobenius error below 3e-2 (BF16 activations, FP32
+    accumulation)."""
     import torch
     from anuga_cuda_amd import learned as L
 
-    mesh = _mesh()
-    graph = L.EdgeGraph(mesh, closed_tags=("left", "right"))
+    if case == "lone_cells":
+        mesh = _mesh()
+        graph = L.EdgeGraph(mesh, closed_tags=("left", "right"))
+        assert graph.group is None
+    else:
+        mesh = _sliver_mesh()
+        r = 2.0 * mesh.areas / mesh.edgelengths.sum(1)
+        graph = L.EdgeGraph(mesh, closed_tags=(), merge_radius=float(np.percentile(r, 40)))
+        assert 0 < graph.merged < graph.N
     net = L.EdgeGNN(seed=5, depth_scale=2.0, momentum_scale=1.0, length_scale=10.0, flux_scale=0.5,
                     sigma=(0.05, 0.05, 0.05))
     rng = np.random.default_rng(9)
@@ -90,7 +112,10 @@ def test_gnn_forward_and_gradients_match_cpu_reference():
 def test_gnn_rollout_conserves_mass_in_a_closed_basin():
     """With every boundary a wall, the learned step moves mass only between
     triangles: total mass sum(A * h) stays constant to 1e-12 (relative) over 30
-    steps of an untrained, randomly weighted network."""
+    steps of an untrained, randomly weighted network, without and with the
+    positivity limiter and dry-cell protect of the inference step; with the
+    limiter no depth goes below zero (to rounding), and its factors match the
+    numpy restatement on the first step."""
     import torch
     from anuga_cuda_amd import learned as L
 
@@ -102,19 +127,35 @@ def test_gnn_rollout_conserves_mass_in_a_closed_basin():
     net.cast()
     w, uh, vh, z = _state(mesh, 3)
     dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
-    tw, tuh, tvh, tz = dev(w), dev(uh), dev(vh), dev(z)
     area = graph.area
-    m0 = float(torch.sum(area * (tw - tz)))
-    ws = L.Workspace(graph)
     delta = dev([0.5])
-    for _ in range(30):
-        L.forward(net, graph, ws, tw, tuh, tvh, tz)
-        L.apply(net, graph, ws, delta, tw, tuh, tvh, tw, tuh, tvh)
-    m1 = float(torch.sum(area * (tw - tz)))
-    moved = float(torch.sum(area * torch.abs(tw - dev(w))))
-    print("mass %.17g -> %.17g, relative change %.2e; mass moved %.3e" % (m0, m1, abs(m1 - m0) / m0, moved))
-    assert moved > 1e-3 * m0  # the network does move water around
-    assert abs(m1 - m0) <= 1e-12 * m0
+    for limited in (False, True):
+        tw, tuh, tvh, tz = dev(w), dev(uh), dev(vh), dev(z)
+        m0 = float(torch.sum(area * (tw - tz)))
+        ws = L.Workspace(graph)
+        min_depth = np.inf
+        for step in range(30):
+            if limited:
+                if step == 0:
+                    L.forward(net, graph, ws, tw, tuh, tvh, tz, save=False)
+                    theta = L.limit(net, graph, ws, delta, tw, tz).cpu().numpy()
+                    ref_theta = ref.limiter(_graph_np(graph), ws.F[:graph.E].cpu().numpy().astype(np.float64), 0.5,
+                                            0.05, w, z)
+                    assert np.all(theta <= 1.0) and np.min(theta) < 1.0  # the limiter acts
+                    np.testing.assert_allclose(theta, ref_theta, rtol=1e-5, atol=1e-7)
+                L.learned_step(net, graph, ws, delta, tw, tuh, tvh, tz)
+                min_depth = min(min_depth, float(torch.min(tw - tz)))
+            else:
+                L.forward(net, graph, ws, tw, tuh, tvh, tz)
+                L.apply(net, graph, ws, delta, tw, tuh, tvh, tw, tuh, tvh)
+        m1 = float(torch.sum(area * (tw - tz)))
+        moved = float(torch.sum(area * torch.abs(tw - dev(w))))
+        print("limited=%s: mass %.17g -> %.17g, relative change %.2e; mass moved %.3e; min depth %.3g" % (
+            limited, m0, m1, abs(m1 - m0) / m0, moved, min_depth))
+        assert moved > 1e-3 * m0  # the network does move water around
+        assert abs(m1 - m0) <= 1e-12 * m0
+        if limited:
+            assert min_depth > -1e-12
 
 
 def test_training_on_fv_trajectories_beats_the_untrained_network():
