@@ -125,6 +125,23 @@ def exported_symbols():
     return sorted(_SIGNATURES)
 
 
+def share_hip_runtime():
+    """Makes the HIP runtime that torch ships the one this process uses.
+
+    torch's ROCm build carries its own copy of the HIP and HSA runtimes under
+    the same soname (libamdhip64.so.7) as /opt/rocm's.  Loaded after torch,
+    the HIP libraries of this package bind to torch's copy and device
+    pointers and streams pass freely between the two.  Loaded before torch,
+    they map /opt/rocm's copy, torch then maps its own as a second runtime,
+    and that second runtime does not see the GPU (torch.cuda.is_available()
+    is False).  Importing torch first, when it is installed, keeps one runtime
+    per process.  C callers of the library are not affected."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
     """Load the HIP library (raises if it has not been built)."""
     global _lib
@@ -134,6 +151,7 @@ def load():
         raise ImportError(
             "libanuga_sw.so not built (run anuga_cuda_amd/csrc/build.sh or "
             "__graft_entry__.build()); there is no CPU fallback")
+    share_hip_runtime()
     lib = C.CDLL(LIB_PATH)
     for name, (argtypes, restype) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -48,6 +48,9 @@ def _load():
     if _gnn is None:
         if not os.path.exists(GNN_LIB_PATH):
             raise ImportError("libanuga_gnn.so not built (run anuga_cuda_amd/csrc/build.sh); there is no CPU fallback")
+        from ._lib import share_hip_runtime
+
+        share_hip_runtime()
         _gnn = C.CDLL(GNN_LIB_PATH)
         for name in ("gnn_features", "gnn_linear_fwd", "gnn_linear_bwd_x", "gnn_linear_bwd_w", "gnn_silu_bwd",
                      "gnn_node_sum", "gnn_concat", "gnn_node_grad", "gnn_edge_grad", "gnn_apply", "gnn_apply_bwd",

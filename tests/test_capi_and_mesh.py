@@ -115,3 +115,19 @@ def test_product_path_has_no_cpu_fallback(monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError):
         _lib.load()
+
+
+def test_one_hip_runtime_per_process():
+    """Loading the package's HIP libraries before torch must not leave two HIP
+    runtimes mapped (the second would not see the GPU).  Runs in a fresh
+    interpreter so that nothing else has imported torch yet."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libanuga_sw.so not built")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from anuga_cuda_amd import _lib, learned\n"
+            "_lib.load(); learned._load()\n"
+            "import torch\n"
+            "maps = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(maps))\n" % ROOT)
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True, check=True)
+    assert out.stdout.strip().splitlines()[-1] == "1", out.stdout + out.stderr
