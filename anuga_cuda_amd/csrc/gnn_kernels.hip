@@ -23,11 +23,15 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kB = 256;      // threads per workgroup
 constexpr int kParts = 256;  // workgroups of the weight-gradient reduction
 constexpr int kFeat = 32;    // edge input features (padded)
-constexpr int kOut = 32;     // head outputs (padded; 5 used)
+constexpr int kOut = 32;     // head weight columns and output gradient row (padded; 5 used)
+constexpr int kFOut = 8;     // row of the FP32 edge outputs F and of the physical part (5 used)
 
 #define GNN_DEV __device__ __forceinline__
 
-GNN_DEV float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// Logistic function with the hardware reciprocal (v_rcp_f32, 1 ulp) rather
+// than an IEEE division: the layers' epilogues are bound by these
+// instructions, and BF16 activations do not need the last bit.
+GNN_DEV float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 GNN_DEV float silu(float z) { return z * sigmoid(z); }
 GNN_DEV float silu_grad(float z) {
     const float s = sigmoid(z);
@@ -38,15 +42,12 @@ GNN_DEV bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p);
 GNN_DEV void st8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
 
 // ---------------------------------------------------------------------------
-// Edge features of the current state, in the edge's normal frame.
-__global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
-                                                 const float *nx, const float *ny, const float *len,
-                                                 const uint8_t *closed, const double *w, const double *uh,
-                                                 const double *vh, const double *z, float hs, float qs, float ls,
-                                                 bf16 *X) {
-    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
-    if (e >= Epad) return;
-    float f[kFeat];
+// Edge features of the current state, in the edge's normal frame (zero for
+// the padding rows e >= E).
+GNN_DEV void edge_features(int64_t e, int64_t E, const int32_t *ea, const int32_t *eb, const float *nx,
+                           const float *ny, const float *len, const uint8_t *closed, const double *w,
+                           const double *uh, const double *vh, const double *z, float hs, float qs, float ls,
+                           float f[kFeat]) {
 #pragma unroll
     for (int i = 0; i < kFeat; i++) f[i] = 0.0f;
     if (e < E) {
@@ -105,6 +106,17 @@ __global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const 
         f[18] = 0.5f * g * (float(ha * ha) - sa * sa) / ps;
         f[19] = 0.5f * g * (float(hb * hb) - sb * sb) / ps;
     }
+}
+
+__global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                                 const float *nx, const float *ny, const float *len,
+                                                 const uint8_t *closed, const double *w, const double *uh,
+                                                 const double *vh, const double *z, float hs, float qs, float ls,
+                                                 bf16 *X) {
+    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (e >= Epad) return;
+    float f[kFeat];
+    edge_features(e, E, ea, eb, nx, ny, len, closed, w, uh, vh, z, hs, qs, ls, f);
 #pragma unroll
     for (int c = 0; c < kFeat / 8; c++) {
         bf16x8 v;
@@ -115,18 +127,70 @@ __global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const 
 }
 
 // ---------------------------------------------------------------------------
-// Y = act(X W + b) (+ R): one wavefront per 16 rows; the 16 rows' A
-// fragments stay in registers across the Nout/16 output blocks.
+// Physical part of the edge outputs: the first-order central-upwind flux
+// (Kurganov, Noelle and Petrova, SIAM J. Sci. Comput. 23, 2001) of the two
+// centroid states after hydrostatic reconstruction (Audusse, Bouchut,
+// Bristeau, Klein and Perthame, SIAM J. Sci. Comput. 25, 2004), FP64, added to
+// the network's outputs in their units (divided by the flux scale):
+//   0: mass flux from a to b,
+//   1, 2: normal / tangential momentum change of side a per unit length,
+//   3, 4: the same for side b.
+// Each side's normal momentum takes the flux minus its own reconstructed
+// pressure 0.5 g s^2.  That drops the term 0.5 g h^2 of the cell's centroid
+// depth, whose sum over a cell's edges is 0.5 g h^2 times the sum of
+// length * normal, zero for a closed polygon.  Written as below, a lake at
+// rest (equal stage, zero momentum, any bed, dry cells included) gives
+// exactly zero on every edge, so the physical part is well balanced without
+// relying on cancellation between edges.
+GNN_DEV void base_flux_edge(int64_t e, const int32_t *ea, const int32_t *eb, const float *nx, const float *ny,
+                            const uint8_t *closed, const double *w, const double *uh, const double *vh,
+                            const double *z, double g, double h_dry, double inv_fs, float *out) {
+    // no fused multiply-adds here: pa and pb must be rounded the same way for
+    // pb - pa to vanish when sa == sb
+#pragma clang fp contract(off)
+    const int a = ea[e], b = eb[e];
+    const double n1 = nx[e], n2 = ny[e];
+    const double wa = w[a], za = z[a], ha = wa - za;
+    const double qna = uh[a] * n1 + vh[a] * n2, qta = -uh[a] * n2 + vh[a] * n1;
+    double wb = wa, zb = za, hb = ha, qnb = closed[e] ? -qna : qna, qtb = qta;
+    if (b >= 0) {
+        wb = w[b];
+        zb = z[b];
+        hb = wb - zb;
+        qnb = uh[b] * n1 + vh[b] * n2;
+        qtb = -uh[b] * n2 + vh[b] * n1;
+    }
+    const double ua = ha > h_dry ? qna / ha : 0.0, va = ha > h_dry ? qta / ha : 0.0;
+    const double ub = hb > h_dry ? qnb / hb : 0.0, vb = hb > h_dry ? qtb / hb : 0.0;
+    const double zm = fmax(za, zb);
+    const double sa = fmax(0.0, wa - zm), sb = fmax(0.0, wb - zm);
+    const double ca = sqrt(g * sa), cb = sqrt(g * sb);
+    const double smax = fmax(fmax(ua + ca, ub + cb), 0.0), smin = fmin(fmin(ua - ca, ub - cb), 0.0);
+    double k0 = 0.0, mna = 0.0, k2 = 0.0, mnb = 0.0;
+    if (smax - smin > 1.0e-12) {
+        const double inv = 1.0 / (smax - smin), sm = smax * smin;
+        const double qa = sa * ua, qb = sb * ub;
+        const double pa = 0.5 * g * sa * sa, pb = 0.5 * g * sb * sb;
+        const double core = smax * qa * ua - smin * qb * ub + sm * (qb - qa);
+        k0 = (smax * qa - smin * qb + sm * (sb - sa)) * inv;
+        mna = -(core - smin * (pb - pa)) * inv;
+        mnb = (core + smax * (pa - pb)) * inv;
+        k2 = (smax * qa * va - smin * qb * vb + sm * (sb * vb - sa * va)) * inv;
+    }
+    out[0] = (float)(k0 * inv_fs);
+    out[1] = (float)(mna * inv_fs);
+    out[2] = (float)(-k2 * inv_fs);
+    out[3] = (float)(mnb * inv_fs);
+    out[4] = (float)(k2 * inv_fs);
+    out[5] = out[6] = out[7] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Y = act(X W + b) (+ R) of the 16 rows m0 .. m0+15 whose A fragments a[]
+// (K/32 of them) are in registers: one wavefront, Nout/16 output blocks.
 template <int K, int NOUT, bool OUTF32>
-__global__ __launch_bounds__(kB) void k_linear_fwd(int64_t M, const bf16 *X, const bf16 *WT, const float *bias,
-                                                   const bf16 *R, bf16 *Zs, void *Yv, int act) {
-    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const int64_t m0 = wave * 16;
-    if (m0 >= M) return;
-    bf16x8 a[K / 32];
-#pragma unroll
-    for (int kk = 0; kk < K / 32; kk++) a[kk] = ld8(X + (m0 + r) * K + kk * 32 + g * 8);
+GNN_DEV void linear_tile(int64_t m0, int r, int g, const bf16x8 *a, const bf16 *WT, const float *bias, const bf16 *R,
+                         bf16 *Zs, void *Yv, int act) {
 #pragma unroll
     for (int nb = 0; nb < NOUT / 16; nb++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -145,6 +209,265 @@ __global__ __launch_bounds__(kB) void k_linear_fwd(int64_t M, const bf16 *X, con
                 static_cast<float *>(Yv)[idx] = y;
             else
                 static_cast<bf16 *>(Yv)[idx] = (bf16)y;
+        }
+    }
+}
+
+// Y = act(X W + b) (+ R): one wavefront per 16 rows; the 16 rows' A
+// fragments stay in registers across the Nout/16 output blocks.
+template <int K, int NOUT, bool OUTF32>
+__global__ __launch_bounds__(kB) void k_linear_fwd(int64_t M, const bf16 *X, const bf16 *WT, const float *bias,
+                                                   const bf16 *R, bf16 *Zs, void *Yv, int act) {
+    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int64_t m0 = wave * 16;
+    if (m0 >= M) return;
+    bf16x8 a[K / 32];
+#pragma unroll
+    for (int kk = 0; kk < K / 32; kk++) a[kk] = ld8(X + (m0 + r) * K + kk * 32 + g * 8);
+    linear_tile<K, NOUT, OUTF32>(m0, r, g, a, WT, bias, R, Zs, Yv, act);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent dense layers of the inference step.  Each wavefront keeps the
+// layer's weight fragments (the B operands of all its output blocks) in
+// registers and walks tiles of 16 rows, so the weights are read once per
+// wavefront rather than once per tile.  Results leave the accumulators (lane
+// (r, g) holds rows 4g .. 4g+3 of column r of each 16-column block) through a
+// per-wavefront LDS tile and are stored as 16-byte row segments: lane (r, g)
+// writes columns 8g .. 8g+7 and 32+8g .. 32+8g+7 of row r, the layout of its
+// own A fragments, so a residual input is already in its registers.  The
+// arithmetic is that of k_linear_fwd (same MFMA order, same roundings).
+constexpr int kTileLd = 68;  // LDS row of a 16 x 64 FP32 tile (+4 words: rows 4 apart hit other banks)
+constexpr int kPersistMax = 1024;  // workgroups of a persistent launch
+
+// Orders one wavefront's LDS writes before its reads of other lanes' data.
+GNN_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// SiLU(acc + bias) of a 16 x 64 tile (acc[nb]: columns nb*16 .. nb*16+15),
+// optionally saving the pre-activation (BF16, rows of 64) to Zs; returns lane
+// (r, g)'s row segments y0 (columns 8g..) and y1 (columns 32+8g..) of row r.
+GNN_DEV void silu_rows(const f32x4 acc[4], const float br[4], int r, int g, float (*T)[kTileLd], bf16 *Zs,
+                       int64_t m0, float y0[8], float y1[8]) {
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float zz = acc[nb][j] + br[nb];
+            if (Zs) Zs[(m0 + 4 * g + j) * 64 + nb * 16 + r] = (bf16)zz;
+            T[4 * g + j][nb * 16 + r] = silu(zz);
+        }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        y0[i] = T[r][8 * g + i];
+        y1[i] = T[r][32 + 8 * g + i];
+    }
+    wave_lds_sync();
+}
+
+GNN_DEV void store_row_segments(bf16 *Y, int64_t row, int g, const float y0[8], const float y1[8]) {
+    bf16x8 o0, o1;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        o0[i] = (bf16)y0[i];
+        o1[i] = (bf16)y1[i];
+    }
+    st8(Y + row * 64 + 8 * g, o0);
+    st8(Y + row * 64 + 32 + 8 * g, o1);
+}
+
+// Features of the state, the physical part of the edge outputs (P, may be
+// NULL) and the encoder layer H1 = SiLU(X0 W + b) in one pass; X0s and Zs
+// (may be NULL) receive the features and the pre-activation (training).  A
+// wavefront takes 64 edges at a time: each lane forms the features and the
+// physical part of one edge, the feature rows go through LDS to the A
+// fragments of four 16-row MFMA tiles.  Epad must be a multiple of 64.
+constexpr int kXLd = kFeat + 8;  // LDS row of the feature tile (BF16; +8: conflict-free 16-byte reads)
+
+__global__ __launch_bounds__(kB) void k_feat_enc(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                                 const float *nx, const float *ny, const float *len,
+                                                 const uint8_t *closed, const double *w, const double *uh,
+                                                 const double *vh, const double *z, float hs, float qs, float ls,
+                                                 const bf16 *WT, const float *bias, double g_acc, double h_dry,
+                                                 double inv_fs, float *P, bf16 *X0s, bf16 *Zs, bf16 *H1) {
+    __shared__ float tiles[kB / 64][16][kTileLd];
+    __shared__ bf16 xtile[kB / 64][64][kXLd];
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float(*T)[kTileLd] = tiles[threadIdx.x >> 6];
+    bf16(*X)[kXLd] = xtile[threadIdx.x >> 6];
+    const int64_t wave0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6, nwaves = (int64_t)gridDim.x * (kB / 64);
+    bf16x8 bw[4];
+    float br[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+        bw[nb] = ld8(WT + (nb * 16 + r) * kFeat + g * 8);
+        br[nb] = bias[nb * 16 + r];
+    }
+    const int64_t ntiles = Epad / 64;
+    for (int64_t t = wave0; t < ntiles; t += nwaves) {
+        const int64_t m64 = t * 64, e = m64 + lane;
+        float f[kFeat];
+        edge_features(e, E, ea, eb, nx, ny, len, closed, w, uh, vh, z, hs, qs, ls, f);
+#pragma unroll
+        for (int c = 0; c < kFeat / 8; c++) {
+            bf16x8 v;
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = (bf16)f[c * 8 + i];
+            *reinterpret_cast<bf16x8 *>(&X[lane][c * 8]) = v;
+            if (X0s) st8(X0s + e * kFeat + c * 8, v);
+        }
+        if (P) {
+            float o[kFOut] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (e < E) base_flux_edge(e, ea, eb, nx, ny, closed, w, uh, vh, z, g_acc, h_dry, inv_fs, o);
+            float4 *p = reinterpret_cast<float4 *>(P + e * kFOut);
+            p[0] = make_float4(o[0], o[1], o[2], o[3]);
+            p[1] = make_float4(o[4], o[5], o[6], o[7]);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            const int64_t m0 = m64 + 16 * sub;
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&X[16 * sub + r][8 * g]);
+            f32x4 acc[4];
+#pragma unroll
+            for (int nb = 0; nb < 4; nb++) acc[nb] = mfma(a, bw[nb], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            float y0[8], y1[8];
+            silu_rows(acc, br, r, g, T, Zs, m0, y0, y1);
+            store_row_segments(H1, m0 + r, g, y0, y1);
+        }
+        wave_lds_sync();
+    }
+}
+
+// One message-passing layer with the concatenation fused in:
+// Hout = H + SiLU([H, S_ea + S_eb] W + b); the second half of each A row is
+// formed from the node sums as k_concat forms it (FP32 sum rounded to BF16;
+// rows E .. Epad-1 zero), so the result is that of k_concat followed by
+// k_linear_fwd<128, 64> with residual H, without the concatenation's round
+// trip through HBM.  The workgroup's four wavefronts share the weights in
+// LDS (rows padded by 16 bytes: the 16 rows a fragment load reads fall in
+// different banks).  With WTh (the head's weights, [32][64]) the head
+// follows in registers: F[Epad][kFOut] = the first kFOut columns of
+// Hout W_h + b_h (+ P), bitwise what k_head computes from the stored Hout;
+// Hout (may then be NULL) is stored only if given.
+constexpr int kWLd = 128 + 8;  // LDS row of the layer's weights (BF16)
+
+__global__ __launch_bounds__(kB) void k_mp_layer(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
+                                                 const bf16 *H, const bf16 *S, const bf16 *WT, const float *bias,
+                                                 bf16 *Zs, bf16 *Hout, const bf16 *WTh, const float *bias_h,
+                                                 const float *P, float *F) {
+    __shared__ float tiles[kB / 64][16][kTileLd];
+    __shared__ bf16 wts[64][kWLd];
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float(*T)[kTileLd] = tiles[threadIdx.x >> 6];
+    for (int i = threadIdx.x; i < 64 * 16; i += kB) {
+        const int row = i >> 4, col = (i & 15) * 8;
+        *reinterpret_cast<bf16x8 *>(&wts[row][col]) = ld8(WT + row * 128 + col);
+    }
+    __syncthreads();
+    const int64_t wave0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6, nwaves = (int64_t)gridDim.x * (kB / 64);
+    float br[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) br[nb] = bias[nb * 16 + r];
+    bf16x8 bh0, bh1;
+    float bhn = 0.0f;
+    if (WTh) {
+        bh0 = ld8(WTh + r * 64 + g * 8);
+        bh1 = ld8(WTh + r * 64 + 32 + g * 8);
+        bhn = bias_h[r];
+    }
+    const int64_t ntiles = Epad / 16;
+    for (int64_t t = wave0; t < ntiles; t += nwaves) {
+        const int64_t m0 = t * 16, e = m0 + r;
+        const bf16x8 h0 = ld8(H + e * 64 + g * 8), h1 = ld8(H + e * 64 + 32 + g * 8);
+        bf16x8 a[4];
+        if (e < E) {
+            a[0] = h0;
+            a[1] = h1;
+            const int ia = ea[e], ib = eb[e];
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                const int c = kk * 32 + g * 8;
+                const bf16x8 sa = ld8(S + (int64_t)ia * 64 + c);
+                if (ib >= 0) {
+                    const bf16x8 sb = ld8(S + (int64_t)ib * 64 + c);
+                    bf16x8 o;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) o[j] = (bf16)((float)sa[j] + (float)sb[j]);
+                    a[2 + kk] = o;
+                } else {
+                    a[2 + kk] = sa;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) a[kk][j] = (bf16)0.0f;
+        }
+        f32x4 acc[4];
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) {
+            acc[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+                acc[nb] = mfma(a[kk], *reinterpret_cast<const bf16x8 *>(&wts[nb * 16 + r][kk * 32 + g * 8]), acc[nb]);
+        }
+        float y0[8], y1[8];
+        silu_rows(acc, br, r, g, T, Zs, m0, y0, y1);
+        bf16x8 o0, o1;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            o0[i] = (bf16)(y0[i] + (float)h0[i]);
+            o1[i] = (bf16)(y1[i] + (float)h1[i]);
+        }
+        if (Hout) {
+            st8(Hout + e * 64 + 8 * g, o0);
+            st8(Hout + e * 64 + 32 + 8 * g, o1);
+        }
+        if (WTh) {
+            // the new rows are the A fragments of the head
+            f32x4 ah = mfma(o0, bh0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            ah = mfma(o1, bh1, ah);
+            if (r < kFOut) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int64_t idx = (m0 + 4 * g + j) * kFOut + r;
+                    float v = ah[j] + bhn;
+                    if (P) v += P[idx];
+                    F[idx] = v;
+                }
+            }
+        }
+    }
+}
+
+// Head: F[Epad][kFOut] (FP32) = the first kFOut columns of H4 W + b, plus the
+// physical part P (may be NULL).
+__global__ __launch_bounds__(kB) void k_head(int64_t Epad, const bf16 *H4, const bf16 *WT, const float *bias,
+                                             const float *P, float *F) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int64_t wave0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6, nwaves = (int64_t)gridDim.x * (kB / 64);
+    const bf16x8 b0 = ld8(WT + r * 64 + g * 8), b1 = ld8(WT + r * 64 + 32 + g * 8);
+    const float bn = bias[r];
+    const int64_t ntiles = Epad / 16;
+    for (int64_t t = wave0; t < ntiles; t += nwaves) {
+        const int64_t m0 = t * 16, e = m0 + r;
+        f32x4 acc = mfma(ld8(H4 + e * 64 + g * 8), b0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        acc = mfma(ld8(H4 + e * 64 + 32 + g * 8), b1, acc);
+        if (r < kFOut) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t idx = (m0 + 4 * g + j) * kFOut + r;
+                float v = acc[j] + bn;
+                if (P) v += P[idx];
+                F[idx] = v;
+            }
         }
     }
 }
@@ -344,6 +667,20 @@ __global__ __launch_bounds__(kB) void k_edge_grad(int64_t E, int64_t Epad, const
     st8(dH + e * 64 + c, o);
 }
 
+// The physical part of rows 0 .. E-1 into P[E][kFOut].
+__global__ __launch_bounds__(kB) void k_base_flux(int64_t E, const int32_t *ea, const int32_t *eb, const float *nx,
+                                                  const float *ny, const uint8_t *closed, const double *w,
+                                                  const double *uh, const double *vh, const double *z, double g,
+                                                  double h_dry, double inv_fs, float *P) {
+    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (e >= E) return;
+    float o[kFOut];
+    base_flux_edge(e, ea, eb, nx, ny, closed, w, uh, vh, z, g, h_dry, inv_fs, o);
+    float4 *p = reinterpret_cast<float4 *>(P + e * kFOut);
+    p[0] = make_float4(o[0], o[1], o[2], o[3]);
+    p[1] = make_float4(o[4], o[5], o[6], o[7]);
+}
+
 // ---------------------------------------------------------------------------
 // State update from the head's edge outputs (FP64 accumulation), loss and its
 // gradient with respect to the state change.
@@ -364,7 +701,7 @@ __device__ __forceinline__ void edge_sums(int m, const int32_t *tri_edges, const
         const bool side_a = ea[e] == m;
         const double L = (double)len[e] * (double)fs;
         const double n1 = nx[e], n2 = ny[e];
-        const float *f = F + (int64_t)e * kOut;
+        const float *f = F + (int64_t)e * kFOut;
         // the mass flux of an edge: one value, added to one side, subtracted from the other
         const int other = side_a ? eb[e] : ea[e];
         const bool internal = mem && other >= 0 &&
@@ -394,10 +731,13 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
     if (i >= N) return;
     double sh = 0.0, sx = 0.0, sy = 0.0;
     double A = area[i];
+    int mem[4] = {(int)i, -1, -1, -1};
     if (group && group[i * 4 + 1] >= 0) {
-        // a merged group of small cells: every member takes the group's
-        // change, summed over the members in the same order
-        const int mem[4] = {group[i * 4], group[i * 4 + 1], group[i * 4 + 2], group[i * 4 + 3]};
+        // a merged group of small cells: its first member forms the group's
+        // change (summed over the members in order) and every member takes it
+        if (group[i * 4] != i) return;
+#pragma unroll
+        for (int j = 0; j < 4; j++) mem[j] = group[i * 4 + j];
 #pragma unroll
         for (int j = 0; j < 4; j++)
             if (mem[j] >= 0)
@@ -408,26 +748,30 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
     }
     const double s = delta[graph_of[i]] / A;
     const double dh = s * sh, dx = s * sx, dy = s * sy;
-    w1[i] = w0[i] + dh;
-    uh1[i] = uh0[i] + dx;
-    vh1[i] = vh0[i] + dy;
-    if (dw_t) {
-        // Area-weighted (the error of the volume and momentum a triangle
-        // gains) Huber loss of the normalised error: quadratic up to kHuber
-        // standard deviations, linear beyond, so the few violently changing
-        // triangles of a trajectory do not dominate.
-        constexpr double kHuber = 3.0;
-        const double wgt = area[i] / area_ref;
-        const double e[3] = {(dh - dw_t[i]) / sigma[0], (dx - duh_t[i]) / sigma[1], (dy - dvh_t[i]) / sigma[2]};
-        double l = 0.0;
+    for (int j = 0; j < 4; j++) {
+        const int m = mem[j];
+        if (m < 0) continue;
+        w1[m] = w0[m] + dh;
+        uh1[m] = uh0[m] + dx;
+        vh1[m] = vh0[m] + dy;
+        if (dw_t) {
+            // Area-weighted (the error of the volume and momentum a triangle
+            // gains) Huber loss of the normalised error: quadratic up to
+            // kHuber standard deviations, linear beyond, so the few violently
+            // changing triangles of a trajectory do not dominate.
+            constexpr double kHuber = 3.0;
+            const double wgt = area[m] / area_ref;
+            const double e[3] = {(dh - dw_t[m]) / sigma[0], (dx - duh_t[m]) / sigma[1], (dy - dvh_t[m]) / sigma[2]};
+            double l = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double ae = fabs(e[c]);
-            l += ae <= kHuber ? e[c] * e[c] : kHuber * (2.0 * ae - kHuber);
-            const double d = ae <= kHuber ? 2.0 * e[c] : 2.0 * kHuber * (e[c] > 0 ? 1.0 : -1.0);
-            gq[i * 3 + c] = (float)(wgt * d / ((double)N * sigma[c]));
+            for (int c = 0; c < 3; c++) {
+                const double ae = fabs(e[c]);
+                l += ae <= kHuber ? e[c] * e[c] : kHuber * (2.0 * ae - kHuber);
+                const double d = ae <= kHuber ? 2.0 * e[c] : 2.0 * kHuber * (e[c] > 0 ? 1.0 : -1.0);
+                gq[(int64_t)m * 3 + c] = (float)(wgt * d / ((double)N * sigma[c]));
+            }
+            loss_terms[m] = (float)(wgt * l);
         }
-        loss_terms[i] = (float)(wgt * l);
     }
 }
 
@@ -445,8 +789,10 @@ __global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edge
     const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (i >= N) return;
     int mem[4] = {(int)i, -1, -1, -1};
-    if (group && group[i * 4 + 1] >= 0)
+    if (group && group[i * 4 + 1] >= 0) {
+        if (group[i * 4] != i) return;  // the group's first member sets the factor of every member
         for (int j = 0; j < 4; j++) mem[j] = group[i * 4 + j];
+    }
     double vol = 0.0, out = 0.0;
     for (int j = 0; j < 4; j++) {
         const int m = mem[j];
@@ -459,12 +805,14 @@ __global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edge
             const bool side_a = ea[e] == m;
             const int other = side_a ? eb[e] : ea[e];
             if (other >= 0 && (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3])) continue;
-            const double c = (double)F[(int64_t)e * kOut] * (double)len[e] * (double)fs;
+            const double c = (double)F[(int64_t)e * kFOut] * (double)len[e] * (double)fs;
             out += side_a ? fmax(c, 0.0) : fmax(-c, 0.0);
         }
     }
     out *= delta[graph_of[i]];
-    theta[i] = out > vol ? vol / out : 1.0;
+    const double th = out > vol ? vol / out : 1.0;
+    for (int j = 0; j < 4; j++)
+        if (mem[j] >= 0) theta[mem[j]] = th;
 }
 
 // Momentum of cells shallower than h_dry set to zero (as the solver's protect).
@@ -602,6 +950,50 @@ int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, cons
     return -1;
 }
 
+inline unsigned persist_blocks(int64_t rows) {
+    const int64_t tiles = rows / 16, b = (tiles + kB / 64 - 1) / (kB / 64);
+    return (unsigned)(b < kPersistMax ? (b > 0 ? b : 1) : kPersistMax);
+}
+
+int gnn_feat_enc(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
+                 const float *ny, const float *len, const uint8_t *closed, const double *w, const double *uh,
+                 const double *vh, const double *z, float depth_scale, float momentum_scale, float length_scale,
+                 const void *WT, const float *b, double g, double h_dry, float flux_scale, float *P, void *X0save,
+                 void *Zsave, void *H1) {
+    if (Epad % 64 || E > Epad || !(flux_scale > 0.0f)) return -1;
+    hipLaunchKernelGGL(k_feat_enc, dim3(persist_blocks(Epad / 4)), dim3(kB), 0, S(stream), E, Epad, ea, eb, nx, ny, len,
+                       closed, w, uh, vh, z, depth_scale, momentum_scale, length_scale, static_cast<const bf16 *>(WT),
+                       b, g, h_dry, 1.0 / (double)flux_scale, P, static_cast<bf16 *>(X0save),
+                       static_cast<bf16 *>(Zsave), static_cast<bf16 *>(H1));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_mp_layer(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
+                 const void *Sv, const void *WT, const float *b, void *Zsave, void *Hout) {
+    if (Epad % 16 || E > Epad || H == Hout || !Hout) return -1;
+    hipLaunchKernelGGL(k_mp_layer, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb,
+                       static_cast<const bf16 *>(H), static_cast<const bf16 *>(Sv), static_cast<const bf16 *>(WT), b,
+                       static_cast<bf16 *>(Zsave), static_cast<bf16 *>(Hout), nullptr, nullptr, nullptr, nullptr);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_mp_layer_head(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
+                      const void *Sv, const void *WT, const float *b, void *Hout, const void *WTh, const float *bh,
+                      const float *P, float *F) {
+    if (Epad % 16 || E > Epad || H == Hout || !WTh || !bh || !F) return -1;
+    hipLaunchKernelGGL(k_mp_layer, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb,
+                       static_cast<const bf16 *>(H), static_cast<const bf16 *>(Sv), static_cast<const bf16 *>(WT), b,
+                       nullptr, static_cast<bf16 *>(Hout), static_cast<const bf16 *>(WTh), bh, P, F);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_head(void *stream, int64_t Epad, const void *H4, const void *WT, const float *b, const float *P, float *F) {
+    if (Epad % 16) return -1;
+    hipLaunchKernelGGL(k_head, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), Epad, static_cast<const bf16 *>(H4),
+                       static_cast<const bf16 *>(WT), b, P, F);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int gnn_linear_bwd_x(void *stream, int64_t M, int K, int Nout, const void *dZ, const void *W, void *dX) {
     if (M % 16) return -1;
     const dim3 grid(blocks(M / 16 * 64)), blk(kB);
@@ -670,6 +1062,15 @@ int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *
     hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, eb, nx, ny, len, closed,
                        area, group, group_area, delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1,
                        dw_t, duh_t, dvh_t, sigma, area_ref, gq, loss_terms);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int gnn_base_flux(void *stream, int64_t E, const int32_t *ea, const int32_t *eb, const float *nx, const float *ny,
+                  const uint8_t *closed, const double *w, const double *uh, const double *vh, const double *z,
+                  double g, double h_dry, float flux_scale, float *P) {
+    if (!(flux_scale > 0.0f)) return -1;
+    hipLaunchKernelGGL(k_base_flux, dim3(blocks(E)), dim3(kB), 0, S(stream), E, ea, eb, nx, ny, closed, w, uh, vh, z,
+                       g, h_dry, 1.0 / (double)flux_scale, P);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -1082,6 +1082,28 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
     return sync_all(h, d);
 }
 
+// Steps to launch before the next host check: the estimated number of steps
+// to the nearest yield or final time of the running instances (at the CFL
+// timestep of the last step, +10% and two steps of margin), in whole graphs,
+// at most kBatchSteps.  Steps queued past a yield only run the halted early
+// exits, so this bounds that waste; a low estimate costs another check.
+static int64_t batch_estimate(const sw_handle *h, const std::vector<DevState> &d) {
+    int64_t est = kGraphSteps;
+    for (int i = 0; i < h->n_inst && i < (int)d.size(); i++) {
+        const DevState &s = d[i];
+        if (s.event != SW_EVENT_NONE || s.halt || s.error) continue;
+        const double dt = std::min(h->params.CFL * s.flux_timestep, h->params.evolve_max_timestep);
+        if (!(dt > 0.0)) return kBatchSteps;
+        double target = s.yieldtime;
+        if (s.has_finaltime && s.finaltime < target) target = s.finaltime;
+        const double n = (target - s.time) / dt;
+        if (!(n < (double)kBatchSteps)) return kBatchSteps;
+        est = std::max<int64_t>(est, (int64_t)(1.1 * n) + 2);
+    }
+    est = (est + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
+    return std::min<int64_t>(est, kBatchSteps);
+}
+
 int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_t *event) {
     if (!h || max_steps < 0) return SW_ERR_ARG;
     std::vector<DevState> d0, d;
@@ -1103,8 +1125,9 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     }
     // Launch in batches between host checks so a yield early in a long request
     // does not leave many idle steps queued.
-    const int64_t batch = kBatchSteps;
+    d = d0;
     while (launched < max_steps && rc == SW_OK) {
+        const int64_t batch = batch_estimate(h, d);
         int64_t this_batch = max_steps - launched < batch ? max_steps - launched : batch;
         int64_t done_here = 0;
         if (h->timing) {

@@ -19,8 +19,15 @@ configuration of the learned workload):
         S_i  = sum of H_l over the edges of triangle i
         C_l  = [H_l, S_a + S_b]            128
         H_l+1 = H_l + SiLU(C_l W_l + b_l)  64
-    F   = H4 W4 + b4                        32   (5 used)
+    F   = H4 W4 + b4 + P                    32   (5 used)
     q_i(t + delta) = q_i(t) + delta / A_i * sum of side-signed, rotated F * len
+
+P is the physical part of the edge outputs (gnn_base_flux): the first-order
+central-upwind flux of the centroid states after hydrostatic reconstruction,
+in FP64, exactly zero for a lake at rest.  The network learns the correction
+that turns one long first-order step into the FV solver's many short
+second-order steps; the untrained network (zero head) is the first-order
+scheme with the long step.
 
 Every layer, its backward pass, the message passing, the state update (FP64)
 and Adam run in the hand-written HIP kernels of ``csrc/gnn_kernels.hip``
@@ -38,6 +45,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 GNN_LIB_PATH = os.path.join(_HERE, "libanuga_gnn.so")
 
 FEAT, HID, CAT, OUT = 32, 64, 128, 32
+FOUT = 8  # row of the FP32 edge outputs (5 used)
+GRAVITY = 9.8
+H_DRY = 1.0e-3  # depth below which a cell's velocity is taken as zero (the solver's minimum_allowed_height)
 LAYERS = [("enc", FEAT, HID), ("mp1", CAT, HID), ("mp2", CAT, HID), ("mp3", CAT, HID), ("head", HID, OUT)]
 
 _gnn = None
@@ -54,7 +64,8 @@ def _load():
         _gnn = C.CDLL(GNN_LIB_PATH)
         for name in ("gnn_features", "gnn_linear_fwd", "gnn_linear_bwd_x", "gnn_linear_bwd_w", "gnn_silu_bwd",
                      "gnn_node_sum", "gnn_concat", "gnn_node_grad", "gnn_edge_grad", "gnn_apply", "gnn_apply_bwd",
-                     "gnn_limit", "gnn_protect", "gnn_adam", "gnn_cast_weights"):
+                     "gnn_limit", "gnn_protect", "gnn_adam", "gnn_cast_weights", "gnn_base_flux", "gnn_mp_layer",
+                     "gnn_feat_enc", "gnn_head", "gnn_mp_layer_head"):
             getattr(_gnn, name).restype = C.c_int
         f, i64, i32, p = C.c_float, C.c_int64, C.c_int, C.c_void_p
         _gnn.gnn_features.argtypes = [p, i64, i64] + [p] * 10 + [f, f, f, p]
@@ -72,6 +83,11 @@ def _load():
         _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 11 + [f, p, p]
         _gnn.gnn_adam.argtypes = [p, i64, p, p, p, p, f, f, f, f, i32]
         _gnn.gnn_cast_weights.argtypes = [p, i32, i32, p, p, p]
+        _gnn.gnn_base_flux.argtypes = [p, i64] + [p] * 9 + [C.c_double, C.c_double, f, p]
+        _gnn.gnn_mp_layer.argtypes = [p, i64, i64] + [p] * 8
+        _gnn.gnn_feat_enc.argtypes = [p, i64, i64] + [p] * 10 + [f, f, f, p, p, C.c_double, C.c_double, f] + [p] * 4
+        _gnn.gnn_head.argtypes = [p, i64] + [p] * 5
+        _gnn.gnn_mp_layer_head.argtypes = [p, i64, i64] + [p] * 11
     return _gnn
 
 
@@ -229,14 +245,18 @@ class EdgeGNN:
     """The network (FP32 master weights, BF16 copies) and its scales."""
 
     def __init__(self, seed=0, depth_scale=5.0, momentum_scale=5.0, length_scale=10.0, flux_scale=1.0,
-                 sigma=(1e-2, 1e-2, 1e-2), merge_radius=0.0):
+                 sigma=(1e-2, 1e-2, 1e-2), merge_radius=0.0, physics_base=True):
         torch = _torch()
         self.lib = _load()
         g = np.random.default_rng(seed)
         # merge_radius: the small-cell agglomeration radius of the graphs the
-        # network is trained and run on (small_cell_groups)
+        # network is trained and run on (small_cell_groups).  physics_base:
+        # the edge outputs are the first-order central-upwind flux
+        # (gnn_base_flux) plus the network's correction; without it the
+        # network predicts the whole flux.
         self.scales = dict(depth=float(depth_scale), momentum=float(momentum_scale), length=float(length_scale),
-                           flux=float(flux_scale), merge_radius=float(merge_radius))
+                           flux=float(flux_scale), merge_radius=float(merge_radius),
+                           physics_base=1.0 if physics_base else 0.0)
         self.sigma = torch.tensor(np.asarray(sigma, dtype=np.float32), device="cuda")
         self.W, self.b = {}, {}
         for name, K, Nn in LAYERS:
@@ -309,7 +329,8 @@ class Workspace:
         self.H = [torch.empty(E, HID, **bf) for _ in range(5)]  # H[1..4]
         self.Cc = [torch.empty(E, CAT, **bf) for _ in range(4)]  # C[1..3]
         self.S = torch.empty(N, HID, **bf)
-        self.F = torch.empty(E, OUT, dtype=torch.float32, device="cuda")
+        self.F = torch.empty(E, FOUT, dtype=torch.float32, device="cuda")
+        self.P = torch.empty(E, FOUT, dtype=torch.float32, device="cuda")  # physical part of F
         # backward
         self.dF = torch.empty(E, OUT, **bf)
         self.dH = torch.empty(E, HID, **bf)
@@ -326,24 +347,48 @@ class Workspace:
 
 
 def forward(net, graph, ws, w, uh, vh, z, save=True):
-    """Edge outputs F (FP32 [Epad][32]) of the state (FP64 device tensors).
-    save=False (inference) skips the pre-activations the backward pass needs."""
+    """Edge outputs F (FP32 [Epad][8]) of the state (FP64 device tensors).
+    save=True (training) keeps the features, concatenated inputs and
+    pre-activations the backward pass needs; save=False (inference) runs the
+    persistent fused kernels (features + physical part + encoder, then
+    message-passing layers with the concatenation fused in), with bitwise the
+    same result."""
     L, s = net.lib, net.stream()
     g = graph
-    _ck(L.gnn_features(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.len), _ptr(g.closed),
-                       _ptr(w), _ptr(uh), _ptr(vh), _ptr(z), net.scales["depth"], net.scales["momentum"],
-                       net.scales["length"], _ptr(ws.X0)), "features")
-    _ck(L.gnn_linear_fwd(s, g.Epad, FEAT, HID, _ptr(ws.X0), _ptr(net.WTb["enc"]), _ptr(net.b["enc"]), None,
-                         _ptr(ws.Z[0]) if save else None, _ptr(ws.H[1]), 1, 0), "encoder")
+    base = net.scales["physics_base"] != 0.0
+    P = _ptr(ws.P) if base else None
+    sc = (net.scales["depth"], net.scales["momentum"], net.scales["length"])
+    if save:
+        _ck(L.gnn_features(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.len),
+                           _ptr(g.closed), _ptr(w), _ptr(uh), _ptr(vh), _ptr(z), *sc, _ptr(ws.X0)), "features")
+        if base:
+            _ck(L.gnn_base_flux(s, g.E, _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.closed), _ptr(w),
+                                _ptr(uh), _ptr(vh), _ptr(z), GRAVITY, H_DRY, net.scales["flux"], P), "base flux")
+        _ck(L.gnn_linear_fwd(s, g.Epad, FEAT, HID, _ptr(ws.X0), _ptr(net.WTb["enc"]), _ptr(net.b["enc"]), None,
+                             _ptr(ws.Z[0]), _ptr(ws.H[1]), 1, 0), "encoder")
+    else:
+        _ck(L.gnn_feat_enc(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(g.nx), _ptr(g.ny), _ptr(g.len),
+                           _ptr(g.closed), _ptr(w), _ptr(uh), _ptr(vh), _ptr(z), *sc, _ptr(net.WTb["enc"]),
+                           _ptr(net.b["enc"]), GRAVITY, H_DRY, net.scales["flux"], P, None, None, _ptr(ws.H[1])),
+            "features + encoder")
     for l in (1, 2, 3):
         name = "mp%d" % l
         _ck(L.gnn_node_sum(s, g.N, _ptr(g.tri_edges), _ptr(ws.H[l]), _ptr(ws.S)), "node sum")
-        _ck(L.gnn_concat(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(ws.H[l]), _ptr(ws.S), _ptr(ws.Cc[l])),
-            "concat")
-        _ck(L.gnn_linear_fwd(s, g.Epad, CAT, HID, _ptr(ws.Cc[l]), _ptr(net.WTb[name]), _ptr(net.b[name]),
-                             _ptr(ws.H[l]), _ptr(ws.Z[l]) if save else None, _ptr(ws.H[l + 1]), 1, 0), name)
-    _ck(L.gnn_linear_fwd(s, g.Epad, HID, OUT, _ptr(ws.H[4]), _ptr(net.WTb["head"]), _ptr(net.b["head"]), None, None,
-                         _ptr(ws.F), 0, 1), "head")
+        if save:  # training keeps the concatenated inputs for the weight gradients
+            _ck(L.gnn_concat(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(ws.H[l]), _ptr(ws.S), _ptr(ws.Cc[l])),
+                "concat")
+            _ck(L.gnn_linear_fwd(s, g.Epad, CAT, HID, _ptr(ws.Cc[l]), _ptr(net.WTb[name]), _ptr(net.b[name]),
+                                 _ptr(ws.H[l]), _ptr(ws.Z[l]), _ptr(ws.H[l + 1]), 1, 0), name)
+        elif l < 3:
+            _ck(L.gnn_mp_layer(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(ws.H[l]), _ptr(ws.S),
+                               _ptr(net.WTb[name]), _ptr(net.b[name]), None, _ptr(ws.H[l + 1])), name)
+        else:  # the last layer with the head in its epilogue (H4 is not stored)
+            _ck(L.gnn_mp_layer_head(s, g.E, g.Epad, _ptr(g.ea), _ptr(g.eb), _ptr(ws.H[l]), _ptr(ws.S),
+                                    _ptr(net.WTb[name]), _ptr(net.b[name]), None, _ptr(net.WTb["head"]),
+                                    _ptr(net.b["head"]), P, _ptr(ws.F)), name + " + head")
+    if save:
+        _ck(L.gnn_head(s, g.Epad, _ptr(ws.H[4]), _ptr(net.WTb["head"]), _ptr(net.b["head"]), P, _ptr(ws.F)),
+            "head")
     return ws.F
 
 
@@ -431,39 +476,46 @@ def training_domain(mesh, seed, extent, base_stage=5.0, friction=0.025):
     return d
 
 
-def fv_trajectories(meshes, per_mesh, samples, delta, warmup=2, extent=1400.0, seed=0, log=None):
-    """FV trajectories for training: for each mesh, ``per_mesh`` scenarios run
-    together as one ensemble with yields every ``delta`` seconds; after
-    ``warmup`` yields each records ``samples + 1`` states, one per yield.
-    Trajectories with non-finite or implausible values (depth > 100 m,
+def fv_trajectories(meshes, per_mesh, samples, delta, horizon=5.0, warmup=1, extent=1400.0, seed=0, log=None):
+    """FV training pairs: for each mesh, ``per_mesh`` scenarios run together as
+    one ensemble, yielding every ``delta`` seconds up to ``horizon`` seconds.
+    ``samples`` pairs (the state at a yield and at the next one) are recorded
+    per scenario, their first yields spread evenly over [warmup, horizon -
+    delta], so the pairs cover the whole time span a rollout of that length
+    sees.  Trajectories with non-finite or implausible values (depth > 100 m,
     |momentum| > 200 m^2/s) are dropped.  Returns one dict per mesh: z [T, N],
-    states [T, samples + 1, 3, N] (stage, xmomentum, ymomentum; FP64, on the
-    device), delta [T]."""
+    states [T, 2 * samples, 3, N] (pair j at rows 2j and 2j + 1; stage,
+    xmomentum, ymomentum; FP64, on the device), delta [T]."""
     torch = _torch()
     from . import _lib
     from .ensemble import Ensemble
 
+    nyields = max(int(round(horizon / delta)), warmup + 1)
+    starts = np.unique(np.linspace(warmup, nyields - 1, samples).round().astype(int))
+    record = {}
+    for j, y in enumerate(starts):
+        record.setdefault(int(y), []).append(2 * j)
+        record.setdefault(int(y) + 1, []).append(2 * j + 1)
+    S1 = 2 * len(starts)
     data = []
     for mi, mesh in enumerate(meshes):
         N = mesh.number_of_elements
         doms = [training_domain(mesh, seed * 100000 + mi * 1000 + t, extent) for t in range(per_mesh)]
         ens = Ensemble(doms)
         z = torch.empty(per_mesh, N, dtype=torch.float64, device="cuda")
-        states = torch.empty(per_mesh, samples + 1, 3, N, dtype=torch.float64, device="cuda")
+        states = torch.empty(per_mesh, S1, 3, N, dtype=torch.float64, device="cuda")
         ok = True
         try:
             ens.prepare(delta, None)
-            for _ in range(warmup):
-                ens.evolve_steps(1 << 30)
-                ens.handle.call("sw_advance_yield")
             torch.cuda.synchronize()
             for t, d in enumerate(doms):
                 d._handle.download_device(_lib.SW_ELEVATION, _lib.SW_CENTROID, z[t].data_ptr(), N)
-            for s in range(samples + 1):
-                for t, d in enumerate(doms):
-                    for c, q in enumerate((_lib.SW_STAGE, _lib.SW_XMOM, _lib.SW_YMOM)):
-                        d._handle.download_device(q, _lib.SW_CENTROID, states[t, s, c].data_ptr(), N)
-                if s == samples:
+            for y in range(max(record) + 1):
+                for row in record.get(y, ()):
+                    for t, d in enumerate(doms):
+                        for c, q in enumerate((_lib.SW_STAGE, _lib.SW_XMOM, _lib.SW_YMOM)):
+                            d._handle.download_device(q, _lib.SW_CENTROID, states[t, row, c].data_ptr(), N)
+                if y == max(record):
                     break
                 ens.evolve_steps(1 << 30)
                 ens.handle.call("sw_advance_yield")
@@ -482,10 +534,10 @@ def fv_trajectories(meshes, per_mesh, samples, delta, warmup=2, extent=1400.0, s
             continue
         data.append(dict(z=z[keep].contiguous(), states=states[keep].contiguous(),
                          delta=torch.full((keep.numel(),), float(delta), dtype=torch.float64, device="cuda"),
-                         mesh=mesh))
+                         mesh=mesh, pair_start_times=starts * float(delta)))
         if log:
-            log("mesh %d: %d triangles, %d of %d trajectories kept x %d samples" % (
-                mi, N, keep.numel(), per_mesh, samples))
+            log("mesh %d: %d triangles, %d of %d trajectories kept x %d pairs (t = %.2f .. %.2f s)" % (
+                mi, N, keep.numel(), per_mesh, len(starts), starts[0] * delta, starts[-1] * delta))
     return data
 
 
@@ -497,7 +549,7 @@ def fit_scales(net, data):
         st = d["states"]
         h.append((st[:, :, 0, :] - d["z"][:, None, :]).clamp(min=0).flatten())
         q.append(st[:, :, 1:, :].flatten())
-        dq.append((st[:, 1:] - st[:, :-1]).transpose(0, 2).reshape(3, -1))
+        dq.append((st[:, 1::2] - st[:, 0::2]).transpose(0, 2).reshape(3, -1))
     h = torch.cat(h)
     q = torch.cat(q)
     dq = torch.cat(dq, dim=1)
@@ -528,7 +580,7 @@ class Trainer:
         d, b = self.data[m], self.buf[m]
         T, S1 = d["states"].shape[:2]
         tr = torch_index(pick[:, 0])
-        sm = torch_index(pick[:, 1])
+        sm = torch_index(2 * pick[:, 1])  # pair j: rows 2j (start) and 2j + 1 (one step later)
         s0 = d["states"][tr, sm]          # [B, 3, N]
         s1 = d["states"][tr, sm + 1]
         b["w"].copy_(s0[:, 0].reshape(-1))
@@ -552,15 +604,15 @@ class Trainer:
         return float(ws.loss_terms.sum()) / g.N
 
     def sample(self, m, holdout=False):
+        """B (trajectory, pair) picks.  The scenarios of every fourth
+        trajectory index are held out for evaluation (all their pairs)."""
         d = self.data[m]
         T, S1 = d["states"].shape[:2]
-        S = S1 - 1
-        # the last sample of every trajectory is held out for evaluation
-        if holdout:
-            tr = self.rng.integers(0, T, self.B)
-            return np.stack([tr, np.full(self.B, S - 1)], 1)
-        tr = self.rng.integers(0, T, self.B)
-        sm = self.rng.integers(0, max(S - 1, 1), self.B)
+        P = S1 // 2
+        held = np.arange(T) % 4 == 3
+        pool = np.nonzero(held if (holdout and held.any()) else ~held if (~held).any() else np.ones(T, bool))[0]
+        tr = pool[self.rng.integers(0, len(pool), self.B)]
+        sm = self.rng.integers(0, P, self.B)
         return np.stack([tr, sm], 1)
 
     def train(self, iters, log=None, every=100, cosine=True):
@@ -580,13 +632,16 @@ class Trainer:
         self.lr = lr0
         return hist
 
-    def evaluate(self, batches=8):
+    def evaluate(self, batches=8, seed=12345):
         """Mean held-out loss (1.0 = no better than leaving the state unchanged
-        for unit-variance changes)."""
+        for unit-variance changes) over ``batches`` batches of held-out pairs,
+        the same pairs for the same seed."""
+        rng, self.rng = self.rng, np.random.default_rng(seed)
         ls = []
         for _ in range(batches):
             m = int(self.rng.integers(0, len(self.data)))
             ls.append(self.loss(m, self.sample(m, holdout=True), train=False))
+        self.rng = rng
         return float(np.mean(ls))
 
 
@@ -616,7 +671,7 @@ def learned_step(net, graph, ws, delta, w, uh, vh, z, h_dry=1e-3):
             "protect")
 
 
-def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, log=None):
+def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, judge_precision="fp64", log=None):
     """Learned rollout against the FP64 FV path on ``domain`` (an FV Domain,
     initial state set), in steps of ``delta`` seconds.
 
@@ -632,8 +687,15 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
        window is recomputed with the FV loop from the window's start and the
        FV loop also runs the next window (fallback).  Should the FV loop fail
        from a window start, it restarts from the window start before.
-    Returns timings, per-step stage errors against the reference, the mass
-    change against the reference and the judge's record."""
+    ``judge_precision`` 'mixed' runs the judge's FV step with the
+    mixed-precision loop (FP32 flux and extrapolation, FP64 accumulation,
+    timestep and depth tests: about 1e-7 relative of the FP64 loop, five
+    orders below a 1% threshold); the reference trajectory and every
+    fallback recomputation run in FP64.
+    Returns timings (with a breakdown into learned steps, judges,
+    recomputation and the rest), per-step stage errors against the
+    reference, the mass change against the reference and the judge's
+    record."""
     import time
 
     torch = _torch()
@@ -641,6 +703,19 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
 
     N = domain.number_of_elements
     h = domain._dev()
+    if judge_precision not in ("fp64", "mixed"):
+        raise ValueError("judge_precision must be 'fp64' or 'mixed'")
+    p64 = _lib.SwParams.from_buffer_copy(h.params)
+    p64.precision = _lib.SW_PRECISION_FP64
+    pjudge = _lib.SwParams.from_buffer_copy(p64)
+    if judge_precision == "mixed":
+        pjudge.precision = _lib.SW_PRECISION_MIXED
+
+    def use(p):
+        if h.params.precision != p.precision:
+            h.set_params(_lib.SwParams.from_buffer_copy(p))
+
+    use(p64)
     h.call("sw_evolve_prepare", 1e9, 0.0, 0)
     dev = lambda: torch.empty(N, dtype=torch.float64, device="cuda")
     w0, uh0, vh0, z = dev(), dev(), dev(), dev()
@@ -675,8 +750,10 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
         h.set_state(s)
         h.call("sw_evolve_prepare", delta, 0.0, 0)
 
-    def fv_advance(w, uh, vh, t, nyields):
-        """FV over nyields yields of delta from (w, uh, vh) at time t; result in place."""
+    def fv_advance(w, uh, vh, t, nyields, judge=False):
+        """FV over nyields yields of delta from (w, uh, vh) at time t; result
+        in place (the judge's precision if ``judge``, else FP64)."""
+        use(pjudge if judge else p64)
         set_state(w, uh, vh, t)
         total = 0
         for k in range(nyields):
@@ -712,6 +789,21 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
     graph = EdgeGraph(domain.mesh, copies=1, closed_tags=(), merge_radius=net.scales["merge_radius"])
     ws = Workspace(graph)
     dl = torch.tensor([delta], dtype=torch.float64, device="cuda")
+    # cost of one learned step on this domain (events on the stream the
+    # kernels run on), from a scratch copy of the initial state
+    sw, suh, svh = w0.clone(), uh0.clone(), vh0.clone()
+    learned_step(net, graph, ws, dl, sw, suh, svh, z)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        learned_step(net, graph, ws, dl, sw, suh, svh, z)
+    ev1.record()
+    torch.cuda.synchronize()
+    step_ms = ev0.elapsed_time(ev1) / 10.0
+    del sw, suh, svh
+    if log:
+        log("learned step: %.3f ms (%d edges, %d of %d triangles in merged groups); FV: %.3f ms per %g s" % (
+            step_ms, graph.E, graph.merged, N, 1000.0 * fv_time / steps, delta))
     w, uh, vh = w0.clone(), uh0.clone(), vh0.clone()
     errors, judged = [], []
     fv_windows = 0
@@ -719,8 +811,13 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
     fallback = False
     health_steps = 2.0 * fv_steps / steps  # a judge needing more FV steps than this saw a damaged state
     history = [(0, (w0, uh0, vh0))]  # window starts, oldest first
-    torch.cuda.synchronize()
-    tic = time.perf_counter()
+    spent = dict(learned=0.0, judge=0.0, recompute=0.0)
+
+    def now():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+
+    tic = now()
     k = 0
 
     def recompute(k_end):
@@ -743,29 +840,37 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
         n = min(judge_every, steps - k)
         tw = t0 + k * delta
         if fallback:
+            t1 = now()
             (w, uh, vh), done = recompute(k + n)
+            spent["recompute"] += now() - t1
             fv_in_rollout += done
             fv_windows += 1
             fallback = False
         else:
+            t1 = now()
             for j in range(n - 1):
                 learned_step(net, graph, ws, dl, w, uh, vh, z)
             last = (w.clone(), uh.clone(), vh.clone())
             learned_step(net, graph, ws, dl, w, uh, vh, z)
+            t2 = now()
+            spent["learned"] += t2 - t1
             try:
-                done = fv_advance(last[0], last[1], last[2], tw + (n - 1) * delta, 1)
+                done = fv_advance(last[0], last[1], last[2], tw + (n - 1) * delta, 1, judge=True)
                 fv_in_rollout += done
                 err = wl2(w - last[0]) / wl2(last[0] - z)
                 if done > health_steps:
                     err = float("inf")
             except _lib.SwError:
                 done, err = -1, float("inf")
+            spent["judge"] += now() - t2
             judged.append((k + n - 1, err))
             if log:
                 log("window ending at step %d: judge error %.3g (%d FV steps) -> %s" % (
                     k + n - 1, err, done, "recomputed by FV" if err > threshold else "accepted"))
             if err > threshold:
+                t1 = now()
                 (w, uh, vh), done = recompute(k + n)
+                spent["recompute"] += now() - t1
                 fv_in_rollout += done
                 fv_windows += 1
                 fallback = True
@@ -774,8 +879,8 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
         k += n
         history.append((k, (w.clone(), uh.clone(), vh.clone())))
         errors.append((k - 1, w - ref_w[k - 1]))
-    torch.cuda.synchronize()
-    learned_time = time.perf_counter() - tic
+    learned_time = now() - tic
+    use(p64)
     err_l2 = [(i, wl2(e) / wl2(ref_w[i] - z)) for i, e in errors]
     mass_final = float(torch.sum(area * (w - z)))
     out = dict(triangles=N, steps=steps, delta=delta, fv_steps=fv_steps, fv_seconds=fv_time,
@@ -783,12 +888,18 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, l
                learned_rate=N * fv_steps / learned_time, speedup=fv_time / learned_time,
                stage_rel_l2=err_l2, max_stage_rel_l2=max(e for _, e in err_l2), final_stage_rel_l2=err_l2[-1][1],
                mass_change_vs_fv=abs(mass_final - ref_mass) / abs(mass0), judged=judged,
-               fallback_windows=fv_windows, fv_steps_in_rollout=fv_in_rollout)
+               fallback_windows=fv_windows, fv_steps_in_rollout=fv_in_rollout, learned_step_ms=step_ms,
+               fv_ms_per_learned_step=1000.0 * fv_time / steps, merged_triangles=graph.merged,
+               judge_precision=judge_precision,
+               seconds_breakdown=dict(spent, other=learned_time - sum(spent.values())))
     if log:
-        log("learned: %d steps in %.3f s (%.3e FV-equivalent triangle-steps/s, %.2fx FV); stage rel L2 at window "
-            "ends max %.3g final %.3g; %d windows judged, %d recomputed by FV" % (
-                steps, learned_time, out["learned_rate"], out["speedup"], out["max_stage_rel_l2"],
-                out["final_stage_rel_l2"], len(judged), fv_windows))
+        log("learned (%s judge): %d steps in %.3f s (%.3e FV-equivalent triangle-steps/s, %.2fx FV); stage rel "
+            "L2 at window ends max %.3g final %.3g; %d windows judged, %d recomputed by FV; seconds: %s" % (
+                judge_precision, steps, learned_time, out["learned_rate"], out["speedup"], out["max_stage_rel_l2"],
+                out["final_stage_rel_l2"], len(judged), fv_windows,
+                {k: round(v, 4) for k, v in out["seconds_breakdown"].items()}))
+    # leave the domain in its initial state
+    set_state(w0, uh0, vh0, t0)
     return out
 
 
@@ -804,7 +915,8 @@ def training_meshes(count, extent=1400.0, density=5000.0, seed=7):
 
 def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.05, iters=3000, batch=16, lr=1e-3,
                          eval_points=500000, eval_steps=100, judge_every=8, threshold=0.01, extent=1400.0,
-                         merge_radius=1.0, log=None):
+                         merge_radius=2.0, horizon=None, save_net=None, load_net=None, judge_precision="mixed",
+                         log=None):
     """Train on FV trajectories of configuration-2-style windows, then roll the
     network out on the configuration-2 workload (1M triangles) against the
     FP64 FV path.  Returns a dict of the results."""
@@ -812,30 +924,93 @@ def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.05, it
 
     from . import scenarios
 
-    t0 = time.time()
-    meshes = training_meshes(train_meshes, extent=extent)
-    data = fv_trajectories(meshes, per_mesh=per_mesh, samples=samples, delta=delta, extent=extent, log=None)
-    n_traj = sum(d["states"].shape[0] for d in data)
-    t_data = time.time() - t0
-    if log:
-        log("training data: %d trajectories x %d samples on %d meshes of %d triangles (%.1f s)" % (
-            n_traj, samples, len(data), meshes[0].number_of_elements, t_data))
-    net = EdgeGNN(seed=1, merge_radius=merge_radius)
-    fit_scales(net, data)
-    tr = Trainer(net, data, batch=batch, lr=lr)
-    untrained = tr.evaluate(16)
-    t0 = time.time()
-    hist = tr.train(iters, log=log, every=max(iters // 6, 1))
-    _torch().cuda.synchronize()
-    t_train = time.time() - t0
-    trained = tr.evaluate(16)
-    if log:
-        log("held-out one-step loss: untrained %.4f, trained %.4f (%d iterations, %.1f s)" % (
-            untrained, trained, iters, t_train))
+    horizon = eval_steps * delta if horizon is None else horizon
+    train = dict(train_pairs=samples, train_horizon=horizon, train_iters=iters)
+    if load_net:
+        # a network trained earlier (its scales and merge radius come with it)
+        net = EdgeGNN.load(load_net)
+        train.update(loaded_network=os.path.basename(load_net))
+        if log:
+            log("network loaded from %s (no training in this run)" % load_net)
+    else:
+        t0 = time.time()
+        meshes = training_meshes(train_meshes, extent=extent)
+        data = fv_trajectories(meshes, per_mesh=per_mesh, samples=samples, delta=delta, horizon=horizon,
+                               extent=extent, log=log)
+        n_traj = sum(d["states"].shape[0] for d in data)
+        t_data = time.time() - t0
+        if log:
+            log("training data: %d trajectories x %d pairs on %d meshes of %d triangles (%.1f s)" % (
+                n_traj, samples, len(data), meshes[0].number_of_elements, t_data))
+        net = EdgeGNN(seed=1, merge_radius=merge_radius)
+        fit_scales(net, data)
+        tr = Trainer(net, data, batch=batch, lr=lr)
+        untrained = tr.evaluate(16)
+        t0 = time.time()
+        hist = tr.train(iters, log=log, every=max(iters // 6, 1))
+        _torch().cuda.synchronize()
+        t_train = time.time() - t0
+        trained = tr.evaluate(16)
+        if save_net:
+            net.save(save_net)
+        if log:
+            log("held-out one-step loss: untrained (first-order scheme) %.4f, trained %.4f (%d iterations, %.1f s)"
+                % (untrained, trained, iters, t_train))
+        train.update(train_trajectories=n_traj, train_seconds=t_train, data_seconds=t_data,
+                     holdout_loss_untrained=untrained, holdout_loss_trained=trained,
+                     train_loss_last=float(np.mean(hist[-50:])))
+        del data, tr
     dom = scenarios.delaunay_terrain(npoints=eval_points)
-    res = evaluate_rollout(net, dom, eval_steps, delta, judge_every=judge_every, threshold=threshold, log=log)
+    res = evaluate_rollout(net, dom, eval_steps, delta, judge_every=judge_every, threshold=threshold,
+                           judge_precision=judge_precision, log=log)
+    other = "fp64" if judge_precision == "mixed" else "mixed"
+    alt = evaluate_rollout(net, dom, eval_steps, delta, judge_every=judge_every, threshold=threshold,
+                           judge_precision=other, log=log)
+    res["with_%s_judge" % other] = {k: alt[k] for k in (
+        "speedup", "learned_rate", "fv_rate", "final_stage_rel_l2", "max_stage_rel_l2", "fallback_windows",
+        "seconds_breakdown")}
     dom.close()
-    res.update(train_trajectories=n_traj, train_samples=samples, train_iters=iters, train_seconds=t_train,
-               data_seconds=t_data, holdout_loss_untrained=untrained, holdout_loss_trained=trained,
-               train_loss_last=float(np.mean(hist[-50:])))
+    res.update(train)
     return res
+
+
+def time_learned_step(points=500000, steps=20, delta=0.05, merge_radius=2.0):
+    """Milliseconds per learned inference step on the seeded Delaunay workload
+    of ``points`` points (an untrained network: the same kernels and cost as a
+    trained one)."""
+    torch = _torch()
+    from . import _lib, scenarios
+
+    dom = scenarios.delaunay_terrain(npoints=points)
+    N = dom.number_of_elements
+    h = dom._dev()
+    buf = {q: torch.empty(N, dtype=torch.float64, device="cuda")
+           for q in (_lib.SW_STAGE, _lib.SW_XMOM, _lib.SW_YMOM, _lib.SW_ELEVATION)}
+    for q, b in buf.items():
+        h.download_device(q, _lib.SW_CENTROID, b.data_ptr(), N)
+    net = EdgeGNN(seed=1, merge_radius=merge_radius)
+    graph = EdgeGraph(dom.mesh, closed_tags=(), merge_radius=merge_radius)
+    ws = Workspace(graph)
+    dl = torch.tensor([delta], dtype=torch.float64, device="cuda")
+    w, uh, vh, z = (buf[q] for q in (_lib.SW_STAGE, _lib.SW_XMOM, _lib.SW_YMOM, _lib.SW_ELEVATION))
+    learned_step(net, graph, ws, dl, w, uh, vh, z)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        learned_step(net, graph, ws, dl, w, uh, vh, z)
+    ev1.record()
+    torch.cuda.synchronize()
+    dom.close()
+    return ev0.elapsed_time(ev1) / steps, N, graph.E
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="time the learned inference step")
+    ap.add_argument("--points", type=int, default=500000)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--merge-radius", type=float, default=2.0)
+    a = ap.parse_args()
+    ms, n, e = time_learned_step(a.points, a.steps, merge_radius=a.merge_radius)
+    print("learned step: %.3f ms on %d triangles, %d edges" % (ms, n, e), flush=True)

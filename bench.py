@@ -93,6 +93,11 @@ def main():
     ap.add_argument("--delta", type=float, default=0.05, help="learned: seconds per learned step")
     ap.add_argument("--train-iters", type=int, default=3000, help="learned: training iterations")
     ap.add_argument("--train-meshes", type=int, default=40, help="learned: training meshes (50 trajectories each)")
+    ap.add_argument("--save-net", default=None, help="learned: write the trained network (.npz)")
+    ap.add_argument("--load-net", default=None, help="learned: use this trained network (.npz) instead of training")
+    ap.add_argument("--judge-precision", default="mixed", choices=["mixed", "fp64"],
+                    help="learned: precision of the FV judge's step (the reference and fallbacks are FP64; the "
+                         "other choice is reported beside it)")
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 200; ensemble 300)")
     ap.add_argument("--warmup", type=int, default=20)
@@ -318,7 +323,8 @@ def bench_learned(args):
     from anuga_cuda_amd import learned
 
     res = learned.run_learned_workload(train_meshes=args.train_meshes, delta=args.delta, iters=args.train_iters,
-                                       eval_points=args.points, log=log)
+                                       eval_points=args.points, save_net=args.save_net, load_net=args.load_net,
+                                       judge_precision=args.judge_precision, log=log)
     out = {
         "metric": "triangle-steps/s at 1M tris, FV vs learned at stated error; % HBM peak",
         "value": res["learned_rate"],
@@ -334,7 +340,8 @@ def bench_learned(args):
         "data": "synthetic",
         "config": {"workload": "learned edge-GNN surrogate (3 message-passing layers, width 64, BF16 MFMA) on "
                                "the seeded 1M-triangle Delaunay workload; learned steps of %g s; FV judge "
-                               "every 8 steps, fallback above 1%% stage L2 error" % args.delta,
+                               "(%s step) every 8 steps, FP64 FV fallback above 1%% stage L2 error; error "
+                               "against the FP64 FV path" % (args.delta, args.judge_precision),
                    "triangles": res["triangles"]},
         "learned": {k: v for k, v in res.items() if k not in ("stage_rel_l2", "judged")},
         "stage_rel_l2_at_window_ends": res["stage_rel_l2"],

@@ -13,7 +13,8 @@
  * All pointers are device pointers; every launch goes to `stream` (a
  * hipStream_t, 0 for the default stream).  Matrices are row-major.  Row
  * counts M are multiples of 16 (callers pad), K is 32, 64 or 128, Nout is 32
- * or 64.  Returns 0, or -1 for an unsupported shape.
+ * or 64.  Returns 0, or -1 for an unsupported shape.  The edge outputs F are
+ * FP32 rows of 8 (5 used); their gradient dF is BF16 rows of 32.
  */
 #ifndef ANUGA_GNN_H
 #define ANUGA_GNN_H
@@ -41,6 +42,44 @@ int gnn_features(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const
 int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, const void *WT, const float *b,
                    const void *R, void *Zsave, void *Y, int act, int out_f32);
 
+/* The inference layers below are persistent kernels: each wavefront keeps
+ * its layer's weights in registers over many 16-row tiles.  Their arithmetic
+ * is that of gnn_features / gnn_concat / gnn_linear_fwd, bit for bit. */
+
+/* Features, physical part and encoder in one pass: H1[Epad][64] =
+ * SiLU(X0 W + b) with X0 the edge features of gnn_features (WT [64][32]),
+ * and P[Epad][8] (may be NULL) the physical part of the edge outputs
+ * (gnn_base_flux; rows E .. Epad-1 zero).  X0save and Zsave (may be NULL)
+ * receive the features and the pre-activation. */
+int gnn_feat_enc(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
+                 const float *ny, const float *len, const uint8_t *closed, const double *w, const double *uh,
+                 const double *vh, const double *z, float depth_scale, float momentum_scale, float length_scale,
+                 const void *WT, const float *b, double g, double h_dry, float flux_scale, float *P, void *X0save,
+                 void *Zsave, void *H1);
+
+/* One message-passing layer, the concatenation fused in:
+ * Hout = H + SiLU([H, S[ea] + S[eb]] W + b) for the Epad (multiple of 16)
+ * edge rows, with WT [64][128] and the node sums S [N][64] (BF16); rows E ..
+ * Epad-1 take a zero input row.  Same result as gnn_concat followed by
+ * gnn_linear_fwd (K 128, Nout 64, SiLU, residual H).  Zsave (may be NULL)
+ * receives the pre-activation.  Hout must not alias H. */
+int gnn_mp_layer(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
+                 const void *Sv, const void *WT, const float *b, void *Zsave, void *Hout);
+
+/* The last message-passing layer with the head fused in (gnn_mp_layer, then
+ * gnn_head on its output rows while they are in registers): F as gnn_head
+ * computes it, bit for bit.  Hout (may be NULL) receives the layer's output.
+ * The persistent kernels above and this one need Epad a multiple of 16
+ * (gnn_feat_enc: of 64). */
+int gnn_mp_layer_head(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
+                      const void *Sv, const void *WT, const float *b, void *Hout, const void *WTh, const float *bh,
+                      const float *P, float *F);
+
+/* Head: F[Epad][8] (FP32) = the first 8 columns of H4 W + b (WT [32][64]),
+ * plus P[Epad][8] (may be NULL: the physical part).  Columns 0..4 are used
+ * (gnn_apply). */
+int gnn_head(void *stream, int64_t Epad, const void *H4, const void *WT, const float *b, const float *P, float *F);
+
 /* dX[M][K] = dZ[M][Nout] W^T, with W [K][Nout] (BF16). */
 int gnn_linear_bwd_x(void *stream, int64_t M, int K, int Nout, const void *dZ, const void *W, void *dX);
 
@@ -65,7 +104,7 @@ int gnn_node_grad(void *stream, int64_t N, const int32_t *tri_edges, const void 
 int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *dHnext,
                   const void *dC, const void *dS, void *dH);
 
-/* Apply the head's outputs F[E][32] (FP32; columns 0: mass flux, 1-2: side-a
+/* Apply the head's outputs F[E][8] (FP32; columns 0: mass flux, 1-2: side-a
  * momentum (normal, tangential), 3-4: side-b momentum) over a step delta:
  * dq[i] = delta / area[i] * sum over edges of side-signed, rotated
  * contributions * len * flux_scale; closed[e] != 0 zeroes the mass flux of a
@@ -96,6 +135,19 @@ int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *
               const float *len, const uint8_t *closed, const double *area, const int32_t *group, const double *delta,
               const int32_t *graph_of, const float *F, float flux_scale, const double *w, const double *z,
               double *theta);
+
+/* The physical part of the edge outputs into P[E][8] (columns 0..4, 5..7
+ * zero; in the outputs' units: divided by flux_scale; gnn_head adds it to the
+ * network's outputs): the first-order central-upwind
+ * flux of the two centroid states after hydrostatic reconstruction, FP64
+ * (mass flux from ea to eb; normal / tangential momentum change per unit
+ * length of side ea, then of side eb, each side's normal momentum without its
+ * own reconstructed pressure, which sums to zero around a cell).  Velocities
+ * are zero where the depth is at most h_dry.  Exactly zero for a lake at
+ * rest. */
+int gnn_base_flux(void *stream, int64_t E, const int32_t *ea, const int32_t *eb, const float *nx, const float *ny,
+                  const uint8_t *closed, const double *w, const double *uh, const double *vh, const double *z,
+                  double g, double h_dry, float flux_scale, float *P);
 
 /* uh = vh = 0 where w - z < h_dry (the solver's protect, without its mass
  * change). */
