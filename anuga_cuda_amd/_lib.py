@@ -94,6 +94,8 @@ _SIGNATURES = {
     "sw_download": ([C.c_void_p, C.c_int, C.c_int, _F64P, C.c_int64], C.c_int),
     "sw_upload_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64], C.c_int),
     "sw_download_device": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64], C.c_int),
+    "sw_restart_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double], C.c_int),
+    "sw_download_conserved_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "sw_get_state": ([C.c_void_p, C.POINTER(SwState)], C.c_int),
     "sw_set_state": ([C.c_void_p, C.POINTER(SwState)], C.c_int),
     "sw_download_max_speed": ([C.c_void_p, _F64P, C.c_int64], C.c_int),
@@ -304,6 +306,16 @@ class Handle:
     def download_device(self, q, loc, ptr, count):
         """Download into a device buffer (address ``ptr``, ``count`` doubles)."""
         self.call("sw_download_device", int(q), int(loc), C.c_void_p(int(ptr)), int(count))
+
+    def restart_device(self, w, uh, vh, time, yieldstep):
+        """Restart from device buffers of the conserved centroid values at
+        ``time``, yields every ``yieldstep`` (sw_restart_device)."""
+        self.call("sw_restart_device", C.c_void_p(int(w)), C.c_void_p(int(uh)), C.c_void_p(int(vh)), float(time),
+                  float(yieldstep))
+
+    def download_conserved_device(self, w, uh, vh):
+        """Conserved centroid values into three device buffers."""
+        self.call("sw_download_conserved_device", C.c_void_p(int(w)), C.c_void_p(int(uh)), C.c_void_p(int(vh)))
 
     def get_state(self, check=True):
         """The evolve state of the selected instance.  With ``check`` a latched

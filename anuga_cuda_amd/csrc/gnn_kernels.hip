@@ -37,6 +37,24 @@ GNN_DEV float silu_grad(float z) {
     const float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
+// Workgroup b of nb renumbered so that each of the 8 XCDs (workgroups are
+// dispatched to them round-robin) takes one contiguous range of the index
+// space: the neighbour gathers of a range then hit that XCD's own L2.
+GNN_DEV int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t per = nb >> 3, rem = nb & 7, x = b & 7, idx = b >> 3;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+// Global thread index of a one-thread-per-item kernel, in XCD-contiguous order.
+GNN_DEV int64_t gidx() { return xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x; }
+// Tiles of a persistent kernel: wavefront w of the launch takes tiles w,
+// w + nwaves, ...  All wavefronts sweep the array together, so the rows they
+// gather from nearby edges are shared in the last-level cache while they are
+// hot (measured faster than giving each workgroup a contiguous range).
+GNN_DEV void wave_tiles(int64_t &t0, int64_t &step) {
+    t0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+    step = (int64_t)gridDim.x * (kB / 64);
+}
+
 GNN_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 GNN_DEV bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 GNN_DEV void st8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
@@ -113,7 +131,7 @@ __global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const 
                                                  const uint8_t *closed, const double *w, const double *uh,
                                                  const double *vh, const double *z, float hs, float qs, float ls,
                                                  bf16 *X) {
-    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t e = gidx();
     if (e >= Epad) return;
     float f[kFeat];
     edge_features(e, E, ea, eb, nx, ny, len, closed, w, uh, vh, z, hs, qs, ls, f);
@@ -218,7 +236,7 @@ GNN_DEV void linear_tile(int64_t m0, int r, int g, const bf16x8 *a, const bf16 *
 template <int K, int NOUT, bool OUTF32>
 __global__ __launch_bounds__(kB) void k_linear_fwd(int64_t M, const bf16 *X, const bf16 *WT, const float *bias,
                                                    const bf16 *R, bf16 *Zs, void *Yv, int act) {
-    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+    const int64_t wave = (gidx()) >> 6;
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int64_t m0 = wave * 16;
     if (m0 >= M) return;
@@ -300,7 +318,6 @@ __global__ __launch_bounds__(kB) void k_feat_enc(int64_t E, int64_t Epad, const 
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     float(*T)[kTileLd] = tiles[threadIdx.x >> 6];
     bf16(*X)[kXLd] = xtile[threadIdx.x >> 6];
-    const int64_t wave0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6, nwaves = (int64_t)gridDim.x * (kB / 64);
     bf16x8 bw[4];
     float br[4];
 #pragma unroll
@@ -308,8 +325,10 @@ __global__ __launch_bounds__(kB) void k_feat_enc(int64_t E, int64_t Epad, const 
         bw[nb] = ld8(WT + (nb * 16 + r) * kFeat + g * 8);
         br[nb] = bias[nb * 16 + r];
     }
-    const int64_t ntiles = Epad / 64;
-    for (int64_t t = wave0; t < ntiles; t += nwaves) {
+    int64_t t0, step;
+    wave_tiles(t0, step);
+    const int64_t t1 = Epad / 64;
+    for (int64_t t = t0; t < t1; t += step) {
         const int64_t m64 = t * 64, e = m64 + lane;
         float f[kFeat];
         edge_features(e, E, ea, eb, nx, ny, len, closed, w, uh, vh, z, hs, qs, ls, f);
@@ -344,23 +363,26 @@ __global__ __launch_bounds__(kB) void k_feat_enc(int64_t E, int64_t Epad, const 
     }
 }
 
-// One message-passing layer with the concatenation fused in:
-// Hout = H + SiLU([H, S_ea + S_eb] W + b); the second half of each A row is
-// formed from the node sums as k_concat forms it (FP32 sum rounded to BF16;
-// rows E .. Epad-1 zero), so the result is that of k_concat followed by
-// k_linear_fwd<128, 64> with residual H, without the concatenation's round
-// trip through HBM.  The workgroup's four wavefronts share the weights in
-// LDS (rows padded by 16 bytes: the 16 rows a fragment load reads fall in
-// different banks).  With WTh (the head's weights, [32][64]) the head
-// follows in registers: F[Epad][kFOut] = the first kFOut columns of
-// Hout W_h + b_h (+ P), bitwise what k_head computes from the stored Hout;
-// Hout (may then be NULL) is stored only if given.
-constexpr int kWLd = 128 + 8;  // LDS row of the layer's weights (BF16)
+constexpr int kWLd = 128 + 8;  // LDS row of a message-passing layer's weights (BF16)
 
-__global__ __launch_bounds__(kB) void k_mp_layer(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
-                                                 const bf16 *H, const bf16 *S, const bf16 *WT, const float *bias,
-                                                 bf16 *Zs, bf16 *Hout, const bf16 *WTh, const float *bias_h,
-                                                 const float *P, float *F) {
+// One message-passing layer, node sums and concatenation folded into the
+// matrix product: [H_e, S_ea + S_eb] W = H_e W1 + sum over the edges t of
+// both triangles of H_t W2 (W1, W2 the first and second 64 rows of W), so
+// the gathered rows go to the matrix cores as they are (K = 7 x 64: the
+// edge's own row, then the three edges of ea and the three of eb in
+// tri_edges order from nbr6[e], -1 for a missing triangle), with the sums
+// exact in FP32 rather than rounded to BF16 first.  The workgroup's four
+// wavefronts share the weights in LDS (rows padded by 16 bytes: the 16 rows a
+// fragment load reads fall in different banks); each wavefront walks tiles of
+// 16 edges, the next tile's rows and neighbour lists requested before this
+// tile's gathers.  Zs (may be NULL) receives the pre-activation; Hout (may be
+// NULL when the head follows) the new rows H + SiLU(Z).  With WTh (the head's
+// weights, [32][64]) the head follows in registers: F[Epad][kFOut] = the
+// first kFOut columns of Hout W_h + b_h (+ P), bitwise what k_head computes
+// from the stored rows.
+__global__ __launch_bounds__(kB) void k_mp_direct(int64_t E, int64_t Epad, const int32_t *nbr6, const bf16 *H,
+                                                  const bf16 *WT, const float *bias, bf16 *Zs, bf16 *Hout,
+                                                  const bf16 *WTh, const float *bias_h, const float *P, float *F) {
     __shared__ float tiles[kB / 64][16][kTileLd];
     __shared__ bf16 wts[64][kWLd];
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -370,7 +392,6 @@ __global__ __launch_bounds__(kB) void k_mp_layer(int64_t E, int64_t Epad, const 
         *reinterpret_cast<bf16x8 *>(&wts[row][col]) = ld8(WT + row * 128 + col);
     }
     __syncthreads();
-    const int64_t wave0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6, nwaves = (int64_t)gridDim.x * (kB / 64);
     float br[4];
 #pragma unroll
     for (int nb = 0; nb < 4; nb++) br[nb] = bias[nb * 16 + r];
@@ -381,57 +402,67 @@ __global__ __launch_bounds__(kB) void k_mp_layer(int64_t E, int64_t Epad, const 
         bh1 = ld8(WTh + r * 64 + 32 + g * 8);
         bhn = bias_h[r];
     }
-    const int64_t ntiles = Epad / 16;
-    for (int64_t t = wave0; t < ntiles; t += nwaves) {
+    const bf16x8 zero8 = {};
+    int64_t t0, step;
+    wave_tiles(t0, step);
+    const int64_t t1 = Epad / 16;
+    bf16x8 h0 = {}, h1 = {};
+    int4 nl0 = make_int4(-1, -1, -1, -1), nl1 = make_int4(-1, -1, -1, -1);
+    if (t0 < t1) {
+        const int64_t e = t0 * 16 + r;
+        h0 = ld8(H + e * 64 + g * 8);
+        h1 = ld8(H + e * 64 + 32 + g * 8);
+        nl0 = reinterpret_cast<const int4 *>(nbr6)[2 * e];
+        nl1 = reinterpret_cast<const int4 *>(nbr6)[2 * e + 1];
+    }
+    for (int64_t t = t0; t < t1; t += step) {
         const int64_t m0 = t * 16, e = m0 + r;
-        const bf16x8 h0 = ld8(H + e * 64 + g * 8), h1 = ld8(H + e * 64 + 32 + g * 8);
-        bf16x8 a[4];
-        if (e < E) {
-            a[0] = h0;
-            a[1] = h1;
-            const int ia = ea[e], ib = eb[e];
+        const bf16x8 c0 = h0, c1 = h1;
+        const bool real = e < E;
+        const int nbr[6] = {real ? nl0.x : -1, real ? nl0.y : -1, real ? nl0.z : -1,
+                            real ? nl0.w : -1, real ? nl1.x : -1, real ? nl1.y : -1};
+        bf16x8 n0[6], n1[6];
 #pragma unroll
-            for (int kk = 0; kk < 2; kk++) {
-                const int c = kk * 32 + g * 8;
-                const bf16x8 sa = ld8(S + (int64_t)ia * 64 + c);
-                if (ib >= 0) {
-                    const bf16x8 sb = ld8(S + (int64_t)ib * 64 + c);
-                    bf16x8 o;
-#pragma unroll
-                    for (int j = 0; j < 8; j++) o[j] = (bf16)((float)sa[j] + (float)sb[j]);
-                    a[2 + kk] = o;
-                } else {
-                    a[2 + kk] = sa;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int kk = 0; kk < 4; kk++)
-#pragma unroll
-                for (int j = 0; j < 8; j++) a[kk][j] = (bf16)0.0f;
+        for (int q = 0; q < 6; q++) {
+            n0[q] = nbr[q] >= 0 ? ld8(H + (int64_t)nbr[q] * 64 + g * 8) : zero8;
+            n1[q] = nbr[q] >= 0 ? ld8(H + (int64_t)nbr[q] * 64 + 32 + g * 8) : zero8;
         }
+        const int64_t tn = t + step;
+        if (tn < t1) {
+            const int64_t en = tn * 16 + r;
+            h0 = ld8(H + en * 64 + g * 8);
+            h1 = ld8(H + en * 64 + 32 + g * 8);
+            nl0 = reinterpret_cast<const int4 *>(nbr6)[2 * en];
+            nl1 = reinterpret_cast<const int4 *>(nbr6)[2 * en + 1];
+        }
+        const bf16x8 a0 = real ? c0 : zero8, a1 = real ? c1 : zero8;
         f32x4 acc[4];
 #pragma unroll
         for (int nb = 0; nb < 4; nb++) {
-            acc[nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            const bf16x8 *wrow = reinterpret_cast<const bf16x8 *>(&wts[nb * 16 + r][g * 8]);
+            const bf16x8 w1a = wrow[0], w1b = wrow[4], w2a = wrow[8], w2b = wrow[12];  // k-blocks 0..3
+            f32x4 c = mfma(a0, w1a, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            c = mfma(a1, w1b, c);
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++)
-                acc[nb] = mfma(a[kk], *reinterpret_cast<const bf16x8 *>(&wts[nb * 16 + r][kk * 32 + g * 8]), acc[nb]);
+            for (int q = 0; q < 6; q++) {
+                c = mfma(n0[q], w2a, c);
+                c = mfma(n1[q], w2b, c);
+            }
+            acc[nb] = c;
         }
         float y0[8], y1[8];
         silu_rows(acc, br, r, g, T, Zs, m0, y0, y1);
         bf16x8 o0, o1;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            o0[i] = (bf16)(y0[i] + (float)h0[i]);
-            o1[i] = (bf16)(y1[i] + (float)h1[i]);
+            o0[i] = (bf16)(y0[i] + (float)c0[i]);
+            o1[i] = (bf16)(y1[i] + (float)c1[i]);
         }
         if (Hout) {
             st8(Hout + e * 64 + 8 * g, o0);
             st8(Hout + e * 64 + 32 + 8 * g, o1);
         }
         if (WTh) {
-            // the new rows are the A fragments of the head
             f32x4 ah = mfma(o0, bh0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
             ah = mfma(o1, bh1, ah);
             if (r < kFOut) {
@@ -452,11 +483,12 @@ __global__ __launch_bounds__(kB) void k_mp_layer(int64_t E, int64_t Epad, const 
 __global__ __launch_bounds__(kB) void k_head(int64_t Epad, const bf16 *H4, const bf16 *WT, const float *bias,
                                              const float *P, float *F) {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const int64_t wave0 = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6, nwaves = (int64_t)gridDim.x * (kB / 64);
     const bf16x8 b0 = ld8(WT + r * 64 + g * 8), b1 = ld8(WT + r * 64 + 32 + g * 8);
     const float bn = bias[r];
-    const int64_t ntiles = Epad / 16;
-    for (int64_t t = wave0; t < ntiles; t += nwaves) {
+    int64_t t0, step;
+    wave_tiles(t0, step);
+    const int64_t t1 = Epad / 16;
+    for (int64_t t = t0; t < t1; t += step) {
         const int64_t m0 = t * 16, e = m0 + r;
         f32x4 acc = mfma(ld8(H4 + e * 64 + g * 8), b0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
         acc = mfma(ld8(H4 + e * 64 + 32 + g * 8), b1, acc);
@@ -475,7 +507,7 @@ __global__ __launch_bounds__(kB) void k_head(int64_t Epad, const bf16 *H4, const
 // dX = dZ W^T (W is [K][NOUT]).
 template <int K, int NOUT>
 __global__ __launch_bounds__(kB) void k_linear_bwd_x(int64_t M, const bf16 *dZ, const bf16 *W, bf16 *dX) {
-    const int64_t wave = ((int64_t)blockIdx.x * kB + threadIdx.x) >> 6;
+    const int64_t wave = (gidx()) >> 6;
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int64_t m0 = wave * 16;
     if (m0 >= M) return;
@@ -556,7 +588,7 @@ __global__ __launch_bounds__(kB) void k_linear_bwd_w(int64_t M, const bf16 *X, c
 // Sum of the per-workgroup partials (fixed order: deterministic).
 __global__ __launch_bounds__(kB) void k_reduce_partials(int n, int nparts, const float *partial, float *dW, float *db,
                                                         int nW) {
-    const int i = blockIdx.x * kB + threadIdx.x;
+    const int i = (int)gidx();
     if (i >= n) return;
     float s = 0.0f;
     for (int p = 0; p < nparts; p++) s += partial[(int64_t)p * n + i];
@@ -567,7 +599,7 @@ __global__ __launch_bounds__(kB) void k_reduce_partials(int n, int nparts, const
 }
 
 __global__ __launch_bounds__(kB) void k_silu_bwd(int64_t n, const bf16 *dY, const bf16 *Z, bf16 *dZ) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= n) return;
     dZ[i] = (bf16)((float)dY[i] * silu_grad((float)Z[i]));
 }
@@ -575,7 +607,7 @@ __global__ __launch_bounds__(kB) void k_silu_bwd(int64_t n, const bf16 *dY, cons
 // ---------------------------------------------------------------------------
 // Message passing: node sums over a triangle's edges, edge concatenation.
 __global__ __launch_bounds__(kB) void k_node_sum(int64_t N, const int32_t *tri_edges, const bf16 *H, bf16 *S) {
-    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t t = gidx();
     if (t >= N * 8) return;
     const int64_t i = t >> 3;
     const int c = (int)(t & 7) * 8;
@@ -596,7 +628,7 @@ __global__ __launch_bounds__(kB) void k_node_sum(int64_t N, const int32_t *tri_e
 
 __global__ __launch_bounds__(kB) void k_concat(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
                                                const bf16 *H, const bf16 *S, bf16 *C) {
-    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t t = gidx();
     if (t >= Epad * 16) return;
     const int64_t e = t >> 4;
     const int c = (int)(t & 15) * 8;
@@ -621,7 +653,7 @@ __global__ __launch_bounds__(kB) void k_concat(int64_t E, int64_t Epad, const in
 }
 
 __global__ __launch_bounds__(kB) void k_node_grad(int64_t N, const int32_t *tri_edges, const bf16 *dC, bf16 *dS) {
-    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t t = gidx();
     if (t >= N * 8) return;
     const int64_t i = t >> 3;
     const int c = (int)(t & 7) * 8;
@@ -642,7 +674,7 @@ __global__ __launch_bounds__(kB) void k_node_grad(int64_t N, const int32_t *tri_
 
 __global__ __launch_bounds__(kB) void k_edge_grad(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
                                                   const bf16 *dHn, const bf16 *dC, const bf16 *dS, bf16 *dH) {
-    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t t = gidx();
     if (t >= Epad * 8) return;
     const int64_t e = t >> 3;
     const int c = (int)(t & 7) * 8;
@@ -672,7 +704,7 @@ __global__ __launch_bounds__(kB) void k_base_flux(int64_t E, const int32_t *ea, 
                                                   const float *ny, const uint8_t *closed, const double *w,
                                                   const double *uh, const double *vh, const double *z, double g,
                                                   double h_dry, double inv_fs, float *P) {
-    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t e = gidx();
     if (e >= E) return;
     float o[kFOut];
     base_flux_edge(e, ea, eb, nx, ny, closed, w, uh, vh, z, g, h_dry, inv_fs, o);
@@ -684,67 +716,84 @@ __global__ __launch_bounds__(kB) void k_base_flux(int64_t E, const int32_t *ea, 
 // ---------------------------------------------------------------------------
 // State update from the head's edge outputs (FP64 accumulation), loss and its
 // gradient with respect to the state change.
+// Packed edge record (32 bytes, int32[8]) read by the per-triangle kernels:
+// ea, eb, bits of nx, ny and len (FP32), closed flag, two zero words.  One
+// edge is two 16-byte loads instead of six scattered ones.
+struct EdgeR {
+    int a, b;
+    float nx, ny, len;
+    bool closed;
+};
+GNN_DEV EdgeR load_edge(const int32_t *rec, int64_t e) {
+    const int4 r0 = reinterpret_cast<const int4 *>(rec)[2 * e];
+    const int4 r1 = reinterpret_cast<const int4 *>(rec)[2 * e + 1];
+    return EdgeR{r0.x, r0.y, __int_as_float(r0.z), __int_as_float(r0.w), __int_as_float(r1.x), r1.y != 0};
+}
+
 // Contributions of the three edges of triangle m to its integrated change:
 // mass (side-signed edge flux), x and y momentum (rotated side outputs).
 // With a group, the mass flux of an edge whose other side is in the group is
 // left out (it would cancel in the group's sum).
 // theta (may be NULL) scales the mass flux of each edge by the limiter factor
 // of the cell it drains (k_limit).
-__device__ __forceinline__ void edge_sums(int m, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
-                                          const float *nx, const float *ny, const float *len, const uint8_t *closed,
-                                          const float *F, float fs, const double *theta, const int *mem, double &sh,
-                                          double &sx, double &sy) {
+__device__ __forceinline__ void edge_sums(int m, const int32_t *tri_edges, const int32_t *rec, const float *F,
+                                          float fs, const double *theta, const int *mem, double &sh, double &sx,
+                                          double &sy) {
 #pragma unroll
     for (int q = 0; q < 3; q++) {
         const int e = tri_edges[(int64_t)m * 3 + q];
         if (e < 0) continue;
-        const bool side_a = ea[e] == m;
-        const double L = (double)len[e] * (double)fs;
-        const double n1 = nx[e], n2 = ny[e];
-        const float *f = F + (int64_t)e * kFOut;
+        const EdgeR ed = load_edge(rec, e);
+        const float4 f03 = *reinterpret_cast<const float4 *>(F + (int64_t)e * kFOut);
+        const float f4 = F[(int64_t)e * kFOut + 4];
+        const bool side_a = ed.a == m;
+        const double L = (double)ed.len * (double)fs;
+        const double n1 = ed.nx, n2 = ed.ny;
         // the mass flux of an edge: one value, added to one side, subtracted from the other
-        const int other = side_a ? eb[e] : ea[e];
+        const int other = side_a ? ed.b : ed.a;
         const bool internal = mem && other >= 0 &&
                               (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3]);
-        double c = (closed[e] || internal) ? 0.0 : (double)f[0] * L;
+        double c = (ed.closed || internal) ? 0.0 : (double)f03.x * L;
         if (theta) {
-            const int donor = c > 0.0 ? ea[e] : eb[e];
+            const int donor = c > 0.0 ? ed.a : ed.b;
             if (donor >= 0) c *= theta[donor];
         }
         sh += side_a ? -c : c;
-        const double mn = side_a ? f[1] : f[3], mt = side_a ? f[2] : f[4];
+        const double mn = side_a ? f03.y : f03.w, mt = side_a ? f03.z : f4;
         sx += (mn * n1 - mt * n2) * L;
         sy += (mn * n2 + mt * n1) * L;
     }
 }
 
-__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
-                                              const float *nx, const float *ny, const float *len,
-                                              const uint8_t *closed, const double *area, const int32_t *group,
+__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *rec,
+                                              const double *area, const int32_t *group,
                                               const double *group_area, const double *delta, const int32_t *graph_of,
                                               const float *F, float fs, const double *theta, const double *w0,
                                               const double *uh0,
                                               const double *vh0, double *w1, double *uh1, double *vh1,
                                               const double *dw_t, const double *duh_t, const double *dvh_t,
                                               const float *sigma, double area_ref, float *gq, float *loss_terms) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= N) return;
     double sh = 0.0, sx = 0.0, sy = 0.0;
     double A = area[i];
     int mem[4] = {(int)i, -1, -1, -1};
-    if (group && group[i * 4 + 1] >= 0) {
+    const int4 grow = group ? reinterpret_cast<const int4 *>(group)[i] : make_int4((int)i, -1, -1, -1);
+    if (grow.y >= 0) {
         // a merged group of small cells: its first member forms the group's
         // change (summed over the members in order) and every member takes it
-        if (group[i * 4] != i) return;
-#pragma unroll
-        for (int j = 0; j < 4; j++) mem[j] = group[i * 4 + j];
+        if (grow.x != i) return;
+        mem[0] = grow.x;
+        mem[1] = grow.y;
+        mem[2] = grow.z;
+        mem[3] = grow.w;
 #pragma unroll
         for (int j = 0; j < 4; j++)
             if (mem[j] >= 0)
-                edge_sums(mem[j], tri_edges, ea, eb, nx, ny, len, closed, F, fs, theta, mem, sh, sx, sy);
+                edge_sums(mem[j], tri_edges, rec, F, fs, theta, mem, sh, sx, sy);
         A = group_area[i];
     } else {
-        edge_sums((int)i, tri_edges, ea, eb, nx, ny, len, closed, F, fs, theta, nullptr, sh, sx, sy);
+        edge_sums((int)i, tri_edges, rec, F, fs, theta, nullptr, sh, sx, sy);
     }
     const double s = delta[graph_of[i]] / A;
     const double dh = s * sh, dx = s * sx, dy = s * sy;
@@ -781,17 +830,20 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
 // outgoing mass fluxes would take more, all of them are scaled by
 // theta = volume / outflow.  Each edge takes the factor of the cell it drains,
 // on both sides, so mass stays conserved and no cell's volume goes below 0.
-__global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edges, const int32_t *ea,
-                                              const int32_t *eb, const float *len, const uint8_t *closed,
+__global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edges, const int32_t *rec,
                                               const double *area, const int32_t *group, const double *delta,
                                               const int32_t *graph_of, const float *F, float fs, const double *w,
                                               const double *z, double *theta) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= N) return;
     int mem[4] = {(int)i, -1, -1, -1};
-    if (group && group[i * 4 + 1] >= 0) {
-        if (group[i * 4] != i) return;  // the group's first member sets the factor of every member
-        for (int j = 0; j < 4; j++) mem[j] = group[i * 4 + j];
+    const int4 grow = group ? reinterpret_cast<const int4 *>(group)[i] : make_int4((int)i, -1, -1, -1);
+    if (grow.y >= 0) {
+        if (grow.x != i) return;  // the group's first member sets the factor of every member
+        mem[0] = grow.x;
+        mem[1] = grow.y;
+        mem[2] = grow.z;
+        mem[3] = grow.w;
     }
     double vol = 0.0, out = 0.0;
     for (int j = 0; j < 4; j++) {
@@ -801,11 +853,13 @@ __global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edge
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const int e = tri_edges[(int64_t)m * 3 + q];
-            if (e < 0 || closed[e]) continue;
-            const bool side_a = ea[e] == m;
-            const int other = side_a ? eb[e] : ea[e];
+            if (e < 0) continue;
+            const EdgeR ed = load_edge(rec, e);
+            if (ed.closed) continue;
+            const bool side_a = ed.a == m;
+            const int other = side_a ? ed.b : ed.a;
             if (other >= 0 && (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3])) continue;
-            const double c = (double)F[(int64_t)e * kFOut] * (double)len[e] * (double)fs;
+            const double c = (double)F[(int64_t)e * kFOut] * (double)ed.len * (double)fs;
             out += side_a ? fmax(c, 0.0) : fmax(-c, 0.0);
         }
     }
@@ -818,7 +872,7 @@ __global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edge
 // Momentum of cells shallower than h_dry set to zero (as the solver's protect).
 __global__ __launch_bounds__(kB) void k_protect_dry(int64_t N, const double *w, const double *z, double *uh,
                                                     double *vh, double h_dry) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= N) return;
     if (w[i] - z[i] < h_dry) {
         uh[i] = 0.0;
@@ -849,20 +903,20 @@ __device__ __forceinline__ void group_grad(int a, const int32_t *group, const do
     }
 }
 
-__global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb,
-                                                  const float *nx, const float *ny, const float *len,
-                                                  const uint8_t *closed, const double *area, const int32_t *group,
+__global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const int32_t *rec,
+                                                  const double *area, const int32_t *group,
                                                   const double *group_area, const double *delta,
                                                   const int32_t *graph_of, float fs, const float *gq, bf16 *dF) {
-    const int64_t e = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t e = gidx();
     if (e >= Epad) return;
     float d[kOut];
 #pragma unroll
     for (int i = 0; i < kOut; i++) d[i] = 0.0f;
     if (e < E) {
-        const int a = ea[e], b = eb[e];
-        const double L = (double)len[e] * (double)fs;
-        const double n1 = nx[e], n2 = ny[e];
+        const EdgeR ed = load_edge(rec, e);
+        const int a = ed.a, b = ed.b;
+        const double L = (double)ed.len * (double)fs;
+        const double n1 = ed.nx, n2 = ed.ny;
         double ga0, ga1, ga2, Aa;
         group_grad(a, group, group_area, area, gq, ga0, ga1, ga2, Aa);
         const double sa = delta[graph_of[a]] / Aa * L;
@@ -880,7 +934,7 @@ __global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const
             d[4] = (float)(sb * (-gb1 * n2 + gb2 * n1));
         }
         if (internal) d0 = 0.0;
-        d[0] = closed[e] ? 0.0f : (float)d0;
+        d[0] = ed.closed ? 0.0f : (float)d0;
     }
 #pragma unroll
     for (int c = 0; c < kOut / 8; c++) {
@@ -894,7 +948,7 @@ __global__ __launch_bounds__(kB) void k_apply_bwd(int64_t E, int64_t Epad, const
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kB) void k_adam(int64_t n, float *p, const float *g, float *m, float *v, float lr,
                                              float b1, float b2, float eps, float c1, float c2) {
-    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= n) return;
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
@@ -905,7 +959,7 @@ __global__ __launch_bounds__(kB) void k_adam(int64_t n, float *p, const float *g
 }
 
 __global__ __launch_bounds__(kB) void k_cast_weights(int K, int NOUT, const float *W, bf16 *Wb, bf16 *WTb) {
-    const int i = blockIdx.x * kB + threadIdx.x;
+    const int i = (int)gidx();
     if (i >= K * NOUT) return;
     const int k = i / NOUT, n = i % NOUT;
     const bf16 x = (bf16)W[i];
@@ -968,22 +1022,13 @@ int gnn_feat_enc(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_mp_layer(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
-                 const void *Sv, const void *WT, const float *b, void *Zsave, void *Hout) {
-    if (Epad % 16 || E > Epad || H == Hout || !Hout) return -1;
-    hipLaunchKernelGGL(k_mp_layer, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb,
-                       static_cast<const bf16 *>(H), static_cast<const bf16 *>(Sv), static_cast<const bf16 *>(WT), b,
-                       static_cast<bf16 *>(Zsave), static_cast<bf16 *>(Hout), nullptr, nullptr, nullptr, nullptr);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int gnn_mp_layer_head(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
-                      const void *Sv, const void *WT, const float *b, void *Hout, const void *WTh, const float *bh,
-                      const float *P, float *F) {
-    if (Epad % 16 || E > Epad || H == Hout || !WTh || !bh || !F) return -1;
-    hipLaunchKernelGGL(k_mp_layer, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb,
-                       static_cast<const bf16 *>(H), static_cast<const bf16 *>(Sv), static_cast<const bf16 *>(WT), b,
-                       nullptr, static_cast<bf16 *>(Hout), static_cast<const bf16 *>(WTh), bh, P, F);
+int gnn_mp_direct(void *stream, int64_t E, int64_t Epad, const int32_t *nbr6, const void *H, const void *WT,
+                  const float *b, void *Zsave, void *Hout, const void *WTh, const float *bh, const float *P,
+                  float *F) {
+    if (Epad % 16 || E > Epad || H == Hout || (!Hout && !WTh) || (WTh && (!bh || !F))) return -1;
+    hipLaunchKernelGGL(k_mp_direct, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, nbr6,
+                       static_cast<const bf16 *>(H), static_cast<const bf16 *>(WT), b, static_cast<bf16 *>(Zsave),
+                       static_cast<bf16 *>(Hout), static_cast<const bf16 *>(WTh), bh, P, F);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1052,16 +1097,15 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
-              const float *nx, const float *ny, const float *len, const uint8_t *closed, const double *area,
+int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
               const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
               const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
               const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
               const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms) {
     if (group && !group_area) return -1;
-    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, eb, nx, ny, len, closed,
-                       area, group, group_area, delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1,
-                       dw_t, duh_t, dvh_t, sigma, area_ref, gq, loss_terms);
+    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, rec, area, group, group_area,
+                       delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma,
+                       area_ref, gq, loss_terms);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1074,12 +1118,11 @@ int gnn_base_flux(void *stream, int64_t E, const int32_t *ea, const int32_t *eb,
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
-              const float *len, const uint8_t *closed, const double *area, const int32_t *group, const double *delta,
-              const int32_t *graph_of, const float *F, float flux_scale, const double *w, const double *z,
-              double *theta) {
-    hipLaunchKernelGGL(k_limit, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, ea, eb, len, closed, area,
-                       group, delta, graph_of, F, flux_scale, w, z, theta);
+int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
+              const int32_t *group, const double *delta, const int32_t *graph_of, const float *F, float flux_scale,
+              const double *w, const double *z, double *theta) {
+    hipLaunchKernelGGL(k_limit, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, rec, area, group, delta,
+                       graph_of, F, flux_scale, w, z, theta);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1088,13 +1131,12 @@ int gnn_protect(void *stream, int64_t N, const double *w, const double *z, doubl
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
-                  const float *ny, const float *len, const uint8_t *closed, const double *area, const int32_t *group,
-                  const double *group_area, const double *delta, const int32_t *graph_of, float flux_scale,
-                  const float *gq, void *dF) {
+int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *rec, const double *area,
+                  const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
+                  float flux_scale, const float *gq, void *dF) {
     if (group && !group_area) return -1;
-    hipLaunchKernelGGL(k_apply_bwd, dim3(blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, ea, eb, nx, ny, len, closed,
-                       area, group, group_area, delta, graph_of, flux_scale, gq, static_cast<bf16 *>(dF));
+    hipLaunchKernelGGL(k_apply_bwd, dim3(blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, rec, area, group,
+                       group_area, delta, graph_of, flux_scale, gq, static_cast<bf16 *>(dF));
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

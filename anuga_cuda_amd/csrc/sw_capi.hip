@@ -824,10 +824,9 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
 
 // Device-memory forms of sw_upload / sw_download: the caller's array (same
 // layout and order as for the host forms) lives in device memory.
-int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsrc, int64_t count) {
-    if (!h || !dsrc) return SW_ERR_ARG;
-    const int i = selected(h);
-    if (i < 0) return i;
+// Device-to-device transfers of one array between a caller's buffer (caller
+// order) and the handle, enqueued on the handle's stream without waiting.
+static int upload_async(sw_handle *h, int i, int quantity, int location, const double *dsrc, int64_t count) {
     const Inst &I = h->inst[i];
     Loc L;
     if (!locate(h, i, quantity, location, &L))
@@ -839,19 +838,14 @@ int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsr
     if (L.count == 0) return SW_OK;
     if (!L.per_triangle) {
         HIPCHECK(h, hipMemcpyAsync(L.ptr, dsrc, L.count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    } else {
-        hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
-                           (long long)h->N, (long long)I.off, L.slots, L.comp, h->cal_of + I.off, dsrc, L.ptr);
-        if (int rc = launch_check(h, "k_to_device")) return rc;
+        return SW_OK;
     }
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
-    return SW_OK;
+    hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                       (long long)h->N, (long long)I.off, L.slots, L.comp, h->cal_of + I.off, dsrc, L.ptr);
+    return launch_check(h, "k_to_device");
 }
 
-int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, int64_t count) {
-    if (!h || !ddst) return SW_ERR_ARG;
-    const int i = selected(h);
-    if (i < 0) return i;
+static int download_async(sw_handle *h, int i, int quantity, int location, double *ddst, int64_t count) {
     const Inst &I = h->inst[i];
     Loc L;
     if (!locate(h, i, quantity, location, &L))
@@ -861,11 +855,59 @@ int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, i
     if (L.count == 0) return SW_OK;
     if (!L.per_triangle) {
         HIPCHECK(h, hipMemcpyAsync(ddst, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    } else {
-        hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
-                           (long long)h->N, (long long)I.off, L.slots, L.comp, h->dev_of + I.poff, L.ptr, ddst);
-        if (int rc = launch_check(h, "k_to_host")) return rc;
+        return SW_OK;
     }
+    hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
+                       (long long)h->N, (long long)I.off, L.slots, L.comp, h->dev_of + I.poff, L.ptr, ddst);
+    return launch_check(h, "k_to_host");
+}
+
+int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsrc, int64_t count) {
+    if (!h || !dsrc) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    if (int rc = upload_async(h, i, quantity, location, dsrc, count)) return rc;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return SW_OK;
+}
+
+int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, int64_t count) {
+    if (!h || !ddst) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    if (int rc = download_async(h, i, quantity, location, ddst, count)) return rc;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return SW_OK;
+}
+
+int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, const double *ymom, double time,
+                      double yieldstep) {
+    if (!h || !stage || !xmom || !ymom) return SW_ERR_ARG;
+    if (h->n_inst != 1) return fail(h, SW_ERR_STATE, "sw_restart_device restarts a single domain, not an ensemble");
+    const int64_t n = h->inst[0].n;
+    if (int rc = upload_async(h, 0, SW_STAGE, SW_CENTROID, stage, n)) return rc;
+    if (int rc = upload_async(h, 0, SW_XMOM, SW_CENTROID, xmom, n)) return rc;
+    if (int rc = upload_async(h, 0, SW_YMOM, SW_CENTROID, ymom, n)) return rc;
+    DevState d;  // read after the uploads; a latched error of an earlier run is cleared below
+    HIPCHECK(h, hipMemcpyAsync(&d, h->st, sizeof d, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    d.time = time;
+    d.starttime = time;
+    d.event = SW_EVENT_NONE;
+    d.error = 0;
+    d.halt = 0;
+    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
+    return sw_evolve_prepare(h, yieldstep, 0.0, 0);
+}
+
+int sw_download_conserved_device(sw_handle *h, double *stage, double *xmom, double *ymom) {
+    if (!h || !stage || !xmom || !ymom) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const int64_t n = h->inst[i].n;
+    if (int rc = download_async(h, i, SW_STAGE, SW_CENTROID, stage, n)) return rc;
+    if (int rc = download_async(h, i, SW_XMOM, SW_CENTROID, xmom, n)) return rc;
+    if (int rc = download_async(h, i, SW_YMOM, SW_CENTROID, ymom, n)) return rc;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     return SW_OK;
 }

@@ -90,7 +90,9 @@ def main():
     ap.add_argument("--workload", default="fv", choices=["fv", "ensemble", "learned"],
                     help="fv: the 1M-triangle plain FV step (default); ensemble: the batched ensemble; "
                          "learned: train the edge-GNN surrogate on FV trajectories and roll it out at 1M")
-    ap.add_argument("--delta", type=float, default=0.05, help="learned: seconds per learned step")
+    ap.add_argument("--delta", type=float, default=0.0625, help="learned: seconds per learned step")
+    ap.add_argument("--merge-radius", type=float, default=2.5,
+                    help="learned: inscribed radius (m) below which triangles are merged for the learned update")
     ap.add_argument("--train-iters", type=int, default=3000, help="learned: training iterations")
     ap.add_argument("--train-meshes", type=int, default=40, help="learned: training meshes (50 trajectories each)")
     ap.add_argument("--save-net", default=None, help="learned: write the trained network (.npz)")
@@ -324,7 +326,7 @@ def bench_learned(args):
 
     res = learned.run_learned_workload(train_meshes=args.train_meshes, delta=args.delta, iters=args.train_iters,
                                        eval_points=args.points, save_net=args.save_net, load_net=args.load_net,
-                                       judge_precision=args.judge_precision, log=log)
+                                       judge_precision=args.judge_precision, merge_radius=args.merge_radius, log=log)
     out = {
         "metric": "triangle-steps/s at 1M tris, FV vs learned at stated error; % HBM peak",
         "value": res["learned_rate"],

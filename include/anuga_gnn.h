@@ -42,12 +42,12 @@ int gnn_features(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const
 int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, const void *WT, const float *b,
                    const void *R, void *Zsave, void *Y, int act, int out_f32);
 
-/* The inference layers below are persistent kernels: each wavefront keeps
- * its layer's weights in registers over many 16-row tiles.  Their arithmetic
- * is that of gnn_features / gnn_concat / gnn_linear_fwd, bit for bit. */
+/* The layers below are persistent kernels: each wavefront keeps its
+ * layer's weights in registers or LDS over many 16-row tiles. */
 
 /* Features, physical part and encoder in one pass: H1[Epad][64] =
- * SiLU(X0 W + b) with X0 the edge features of gnn_features (WT [64][32]),
+ * SiLU(X0 W + b) with X0 the edge features of gnn_features (WT [64][32];
+ * bit for bit gnn_features, gnn_base_flux and gnn_linear_fwd),
  * and P[Epad][8] (may be NULL) the physical part of the edge outputs
  * (gnn_base_flux; rows E .. Epad-1 zero).  X0save and Zsave (may be NULL)
  * receive the features and the pre-activation. */
@@ -57,23 +57,21 @@ int gnn_feat_enc(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const
                  const void *WT, const float *b, double g, double h_dry, float flux_scale, float *P, void *X0save,
                  void *Zsave, void *H1);
 
-/* One message-passing layer, the concatenation fused in:
- * Hout = H + SiLU([H, S[ea] + S[eb]] W + b) for the Epad (multiple of 16)
- * edge rows, with WT [64][128] and the node sums S [N][64] (BF16); rows E ..
- * Epad-1 take a zero input row.  Same result as gnn_concat followed by
- * gnn_linear_fwd (K 128, Nout 64, SiLU, residual H).  Zsave (may be NULL)
- * receives the pre-activation.  Hout must not alias H. */
-int gnn_mp_layer(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
-                 const void *Sv, const void *WT, const float *b, void *Zsave, void *Hout);
-
-/* The last message-passing layer with the head fused in (gnn_mp_layer, then
- * gnn_head on its output rows while they are in registers): F as gnn_head
- * computes it, bit for bit.  Hout (may be NULL) receives the layer's output.
- * The persistent kernels above and this one need Epad a multiple of 16
- * (gnn_feat_enc: of 64). */
-int gnn_mp_layer_head(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *H,
-                      const void *Sv, const void *WT, const float *b, void *Hout, const void *WTh, const float *bh,
-                      const float *P, float *F);
+/* One message-passing layer with the node sums and the concatenation folded
+ * into the matrix product: Hout = H + SiLU(H W1 + sum over the edges t of
+ * both triangles of H[t] W2 + b) (W1, W2: the first and second 64 rows of W;
+ * WT [64][128]), the edges listed per edge in nbr6[Epad][8] (int32: the
+ * three edges of ea in tri_edges order, then those of eb, -1 for a boundary
+ * edge's missing triangle and for padding rows, whose input row is zero).
+ * This is [H, S[ea] + S[eb]] W + b with the node sums exact in FP32 (the
+ * concatenation path rounds them to BF16 first).  Zsave (may be NULL)
+ * receives the pre-activation, Hout (may be NULL if WTh is given) the rows.
+ * With WTh (the head's weights, [32][64]) and bh, the head follows while the
+ * rows are in registers: F as gnn_head computes it from them, bit for bit.
+ * The persistent kernels need Epad a multiple of 16 (gnn_feat_enc: of 64). */
+int gnn_mp_direct(void *stream, int64_t E, int64_t Epad, const int32_t *nbr6, const void *H, const void *WT,
+                  const float *b, void *Zsave, void *Hout, const void *WTh, const float *bh, const float *P,
+                  float *F);
 
 /* Head: F[Epad][8] (FP32) = the first 8 columns of H4 W + b (WT [32][64]),
  * plus P[Epad][8] (may be NULL: the physical part).  Columns 0..4 are used
@@ -104,7 +102,10 @@ int gnn_node_grad(void *stream, int64_t N, const int32_t *tri_edges, const void 
 int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *dHnext,
                   const void *dC, const void *dS, void *dH);
 
-/* Apply the head's outputs F[E][8] (FP32; columns 0: mass flux, 1-2: side-a
+/* The per-triangle kernels below read each edge from rec[E][8] (int32; 32
+ * bytes): ea, eb, the FP32 bits of nx, ny and len, closed, 0, 0.
+ *
+ * Apply the head's outputs F[E][8] (FP32; columns 0: mass flux, 1-2: side-a
  * momentum (normal, tangential), 3-4: side-b momentum) over a step delta:
  * dq[i] = delta / area[i] * sum over edges of side-signed, rotated
  * contributions * len * flux_scale; closed[e] != 0 zeroes the mass flux of a
@@ -119,8 +120,7 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
  * sum over components of huber((dq - dq_true) / sigma) (quadratic up to 3,
  * linear beyond) and their gradient gq[i][3] with respect to dq, for the loss
  * sum(terms) / Ntri. */
-int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
-              const float *nx, const float *ny, const float *len, const uint8_t *closed, const double *area,
+int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
               const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
               const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
               const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
@@ -131,10 +131,9 @@ int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *
  * triangle i's group (or of i), volume = sum of area * max(w - z, 0),
  * outflow = delta * the outgoing mass fluxes through the group's outer,
  * open edges.  Used at inference; training treats theta as 1. */
-int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *ea, const int32_t *eb,
-              const float *len, const uint8_t *closed, const double *area, const int32_t *group, const double *delta,
-              const int32_t *graph_of, const float *F, float flux_scale, const double *w, const double *z,
-              double *theta);
+int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
+              const int32_t *group, const double *delta, const int32_t *graph_of, const float *F, float flux_scale,
+              const double *w, const double *z, double *theta);
 
 /* The physical part of the edge outputs into P[E][8] (columns 0..4, 5..7
  * zero; in the outputs' units: divided by flux_scale; gnn_head adds it to the
@@ -155,10 +154,9 @@ int gnn_protect(void *stream, int64_t N, const double *w, const double *z, doubl
 
 /* dF[Epad][32] (BF16) from gq: the transpose of gnn_apply's linear map (same
  * group arguments). */
-int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
-                  const float *ny, const float *len, const uint8_t *closed, const double *area, const int32_t *group,
-                  const double *group_area, const double *delta, const int32_t *graph_of, float flux_scale,
-                  const float *gq, void *dF);
+int gnn_apply_bwd(void *stream, int64_t E, int64_t Epad, const int32_t *rec, const double *area,
+                  const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
+                  float flux_scale, const float *gq, void *dF);
 
 /* Adam step on n FP32 parameters, then BF16 copies in both layouts:
  * W [K][Nout] -> Wb (same layout) and WTb ([Nout][K]); for biases pass K = 1

@@ -136,6 +136,18 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
  * no counterpart in the reference. */
 int sw_upload_device(sw_handle *h, int quantity, int location, const double *device_src, int64_t count);
 int sw_download_device(sw_handle *h, int quantity, int location, double *device_dst, int64_t count);
+/* Restart of a single domain from device buffers: uploads the centroid
+ * values of stage, xmomentum and ymomentum (caller order), sets time and
+ * starttime to `time`, clears a latched event or error and prepares the
+ * evolve loop for yields every `yieldstep` (sw_evolve_prepare, no final
+ * time); vertex values of cells in dry neighbourhoods keep their contents.
+ * The matching download of the three conserved centroid arrays waits once.
+ * For drivers that restart the solver from states held on the device, such
+ * as the learned surrogate's judged rollout; no counterpart in the
+ * reference. */
+int sw_restart_device(sw_handle *h, const double *stage, const double *xmomentum, const double *ymomentum,
+                      double time, double yieldstep);
+int sw_download_conserved_device(sw_handle *h, double *stage, double *xmomentum, double *ymomentum);
 int sw_get_state(sw_handle *h, sw_state *state);
 int sw_set_state(sw_handle *h, const sw_state *state);
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count);

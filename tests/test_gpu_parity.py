@@ -166,6 +166,39 @@ def test_operators_match_oracle_one_by_one():
     print({k: v for k, v in st.items() if v[0]})
 
 
+def test_restart_from_device_buffers_matches_oracle_restart():
+    """sw_restart_device / sw_download_conserved_device: after 6 lockstep
+    steps, the conserved centroid state goes to torch device buffers, is
+    perturbed (a 1 cm rise of stage where wet), and the solver restarts from
+    it at the current time; the oracle restarts from the same values
+    (starttime = time, evolve_prepare).  6 more steps match the oracle to the
+    FP64 bar, timesteps included."""
+    import torch
+
+    d = _delaunay("morton")
+    od, _ = _lockstep(d, 6, True, {})
+    h = d._handle
+    N = d.number_of_elements
+    w, uh, vh = (torch.empty(N, dtype=torch.float64, device="cuda") for _ in range(3))
+    h.download_conserved_device(w.data_ptr(), uh.data_ptr(), vh.data_ptr())
+    for name, t in (("stage", w), ("xmomentum", uh), ("ymomentum", vh)):
+        compare(t.cpu().numpy(), od.q(name, "centroid_values"), "download." + name, TOL)
+    bed = torch.tensor(od.q("elevation", "centroid_values"), device="cuda")
+    w += torch.where(w - bed > 1e-3, 0.01, 0.0)
+    t_now = od.d.time
+    h.restart_device(w.data_ptr(), uh.data_ptr(), vh.data_ptr(), t_now, 1e9)
+    od.q("stage", "centroid_values")[:] = w.cpu().numpy()
+    od.d.starttime = t_now
+    od.evolve_prepare(1e9, None)
+    compare_state(d, od, TOL, tag="restart.")
+    for step in range(6):
+        done, ev = h.evolve_steps(1)
+        od.evolve_step()
+        s = h.get_state()
+        compare([s.timestep, s.time], [od.d.timestep, od.d.time], "timestep/time", TOL)
+        compare_state(d, od, TOL, tag="after restart.")
+
+
 def test_evolve_generator_yields_match_oracle():
     d = _dam()
     od = OracleDomain.from_domain(d)
