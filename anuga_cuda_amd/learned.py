@@ -1044,7 +1044,32 @@ def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.0625, 
     return res
 
 
-def time_learned_step(points=500000, steps=20, delta=0.05, merge_radius=2.0):
+def learned_kernel_bytes(N, E):
+    """Algorithmic HBM bytes per launch of the inference kernels on N
+    triangles and E edges: every array a kernel reads or writes, counted once
+    (rows gathered from neighbours as cache hits; of a packed record or an
+    output row only the fields used).  The three message-passing launches
+    are averaged (the third reads the physical part and writes F instead of
+    the hidden rows)."""
+    return {
+        # ea, eb, nx, ny, len, closed; state (w, uh, vh, z) once per triangle;
+        # writes H1 (64 BF16) and the physical part (8 FP32)
+        "k_feat_enc": E * (8 + 12 + 1 + 128 + 32) + N * 32,
+        # own row 128 + neighbour list 32, writes the new row 128; third layer:
+        # + physical part 32, writes F 32 instead of the row
+        "k_mp_direct": E * (2 * (128 + 32 + 128) + (128 + 32 + 32 + 32)) // 3,
+        # tri_edges 12, group row 16, area, w, z 24, theta 8, graph_of 4;
+        # per edge ea, eb, len, closed 16 and the mass flux 4
+        "k_limit": N * 64 + E * 20,
+        # tri_edges 12, group 16, areas 16, graph_of 4, theta 8, state in
+        # 24 and out 24; per edge the record 32 and five outputs 20
+        "k_apply": N * 104 + E * 52,
+        # w and z read, both momenta written (at most: only where dry)
+        "k_protect_dry": N * 32,
+    }
+
+
+def time_learned_step(points=500000, steps=20, delta=0.0625, merge_radius=2.5):
     """Milliseconds per learned inference step on the seeded Delaunay workload
     of ``points`` points (an untrained network: the same kernels and cost as a
     trained one)."""
@@ -1080,7 +1105,19 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="time the learned inference step")
     ap.add_argument("--points", type=int, default=500000)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--merge-radius", type=float, default=2.0)
+    ap.add_argument("--merge-radius", type=float, default=2.5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (scale for rocprof FETCH/WRITE_SIZE)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        torch = _torch()
+        src = torch.ones(a.copy_calibration * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ms, n, e = time_learned_step(a.points, a.steps, merge_radius=a.merge_radius)
     print("learned step: %.3f ms on %d triangles, %d edges" % (ms, n, e), flush=True)
+    for k, v in learned_kernel_bytes(n, e).items():
+        print("algorithmic bytes per launch: %s=%d" % (k, v))

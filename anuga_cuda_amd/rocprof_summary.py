@@ -15,6 +15,10 @@ Usage::
   read requests of the memory fabric at a smaller size, so its factor is not 1.
 * ``--json``: writes {kernel short name: {avg_us, fetch_bytes, write_bytes,
   traffic_bytes}} for bench.py's roofline ``traffic`` field.
+* ``--bytes NAME=B`` (repeatable): algorithmic bytes per launch of the
+  kernels whose name contains NAME (mangled names included); prints their
+  achieved GB/s
+  (B / average duration) against the 8 TB/s HBM peak.
 """
 from __future__ import annotations
 
@@ -57,6 +61,7 @@ def main(argv=None):
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     ap.add_argument("--triangles", type=int, default=0, help="triangles per launch, for bytes per triangle")
     ap.add_argument("--json")
+    ap.add_argument("--bytes", action="append", default=[], metavar="NAME=B")
     a = ap.parse_args(argv)
 
     ks = kernel_stats(a.trace)
@@ -93,6 +98,17 @@ def main(argv=None):
             print(line)
             d = out.setdefault(short_name(name), {})
             d.update({"fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb})
+    if a.bytes:
+        print()
+        print("%-44s %12s %10s %10s %8s" % ("kernel", "alg_bytes", "avg_us", "GB/s", "of_8TB/s"))
+        for spec in a.bytes:
+            prefix, b = spec.split("=")
+            for name, (n, avg, mn, mx) in sorted(ks.items()):
+                if prefix in name:
+                    gbs = float(b) / (avg * 1e-6) / 1e9
+                    print("%-44s %12.4g %10.2f %10.1f %8.3f" % (short_name(name)[:44], float(b), avg, gbs,
+                                                                gbs / 8000.0))
+                    out.setdefault(short_name(name), {})["algorithmic_bytes"] = float(b)
     if a.json:
         with open(a.json, "w") as fh:
             json.dump({"fetch_scale": sf if a.fetch else None, "write_scale": sw if a.write else None,
