@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kB) void k_features(int64_t E, int64_t Epad, const 
 // Physical part of the edge outputs: the first-order central-upwind flux
 // (Kurganov, Noelle and Petrova, SIAM J. Sci. Comput. 23, 2001) of the two
 // centroid states after hydrostatic reconstruction (Audusse, Bouchut,
-// Bristeau, Klein and Perthame, SIAM J. Sci. Comput. 25, 2004), FP64, added to
+// Bristeau, Klein and Perthame, SIAM J. Sci. Comput. 25, 2004), added to
 // the network's outputs in their units (divided by the flux scale):
 //   0: mass flux from a to b,
 //   1, 2: normal / tangential momentum change of side a per unit length,
@@ -168,6 +168,9 @@ GNN_DEV void base_flux_edge(int64_t e, const int32_t *ea, const int32_t *eb, con
 #pragma clang fp contract(off)
     const int a = ea[e], b = eb[e];
     const double n1 = nx[e], n2 = ny[e];
+    // depths and normal-frame momenta from the FP64 state (no cancellation
+    // in FP32), then the flux in FP32: equal inputs still give equal
+    // pressures, so a lake at rest stays exactly zero
     const double wa = w[a], za = z[a], ha = wa - za;
     const double qna = uh[a] * n1 + vh[a] * n2, qta = -uh[a] * n2 + vh[a] * n1;
     double wb = wa, zb = za, hb = ha, qnb = closed[e] ? -qna : qna, qtb = qta;
@@ -178,28 +181,30 @@ GNN_DEV void base_flux_edge(int64_t e, const int32_t *ea, const int32_t *eb, con
         qnb = uh[b] * n1 + vh[b] * n2;
         qtb = -uh[b] * n2 + vh[b] * n1;
     }
-    const double ua = ha > h_dry ? qna / ha : 0.0, va = ha > h_dry ? qta / ha : 0.0;
-    const double ub = hb > h_dry ? qnb / hb : 0.0, vb = hb > h_dry ? qtb / hb : 0.0;
     const double zm = fmax(za, zb);
-    const double sa = fmax(0.0, wa - zm), sb = fmax(0.0, wb - zm);
-    const double ca = sqrt(g * sa), cb = sqrt(g * sb);
-    const double smax = fmax(fmax(ua + ca, ub + cb), 0.0), smin = fmin(fmin(ua - ca, ub - cb), 0.0);
-    double k0 = 0.0, mna = 0.0, k2 = 0.0, mnb = 0.0;
-    if (smax - smin > 1.0e-12) {
-        const double inv = 1.0 / (smax - smin), sm = smax * smin;
-        const double qa = sa * ua, qb = sb * ub;
-        const double pa = 0.5 * g * sa * sa, pb = 0.5 * g * sb * sb;
-        const double core = smax * qa * ua - smin * qb * ub + sm * (qb - qa);
+    const float sa = (float)fmax(0.0, wa - zm), sb = (float)fmax(0.0, wb - zm);
+    const float ua = ha > h_dry ? (float)qna / (float)ha : 0.0f, va = ha > h_dry ? (float)qta / (float)ha : 0.0f;
+    const float ub = hb > h_dry ? (float)qnb / (float)hb : 0.0f, vb = hb > h_dry ? (float)qtb / (float)hb : 0.0f;
+    const float gf = (float)g;
+    const float ca = sqrtf(gf * sa), cb = sqrtf(gf * sb);
+    const float smax = fmaxf(fmaxf(ua + ca, ub + cb), 0.0f), smin = fminf(fminf(ua - ca, ub - cb), 0.0f);
+    float k0 = 0.0f, mna = 0.0f, k2 = 0.0f, mnb = 0.0f;
+    if (smax - smin > 1.0e-6f) {
+        const float inv = 1.0f / (smax - smin), sm = smax * smin;
+        const float qa = sa * ua, qb = sb * ub;
+        const float pa = 0.5f * gf * sa * sa, pb = 0.5f * gf * sb * sb;
+        const float core = smax * qa * ua - smin * qb * ub + sm * (qb - qa);
         k0 = (smax * qa - smin * qb + sm * (sb - sa)) * inv;
         mna = -(core - smin * (pb - pa)) * inv;
         mnb = (core + smax * (pa - pb)) * inv;
         k2 = (smax * qa * va - smin * qb * vb + sm * (sb * vb - sa * va)) * inv;
     }
-    out[0] = (float)(k0 * inv_fs);
-    out[1] = (float)(mna * inv_fs);
-    out[2] = (float)(-k2 * inv_fs);
-    out[3] = (float)(mnb * inv_fs);
-    out[4] = (float)(k2 * inv_fs);
+    const float s = (float)inv_fs;
+    out[0] = k0 * s;
+    out[1] = mna * s;
+    out[2] = -k2 * s;
+    out[3] = mnb * s;
+    out[4] = k2 * s;
     out[5] = out[6] = out[7] = 0.0f;
 }
 
@@ -765,6 +770,40 @@ __device__ __forceinline__ void edge_sums(int m, const int32_t *tri_edges, const
     }
 }
 
+// New state and (with targets) loss terms of triangle i from its change.
+__device__ __forceinline__ void finish_cell(int64_t N, int64_t i, double dh, double dx, double dy,
+                                            const double *area, const double *w0, const double *uh0,
+                                            const double *vh0, double *w1, double *uh1, double *vh1,
+                                            const double *dw_t, const double *duh_t, const double *dvh_t,
+                                            const float *sigma, double area_ref, float *gq, float *loss_terms) {
+    w1[i] = w0[i] + dh;
+    uh1[i] = uh0[i] + dx;
+    vh1[i] = vh0[i] + dy;
+    if (dw_t) {
+        // Area-weighted (the error of the volume and momentum a triangle
+        // gains) Huber loss of the normalised error: quadratic up to kHuber
+        // standard deviations, linear beyond, so the few violently changing
+        // triangles of a trajectory do not dominate.
+        constexpr double kHuber = 3.0;
+        const double wgt = area[i] / area_ref;
+        const double e[3] = {(dh - dw_t[i]) / sigma[0], (dx - duh_t[i]) / sigma[1], (dy - dvh_t[i]) / sigma[2]};
+        double l = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double ae = fabs(e[c]);
+            l += ae <= kHuber ? e[c] * e[c] : kHuber * (2.0 * ae - kHuber);
+            const double d = ae <= kHuber ? 2.0 * e[c] : 2.0 * kHuber * (e[c] > 0 ? 1.0 : -1.0);
+            gq[i * 3 + c] = (float)(wgt * d / ((double)N * sigma[c]));
+        }
+        loss_terms[i] = (float)(wgt * l);
+    }
+}
+
+// The update in two passes, so that every lane does the same work: first
+// each triangle sums its own three edges (the mass flux of an edge whose
+// other side is in its group left out); a lone triangle takes its update at
+// once, a grouped one leaves its sums in M.  Then every member of a group
+// adds up its members' sums in member order and takes the group's change.
 __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *rec,
                                               const double *area, const int32_t *group,
                                               const double *group_area, const double *delta, const int32_t *graph_of,
@@ -772,56 +811,49 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
                                               const double *uh0,
                                               const double *vh0, double *w1, double *uh1, double *vh1,
                                               const double *dw_t, const double *duh_t, const double *dvh_t,
-                                              const float *sigma, double area_ref, float *gq, float *loss_terms) {
+                                              const float *sigma, double area_ref, float *gq, float *loss_terms,
+                                              double4 *M) {
+    (void)group_area;
     const int64_t i = gidx();
     if (i >= N) return;
     double sh = 0.0, sx = 0.0, sy = 0.0;
-    double A = area[i];
-    int mem[4] = {(int)i, -1, -1, -1};
     const int4 grow = group ? reinterpret_cast<const int4 *>(group)[i] : make_int4((int)i, -1, -1, -1);
     if (grow.y >= 0) {
-        // a merged group of small cells: its first member forms the group's
-        // change (summed over the members in order) and every member takes it
-        if (grow.x != i) return;
-        mem[0] = grow.x;
-        mem[1] = grow.y;
-        mem[2] = grow.z;
-        mem[3] = grow.w;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (mem[j] >= 0)
-                edge_sums(mem[j], tri_edges, rec, F, fs, theta, mem, sh, sx, sy);
-        A = group_area[i];
-    } else {
-        edge_sums((int)i, tri_edges, rec, F, fs, theta, nullptr, sh, sx, sy);
+        const int mem[4] = {grow.x, grow.y, grow.z, grow.w};
+        edge_sums((int)i, tri_edges, rec, F, fs, theta, mem, sh, sx, sy);
+        M[i] = make_double4(sh, sx, sy, 0.0);
+        return;
     }
-    const double s = delta[graph_of[i]] / A;
-    const double dh = s * sh, dx = s * sx, dy = s * sy;
-    for (int j = 0; j < 4; j++) {
-        const int m = mem[j];
-        if (m < 0) continue;
-        w1[m] = w0[m] + dh;
-        uh1[m] = uh0[m] + dx;
-        vh1[m] = vh0[m] + dy;
-        if (dw_t) {
-            // Area-weighted (the error of the volume and momentum a triangle
-            // gains) Huber loss of the normalised error: quadratic up to
-            // kHuber standard deviations, linear beyond, so the few violently
-            // changing triangles of a trajectory do not dominate.
-            constexpr double kHuber = 3.0;
-            const double wgt = area[m] / area_ref;
-            const double e[3] = {(dh - dw_t[m]) / sigma[0], (dx - duh_t[m]) / sigma[1], (dy - dvh_t[m]) / sigma[2]};
-            double l = 0.0;
+    edge_sums((int)i, tri_edges, rec, F, fs, theta, nullptr, sh, sx, sy);
+    const double s = delta[graph_of[i]] / area[i];
+    finish_cell(N, i, s * sh, s * sx, s * sy, area, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, area_ref,
+                gq, loss_terms);
+}
+
+__global__ __launch_bounds__(kB) void k_apply_groups(int64_t N, const double *area, const int32_t *group,
+                                                     const double *group_area, const double *delta,
+                                                     const int32_t *graph_of, const double4 *M, const double *w0,
+                                                     const double *uh0, const double *vh0, double *w1, double *uh1,
+                                                     double *vh1, const double *dw_t, const double *duh_t,
+                                                     const double *dvh_t, const float *sigma, double area_ref,
+                                                     float *gq, float *loss_terms) {
+    const int64_t i = gidx();
+    if (i >= N) return;
+    const int4 grow = reinterpret_cast<const int4 *>(group)[i];
+    if (grow.y < 0) return;
+    const int mem[4] = {grow.x, grow.y, grow.z, grow.w};
+    double4 t = M[mem[0]];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double ae = fabs(e[c]);
-                l += ae <= kHuber ? e[c] * e[c] : kHuber * (2.0 * ae - kHuber);
-                const double d = ae <= kHuber ? 2.0 * e[c] : 2.0 * kHuber * (e[c] > 0 ? 1.0 : -1.0);
-                gq[(int64_t)m * 3 + c] = (float)(wgt * d / ((double)N * sigma[c]));
-            }
-            loss_terms[m] = (float)(wgt * l);
-        }
+    for (int j = 1; j < 4; j++) {
+        if (mem[j] < 0) continue;
+        const double4 m = M[mem[j]];
+        t.x += m.x;
+        t.y += m.y;
+        t.z += m.z;
     }
+    const double s = delta[graph_of[i]] / group_area[i];
+    finish_cell(N, i, s * t.x, s * t.y, s * t.z, area, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma,
+                area_ref, gq, loss_terms);
 }
 
 // Positivity limiter of the mass fluxes (the draining-time idea of
@@ -833,40 +865,53 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
 __global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edges, const int32_t *rec,
                                               const double *area, const int32_t *group, const double *delta,
                                               const int32_t *graph_of, const float *F, float fs, const double *w,
-                                              const double *z, double *theta) {
+                                              const double *z, double *theta, double4 *M) {
     const int64_t i = gidx();
     if (i >= N) return;
-    int mem[4] = {(int)i, -1, -1, -1};
     const int4 grow = group ? reinterpret_cast<const int4 *>(group)[i] : make_int4((int)i, -1, -1, -1);
-    if (grow.y >= 0) {
-        if (grow.x != i) return;  // the group's first member sets the factor of every member
-        mem[0] = grow.x;
-        mem[1] = grow.y;
-        mem[2] = grow.z;
-        mem[3] = grow.w;
-    }
-    double vol = 0.0, out = 0.0;
-    for (int j = 0; j < 4; j++) {
-        const int m = mem[j];
-        if (m < 0) continue;
-        vol += area[m] * fmax(w[m] - z[m], 0.0);
+    const bool grouped = grow.y >= 0;
+    const double vol = area[i] * fmax(w[i] - z[i], 0.0);
+    double out = 0.0;
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int e = tri_edges[(int64_t)m * 3 + q];
-            if (e < 0) continue;
-            const EdgeR ed = load_edge(rec, e);
-            if (ed.closed) continue;
-            const bool side_a = ed.a == m;
-            const int other = side_a ? ed.b : ed.a;
-            if (other >= 0 && (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3])) continue;
-            const double c = (double)F[(int64_t)e * kFOut] * (double)ed.len * (double)fs;
-            out += side_a ? fmax(c, 0.0) : fmax(-c, 0.0);
-        }
+    for (int q = 0; q < 3; q++) {
+        const int e = tri_edges[i * 3 + q];
+        if (e < 0) continue;
+        const EdgeR ed = load_edge(rec, e);
+        if (ed.closed) continue;
+        const bool side_a = ed.a == i;
+        const int other = side_a ? ed.b : ed.a;
+        if (grouped && other >= 0 && (other == grow.x || other == grow.y || other == grow.z || other == grow.w))
+            continue;
+        const double c = (double)F[(int64_t)e * kFOut] * (double)ed.len * (double)fs;
+        out += side_a ? fmax(c, 0.0) : fmax(-c, 0.0);
+    }
+    if (grouped) {  // the group's factor is formed by k_limit_groups
+        M[i] = make_double4(vol, out, 0.0, 0.0);
+        return;
     }
     out *= delta[graph_of[i]];
-    const double th = out > vol ? vol / out : 1.0;
-    for (int j = 0; j < 4; j++)
-        if (mem[j] >= 0) theta[mem[j]] = th;
+    theta[i] = out > vol ? vol / out : 1.0;
+}
+
+// The factor of a merged group: its members' volumes and outflows summed in
+// member order (each member forms it).
+__global__ __launch_bounds__(kB) void k_limit_groups(int64_t N, const int32_t *group, const double *delta,
+                                                     const int32_t *graph_of, const double4 *M, double *theta) {
+    const int64_t i = gidx();
+    if (i >= N) return;
+    const int4 grow = reinterpret_cast<const int4 *>(group)[i];
+    if (grow.y < 0) return;
+    const int mem[4] = {grow.x, grow.y, grow.z, grow.w};
+    double vol = 0.0, out = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (mem[j] < 0) continue;
+        const double4 m = M[mem[j]];
+        vol += m.x;
+        out += m.y;
+    }
+    out *= delta[graph_of[i]];
+    theta[i] = out > vol ? vol / out : 1.0;
 }
 
 // Momentum of cells shallower than h_dry set to zero (as the solver's protect).
@@ -1101,11 +1146,17 @@ int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *
               const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
               const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
               const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
-              const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms) {
-    if (group && !group_area) return -1;
+              const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms,
+              double *cell_sums) {
+    if (group && (!group_area || !cell_sums)) return -1;
+    double4 *M = reinterpret_cast<double4 *>(cell_sums);
     hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, rec, area, group, group_area,
                        delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma,
-                       area_ref, gq, loss_terms);
+                       area_ref, gq, loss_terms, M);
+    if (group)
+        hipLaunchKernelGGL(k_apply_groups, dim3(blocks(N)), dim3(kB), 0, S(stream), N, area, group, group_area, delta,
+                           graph_of, M, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, area_ref, gq,
+                           loss_terms);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1120,9 +1171,14 @@ int gnn_base_flux(void *stream, int64_t E, const int32_t *ea, const int32_t *eb,
 
 int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
               const int32_t *group, const double *delta, const int32_t *graph_of, const float *F, float flux_scale,
-              const double *w, const double *z, double *theta) {
+              const double *w, const double *z, double *theta, double *cell_sums) {
+    if (group && !cell_sums) return -1;
+    double4 *M = reinterpret_cast<double4 *>(cell_sums);
     hipLaunchKernelGGL(k_limit, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, rec, area, group, delta,
-                       graph_of, F, flux_scale, w, z, theta);
+                       graph_of, F, flux_scale, w, z, theta, M);
+    if (group)
+        hipLaunchKernelGGL(k_limit_groups, dim3(blocks(N)), dim3(kB), 0, S(stream), N, group, delta, graph_of, M,
+                           theta);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

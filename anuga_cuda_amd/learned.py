@@ -77,8 +77,8 @@ def _load():
         _gnn.gnn_concat.argtypes = [p, i64, i64, p, p, p, p, p]
         _gnn.gnn_node_grad.argtypes = [p, i64, p, p, p]
         _gnn.gnn_edge_grad.argtypes = [p, i64, i64, p, p, p, p, p, p]
-        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 8 + [f] + [p] * 11 + [C.c_double, p, p]
-        _gnn.gnn_limit.argtypes = [p, i64] + [p] * 7 + [f, p, p, p]
+        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 8 + [f] + [p] * 11 + [C.c_double, p, p, p]
+        _gnn.gnn_limit.argtypes = [p, i64] + [p] * 7 + [f, p, p, p, p]
         _gnn.gnn_protect.argtypes = [p, i64, p, p, p, p, C.c_double]
         _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 6 + [f, p, p]
         _gnn.gnn_adam.argtypes = [p, i64, p, p, p, p, f, f, f, f, i32]
@@ -358,6 +358,7 @@ class Workspace:
         self.gq = torch.empty(N, 3, dtype=torch.float32, device="cuda")
         self.loss_terms = torch.empty(N, dtype=torch.float32, device="cuda")
         self.theta = torch.empty(N, dtype=torch.float64, device="cuda")
+        self.cells = torch.empty(N, 4, dtype=torch.float64, device="cuda")  # per-triangle sums of merged groups
         self.gW = {n: torch.empty(K, Nn, dtype=torch.float32, device="cuda") for n, K, Nn in LAYERS}
         self.gb = {n: torch.empty(Nn, dtype=torch.float32, device="cuda") for n, K, Nn in LAYERS}
 
@@ -415,7 +416,7 @@ def apply(net, graph, ws, delta, w0, uh0, vh0, w1, uh1, vh1, targets=None, theta
     _ck(L.gnn_apply(s, g.N, _ptr(g.tri_edges), _ptr(g.rec), _ptr(g.area), _ptr(g.group), _ptr(g.group_area),
                     _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(theta), _ptr(w0), _ptr(uh0),
                     _ptr(vh0), _ptr(w1), _ptr(uh1), _ptr(vh1), _ptr(dw), _ptr(duh), _ptr(dvh), _ptr(net.sigma),
-                    graph.mean_area, _ptr(ws.gq), _ptr(ws.loss_terms)), "apply")
+                    graph.mean_area, _ptr(ws.gq), _ptr(ws.loss_terms), _ptr(ws.cells)), "apply")
 
 
 def backward(net, graph, ws, delta):
@@ -668,7 +669,7 @@ def limit(net, graph, ws, delta, w, z):
     g = graph
     _ck(net.lib.gnn_limit(net.stream(), g.N, _ptr(g.tri_edges), _ptr(g.rec), _ptr(g.area), _ptr(g.group),
                           _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(w), _ptr(z),
-                          _ptr(ws.theta)), "limit")
+                          _ptr(ws.theta), _ptr(ws.cells)), "limit")
     return ws.theta
 
 

@@ -119,21 +119,27 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
  * changes), also writes the area-weighted loss terms (area[i] / area_ref) *
  * sum over components of huber((dq - dq_true) / sigma) (quadratic up to 3,
  * linear beyond) and their gradient gq[i][3] with respect to dq, for the loss
- * sum(terms) / Ntri. */
+ * sum(terms) / Ntri.  With groups, cell_sums (4 N doubles of scratch) holds
+ * each grouped triangle's own sums between the two passes: first every
+ * triangle sums its own edges, then a group's members add them up in member
+ * order. */
 int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
               const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
               const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
               const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
-              const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms);
+              const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms,
+              double *cell_sums);
 
 /* Positivity limiter of the learned mass fluxes (draining time, Bollermann,
  * Chen, Kurganov and Noelle 2013): theta[i] = min(1, volume / outflow) of
  * triangle i's group (or of i), volume = sum of area * max(w - z, 0),
  * outflow = delta * the outgoing mass fluxes through the group's outer,
- * open edges.  Used at inference; training treats theta as 1. */
+ * open edges.  Used at inference; training treats theta as 1.  With groups,
+ * cell_sums (4 N doubles of scratch) holds each grouped triangle's volume and
+ * outflow between the two passes. */
 int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
               const int32_t *group, const double *delta, const int32_t *graph_of, const float *F, float flux_scale,
-              const double *w, const double *z, double *theta);
+              const double *w, const double *z, double *theta, double *cell_sums);
 
 /* The physical part of the edge outputs into P[E][8] (columns 0..4, 5..7
  * zero; in the outputs' units: divided by flux_scale; gnn_head adds it to the

@@ -113,7 +113,8 @@ def test_gnn_forward_and_gradients_match_cpu_reference(case):
 
 def test_physical_base_flux_matches_reference_and_keeps_a_lake_at_rest():
     """With the head at zero the edge outputs are the physical part alone
-    (k_base_flux): they match the numpy restatement to FP32 rounding on a
+    (k_base_flux, FP32 arithmetic from FP64 depths): they match the numpy
+    restatement to FP32 rounding on a
     random wet/dry state with walls and open edges; for a lake at rest over
     the same rough bed (dry cells above it) every output is exactly zero, and
     30 learned steps leave the state bitwise unchanged; in a closed basin the
@@ -132,7 +133,7 @@ def test_physical_base_flux_matches_reference_and_keeps_a_lake_at_rest():
     got = ws.F[:graph.E, :5].cpu().numpy().astype(np.float64)
     exp = ref.base_flux(_graph_np(graph), w, uh, vh, z, 0.8)
     assert np.max(np.abs(exp)) > 1.0
-    np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-6 * np.max(np.abs(exp)))
+    np.testing.assert_allclose(got, exp, rtol=1e-5, atol=1e-5 * np.max(np.abs(exp)))  # FP32 flux arithmetic
     rest = np.maximum(z, 0.9)
     tw, tuh, tvh, tz = dev(rest), dev(np.zeros_like(w)), dev(np.zeros_like(w)), dev(z)
     L.forward(net, graph, ws, tw, tuh, tvh, tz, save=False)
