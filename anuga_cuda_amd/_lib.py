@@ -34,7 +34,7 @@ class SwParams(C.Structure):
         (n, C.c_int32) for n in (
             "optimise_dry_cells", "extrapolate_velocity_second_order",
             "tight_slope_limiters", "use_centroid_velocities", "use_sloped_mannings",
-            "max_smallsteps", "default_order", "precision", "locality_order", "reserved")]
+            "max_smallsteps", "default_order", "precision", "locality_order", "timestepping_method")]
 
 
 _I64P = C.POINTER(C.c_int64)
@@ -116,6 +116,8 @@ _SIGNATURES = {
     "sw_evolve_steps": ([C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32)], C.c_int),
     "sw_advance_yield": ([C.c_void_p], C.c_int),
     "sw_evolve_one_step_unfused": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
+    "sw_backup_conserved_quantities": ([C.c_void_p], C.c_int),
+    "sw_saxpy_conserved_quantities": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
     "sw_synchronize": ([C.c_void_p], C.c_int),
     "sw_autotune": ([C.c_void_p, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "sw_enable_timing": ([C.c_void_p, C.c_int32], C.c_int),

@@ -138,6 +138,8 @@ int check_params(sw_handle *h, const sw_params *p) {
         return fail(h, SW_ERR_ARG, "unknown precision %d", p->precision);
     if (p->locality_order != SW_ORDER_KEEP && p->locality_order != SW_ORDER_HILBERT)
         return fail(h, SW_ERR_ARG, "unknown locality order %d", p->locality_order);
+    if (p->timestepping_method < 0 || p->timestepping_method > 3)
+        return fail(h, SW_ERR_ARG, "unknown timestepping method %d (1 euler, 2 rk2, 3 rk3)", p->timestepping_method);
     if (p->default_order != 1 && p->default_order != 2)
         return fail(h, SW_ERR_ARG, "default_order must be 1 or 2");
     if (!(p->H0 > 0) || !(p->evolve_max_timestep > 0))
@@ -280,17 +282,18 @@ __global__ void k_set_bed_interp(DevState *st, long long v) { st->bed_interp = v
 // Fused-kernel launches at one of the workgroup sizes the autotuner chooses
 // from (128, 256, 512 lanes).
 template <int B>
-void launch_flux_b(sw_handle *h, bool mixed) {
+void launch_flux_b(sw_handle *h, bool mixed, int first) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
     if (mixed)
-        hipLaunchKernelGGL((k_flux_mixed<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+        hipLaunchKernelGGL((k_flux_mixed<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st, first);
     else
-        hipLaunchKernelGGL((k_flux<double, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+        hipLaunchKernelGGL((k_flux<double, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st,
+                           first);
 }
 template <int B>
-void launch_update_b(sw_handle *h) {
+void launch_update_b(sw_handle *h, int mode, double a, double b) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
-    hipLaunchKernelGGL((k_update<true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    hipLaunchKernelGGL((k_update<true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st, mode, a, b);
 }
 template <int B>
 void launch_distribute_b(sw_handle *h, bool mixed) {
@@ -303,35 +306,69 @@ void launch_distribute_b(sw_handle *h, bool mixed) {
                            h->st);
 }
 
-// Enqueues one fused step.  slot >= 0 records HIP events around every kernel
-// into event group `slot` (timing mode).
+// Stages of a step: 1 (Euler), 2 (rk2) or 3 (rk3).
+int rk_stages(const sw_params &p) {
+    return p.timestepping_method == 2 ? 2 : p.timestepping_method == 3 ? 3 : 1;
+}
+
+// Weights (a, b) of q = a q + b q0 after stage `stage` (1-based, > 1) of an
+// n-stage Runge-Kutta step (ANUGA's evolve_one_rk2_step / evolve_one_rk3_step).
+void rk_weights(int stages, int stage, double *a, double *b) {
+    if (stages == 2) {
+        *a = 0.5, *b = 0.5;
+    } else if (stage == 2) {
+        *a = 0.25, *b = 0.75;
+    } else {
+        *a = 2.0 / 3.0, *b = 1.0 / 3.0;
+    }
+}
+
+// Enqueues one fused step: flux, timestep, update (+ protect), distribute per
+// stage; a Runge-Kutta step keeps its starting values in the first stage's
+// update, combines them into the later stages' updates and finishes the step
+// after the last stage's flux.  slot >= 0 records HIP events around the first
+// stage's kernels into event group `slot` (timing mode).
 int enqueue_fused_step(sw_handle *h, int slot) {
     const bool mixed = h->params.precision == SW_PRECISION_MIXED;
+    const int stages = rk_stages(h->params);
     auto rec = [&](int idx) {
         if (slot >= 0) (void)hipEventRecord(h->ev[slot * (kNumTimed + 1) + idx], h->stream);
     };
-    rec(0);
-    switch (h->blk[0]) {
-        case 128: launch_flux_b<128>(h, mixed); break;
-        case 512: launch_flux_b<512>(h, mixed); break;
-        default: launch_flux_b<256>(h, mixed); break;
+    for (int stage = 1; stage <= stages; stage++) {
+        const bool first = stage == 1;
+        if (first) rec(0);
+        switch (h->blk[0]) {
+            case 128: launch_flux_b<128>(h, mixed, first); break;
+            case 512: launch_flux_b<512>(h, mixed, first); break;
+            default: launch_flux_b<256>(h, mixed, first); break;
+        }
+        if (first) rec(1);
+        if (first || stage == stages) {
+            const int mode = stages == 1 ? kStepEuler : first ? kStageFirst : kStageLast;
+            hipLaunchKernelGGL((k_timestep<true>), dim3(h->n_inst), dim3(kReduceBlock), 0, h->stream, h->mesh,
+                               h->f.wave_min, h->st, h->dp, mode);
+        }
+        if (first) rec(2);
+        int mode = kUpdateEuler;
+        double a = 0.0, b = 0.0;
+        if (stages > 1 && first) mode = kUpdateBackup;
+        if (!first) {
+            mode = kUpdateSaxpy;
+            rk_weights(stages, stage, &a, &b);
+        }
+        switch (h->blk[1]) {
+            case 128: launch_update_b<128>(h, mode, a, b); break;
+            case 512: launch_update_b<512>(h, mode, a, b); break;
+            default: launch_update_b<256>(h, mode, a, b); break;
+        }
+        if (first) rec(3);
+        switch (h->blk[2]) {
+            case 128: launch_distribute_b<128>(h, mixed); break;
+            case 512: launch_distribute_b<512>(h, mixed); break;
+            default: launch_distribute_b<256>(h, mixed); break;
+        }
+        if (first) rec(4);
     }
-    rec(1);
-    hipLaunchKernelGGL((k_timestep<true>), dim3(h->n_inst), dim3(kReduceBlock), 0, h->stream, h->mesh,
-                       h->f.wave_min, h->st, h->dp);
-    rec(2);
-    switch (h->blk[1]) {
-        case 128: launch_update_b<128>(h); break;
-        case 512: launch_update_b<512>(h); break;
-        default: launch_update_b<256>(h); break;
-    }
-    rec(3);
-    switch (h->blk[2]) {
-        case 128: launch_distribute_b<128>(h, mixed); break;
-        case 512: launch_distribute_b<512>(h, mixed); break;
-        default: launch_distribute_b<256>(h, mixed); break;
-    }
-    rec(4);
     return launch_check(h, "fused step");
 }
 
@@ -672,7 +709,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         if (dalloc(h, pp, 3 * N)) return bail();
     for (auto pp : perNb)
         if (dalloc(h, pp, Nb)) return bail();
-    if (dalloc(h, &F.cq, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &F.grav, N) ||
+    if (dalloc(h, &F.cq, N) || dalloc(h, &F.cq0, N) || dalloc(h, &F.eq, 3 * N) || dalloc(h, &F.grav, N) ||
         dalloc(h, &F.wave_min, (size_t)nwaves) || dalloc(h, &F.eqf, 3 * N) || dalloc(h, &h->staging, 3 * Nmax))
         return bail();
     if (dalloc(h, &h->st, n_inst)) return bail();
@@ -954,11 +991,13 @@ int sw_compute_fluxes(sw_handle *h, double *flux_timestep) {
     if (int rc = single(h, "compute_fluxes")) return rc;
     const unsigned g = grid_for(h->N);
     if (h->params.precision == SW_PRECISION_MIXED)
-        hipLaunchKernelGGL((k_flux<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+        hipLaunchKernelGGL((k_flux<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st,
+                           1);
     else
-        hipLaunchKernelGGL((k_flux<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+        hipLaunchKernelGGL((k_flux<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp,
+                           h->st, 1);
     hipLaunchKernelGGL((k_timestep<false>), dim3(1), dim3(kReduceBlock), 0, h->stream, h->mesh, h->f.wave_min,
-                       h->st, h->dp);
+                       h->st, h->dp, kStepEuler);
     if (int rc = launch_check(h, "compute_fluxes")) return rc;
     DevState d;
     int rc = sync_state(h, 0, &d);
@@ -997,10 +1036,25 @@ int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t
 int sw_update_conserved_quantities(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
     if (int rc = single(h, "update_conserved_quantities")) return rc;
-    hipLaunchKernelGGL((k_update<false>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
+    hipLaunchKernelGGL((k_update<false>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st,
+                       kUpdateEuler, 0.0, 0.0);
     if (int rc = launch_check(h, "update")) return rc;
     DevState d;
     return sync_state(h, 0, &d);
+}
+
+int sw_backup_conserved_quantities(sw_handle *h) {
+    if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "backup_conserved_quantities")) return rc;
+    hipLaunchKernelGGL(k_backup, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
+    return launch_check(h, "backup_conserved_quantities");
+}
+
+int sw_saxpy_conserved_quantities(sw_handle *h, double a, double b) {
+    if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, "saxpy_conserved_quantities")) return rc;
+    hipLaunchKernelGGL(k_saxpy, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, a, b);
+    return launch_check(h, "saxpy_conserved_quantities");
 }
 
 int sw_protect(sw_handle *h) {
@@ -1237,15 +1291,30 @@ int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event) {
     DevState d;
     if (int rc = sync_state(h, 0, &d)) return rc;
     if (d.event != SW_EVENT_NONE) return fail(h, SW_ERR_STATE, "a yield is pending (call sw_advance_yield)");
-    double ft;
-    if (int rc = sw_compute_fluxes(h, &ft)) return rc;
-    if (int rc = sw_gravity_wb(h)) return rc;
-    if (int rc = sw_manning_friction(h)) return rc;
-    hipLaunchKernelGGL(k_update_timestep, dim3(1), dim3(1), 0, h->stream, h->st, h->dp);
-    if (int rc = sw_update_conserved_quantities(h)) return rc;
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, h->stream, h->st);
-    if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
-    if (int rc = sw_update_boundary(h)) return rc;
+    // Runge-Kutta stages after the first reuse the first stage's timestep
+    // (ANUGA's evolve_one_rk2_step / evolve_one_rk3_step).  rk3's intermediate
+    // time t0 + dt/2 is read by no operator here (the boundaries and forcing
+    // terms do not depend on time), so the time goes from t0 to t0 + dt.
+    const int stages = rk_stages(h->params);
+    if (stages > 1)
+        if (int rc = sw_backup_conserved_quantities(h)) return rc;
+    for (int stage = 1; stage <= stages; stage++) {
+        double ft;
+        if (int rc = sw_compute_fluxes(h, &ft)) return rc;
+        if (int rc = sw_gravity_wb(h)) return rc;
+        if (int rc = sw_manning_friction(h)) return rc;
+        if (stage == 1) hipLaunchKernelGGL(k_update_timestep, dim3(1), dim3(1), 0, h->stream, h->st, h->dp);
+        if (int rc = sw_update_conserved_quantities(h)) return rc;
+        if (stage == 1) {
+            hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, h->stream, h->st);
+        } else {
+            double a, b;
+            rk_weights(stages, stage, &a, &b);
+            if (int rc = sw_saxpy_conserved_quantities(h, a, b)) return rc;
+        }
+        if (int rc = sw_distribute_to_vertices_and_edges(h)) return rc;
+        if (int rc = sw_update_boundary(h)) return rc;
+    }
     if (int rc = sw_update_other_quantities(h)) return rc;
     hipLaunchKernelGGL(k_finish_step, dim3(1), dim3(1), 0, h->stream, h->st);
     if (int rc = launch_check(h, "step bookkeeping")) return rc;

@@ -96,6 +96,7 @@ struct DevMesh {
 
 struct DevFields {
     Q4 *cq;                               // [N]     centroid w, uh, vh, z
+    Q4 *cq0;                              // [N]     centroid values at the start of a Runge-Kutta step
     Q4 *eq;                               // [3][N]  edge w, uh, vh, z
     double *wv, *uhv, *vhv, *zv;          // [3][N]  vertex values
     double *ze;                           // [3][N]  bed edge values (static copy)
@@ -212,6 +213,28 @@ SW_DEV void finish_step(DevState *st) {
     st->event = event;
 }
 
+// Early exit of a fused kernel.  The first (or only) stage of a step halts an
+// instance whose previous step ended at a yield, the final time or an error;
+// a later Runge-Kutta stage runs only while its step runs (not halted, no
+// error).
+SW_DEV bool stage_skipped(DevState *st, int first) {
+    if (first) {
+        if (st->event || st->error) {
+            if (threadIdx.x == 0) st->halt = 1;
+            return true;
+        }
+        return false;
+    }
+    return st->halt || st->error;
+}
+
+// What k_timestep does in a fused step: the whole Euler step's bookkeeping;
+// the first stage of a Runge-Kutta step (timestep and time advance: every
+// stage uses the first stage's timestep, gpu_domain_advanced.py:2566-2573
+// with ANUGA's evolve_one_rk2_step / evolve_one_rk3_step); the last stage of
+// one (its flux_timestep and the end-of-step bookkeeping).
+enum : int { kStepEuler = 0, kStageFirst = 1, kStageLast = 2 };
+
 // ---------------------------------------------------------------------------
 // compute_fluxes (shallow_water_ext.c:4519-4696).
 //
@@ -231,15 +254,14 @@ SW_DEV void finish_step(DevState *st) {
 //
 // FUSED adds: boundary values evaluated on the fly (reflective / Dirichlet /
 // transmissive) and gravity_wb.
+//
+// first: the first (or only) stage of a step.  A later Runge-Kutta stage runs
+// only if its step is running: not halted and no error.
 template <typename R, bool FUSED, int BLOCK = kBlock>
-__global__ __launch_bounds__(BLOCK) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+__global__ __launch_bounds__(BLOCK) void k_flux(DevMesh M, DevFields F, DevParams P, DevState *st_all, int first) {
     const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
-    if (FUSED && (st->event || st->error)) {
-        // the previous step of this instance ended at a yield / the final time / an error
-        if (threadIdx.x == 0) st->halt = 1;
-        return;
-    }
+    if (FUSED && stage_skipped(st, first)) return;
     const int N = M.N;
     const int k = ln.k;
     double dtk = P.evolve_max_timestep;
@@ -386,7 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(DevMesh M, DevFields F, DevParam
 constexpr int kReduceBlock = 1024;
 template <bool FUSED>
 __global__ __launch_bounds__(kReduceBlock) void k_timestep(DevMesh M, const unsigned long long *wave_min,
-                                                           DevState *st_all, DevParams P) {
+                                                           DevState *st_all, DevParams P, int mode) {
     const int inst = blockIdx.x;
     DevState *st = st_all + inst;
     // state and partial minima are requested together: one memory round trip
@@ -409,15 +431,15 @@ __global__ __launch_bounds__(kReduceBlock) void k_timestep(DevMesh M, const unsi
     if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = b;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    if (FUSED && s.halt) return;
+    if (FUSED && (s.halt || (mode == kStageLast && s.error))) return;
 #pragma unroll
     for (int w = 0; w < kReduceBlock / 64; w++) b = (wmin[w] < b) ? wmin[w] : b;
     s.flux_timestep = bitsd(b);
     if (FUSED) {
-        timestep_logic(&s, P);
+        if (mode != kStageLast) timestep_logic(&s, P);
         if (!(s.error & kErrSmallsteps)) {
-            s.time = s.time + s.timestep;
-            finish_step(&s);
+            if (mode != kStageLast) s.time = s.time + s.timestep;
+            if (mode != kStageFirst) finish_step(&s);
         }
     }
     *st = s;
@@ -429,13 +451,11 @@ __global__ __launch_bounds__(kReduceBlock) void k_timestep(DevMesh M, const unsi
 // FP64.  The bed must be continuous across edges (k_check_bed), so a side's
 // depth is its own edge depth.
 template <int BLOCK = kBlock>
-__global__ __launch_bounds__(BLOCK) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+__global__ __launch_bounds__(BLOCK) void k_flux_mixed(DevMesh M, DevFields F, DevParams P, DevState *st_all,
+                                                      int first) {
     const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
-    if (st->event || st->error) {
-        if (threadIdx.x == 0) st->halt = 1;
-        return;
-    }
+    if (stage_skipped(st, first)) return;
     const int N = M.N;
     const int k = ln.k;
     double dtk = P.evolve_max_timestep;
@@ -689,22 +709,40 @@ SW_DEV double2 gravity_gradient_term(const double2 v[3], const double wv[3], con
     return make_double2(-g * wx * avg_h, -g * wy * avg_h);
 }
 
+// saxpy_centroid_values (saxpy_centroid_values.cu:1-12) on the conserved
+// quantities: q = a q + b q0
+SW_DEV void saxpy_conserved(Q4 &c, const Q4 &c0, double a, double b) {
+    c.w = a * c.w + b * c0.w;
+    c.uh = a * c.uh + b * c0.uh;
+    c.vh = a * c.vh + b * c0.vh;
+}
+
+// What k_update does besides the update in a fused step.
+enum : int { kUpdateEuler = 0, kUpdateBackup = 1, kUpdateSaxpy = 2 };
+
 // update_conserved_quantities (quantity.c:769-820), optionally followed by
-// protect (the first stage of distribute_to_vertices_and_edges).
+// protect (the first stage of distribute_to_vertices_and_edges).  Fused
+// Runge-Kutta stages: the first (kUpdateBackup) keeps the values it starts
+// from in cq0 (backup_conserved_quantities); a later one (kUpdateSaxpy)
+// combines its result with them, q = a q + b q0
+// (saxpy_conserved_quantities), before protect.
 template <bool FUSED, int BLOCK = kBlock>
-__global__ __launch_bounds__(BLOCK) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
+__global__ __launch_bounds__(BLOCK) void k_update(DevMesh M, DevFields F, DevParams P, DevState *st_all, int mode,
+                                                  double a, double b) {
     const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
-    if (FUSED && (st->halt || (st->error & kErrSmallsteps))) return;
+    if (FUSED && (st->halt || (st->error & (mode == kUpdateSaxpy ? ~0ll : kErrSmallsteps)))) return;
     const int k = ln.k;
     if (!ln.valid) return;
     const double dt = st->timestep;
     Q4 c = ldq(F.cq + k);
+    if (FUSED && mode == kUpdateBackup) F.cq0[k] = c;
     double siw = ld_s(F.wsi + k), siu = ld_s(F.uhsi + k), siv = ld_s(F.vhsi + k);
     bool ok = update_one(c.w, ld_s(F.weu + k), siw, dt);
     ok = update_one(c.uh, ld_s(F.uheu + k), siu, dt) && ok;
     ok = update_one(c.vh, ld_s(F.vheu + k), siv, dt) && ok;
     if (!ok) set_error(st, kErrUpdate);
+    if (FUSED && mode == kUpdateSaxpy) saxpy_conserved(c, ldq(F.cq0 + k), a, b);
     if (FUSED) {
         protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
         // The centroid values are final for this step (distribute leaves them
@@ -718,6 +756,22 @@ __global__ __launch_bounds__(BLOCK) void k_update(DevMesh M, DevFields F, DevPar
         F.uhsi[k] = 0.0;
         F.vhsi[k] = 0.0;
     }
+    F.cq[k] = c;
+}
+
+// backup_conserved_quantities (gpu_domain_advanced.py:1449-1475)
+__global__ __launch_bounds__(kBlock) void k_backup(DevMesh M, DevFields F) {
+    const int k = tri_index();
+    if (k >= M.N) return;
+    F.cq0[k] = ldq(F.cq + k);
+}
+
+// saxpy_conserved_quantities (gpu_domain_advanced.py:1481-1519)
+__global__ __launch_bounds__(kBlock) void k_saxpy(DevMesh M, DevFields F, double a, double b) {
+    const int k = tri_index();
+    if (k >= M.N) return;
+    Q4 c = ldq(F.cq + k);
+    saxpy_conserved(c, ldq(F.cq0 + k), a, b);
     F.cq[k] = c;
 }
 

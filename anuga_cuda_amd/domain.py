@@ -72,6 +72,8 @@ _QID = {
     "yvelocity": SW_YVELOCITY,
 }
 CONSERVED = ("stage", "xmomentum", "ymomentum")
+# timestepping methods and their code in sw_params.timestepping_method
+TIMESTEPPING = {"euler": 1, "rk2": 2, "rk3": 3}
 _LOC = {
     "centroid_values": SW_CENTROID,
     "edge_values": SW_EDGE,
@@ -355,11 +357,18 @@ class Domain:
         return self.quantities[name]
 
     def get_timestepping_method(self):
-        return "euler"
+        return self.__dict__.get("_timestepping", "euler")
 
     def set_timestepping_method(self, method):
-        if method != "euler":
-            raise NotImplementedError("only Euler timestepping is built (the reference's profiled path)")
+        """'euler' (default), 'rk2' or 'rk3' (ANUGA's set_timestepping_method;
+        the reference drivers dispatch on it, gpu_domain_advanced.py:2566-2573)."""
+        if method in (1, 2, 3):  # ANUGA also accepts the method's number
+            method = list(TIMESTEPPING)[method - 1]
+        if method not in TIMESTEPPING:
+            raise ValueError("timestepping method must be one of %s" % ", ".join(TIMESTEPPING))
+        self._timestepping = method
+        if self._handle is not None:
+            self._handle.set_params(self._sw_params())
 
     def set_quantities_to_be_stored(self, *args, **kwargs):
         return None
@@ -421,6 +430,7 @@ class Domain:
             setattr(p, k, v)
         p.precision = _lib.SW_PRECISION_MIXED if self.precision == "mixed" else _lib.SW_PRECISION_FP64
         p.locality_order = _lib.SW_ORDER_HILBERT if self.locality_order == "hilbert" else _lib.SW_ORDER_KEEP
+        p.timestepping_method = TIMESTEPPING[self.get_timestepping_method()]
         return p
 
     def _attach(self, handle):
@@ -563,6 +573,15 @@ class Domain:
         self.compute_forcing_terms()
         self.update_timestep(yieldstep, finaltime)
         self.update_conserved_quantities()
+
+    def backup_conserved_quantities(self):
+        """Copies of the conserved centroid values (gpu_domain_advanced.py:1449-1475)."""
+        self._dev().call("sw_backup_conserved_quantities")
+
+    def saxpy_conserved_quantities(self, a, b):
+        """centroid = a * centroid + b * backup for the conserved quantities
+        (gpu_domain_advanced.py:1481-1519)."""
+        self._dev().call("sw_saxpy_conserved_quantities", float(a), float(b))
 
     def update_ghosts(self):
         return None

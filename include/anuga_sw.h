@@ -66,7 +66,9 @@ typedef struct sw_params {
     int32_t tight_slope_limiters, use_centroid_velocities, use_sloped_mannings;
     int32_t max_smallsteps, default_order, precision;
     int32_t locality_order;  /* SW_ORDER_*; arrays at the ABI stay in the caller's order */
-    int32_t reserved;
+    /* timestepping_method (sw_domain.h:49; gpu_domain_advanced.py:2566-2573):
+     * 1 (or 0) Euler, 2 rk2, 3 rk3 */
+    int32_t timestepping_method;
 } sw_params;
 
 /* Mesh arrays (host, borrowed during sw_create). */
@@ -166,6 +168,13 @@ int sw_manning_friction(sw_handle *h);
 int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime);
 /* update_conserved_quantities (gpu_domain_advanced.py:1412-1442; quantity.c:769-820) */
 int sw_update_conserved_quantities(sw_handle *h);
+/* backup_conserved_quantities (gpu_domain_advanced.py:1449-1475; evolve.c:549-561):
+ * keeps a copy of the stage / xmomentum / ymomentum centroid values */
+int sw_backup_conserved_quantities(sw_handle *h);
+/* saxpy_conserved_quantities (gpu_domain_advanced.py:1481-1519;
+ * saxpy_centroid_values.cu:1-12): centroid = a * centroid + b * backup for the
+ * conserved quantities */
+int sw_saxpy_conserved_quantities(sw_handle *h, double a, double b);
 /* protect_against_infinitesimal_and_negative_heights
  * (gpu_domain_advanced.py:1003-1055; shallow_water_ext.c:1638-1705) */
 int sw_protect(sw_handle *h);
@@ -193,14 +202,18 @@ int sw_update_other_quantities(sw_handle *h);
 /* evolve(): preamble up to the first yield (gpu_domain_advanced.py:2473-2545). */
 int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime);
 /* evolve() loop body (gpu_domain_advanced.py:2555-2663 / evolve.c:1466-1937):
- * runs up to max_steps Euler steps on the device with fused kernels replayed
- * from a hipGraph, stopping at the first yield or the final time.  Returns the
- * number of steps taken and the event (SW_EVENT_*). */
+ * runs up to max_steps steps (Euler, rk2 or rk3 per
+ * params.timestepping_method, gpu_domain_advanced.py:2566-2573) on the device
+ * with fused kernels replayed from a hipGraph, stopping at the first yield or
+ * the final time.  Returns the number of steps taken and the event
+ * (SW_EVENT_*). */
 int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_t *event);
 /* Bookkeeping when evolve() resumes after a yield (gpu_domain_advanced.py:2649-2656). */
 int sw_advance_yield(sw_handle *h);
 /* Unfused reference-order step (one launch per operator, as the reference
- * driver's evolve_one_euler_step + distribute + update_boundary). */
+ * driver's evolve_one_euler_step + distribute + update_boundary; for rk2 /
+ * rk3 with backup_conserved_quantities and saxpy_conserved_quantities between
+ * the stages). */
 int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event);
 
 /* Autotuner of the fused step's launch shapes (no counterpart in the

@@ -56,7 +56,11 @@ class OrDomain(C.Structure):
         + [("friction_centroid_values", _F64P)]
         + [(p + "_" + s, _F64P) for p in ("height", "xvelocity", "yvelocity") for s in _LOCS]
         + [("max_speed", _F64P), ("already_computed_flux", _I64P), ("flux_call", C.c_int64)]
+        + [("timestepping_method", C.c_int64)]
+        + [(p + "_centroid_backup", _F64P) for p in ("stage", "xmom", "ymom")]
     )
+
+_METHODS = {"euler": 1, "rk2": 2, "rk3": 3}
 
 
 _lib = None
@@ -92,6 +96,9 @@ def load():
         "or_update_other_quantities": ([P], None),
         "or_update_timestep": ([P], C.c_int),
         "or_evolve_one_euler_step": ([P], C.c_int),
+        "or_backup_conserved_quantities": ([P], None),
+        "or_saxpy_conserved_quantities": ([P, C.c_double, C.c_double], None),
+        "or_evolve_one_rk_step": ([P, C.c_int], C.c_int),
         "or_evolve_prepare": ([P, C.c_double, C.c_double, C.c_int64], C.c_int),
         "or_evolve_step": ([P, _I64P], C.c_int),
         "or_evolve_steps": ([P, C.c_int64, _I64P, _I64P], C.c_int),
@@ -175,6 +182,9 @@ class OracleDomain:
         keep("max_speed", np.zeros(N), np.float64)
         keep("already_computed_flux", np.zeros(3 * N, dtype=np.int64), np.int64)
         d.flux_call = 0
+        d.timestepping_method = _METHODS[params.get("timestepping_method", "euler")]
+        for cname in ("stage", "xmom", "ymom"):
+            keep(cname + "_centroid_backup", np.zeros(N), np.float64)
         d._order_ = int(params["default_order"])
         d.evolve_max_timestep = float(params["evolve_max_timestep"])
         d.recorded_min_timestep = d.evolve_max_timestep
@@ -191,7 +201,8 @@ class OracleDomain:
         for name, Q in domain.quantities.items():
             q[name] = {k: v.copy() for k, v in Q._arrays.items()}
         state = {"time": domain._time, "starttime": domain.starttime, "_order_": domain._order_}
-        return cls(domain.mesh, domain._params, q, domain._kinds, domain._dirichlet, state)
+        params = dict(domain._params, timestepping_method=domain.get_timestepping_method())
+        return cls(domain.mesh, params, q, domain._kinds, domain._dirichlet, state)
 
     # array access by ANUGA quantity name
     def q(self, name, loc):
@@ -254,6 +265,12 @@ class OracleDomain:
 
     def update_other_quantities(self):
         self.lib.or_update_other_quantities(self._p())
+
+    def backup_conserved_quantities(self):
+        self.lib.or_backup_conserved_quantities(self._p())
+
+    def saxpy_conserved_quantities(self, a, b):
+        self.lib.or_saxpy_conserved_quantities(self._p(), float(a), float(b))
 
     def evolve_prepare(self, yieldstep, finaltime):
         self._rc(self.lib.or_evolve_prepare(self._p(), float(yieldstep),

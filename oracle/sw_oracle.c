@@ -797,6 +797,73 @@ int or_evolve_one_euler_step(or_domain *D) {
     return or_update_conserved_quantities(D);
 }
 
+/* backup_conserved_quantities (evolve.c:549-561; gpu_domain_advanced.py:1449-1475):
+ * copies of the conserved centroid values (the reference's memcpy passes the
+ * element count as the byte count; the whole arrays are meant) */
+void or_backup_conserved_quantities(or_domain *D) {
+    const size_t n = (size_t)D->number_of_elements * sizeof(double);
+    memcpy(D->stage_centroid_backup, D->stage_centroid_values, n);
+    memcpy(D->xmom_centroid_backup, D->xmom_centroid_values, n);
+    memcpy(D->ymom_centroid_backup, D->ymom_centroid_values, n);
+}
+
+/* saxpy_conserved_quantities (evolve.c:566-603; saxpy_centroid_values.cu:1-12):
+ * centroid = a * centroid + b * backup for each conserved quantity */
+void or_saxpy_conserved_quantities(or_domain *D, double a, double b) {
+    double *q[3] = {D->stage_centroid_values, D->xmom_centroid_values, D->ymom_centroid_values};
+    const double *p[3] = {D->stage_centroid_backup, D->xmom_centroid_backup, D->ymom_centroid_backup};
+    for (int c = 0; c < 3; c++)
+        for (int64_t k = 0; k < D->number_of_elements; k++) q[c][k] = a * q[c][k] + b * p[c][k];
+}
+
+/* One Euler stage after the first: compute_fluxes, forcing terms and
+ * update_conserved_quantities with the timestep of the first stage. */
+static int or_rk_stage(or_domain *D) {
+    or_compute_fluxes(D);
+    or_gravity_wb(D);
+    or_compute_forcing_terms(D);
+    return or_update_conserved_quantities(D);
+}
+
+/* ANUGA's Generic_Domain.evolve_one_rk2_step / evolve_one_rk3_step, which
+ * the reference drivers call for timestepping_method 'rk2' / 'rk3'
+ * (gpu_domain_advanced.py:2198-2300, 2566-2573; the C stubs evolve.c:1093-1100
+ * are empty).  Restated from ANUGA's published algorithm (the anuga package
+ * is not in the reference): the first stage is the Euler step (with the
+ * timestep and time advance) followed by distribute and update_boundary; rk2
+ * adds one stage and Q = 0.5 Q + 0.5 Q0; rk3 adds Q = 0.25 Q + 0.75 Q0 (time
+ * t0 + dt/2) and Q = 2/3 Q + 1/3 Q0 (time t0 + dt), each followed by
+ * distribute and update_boundary.  Ends with time t0 + dt. */
+int or_evolve_one_rk_step(or_domain *D, int stages) {
+    const double t0 = D->time;
+    or_backup_conserved_quantities(D);
+    int rc = or_evolve_one_euler_step(D);
+    if (rc) return rc;
+    D->time = t0 + D->timestep;
+    rc = or_distribute_to_vertices_and_edges(D);
+    if (rc) return rc;
+    or_update_boundary(D);
+    rc = or_rk_stage(D);
+    if (rc) return rc;
+    if (stages == 2) {
+        or_saxpy_conserved_quantities(D, 0.5, 0.5);
+    } else {
+        or_saxpy_conserved_quantities(D, 0.25, 0.75);
+        D->time = t0 + D->timestep * 0.5;
+        rc = or_distribute_to_vertices_and_edges(D);
+        if (rc) return rc;
+        or_update_boundary(D);
+        rc = or_rk_stage(D);
+        if (rc) return rc;
+        or_saxpy_conserved_quantities(D, 2.0 / 3.0, 1.0 / 3.0);
+        D->time = t0 + D->timestep;
+    }
+    rc = or_distribute_to_vertices_and_edges(D);
+    if (rc) return rc;
+    or_update_boundary(D);
+    return OR_OK;
+}
+
 /* gpu_domain_advanced.py:2473-2545: evolve() preamble up to the first yield */
 int or_evolve_prepare(or_domain *D, double yieldstep, double finaltime, int64_t has_finaltime) {
     int rc = or_distribute_to_vertices_and_edges(D);
@@ -823,12 +890,18 @@ int or_evolve_prepare(or_domain *D, double yieldstep, double finaltime, int64_t 
 int or_evolve_step(or_domain *D, int64_t *event) {
     const double epsilon = 1.0e-12; /* anuga.config.epsilon */
     double initial_time = D->time;
-    int rc = or_evolve_one_euler_step(D);
-    if (rc) return rc;
-    D->time = initial_time + D->timestep;
-    rc = or_distribute_to_vertices_and_edges(D);
-    if (rc) return rc;
-    or_update_boundary(D);
+    int rc;
+    if (D->timestepping_method == 2 || D->timestepping_method == 3) {
+        rc = or_evolve_one_rk_step(D, (int)D->timestepping_method);
+        if (rc) return rc;
+    } else {
+        rc = or_evolve_one_euler_step(D);
+        if (rc) return rc;
+        D->time = initial_time + D->timestep;
+        rc = or_distribute_to_vertices_and_edges(D);
+        if (rc) return rc;
+        or_update_boundary(D);
+    }
     or_update_other_quantities(D);
     D->number_of_steps += 1;
     if (D->_order_ == 1) D->number_of_first_order_steps += 1;

@@ -71,6 +71,10 @@ typedef struct or_domain {
     double *max_speed;
     int64_t *already_computed_flux; /* [N][3] work array, zero-initialised */
     int64_t flux_call;              /* counter behind already_computed_flux */
+    /* timestepping_method (sw_domain.h:49): 1 euler, 2 rk2, 3 rk3; backups
+     * of the conserved centroid values for the multi-stage steps */
+    int64_t timestepping_method;
+    double *stage_centroid_backup, *xmom_centroid_backup, *ymom_centroid_backup;
 } or_domain;
 
 /* status codes */
@@ -102,6 +106,9 @@ void or_update_boundary(or_domain *D);
 void or_update_other_quantities(or_domain *D);
 int or_update_timestep(or_domain *D);
 int or_evolve_one_euler_step(or_domain *D);
+void or_backup_conserved_quantities(or_domain *D);
+void or_saxpy_conserved_quantities(or_domain *D, double a, double b);
+int or_evolve_one_rk_step(or_domain *D, int stages);
 int or_evolve_prepare(or_domain *D, double yieldstep, double finaltime, int64_t has_finaltime);
 int or_evolve_step(or_domain *D, int64_t *event);
 int or_evolve_steps(or_domain *D, int64_t max_steps, int64_t *steps_done, int64_t *event);

@@ -115,6 +115,44 @@ def test_unfused_evolve_matches_oracle(name):
                 name_q + ".boundary", TOL)
 
 
+@pytest.mark.parametrize("method", ["rk2", "rk3"])
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "unfused"])
+def test_runge_kutta_steps_match_oracle(method, fused):
+    """rk2 / rk3 steps (backup, stages at the first stage's timestep, saxpy
+    combinations) in lockstep with the oracle's."""
+    d = SCENARIOS["delaunay_random"]() if fused else _dam()
+    d.set_timestepping_method(method)
+    stats = {}
+    od, dts = _lockstep(d, 16 if fused else 6, fused=fused, stats=stats)
+    assert len(dts) == (16 if fused else 6)
+    compare(d._handle.max_speed(), od.max_speed, "max_speed", TOL)
+
+
+def test_backup_and_saxpy_operators_match_oracle():
+    d = _dam()
+    for _ in d.evolve(yieldstep=0.5, finaltime=1.0):
+        pass
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    s = h.get_state()
+    s.yieldtime = 1e9
+    s.event = 0
+    h.set_state(s)
+    od.d.yieldtime = 1e9
+    h.call("sw_backup_conserved_quantities")
+    od.backup_conserved_quantities()
+    assert h.compute_fluxes() == od.compute_fluxes()
+    h.call("sw_update_timestep", 1e9, 0.0, 0)
+    od.update_timestep()
+    assert h.get_state().timestep == od.d.timestep > 0
+    h.call("sw_update_conserved_quantities")
+    od.update_conserved_quantities()
+    h.call("sw_saxpy_conserved_quantities", 2.0 / 3.0, 1.0 / 3.0)
+    od.saxpy_conserved_quantities(2.0 / 3.0, 1.0 / 3.0)
+    for q in ("stage", "xmomentum", "ymomentum"):
+        compare(device_array(d, q, "centroid_values"), od.q(q, "centroid_values"), q + ".saxpy", TOL)
+
+
 def test_operators_match_oracle_one_by_one():
     d = _dam()
     # advance a few steps so momenta and wet/dry fronts are non-trivial

@@ -12,7 +12,10 @@ in numpy 2 and the file uses the Python-2 C API), so the oracle is pinned by:
     with reflective walls, limiter bounds, Stoker's dam-break solution;
   * closed forms of the semi-implicit update and of Manning's friction term;
   * brute-force edge-by-edge recomputation on a tiny mesh and the reference's
-    4-triangle known-answer case (src/anuga_cuda/anuga_gpu/mytest.py:17-31).
+    4-triangle known-answer case (src/anuga_cuda/anuga_gpu/mytest.py:17-31);
+  * the rk2 / rk3 steps: the reference kernel's saxpy (saxpy_centroid_values.cu:11),
+    the observed temporal order of accuracy (1, 2, 3 for Euler, rk2, rk3 on a
+    smooth wave at fixed timesteps), lake at rest and mass conservation.
 """
 import copy
 import math
@@ -381,3 +384,89 @@ def test_compute_fluxes_brute_force_edge_once():
     got = np.stack([od.q(q, "explicit_update") for q in ("stage", "xmomentum", "ymomentum")], 1)
     np.testing.assert_allclose(got, eu, rtol=1e-13, atol=1e-13 * np.max(np.abs(eu)))
     assert ts == dt
+
+
+# ------------------------------------------------------------ rk2 / rk3
+def test_backup_and_saxpy_follow_the_reference_kernel():
+    """saxpy_centroid_values.cu:11: centroid = a * centroid + b * backup, the
+    backup being the values at backup_conserved_quantities."""
+    d = scenarios.dam_break_channel(6, 6)
+    od = _oracle(d)
+    names = ("stage", "xmomentum", "ymomentum")
+    q0 = {n: od.q(n, "centroid_values").copy() for n in names}
+    od.backup_conserved_quantities()
+    rng = np.random.default_rng(5)
+    q1 = {}
+    for n in names:
+        q1[n] = rng.normal(size=q0[n].size)
+        od.q(n, "centroid_values")[:] = q1[n]
+    od.saxpy_conserved_quantities(0.25, 0.75)
+    for n in names:
+        assert np.array_equal(od.q(n, "centroid_values"), 0.25 * q1[n] + 0.75 * q0[n])
+    od.saxpy_conserved_quantities(0.0, 1.0)  # restores the backup exactly
+    for n in names:
+        assert np.array_equal(od.q(n, "centroid_values"), q0[n])
+
+
+def _smooth_wave(method, dt, order=1, T=2.0):
+    """A 1 mm-per-cm Gaussian hump on 10 m of still water in a walled 100 m
+    square, stepped at the fixed timestep dt (below the CFL limit, so
+    evolve_max_timestep sets it) to time T."""
+    points, tris, boundary = rectangular_cross(16, 16, 100.0, 100.0)
+    d = Domain(mesh=Mesh(points, tris, boundary=boundary), evolve_max_timestep=dt, default_order=order)
+    d.set_quantity("elevation", 0.0)
+    d.set_quantity("stage", lambda x, y: 10.0 + 0.1 * np.exp(-((x - 50) ** 2 + (y - 50) ** 2) / (2 * 15.0 ** 2)))
+    d.set_quantity("friction", 0.0, location="centroids")
+    d.set_boundary({t: Reflective_boundary(d) for t in d.mesh.get_tag_names()})
+    d.set_timestepping_method(method)
+    od = _oracle(d)
+    for _ in od.evolve(T, T):
+        pass
+    assert abs(od.d.time - T) < 1e-12
+    return od.q("stage", "centroid_values").copy()
+
+
+@pytest.fixture(scope="module")
+def fine_wave():
+    return _smooth_wave("rk3", 0.0025)
+
+
+@pytest.mark.parametrize("method,order", [("euler", 1), ("rk2", 2), ("rk3", 3)])
+def test_temporal_order_of_accuracy(fine_wave, method, order):
+    """Halving the timestep divides the error by 2^order (space fixed, error
+    against the rk3 solution at a 32x smaller step).  Wrong stage weights or
+    a stage that skips the flux of its own state lose the order."""
+    errs = [np.max(np.abs(_smooth_wave(method, dt) - fine_wave)) for dt in (0.08, 0.04, 0.02)]
+    observed = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(abs(o - order) < 0.2 for o in observed), (errs, observed)
+
+
+@pytest.mark.parametrize("method", ["rk2", "rk3"])
+def test_rk_lake_at_rest_and_mass_conservation(method):
+    d = scenarios.dam_break_channel(20, 20, high=2.0, low=2.0, bump=0.8)
+    d.set_timestepping_method(method)
+    od = _run(_oracle(d), 30)
+    np.testing.assert_allclose(od.q("stage", "centroid_values"), 2.0, rtol=0, atol=1e-12)
+    assert np.max(np.abs(od.q("xmomentum", "centroid_values"))) < 1e-12
+    d = scenarios.dam_break_channel(20, 20, high=2.0, low=1.2, bump=0.3)
+    d.set_timestepping_method(method)
+    od = _oracle(d)
+    od.evolve_prepare(1e9, None)
+    area = d.mesh.areas
+    v0 = np.sum(area * od.q("stage", "centroid_values"))
+    for _ in range(40):
+        od.evolve_step()
+    v1 = np.sum(area * od.q("stage", "centroid_values"))
+    assert abs(v1 - v0) / v0 < 1e-13
+    assert od.d.number_of_steps == 40
+
+
+def test_timestepping_method_names():
+    d = scenarios.dam_break_channel(4, 4)
+    assert d.get_timestepping_method() == "euler"
+    d.set_timestepping_method(2)
+    assert d.get_timestepping_method() == "rk2"
+    d.set_timestepping_method("rk3")
+    assert _oracle(d).d.timestepping_method == 3
+    with pytest.raises(ValueError):
+        d.set_timestepping_method("rk4")
