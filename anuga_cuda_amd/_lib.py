@@ -117,6 +117,8 @@ _SIGNATURES = {
     "sw_advance_yield": ([C.c_void_p], C.c_int),
     "sw_evolve_one_step_unfused": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
     "sw_backup_conserved_quantities": ([C.c_void_p], C.c_int),
+    "sw_set_sww_points": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
+    "sw_sww_values": ([C.c_void_p, C.c_double, C.c_void_p, C.c_int64], C.c_int),
     "sw_saxpy_conserved_quantities": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
     "sw_synchronize": ([C.c_void_p], C.c_int),
     "sw_autotune": ([C.c_void_p, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
@@ -318,6 +320,24 @@ class Handle:
     def download_conserved_device(self, w, uh, vh):
         """Conserved centroid values into three device buffers."""
         self.call("sw_download_conserved_device", C.c_void_p(int(w)), C.c_void_p(int(uh)), C.c_void_p(int(vh)))
+
+    def set_sww_points(self, number_of_triangles_per_node=None, vertex_value_indices=None):
+        """Output points for sw_sww_values: mesh nodes (both arrays given) or
+        unique vertices (neither)."""
+        if number_of_triangles_per_node is None:
+            self.call("sw_set_sww_points", 0, None, None)
+            return
+        counts = np.ascontiguousarray(number_of_triangles_per_node, dtype=np.int64)
+        idx = np.ascontiguousarray(vertex_value_indices, dtype=np.int64)
+        self.call("sw_set_sww_points", len(counts), counts.ctypes.data, idx.ctypes.data)
+
+    def sww_values(self, npoints, minimum_storable_height, out=None):
+        """FP32 [4][npoints] elevation, stage, xmomentum, ymomentum at the points."""
+        if out is None:
+            out = np.empty((4, npoints), dtype=np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.shape == (4, npoints)
+        self.call("sw_sww_values", float(minimum_storable_height), out.ctypes.data, out.size)
+        return out
 
     def get_state(self, check=True):
         """The evolve state of the selected instance.  With ``check`` a latched

@@ -37,6 +37,14 @@ struct Inst {
     int64_t n = 0, off = 0, nb = 0, boff = 0, poff = 0;
 };
 
+// Output points of one instance (sw_set_sww_points): point p averages the
+// vertex-value slots slot[start[p] .. start[p+1]); out holds the FP32 values.
+struct SwwPoints {
+    int64_t P = 0;
+    int *start = nullptr, *slot = nullptr;
+    float *out = nullptr;
+};
+
 }  // namespace
 
 struct sw_handle {
@@ -63,6 +71,7 @@ struct sw_handle {
     int *cal_of = nullptr;      // [N] caller index (within its instance) of device triangle d
     int *dev_of = nullptr;      // [sum n] device index (within its instance) of caller triangle k
     int blk[3] = {kBlock, kBlock, kBlock};  // workgroup size of the fused flux, update, distribute kernels
+    std::vector<SwwPoints> sww;             // [n_inst] output points
 };
 
 namespace {
@@ -101,6 +110,17 @@ int dalloc(sw_handle *h, T **p, size_t count) {
     e = hipMemset(q, 0, count * sizeof(T));
     if (e != hipSuccess) return fail(h, SW_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
     return SW_OK;
+}
+
+void dfree(sw_handle *h, void *p) {
+    if (!p) return;
+    for (auto &q : h->allocations)
+        if (q == p) {
+            (void)hipFree(p);
+            q = h->allocations.back();
+            h->allocations.pop_back();
+            return;
+        }
 }
 
 DevParams to_dev(const sw_params &p) {
@@ -650,6 +670,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     h->Nb = Nbtot;
     h->n_inst = n_inst;
     h->inst = inst;
+    h->sww.resize(n_inst);
     h->params = *p;
     h->dp = to_dev(*p);
     auto bail = [&](void) -> sw_handle * {
@@ -1041,6 +1062,75 @@ int sw_update_conserved_quantities(sw_handle *h) {
     if (int rc = launch_check(h, "update")) return rc;
     DevState d;
     return sync_state(h, 0, &d);
+}
+
+int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *number_of_triangles_per_node,
+                      const int64_t *vertex_value_indices) {
+    if (!h) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const Inst &I = h->inst[i];
+    const int64_t nv = 3 * I.n;
+    if (number_of_points < 0) return fail(h, SW_ERR_ARG, "number_of_points < 0");
+    // unique vertices: point 3k+j is vertex j of triangle k
+    const bool unique = number_of_points == 0;
+    const int64_t P = unique ? nv : number_of_points;
+    std::vector<int> start(P + 1), flat(nv);
+    if (unique) {
+        for (int64_t p = 0; p <= P; p++) start[p] = (int)p;
+        for (int64_t v = 0; v < nv; v++) flat[v] = (int)v;
+    } else {
+        if (!number_of_triangles_per_node || !vertex_value_indices)
+            return fail(h, SW_ERR_ARG, "node structure arrays are NULL");
+        int64_t acc = 0;
+        for (int64_t p = 0; p < P; p++) {
+            if (number_of_triangles_per_node[p] < 0) return fail(h, SW_ERR_ARG, "negative triangle count at node %lld", (long long)p);
+            start[p] = (int)acc;
+            acc += number_of_triangles_per_node[p];
+            if (acc > nv) break;
+        }
+        if (acc != nv)
+            return fail(h, SW_ERR_ARG, "triangle counts per node sum to %lld, not 3 * %lld", (long long)acc, (long long)I.n);
+        start[P] = (int)acc;
+        std::vector<char> seen(nv, 0);
+        for (int64_t v = 0; v < nv; v++) {
+            const int64_t x = vertex_value_indices[v];
+            if (x < 0 || x >= nv || seen[x]) return fail(h, SW_ERR_ARG, "vertex_value_indices is not a permutation of 0 .. 3N-1");
+            seen[x] = 1;
+            flat[v] = (int)x;
+        }
+    }
+    SwwPoints &S = h->sww[i];
+    dfree(h, S.start);
+    dfree(h, S.slot);
+    dfree(h, S.out);
+    S = SwwPoints{};
+    if (dalloc(h, &S.start, P + 1) || dalloc(h, &S.slot, nv) || dalloc(h, &S.out, 4 * P)) return SW_ERR_HIP;
+    S.P = P;
+    HIPCHECK(h, hipMemcpy(S.start, start.data(), (P + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(S.slot, flat.data(), nv * sizeof(int), hipMemcpyHostToDevice));
+    // caller vertex 3k+j -> device slot j*N + off + dev_of[k]
+    hipLaunchKernelGGL(k_sww_slots, dim3(grid_for(nv)), dim3(kBlock), 0, h->stream, (int)nv, (long long)h->N,
+                       (long long)I.off, h->dev_of + I.poff, S.slot);
+    if (int rc = launch_check(h, "k_sww_slots")) return rc;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return SW_OK;
+}
+
+int sw_sww_values(sw_handle *h, double minimum_storable_height, float *out, int64_t count) {
+    if (!h || !out) return SW_ERR_ARG;
+    const int i = selected(h);
+    if (i < 0) return i;
+    const SwwPoints &S = h->sww[i];
+    if (S.P == 0) return fail(h, SW_ERR_STATE, "no output points (call sw_set_sww_points)");
+    if (count != 4 * S.P) return fail(h, SW_ERR_ARG, "sww values: %lld floats for 4 x %lld points", (long long)count, (long long)S.P);
+    const DevFields &F = h->f;
+    hipLaunchKernelGGL(k_sww_points, dim3(grid_for(S.P)), dim3(kBlock), 0, h->stream, (long long)S.P, S.start, S.slot,
+                       F.zv, F.wv, F.uhv, F.vhv, (float)minimum_storable_height, S.out);
+    if (int rc = launch_check(h, "k_sww_points")) return rc;
+    HIPCHECK(h, hipMemcpyAsync(out, S.out, 4 * S.P * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return SW_OK;
 }
 
 int sw_backup_conserved_quantities(sw_handle *h) {

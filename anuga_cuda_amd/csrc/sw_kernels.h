@@ -1228,6 +1228,53 @@ __global__ __launch_bounds__(kBlock) void k_to_host(long long n, long long strid
     dst[t] = (comp < 0) ? src[pos] : src[4 * pos + comp];
 }
 
+// SWW output points.  Caller vertex 3k+j of an instance -> its device slot
+// j*N + off + dev_of[k] in the [3][N] vertex arrays.
+__global__ __launch_bounds__(kBlock) void k_sww_slots(int nv, long long N, long long off, const int *dev_of,
+                                                      int *slot) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nv) return;
+    const int v = slot[t];
+    slot[t] = (int)((v % 3) * N + off + dev_of[v / 3]);
+}
+
+// FP32 values of the SWW points out[4][P] (elevation, stage, xmomentum,
+// ymomentum).  Point p is the mean of the vertex values in slot[start[p] ..
+// start[p+1]), summed in that order and divided by the count
+// (ANUGA quantity_ext.c _average_vertex_values, the smoothing of
+// Quantity.get_vertex_values), then rounded to FP32; a unique-vertex point
+// lists one slot.  Where stage - elevation < minimum_storable_height (an
+// FP32 test, as the writer's) stage is stored as the elevation and the
+// momenta as 0.  A point with no triangles gets zeros.
+__global__ __launch_bounds__(kBlock) void k_sww_points(long long P, const int *start, const int *slot,
+                                                       const double *zv, const double *wv, const double *uhv,
+                                                       const double *vhv, float min_storable, float *out) {
+    const long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    const int a = start[p], b = start[p + 1];
+    double sz = 0.0, sw = 0.0, su = 0.0, sv = 0.0;
+    for (int i = a; i < b; i++) {
+        const int s = slot[i];
+        sz += zv[s];
+        sw += wv[s];
+        su += uhv[s];
+        sv += vhv[s];
+    }
+    float z = 0.f, w = 0.f, u = 0.f, v = 0.f;
+    if (b > a) {
+        const double k = (double)(b - a);
+        z = (float)(sz / k);
+        w = (float)(sw / k);
+        u = (float)(su / k);
+        v = (float)(sv / k);
+    }
+    const bool storable = (w - z) >= min_storable;
+    out[p] = z;
+    out[P + p] = storable ? w : z;
+    out[2 * P + p] = storable ? u : 0.f;
+    out[3 * P + p] = storable ? v : 0.f;
+}
+
 // Mean bed and the check that the bed edge values are the interpolation of the
 // bed vertex values, over device triangles [off, off + n) of one instance.
 __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, const double *zv, const double *ze,

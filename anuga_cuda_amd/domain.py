@@ -19,6 +19,8 @@ reference, so these values are stated here and every one can be overridden.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from . import _lib
@@ -295,6 +297,13 @@ class Domain:
         self.recorded_min_timestep = self._params["evolve_max_timestep"]
         self._order_ = self._params["default_order"]
         self.smallsteps = 0
+        # SWW output (ANUGA's names; nothing is written unless store is set)
+        self.name = "domain"
+        self.datadir = "."
+        self.store = False
+        self.smooth = True
+        self.minimum_storable_height = 1.0e-5
+        self.writer = None
         self.max_speed = np.zeros(self.number_of_elements)
 
     # ------------------------------------------------------------------ params
@@ -374,13 +383,47 @@ class Domain:
         return None
 
     def set_name(self, name):
-        self.name = name
+        self.name = name[:-4] if name.endswith(".sww") else name
+
+    def get_name(self):
+        return self.name
 
     def set_datadir(self, name):
         self.datadir = name
 
+    def get_datadir(self):
+        return self.datadir
+
     def set_store(self, flag=True):
-        self.store = flag
+        """Write the SWW file (datadir/name.sww) at every yield of evolve."""
+        self.store = bool(flag)
+
+    def set_store_vertices_uniquely(self, flag=True, reduction=None):
+        """Store 3 values per triangle instead of one smoothed value per node."""
+        self.smooth = not flag
+
+    def set_minimum_storable_height(self, minimum_storable_height):
+        self.minimum_storable_height = float(minimum_storable_height)
+
+    def get_minimum_storable_height(self):
+        return self.minimum_storable_height
+
+    def initialise_storage(self):
+        """Create the SWW file with the mesh and the elevation."""
+        from .sww import SWWWriter, sww_path
+
+        if self.writer is not None:
+            self.writer.close()
+        if self.datadir and not os.path.isdir(self.datadir):
+            os.makedirs(self.datadir, exist_ok=True)
+        self.writer = SWWWriter(self, self._dev(), sww_path(self.datadir, self.name), smooth=self.smooth,
+                                minimum_storable_height=self.minimum_storable_height)
+
+    def store_timestep(self):
+        """Append the current state to the SWW file (gpu_domain_advanced.py:2167-2172)."""
+        if self.writer is None:
+            self.initialise_storage()
+        self.writer.store_timestep(self._time)
 
     # ------------------------------------------------------------ quantities
     def set_quantity(self, name, numeric=None, function=None, expression=None,
@@ -621,7 +664,11 @@ class Domain:
                int(finaltime is not None))
         self._host_stale = True
         self._sync_scalars()
+        if self.store:
+            self.initialise_storage()
         if not skip_initial_step:
+            if self.store:
+                self.store_timestep()
             yield self._time
         while True:
             h = self._dev()
@@ -632,6 +679,8 @@ class Domain:
                 while event == _lib.SW_EVENT_NONE:
                     event = h.evolve_one_step_unfused()
             self._sync_scalars()
+            if self.store:
+                self.store_timestep()
             if event == _lib.SW_EVENT_FINAL:
                 yield self._time
                 break
@@ -653,6 +702,9 @@ class Domain:
         return float(np.sum(h * self.mesh.areas))
 
     def close(self):
+        if self.writer is not None:
+            self.writer.close()
+            self.writer = None
         if self._handle is not None:
             self._pull()
             self._handle.close()

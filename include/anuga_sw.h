@@ -154,6 +154,30 @@ int sw_get_state(sw_handle *h, sw_state *state);
 int sw_set_state(sw_handle *h, const sw_state *state);
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count);
 
+/* SWW output at yields.  The reference's store_timestep
+ * (gpu_domain_advanced.py:2167-2172) hands over to ANUGA's SWW writer after
+ * copy_back_necessary_data, which copies nothing (:734-737); here the device
+ * reduces the state to the FP32 point values the writer stores, and only
+ * those cross PCIe.
+ *
+ * Output points of the selected instance: with number_of_points > 0, mesh
+ * nodes (ANUGA's smoothed output): node p averages the vertex values whose
+ * flat caller indices 3k+j are listed, in this order, in
+ * vertex_value_indices[start_p .. start_p + number_of_triangles_per_node[p])
+ * (start_p the running sum of the counts; ANUGA's
+ * build_inverted_triangle_structure arrays, the indices a permutation of
+ * 0 .. 3N-1).  With number_of_points = 0: unique vertices, 3N points, point
+ * 3k+j = vertex j of triangle k (both arrays may be NULL). */
+int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *number_of_triangles_per_node,
+                      const int64_t *vertex_value_indices);
+/* FP32 point values out[4][P] (count = 4 P) of elevation, stage, xmomentum,
+ * ymomentum: per point the mean of its vertex values (summed in the listed
+ * order, divided by the count, rounded to FP32; ANUGA's
+ * Quantity.get_vertex_values(smooth=True)); where stage - elevation <
+ * minimum_storable_height (FP32 test) stage is stored as the elevation and
+ * the momenta as 0 (ANUGA's sww writer). */
+int sw_sww_values(sw_handle *h, double minimum_storable_height, float *out, int64_t count);
+
 /* Step operators, one per overridden ANUGA method. */
 /* compute_fluxes_central_structure (gpu_domain_advanced.py:1612-1646;
  * src/anuga_HMPP/shallow_water_ext.c:4519-4696).  Writes explicit updates and

@@ -125,6 +125,44 @@ class OracleError(RuntimeError):
         self.code = code
 
 
+def average_vertex_values(vertex_value_indices, number_of_triangles_per_node, vertex_values):
+    """ANUGA's quantity_ext.c ``_average_vertex_values`` (the smoothing of
+    Quantity.get_vertex_values(smooth=True)), restated as a plain loop: walk
+    the indices in order, summing the flat vertex values of the current node,
+    and store total / count when the node's count is reached.  (The anuga
+    package is not part of the reference.)  A node with no triangles gets 0."""
+    vv = np.asarray(vertex_values, dtype=np.float64).ravel()
+    counts = [int(c) for c in number_of_triangles_per_node]
+    out = np.zeros(len(counts))
+    node = 0
+    while node < len(counts) and counts[node] == 0:
+        node += 1
+    k = 0
+    total = 0.0
+    for index in vertex_value_indices:
+        k += 1
+        total += float(vv[index])
+        if k == counts[node]:
+            out[node] = total / k
+            total = 0.0
+            k = 0
+            node += 1
+            while node < len(counts) and counts[node] == 0:
+                node += 1
+    return out
+
+
+def sww_point_values(elevation, stage, xmom, ymom, minimum_storable_height):
+    """The SWW writer's stored values from FP64 point values: rounded to FP32;
+    where stage - elevation < minimum_storable_height (FP32 arithmetic, as the
+    writer's numpy test on FP32 arrays) stage becomes the elevation and the
+    momenta 0.  Returns float32 [4][P]."""
+    z, w, u, v = (np.asarray(a, dtype=np.float64).astype(np.float32) for a in (elevation, stage, xmom, ymom))
+    storable = (w - z) >= np.float32(minimum_storable_height)
+    return np.stack([z, np.where(storable, w, z), np.where(storable, u, np.float32(0)),
+                     np.where(storable, v, np.float32(0))]).astype(np.float32)
+
+
 def flux_function_central(ql, qr, zl, zr, n1, n2, epsilon=1e-12, H0=1e-3, g=9.8):
     """One edge flux, as _flux_function_central (h0 = H0^2, threshold 10 H0)."""
     lib = load()
