@@ -123,8 +123,41 @@ _SIGNATURES = {
     "sw_synchronize": ([C.c_void_p], C.c_int),
     "sw_autotune": ([C.c_void_p, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "sw_enable_timing": ([C.c_void_p, C.c_int32], C.c_int),
+    "sw_build_mesh": ([C.c_int, C.c_int64, _F64P, C.c_int64, _I64P, C.c_void_p], C.c_int),
+    "sw_mesh_last_error": ([], C.c_char_p),
     "sw_kernel_times": ([C.c_void_p, _F64P, _I64P, C.c_int32], C.c_int),
 }
+
+
+class SwMeshGeometry(C.Structure):
+    _fields_ = [("triangles", _I64P), ("vertex_coordinates", _F64P), ("centroid_coordinates", _F64P),
+                ("areas", _F64P), ("normals", _F64P), ("edgelengths", _F64P),
+                ("edge_midpoint_coordinates", _F64P), ("radii", _F64P), ("neighbours", _I64P),
+                ("neighbour_edges", _I64P)]
+
+
+def build_mesh_geometry(points, triangles, device=0):
+    """sw_build_mesh: the oriented triangles, derived geometry and neighbours
+    of a mesh, built on GPU ``device``.  Returns a dict of host arrays named
+    as the Mesh attributes."""
+    lib = load()
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    tris = np.ascontiguousarray(triangles, dtype=np.int64)
+    N = len(tris)
+    out = {"triangles": np.empty((N, 3), dtype=np.int64), "vertex_coordinates": np.empty((N, 6)),
+           "centroid_coordinates": np.empty((N, 2)), "areas": np.empty(N), "normals": np.empty((N, 6)),
+           "edgelengths": np.empty((N, 3)), "edge_midpoint_coordinates": np.empty((N, 6)),
+           "radii": np.empty(N), "neighbours": np.empty((N, 3), dtype=np.int64),
+           "neighbour_edges": np.empty((N, 3), dtype=np.int64)}
+    g = SwMeshGeometry()
+    for name, _ in SwMeshGeometry._fields_:
+        a = out[name]
+        setattr(g, name, i64p(a) if a.dtype == np.int64 else f64p(a))
+    rc = lib.sw_build_mesh(int(device), len(pts), f64p(pts), N, i64p(tris), C.byref(g))
+    if rc != 0:
+        msg = (lib.sw_mesh_last_error() or b"").decode()
+        raise (ValueError if rc == -10 else SwError)(msg or "sw_build_mesh failed (%d)" % rc)
+    return out
 
 
 def exported_symbols():

@@ -1,5 +1,6 @@
 #!/bin/bash
-# Builds libanuga_sw.so (the shallow-water step, from sw_capi.hip) and
+# Builds libanuga_sw.so (the shallow-water step, from sw_capi.hip, and the
+# device mesh builder, from mesh_build.hip) and
 # libanuga_gnn.so (the learned edge-flux surrogate, from gnn_kernels.hip) for
 # gfx950, in-tree.
 # -ffp-contract=off: no FMA contraction, so FP64 results follow the C oracle's
@@ -13,7 +14,7 @@ ARCH="${SW_OFFLOAD_ARCH:-gfx950}"
     -ffp-contract=off -fno-fast-math \
     -Xclang -target-feature -Xclang -packed-fp32-ops \
     -Wall -Wno-unused-function \
-    -o "${OUT}.tmp" "${HERE}/sw_capi.hip"
+    -o "${OUT}.tmp" "${HERE}/sw_capi.hip" "${HERE}/mesh_build.hip"
 mv -f "${OUT}.tmp" "${OUT}"
 echo "built ${OUT}"
 GOUT="${HERE}/../libanuga_gnn.so"

@@ -39,6 +39,8 @@ __all__ = [
     "Mesh",
     "rectangular_cross",
     "read_tsh",
+    "read_msh",
+    "read_mesh",
     "delaunay_random",
     "morton_order",
 ]
@@ -101,7 +103,7 @@ def rectangular_cross(m, n, len1=1.0, len2=1.0, origin=(0.0, 0.0)):
     return points, tris, boundary
 
 
-def read_tsh(path):
+def read_tsh(path, full=False):
     """Read ANUGA's ``.tsh`` text mesh format (optionally gzip-compressed).
 
     Sections (see the header lines of
@@ -111,8 +113,9 @@ def read_tsh(path):
     ``<id> <va> <vb> [tag]``, then holes / regions / geo reference.
 
     Returns ``(points, triangles, segments)`` where ``segments`` maps the
-    unordered vertex pair of each tagged segment to its tag.  Neighbour columns
-    are ignored (neighbours are rebuilt from the triangles).
+    unordered vertex pair of each tagged segment to its tag; with ``full``
+    also a dict with the file's ``triangle_neighbors`` (the Mesh rebuilds
+    neighbours from the triangles; the file's columns serve as a check).
     """
     opener = gzip.open if str(path).endswith(".gz") else open
     with opener(path, "rt") as fh:
@@ -140,6 +143,7 @@ def read_tsh(path):
     tri_rows = [lines[pos + t].split() for t in range(ntri)]
     pos += ntri
     triangles = np.array([[int(r[1]), int(r[2]), int(r[3])] for r in tri_rows], dtype=np.int64)
+    neighbors = np.array([[int(r[4]), int(r[5]), int(r[6])] for r in tri_rows], dtype=np.int64)
     fields = header()
     nseg = int(fields[0])
     segments = {}
@@ -148,7 +152,60 @@ def read_tsh(path):
         tag = r[3] if len(r) > 3 else ""
         a, b = int(r[1]), int(r[2])
         segments[(min(a, b), max(a, b))] = tag
+    if full:
+        return points, triangles, segments, {"triangle_neighbors": neighbors, "georeference": {}}
     return points, triangles, segments
+
+
+def read_msh(path, full=False):
+    """Read ANUGA's NetCDF ``.msh`` mesh format (optionally gzip-compressed),
+    as the reference's Cairns scenario ships it (``tests/CUDA/cairns/cairns.msh``):
+    variables ``vertices`` [P][2], ``triangles`` [N][3], ``triangle_neighbors``
+    [N][3], ``segments`` [S][2] with ``segment_tags`` [S][chars] (blank-padded),
+    and the georeference as global attributes.
+
+    Returns ``(points, triangles, segments)`` as ``read_tsh``; with ``full``
+    also a dict with ``triangle_neighbors`` and ``georeference``.  Vertex
+    coordinates are relative to the georeference's ``xllcorner`` /
+    ``yllcorner``, as stored."""
+    from scipy.io import netcdf_file
+
+    if str(path).endswith(".gz"):
+        with gzip.open(path, "rb") as fh:
+            src = io.BytesIO(fh.read())
+    else:
+        src = str(path)
+    f = netcdf_file(src, "r", mmap=False)
+    try:
+        v = f.variables
+        points = np.array(v["vertices"].data, dtype=np.float64)
+        triangles = np.array(v["triangles"].data, dtype=np.int64)
+        segments = {}
+        if "segments" in v:
+            segs = np.array(v["segments"].data, dtype=np.int64)
+            tags = v["segment_tags"].data if "segment_tags" in v else None
+            for i, (a, b) in enumerate(segs.tolist()):
+                tag = b"".join(tags[i]).decode("utf-8", "replace").strip("\0 ") if tags is not None else ""
+                segments[(min(a, b), max(a, b))] = tag
+        extras = {"triangle_neighbors": np.array(v["triangle_neighbors"].data, dtype=np.int64)
+                  if "triangle_neighbors" in v else None,
+                  "georeference": {k: (a.decode() if isinstance(a, bytes) else a.item() if hasattr(a, "item") else a)
+                                   for k, a in f._attributes.items()
+                                   if k in ("xllcorner", "yllcorner", "zone", "false_easting", "false_northing",
+                                            "datum", "projection", "units")}}
+    finally:
+        f.close()
+    return (points, triangles, segments, extras) if full else (points, triangles, segments)
+
+
+def read_mesh(path, full=False):
+    """``read_tsh`` or ``read_msh`` by the file name (``.tsh`` / ``.msh``, optionally ``.gz``)."""
+    name = str(path)[:-3] if str(path).endswith(".gz") else str(path)
+    if name.endswith(".msh"):
+        return read_msh(path, full=full)
+    if name.endswith(".tsh"):
+        return read_tsh(path, full=full)
+    raise ValueError("unknown mesh format: %s" % path)
 
 
 def delaunay_random(npoints, extent=(10000.0, 10000.0), seed=12345):
@@ -229,38 +286,57 @@ class Mesh:
     ``boundary`` maps ``(vol_id, edge_id)`` to a tag for tagged boundary edges;
     untagged boundary edges get the tag ``'exterior'`` (ANUGA's default).
     ``segments`` (as returned by ``read_tsh``) can tag boundary edges by their
-    vertex pair instead.
+    vertex pair instead.  ``build="device"`` computes the oriented triangles,
+    geometry and neighbours on GPU ``device`` (sw_build_mesh; the same bits as
+    the host build); the boundary tagging stays on the host.
     """
 
-    def __init__(self, points, triangles, boundary=None, segments=None):
+    def __init__(self, points, triangles, boundary=None, segments=None, build="host", device=0):
         points = np.ascontiguousarray(points, dtype=np.float64)
         tris = np.ascontiguousarray(triangles, dtype=np.int64).copy()
         if tris.ndim != 2 or tris.shape[1] != 3:
             raise ValueError("triangles must be an (N, 3) index array")
-        if tris.min() < 0 or tris.max() >= len(points):
-            raise ValueError("triangle vertex index out of range")
-        # Orientation: make every triangle counter-clockwise.  Swapping
-        # v1<->v2 keeps v0 and relabels edges 1<->2; boundary keys follow.
-        p = points[tris]
-        cross = (p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) - (
-            p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1])
-        if np.any(cross == 0.0):
-            raise ValueError("degenerate triangle (zero area)")
-        flip = cross < 0
-        if flip.any():
-            tris[flip, 1], tris[flip, 2] = tris[flip, 2].copy(), tris[flip, 1].copy()
-            if boundary:
-                swapped = {}
-                for (k, e), tag in boundary.items():
-                    if flip[k] and e in (1, 2):
-                        e = 3 - e
-                    swapped[(k, e)] = tag
-                boundary = swapped
+        if points.ndim != 2 or points.shape[1] != 2:
+            raise ValueError("points must be an (P, 2) array")
+        if build not in ("host", "device"):
+            raise ValueError("build must be 'host' or 'device'")
+        self._build = build
+        self._device = device
         self.points = points
-        self.triangles = tris
-        self.number_of_elements = N = len(tris)
-        self._build_geometry()
-        self._build_neighbours()
+        self.number_of_elements = len(tris)
+        if build == "device":
+            from . import _lib
+
+            g = _lib.build_mesh_geometry(points, tris, device)
+            flip = g["triangles"][:, 1] != tris[:, 1]
+            for name, a in g.items():
+                setattr(self, name, a)
+            nb, ne = g["neighbours"].ravel(), g["neighbour_edges"].ravel()
+            self._edge_partner = np.where(nb >= 0, 3 * nb + ne, -1)
+        else:
+            if tris.min() < 0 or tris.max() >= len(points):
+                raise ValueError("triangle vertex index out of range")
+            # Orientation: make every triangle counter-clockwise.  Swapping
+            # v1<->v2 keeps v0 and relabels edges 1<->2.
+            p = points[tris]
+            cross = (p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) - (
+                p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1])
+            if np.any(cross == 0.0):
+                raise ValueError("degenerate triangle (zero area)")
+            flip = cross < 0
+            if flip.any():
+                tris[flip, 1], tris[flip, 2] = tris[flip, 2].copy(), tris[flip, 1].copy()
+            self.triangles = tris
+            self._build_geometry()
+            self._build_neighbours()
+        if boundary and flip.any():
+            # boundary keys follow the relabelled edges
+            swapped = {}
+            for (k, e), tag in boundary.items():
+                if flip[k] and e in (1, 2):
+                    e = 3 - e
+                swapped[(k, e)] = tag
+            boundary = swapped
         self._build_boundary(boundary or {}, segments or {})
 
     # ------------------------------------------------------------------
@@ -377,7 +453,7 @@ class Mesh:
         inv = np.empty_like(perm)
         inv[perm] = np.arange(len(perm))
         boundary = {(int(inv[k]), e): tag for (k, e), tag in self.boundary.items()}
-        return Mesh(self.points, self.triangles[perm], boundary=boundary)
+        return Mesh(self.points, self.triangles[perm], boundary=boundary, build=self._build, device=self._device)
 
     def morton_renumbered(self):
         c = self.centroid_coordinates
@@ -386,3 +462,40 @@ class Mesh:
     def random_renumbered(self, seed=0):
         rng = np.random.default_rng(seed)
         return self.renumber(rng.permutation(self.number_of_elements))
+
+
+def main(argv=None):
+    """Times the host and the device build of the seeded Delaunay meshes'
+    derived arrays (geometry, neighbours, boundary) and checks that both give
+    the same arrays.  Prints one JSON line per mesh."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--points", type=int, nargs="+", default=[500000, 2000000])
+    ap.add_argument("--repeats", type=int, default=3)
+    args = ap.parse_args(argv)
+    for npts in args.points:
+        pts, tris = delaunay_random(npts, extent=(10000.0, 10000.0), seed=12345)
+        Mesh(*delaunay_random(100, seed=1), build="device")  # device and runtime warm-up
+        t = {}
+        meshes = {}
+        for build in ("host", "device"):
+            best = 1e30
+            for _ in range(args.repeats):
+                t0 = time.perf_counter()
+                meshes[build] = Mesh(pts, tris, build=build)
+                best = min(best, time.perf_counter() - t0)
+            t[build] = best
+        h, d = meshes["host"], meshes["device"]
+        same = all(np.array_equal(getattr(h, n), getattr(d, n)) for n in (
+            "triangles", "vertex_coordinates", "centroid_coordinates", "areas", "normals", "edgelengths",
+            "edge_midpoint_coordinates", "radii", "neighbours", "neighbour_edges", "surrogate_neighbours",
+            "boundary_cells", "boundary_edges"))
+        print(json.dumps({"points": npts, "triangles": h.number_of_elements, "host_s": t["host"],
+                          "device_s": t["device"], "speedup": t["host"] / t["device"], "identical": same}))
+
+
+if __name__ == "__main__":
+    main()

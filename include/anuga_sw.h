@@ -119,6 +119,37 @@ sw_handle *sw_create_ensemble(const sw_mesh *meshes, int32_t n_instances, const 
 int sw_select_instance(sw_handle *h, int32_t instance);
 int sw_destroy(sw_handle *h);
 const char *sw_last_error(const sw_handle *h);
+
+/* Mesh ingestion on the device.  The reference receives these arrays from
+ * ANUGA's host-side General_mesh / Mesh and uploads them
+ * (gpu_domain_advanced.py:582-733; layout sw_domain.h:60-180); here one
+ * call builds them on GPU `device` from the points and triangles and returns
+ * them in host arrays the caller allocates (row-major):
+ *   triangles[N][3]     the input, v1 and v2 swapped where it is clockwise;
+ *   vertex_coordinates[N][6], centroid_coordinates[N][2], areas[N];
+ *   normals[N][6]       outward unit normal of edge i (opposite vertex i);
+ *   edgelengths[N][3], edge_midpoint_coordinates[N][6];
+ *   radii[N]            smallest centroid-to-edge-midpoint distance;
+ *   neighbours[N][3], neighbour_edges[N][3]: the triangle across edge i and
+ *                       that triangle's edge index, -1 on boundary edges.
+ * Returns SW_ERR_ARG (message via sw_mesh_last_error) for a vertex index out
+ * of range, a degenerate triangle, a zero-length edge or an edge shared by
+ * more than two triangles. */
+typedef struct {
+    int64_t *triangles;
+    double *vertex_coordinates;
+    double *centroid_coordinates;
+    double *areas;
+    double *normals;
+    double *edgelengths;
+    double *edge_midpoint_coordinates;
+    double *radii;
+    int64_t *neighbours;
+    int64_t *neighbour_edges;
+} sw_mesh_geometry;
+int sw_build_mesh(int device, int64_t number_of_points, const double *points, int64_t number_of_elements,
+                  const int64_t *triangles, sw_mesh_geometry *out);
+const char *sw_mesh_last_error(void);
 int sw_set_params(sw_handle *h, const sw_params *params);
 int sw_get_params(const sw_handle *h, sw_params *params);
 

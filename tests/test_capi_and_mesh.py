@@ -131,3 +131,31 @@ def test_one_hip_runtime_per_process():
             "print(len(maps))\n" % ROOT)
     out = subprocess.run(["python", "-c", code], capture_output=True, text=True, check=True)
     assert out.stdout.strip().splitlines()[-1] == "1", out.stdout + out.stderr
+
+
+def test_neighbours_match_the_reference_mesh_files():
+    """The host neighbour build against the neighbour columns the reference's
+    own mesh files hold: tests/CUDA/cairns/cairns.msh (NetCDF) and
+    tests/CUDA/merimbula/merimbula_43200.tsh (text)."""
+    from anuga_cuda_amd.mesh import read_mesh
+
+    for name, ntri in (("cairns.msh.gz", 17831), ("merimbula_43200.tsh.gz", 43200)):
+        p, t, segs, extra = read_mesh(os.path.join(ROOT, "tests", "golden", name), full=True)
+        assert len(t) == ntri
+        m = Mesh(p, t, segments=segs)
+        assert np.array_equal(m.triangles, t)  # both files are counter-clockwise
+        nb = np.where(m.neighbours < 0, -1, m.neighbours)
+        assert np.array_equal(nb, extra["triangle_neighbors"]), name
+
+
+def test_read_msh_cairns_fixture():
+    from anuga_cuda_amd.mesh import read_msh
+
+    p, t, segs, extra = read_msh(os.path.join(ROOT, "tests", "golden", "cairns.msh.gz"), full=True)
+    assert p.shape == (8974, 2) and t.shape == (17831, 3) and len(segs) == 790
+    geo = extra["georeference"]
+    assert geo["zone"] == -1 and geo["false_easting"] == 500000 and geo["datum"] == "wgs84"
+    assert abs(geo["xllcorner"] - 301278.51) < 1e-9
+    m = Mesh(p, t, segments=segs)
+    tags, counts = np.unique(m.boundary_tags, return_counts=True)
+    assert dict(zip(tags.tolist(), counts.tolist())) == {"bottom": 25, "ocean_east": 35, "onshore": 20, "top": 35}
