@@ -31,7 +31,7 @@ COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
 def short_name(name):
-    n = name.split("(")[0]
+    n = name.replace("(anonymous namespace)::", "").split("(")[0]
     if n.startswith("void "):
         n = n[5:]
     return n.replace("sw::", "")
