@@ -19,6 +19,7 @@ SW_CENTROID, SW_EDGE, SW_VERTEX, SW_BOUNDARY, SW_EXPLICIT_UPDATE, SW_SEMI_IMPLIC
 SW_BC_REFLECTIVE, SW_BC_DIRICHLET, SW_BC_TRANSMISSIVE = range(3)
 SW_PRECISION_FP64, SW_PRECISION_MIXED = 0, 1
 SW_ORDER_KEEP, SW_ORDER_HILBERT = 0, 1
+SW_LIMITER_OPTIMISED, SW_LIMITER_VERTEX, SW_LIMITER_EDGE = 0, 1, 2
 SW_EVENT_NONE, SW_EVENT_YIELD, SW_EVENT_FINAL = 0, 1, 2
 
 SW_ERR_UPDATE = -1
@@ -34,7 +35,8 @@ class SwParams(C.Structure):
         (n, C.c_int32) for n in (
             "optimise_dry_cells", "extrapolate_velocity_second_order",
             "tight_slope_limiters", "use_centroid_velocities", "use_sloped_mannings",
-            "max_smallsteps", "default_order", "precision", "locality_order", "timestepping_method")]
+            "max_smallsteps", "default_order", "precision", "locality_order", "timestepping_method",
+            "limiter")] + [("quantity_beta", C.c_double * 3)]
 
 
 _I64P = C.POINTER(C.c_int64)
@@ -117,6 +119,8 @@ _SIGNATURES = {
     "sw_advance_yield": ([C.c_void_p], C.c_int),
     "sw_evolve_one_step_unfused": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
     "sw_backup_conserved_quantities": ([C.c_void_p], C.c_int),
+    "sw_extrapolate_second_order_and_limit_by_vertex": ([C.c_void_p], C.c_int),
+    "sw_extrapolate_second_order_and_limit_by_edge": ([C.c_void_p], C.c_int),
     "sw_set_sww_points": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
     "sw_sww_values": ([C.c_void_p, C.c_double, C.c_void_p, C.c_int64], C.c_int),
     "sw_saxpy_conserved_quantities": ([C.c_void_p, C.c_double, C.c_double], C.c_int),

@@ -149,6 +149,8 @@ DevParams to_dev(const sw_params &p) {
     d.use_sloped_mannings = p.use_sloped_mannings;
     d.max_smallsteps = p.max_smallsteps;
     d.default_order = p.default_order;
+    d.limiter = p.limiter;
+    for (int i = 0; i < 3; i++) d.qbeta[i] = p.quantity_beta[i];
     return d;
 }
 
@@ -158,6 +160,8 @@ int check_params(sw_handle *h, const sw_params *p) {
         return fail(h, SW_ERR_ARG, "unknown precision %d", p->precision);
     if (p->locality_order != SW_ORDER_KEEP && p->locality_order != SW_ORDER_HILBERT)
         return fail(h, SW_ERR_ARG, "unknown locality order %d", p->locality_order);
+    if (p->limiter < SW_LIMITER_OPTIMISED || p->limiter > SW_LIMITER_EDGE)
+        return fail(h, SW_ERR_ARG, "unknown limiter %d", p->limiter);
     if (p->timestepping_method < 0 || p->timestepping_method > 3)
         return fail(h, SW_ERR_ARG, "unknown timestepping method %d (1 euler, 2 rk2, 3 rk3)", p->timestepping_method);
     if (p->default_order != 1 && p->default_order != 2)
@@ -1166,6 +1170,23 @@ int sw_extrapolate_second_order_sw(sw_handle *h) {
     return sync_state(h, 0, &d);
 }
 
+static int extrapolate_limited(sw_handle *h, int limiter, const char *what) {
+    if (!h) return SW_ERR_ARG;
+    if (int rc = single(h, what)) return rc;
+    DevParams dp = h->dp;
+    dp.limiter = limiter;
+    hipLaunchKernelGGL(k_extrapolate_limited, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, dp);
+    return launch_check(h, what);
+}
+
+int sw_extrapolate_second_order_and_limit_by_vertex(sw_handle *h) {
+    return extrapolate_limited(h, SW_LIMITER_VERTEX, "extrapolate_second_order_and_limit_by_vertex");
+}
+
+int sw_extrapolate_second_order_and_limit_by_edge(sw_handle *h) {
+    return extrapolate_limited(h, SW_LIMITER_EDGE, "extrapolate_second_order_and_limit_by_edge");
+}
+
 int sw_extrapolate_first_order(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
     if (int rc = single(h, "extrapolate_first_order")) return rc;
@@ -1195,6 +1216,10 @@ int sw_distribute_to_vertices_and_edges(sw_handle *h) {
     if (int rc = sw_protect(h)) return rc;
     if (d.order == 1) {
         if (int rc = sw_extrapolate_first_order(h)) return rc;
+    } else if (h->params.limiter == SW_LIMITER_VERTEX) {
+        if (int rc = sw_extrapolate_second_order_and_limit_by_vertex(h)) return rc;
+    } else if (h->params.limiter == SW_LIMITER_EDGE) {
+        if (int rc = sw_extrapolate_second_order_and_limit_by_edge(h)) return rc;
     } else {
         if (int rc = sw_extrapolate_second_order_sw(h)) return rc;
     }

@@ -51,6 +51,8 @@ struct DevParams {
     double alpha_balance, CFL, evolve_max_timestep, evolve_min_timestep;
     int optimise_dry_cells, extrapolate_velocity_second_order, tight_slope_limiters;
     int use_centroid_velocities, use_sloped_mannings, max_smallsteps, default_order;
+    int limiter;      // SW_LIMITER_*: distribute path
+    double qbeta[3];  // Quantity.beta of stage, xmomentum, ymomentum (per-quantity limiters)
 };
 
 // Static geometry of edge i of triangle k, one 32-byte record at i*N + k.
@@ -829,6 +831,92 @@ SW_DEV void load_rec(const DevMesh &M, const DevFields &F, int k, bool second_or
     }
 }
 
+// Second-order reconstruction of one conserved quantity of triangle k with the
+// per-quantity limiters (extrapolate_second_order_and_limit_by_vertex /
+// _by_edge, extrapolate_second_order_and_limit_by_vertex_or_edge.cu:694-800):
+// the gradient from the surrogate neighbours' centroids (compute_gradients,
+// :226-316; a boundary edge's surrogate is k itself; two boundary edges: the
+// two-point gradient towards the one neighbour; three: none), the vertex
+// values from the gradient (extrapolate_from_gradient, :172-224), then phi =
+// min over the vertices (by_edge: the edge midpoints) of min(r beta, 1) against
+// the range of qc and the neighbours' centroid values (:5-75, :77-170); by
+// edge, the vertices are rebuilt as e_{i+1} + e_{i+2} - e_i.  qs are the
+// surrogates' centroid values; the range over them equals the range over the
+// true neighbours, a boundary surrogate being k (whose qc is in the range).
+SW_DEV void reconstruct_limited(const RecIn &r, int k, int nob, double qc, const double qs[3], double beta,
+                                bool by_edge, double qv[3]) {
+    double a = 0.0, b = 0.0;
+    if (nob < 2) {
+        if (!(r.sk[0] == r.sk[1] || r.sk[1] == r.sk[2])) {
+            const double x0 = r.xs[0].x, y0 = r.xs[0].y, x1 = r.xs[1].x, y1 = r.xs[1].y;
+            const double x2 = r.xs[2].x, y2 = r.xs[2].y;
+            const double q0 = qs[0], q1 = qs[1], q2 = qs[2];
+            const double det = (y2 - y0) * (x1 - x0) - (y1 - y0) * (x2 - x0);
+            a = ((y2 - y0) * (q1 - q0) - (y1 - y0) * (q2 - q0)) / det;
+            b = ((x1 - x0) * (q2 - q0) - (x2 - x0) * (q1 - q0)) / det;
+        }
+    } else if (nob == 2) {
+        const int i = (r.sk[0] != k) ? 0 : (r.sk[1] != k) ? 1 : 2;
+        if (r.sk[i] != k) {
+            const double x0 = r.xs[i].x, y0 = r.xs[i].y, x1 = r.xy.x, y1 = r.xy.y;
+            const double q0 = qs[i], q1 = qc;
+            const double det = (x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0);
+            a = (x1 - x0) * (q1 - q0) / det;
+            b = (y1 - y0) * (q1 - q0) / det;
+        }
+    }
+    const double x = r.xy.x, y = r.xy.y;
+    double v[3], e[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) v[i] = qc + a * (r.v[i].x - x) + b * (r.v[i].y - y);
+    e[0] = 0.5 * (v[1] + v[2]);
+    e[1] = 0.5 * (v[2] + v[0]);
+    e[2] = 0.5 * (v[0] + v[1]);
+    double qmin = qc, qmax = qc;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        qmin = omin<double>(qmin, qs[i]);
+        qmax = omax<double>(qmax, qs[i]);
+    }
+    double dqa[3], phi = 1.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double rr = 1.0;
+        const double dq = (by_edge ? e[i] : v[i]) - qc;
+        dqa[i] = dq;
+        if (dq > 0.0) rr = (qmax - qc) / dq;
+        if (dq < 0.0) rr = (qmin - qc) / dq;
+        phi = omin<double>(omin<double>(rr * beta, 1.0), phi);
+    }
+    if (by_edge) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) e[i] = qc + phi * dqa[i];
+        qv[0] = e[1] + e[2] - e[0];
+        qv[1] = e[2] + e[0] - e[1];
+        qv[2] = e[0] + e[1] - e[2];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) qv[i] = qc + phi * dqa[i];
+    }
+}
+
+// The three conserved quantities with the per-quantity limiters.
+SW_DEV void reconstruct_limited_all(const DevParams &P, int k, const RecIn &r, double wv[3], double uv[3],
+                                    double vv[3]) {
+    const int nob = nob_of(r.flags);
+    const bool by_edge = P.limiter == 2;
+    double qs[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) qs[i] = r.cs[i].w;
+    reconstruct_limited(r, k, nob, r.c.w, qs, P.qbeta[0], by_edge, wv);
+#pragma unroll
+    for (int i = 0; i < 3; i++) qs[i] = r.cs[i].uh;
+    reconstruct_limited(r, k, nob, r.c.uh, qs, P.qbeta[1], by_edge, uv);
+#pragma unroll
+    for (int i = 0; i < 3; i++) qs[i] = r.cs[i].vh;
+    reconstruct_limited(r, k, nob, r.c.vh, qs, P.qbeta[2], by_edge, vv);
+}
+
 // extrapolate_second_order_sw for triangle k (shallow_water_ext.c:2717-3324),
 // result in wv/uv/vv (momentum).  Returns false on the missing-neighbour error.
 template <typename R>
@@ -969,6 +1057,33 @@ __global__ __launch_bounds__(kBlock) void k_extrapolate_second_order(DevMesh M, 
     }
 }
 
+// extrapolate_second_order_and_limit_by_vertex / _by_edge of the three
+// conserved quantities (vertex and edge values; gpu_domain_advanced.py:1836-1859, 1941-1962)
+__global__ __launch_bounds__(kBlock) void k_extrapolate_limited(DevMesh M, DevFields F, DevParams P) {
+    const int N = M.N;
+    const int k = tri_index();
+    if (k >= N) return;
+    RecIn r;
+    load_rec(M, F, k, true, r);
+    double wv[3], uv[3], vv[3];
+    reconstruct_limited_all(P, k, r, wv, uv, vv);
+    double ew[3], eu[3], ev[3];
+    interpolate_one(wv, ew);
+    interpolate_one(uv, eu);
+    interpolate_one(vv, ev);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        F.wv[i * N + k] = wv[i];
+        F.uhv[i * N + k] = uv[i];
+        F.vhv[i * N + k] = vv[i];
+        Q4 e = F.eq[i * N + k];
+        e.w = ew[i];
+        e.uh = eu[i];
+        e.vh = ev[i];
+        F.eq[i * N + k] = e;
+    }
+}
+
 // extrapolate_first_order (extrapolate_first_order.cu:3-33) of the conserved quantities
 __global__ __launch_bounds__(kBlock) void k_first_order_conserved(DevMesh M, DevFields F) {
     const int N = M.N;
@@ -1075,6 +1190,8 @@ __global__ __launch_bounds__(BLOCK) void k_distribute(DevMesh M, DevFields F, De
         wv[0] = wv[1] = wv[2] = c.w;
         uv[0] = uv[1] = uv[2] = c.uh;
         vv[0] = vv[1] = vv[2] = c.vh;
+    } else if (P.limiter != 0) {
+        reconstruct_limited_all(P, k, r, wv, uv, vv);
     } else {
         if (!extrapolate_one<R>(F, P, N, k, r, wv, uv, vv)) set_error(st, kErrNeighbour);
     }

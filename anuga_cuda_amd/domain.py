@@ -61,7 +61,13 @@ DEFAULT_PARAMETERS = dict(
     use_sloped_mannings=0,
     max_smallsteps=50,
     default_order=2,
+    # distribute path (gpu_domain_advanced.py:1814-1990)
+    use_edge_limiter=0,
+    optimised_gradient_limiter=1,
 )
+
+# parameters that select the distribute path rather than map to a sw_params field
+_LIMITER_PARAMS = ("use_edge_limiter", "optimised_gradient_limiter")
 
 _QID = {
     "stage": SW_STAGE,
@@ -142,6 +148,9 @@ class Quantity:
     def __init__(self, domain, name):
         self.domain = domain
         self.name = name
+        # limiter coefficient of the per-quantity limiters (use_edge_limiter or
+        # optimised_gradient_limiter False); ANUGA's Quantity default
+        self._beta = 1.0
         N = domain.number_of_elements
         Nb = domain.number_of_boundary_elements
         self._arrays = {
@@ -156,6 +165,18 @@ class Quantity:
         if name == "friction":
             self._arrays = {"centroid_values": np.zeros(N), "vertex_values": np.zeros((N, 3)),
                             "edge_values": np.zeros((N, 3))}
+
+    @property
+    def beta(self):
+        return self._beta
+
+    @beta.setter
+    def beta(self, value):
+        self._beta = float(value)
+        self.domain._params_changed()
+
+    def set_beta(self, beta):
+        self.beta = beta
 
     def _get(self, key):
         self.domain._pull()
@@ -467,10 +488,34 @@ class Domain:
             self._handle.set_boundary(kinds, dirichlet)
 
     # ---------------------------------------------------------- host/device
+    def limiter_path(self):
+        """'optimised' (extrapolate_second_order_sw), 'vertex' or 'edge' (the
+        per-quantity limiters), as the reference's distribute chooses."""
+        if self._params["use_edge_limiter"]:
+            return "edge"
+        return "optimised" if self._params["optimised_gradient_limiter"] else "vertex"
+
+    def set_beta(self, beta):
+        """One limiter coefficient for every limiter (ANUGA's set_beta)."""
+        for n in ("beta_w", "beta_w_dry", "beta_uh", "beta_uh_dry", "beta_vh", "beta_vh_dry"):
+            self._params[n] = float(beta)
+        for q in CONSERVED:
+            self.quantities[q]._beta = float(beta)
+        self._params_changed()
+
+    def _params_changed(self):
+        if self.__dict__.get("_handle") is not None:
+            self._handle.set_params(self._sw_params())
+
     def _sw_params(self):
         p = _lib.SwParams()
         for k, v in self._params.items():
-            setattr(p, k, v)
+            if k not in _LIMITER_PARAMS:
+                setattr(p, k, v)
+        p.limiter = {"optimised": _lib.SW_LIMITER_OPTIMISED, "vertex": _lib.SW_LIMITER_VERTEX,
+                     "edge": _lib.SW_LIMITER_EDGE}[self.limiter_path()]
+        for i, q in enumerate(CONSERVED):
+            p.quantity_beta[i] = self.quantities[q].beta if "quantities" in self.__dict__ else 1.0
         p.precision = _lib.SW_PRECISION_MIXED if self.precision == "mixed" else _lib.SW_PRECISION_FP64
         p.locality_order = _lib.SW_ORDER_HILBERT if self.locality_order == "hilbert" else _lib.SW_ORDER_KEEP
         p.timestepping_method = TIMESTEPPING[self.get_timestepping_method()]

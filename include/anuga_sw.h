@@ -54,6 +54,7 @@ enum sw_precision { SW_PRECISION_FP64 = 0, SW_PRECISION_MIXED = 1 };
  * never depends on it: which side evaluates a shared edge and the order in
  * which edge fluxes are summed follow the caller's numbering, as in the C. */
 enum sw_locality_order { SW_ORDER_KEEP = 0, SW_ORDER_HILBERT = 1 };
+enum sw_limiter { SW_LIMITER_OPTIMISED = 0, SW_LIMITER_VERTEX = 1, SW_LIMITER_EDGE = 2 };
 enum sw_event { SW_EVENT_NONE = 0, SW_EVENT_YIELD = 1, SW_EVENT_FINAL = 2 };
 
 /* Domain parameters; names follow struct domain (src/anuga_HMPP/sw_domain.h:16-60)
@@ -69,6 +70,13 @@ typedef struct sw_params {
     /* timestepping_method (sw_domain.h:49; gpu_domain_advanced.py:2566-2573):
      * 1 (or 0) Euler, 2 rk2, 3 rk3 */
     int32_t timestepping_method;
+    /* distribute path (gpu_domain_advanced.py:1814-1990): SW_LIMITER_OPTIMISED
+     * (default: extrapolate_second_order_sw, optimised_gradient_limiter),
+     * SW_LIMITER_VERTEX (per-quantity extrapolate_second_order_and_limit_by_vertex),
+     * SW_LIMITER_EDGE (use_edge_limiter: ..._limit_by_edge) */
+    int32_t limiter;
+    /* Quantity.beta of stage, xmomentum, ymomentum for the per-quantity limiters */
+    double quantity_beta[3];
 } sw_params;
 
 /* Mesh arrays (host, borrowed during sw_create). */
@@ -236,6 +244,13 @@ int sw_protect(sw_handle *h);
 /* extrapolate_second_order_sw (gpu_domain_advanced.py:1061-1161;
  * shallow_water_ext.c:2717-3324) */
 int sw_extrapolate_second_order_sw(sw_handle *h);
+/* extrapolate_second_order_and_limit_by_vertex / _by_edge of stage,
+ * xmomentum and ymomentum with their quantity_beta
+ * (gpu_domain_advanced.py:1836-1859, 1941-1962;
+ * extrapolate_second_order_and_limit_by_vertex_or_edge.cu:694-800): vertex
+ * and edge values */
+int sw_extrapolate_second_order_and_limit_by_vertex(sw_handle *h);
+int sw_extrapolate_second_order_and_limit_by_edge(sw_handle *h);
 /* extrapolate_first_order for the conserved quantities
  * (gpu_domain_advanced.py:1888-1907; extrapolate_first_order.cu:3-33) */
 int sw_extrapolate_first_order(sw_handle *h);

@@ -58,6 +58,8 @@ class OrDomain(C.Structure):
         + [("max_speed", _F64P), ("already_computed_flux", _I64P), ("flux_call", C.c_int64)]
         + [("timestepping_method", C.c_int64)]
         + [(p + "_centroid_backup", _F64P) for p in ("stage", "xmom", "ymom")]
+        + [("limiter", C.c_int64), ("beta_stage", C.c_double), ("beta_xmom", C.c_double),
+           ("beta_ymom", C.c_double)]
     )
 
 _METHODS = {"euler": 1, "rk2": 2, "rk3": 3}
@@ -97,6 +99,7 @@ def load():
         "or_update_timestep": ([P], C.c_int),
         "or_evolve_one_euler_step": ([P], C.c_int),
         "or_backup_conserved_quantities": ([P], None),
+        "or_extrapolate_second_order_and_limit": ([P, _F64P, _F64P, _F64P, C.c_double, C.c_int], None),
         "or_saxpy_conserved_quantities": ([P, C.c_double, C.c_double], None),
         "or_evolve_one_rk_step": ([P, C.c_int], C.c_int),
         "or_evolve_prepare": ([P, C.c_double, C.c_double, C.c_int64], C.c_int),
@@ -223,6 +226,9 @@ class OracleDomain:
         d.timestepping_method = _METHODS[params.get("timestepping_method", "euler")]
         for cname in ("stage", "xmom", "ymom"):
             keep(cname + "_centroid_backup", np.zeros(N), np.float64)
+        d.limiter = {"optimised": 0, "vertex": 1, "edge": 2}[params.get("limiter", "optimised")]
+        betas = params.get("quantity_beta", (1.0, 1.0, 1.0))
+        d.beta_stage, d.beta_xmom, d.beta_ymom = (float(b) for b in betas)
         d._order_ = int(params["default_order"])
         d.evolve_max_timestep = float(params["evolve_max_timestep"])
         d.recorded_min_timestep = d.evolve_max_timestep
@@ -239,7 +245,9 @@ class OracleDomain:
         for name, Q in domain.quantities.items():
             q[name] = {k: v.copy() for k, v in Q._arrays.items()}
         state = {"time": domain._time, "starttime": domain.starttime, "_order_": domain._order_}
-        params = dict(domain._params, timestepping_method=domain.get_timestepping_method())
+        params = dict(domain._params, timestepping_method=domain.get_timestepping_method(),
+                      limiter=domain.limiter_path(),
+                      quantity_beta=tuple(domain.quantities[q].beta for q in ("stage", "xmomentum", "ymomentum")))
         return cls(domain.mesh, params, q, domain._kinds, domain._dirichlet, state)
 
     # array access by ANUGA quantity name
@@ -303,6 +311,14 @@ class OracleDomain:
 
     def update_other_quantities(self):
         self.lib.or_update_other_quantities(self._p())
+
+    def extrapolate_second_order_and_limit(self, name, by_edge, beta):
+        """The per-quantity reconstruction of one conserved quantity into its
+        vertex and edge arrays."""
+        cv, vv, ev = (self.q(name, loc) for loc in ("centroid_values", "vertex_values", "edge_values"))
+        self.lib.or_extrapolate_second_order_and_limit(self._p(), cv.ctypes.data_as(_F64P),
+                                                        vv.ctypes.data_as(_F64P), ev.ctypes.data_as(_F64P),
+                                                        float(beta), int(by_edge))
 
     def backup_conserved_quantities(self):
         self.lib.or_backup_conserved_quantities(self._p())

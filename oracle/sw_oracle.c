@@ -577,6 +577,94 @@ int or_extrapolate_second_order_sw(or_domain *D) {
 }
 
 /* extrapolate_first_order.c:8-40: edge = vertex = centroid */
+/* extrapolate_second_order_and_limit_by_vertex / _by_edge for one quantity
+ * (extrapolate_second_order_and_limit_by_vertex_or_edge.cu:694-800), per
+ * triangle k:
+ *  - gradient (a, b) from the centroid values (compute_gradients, :226-316):
+ *    with fewer than two boundary edges the plane through the three surrogate
+ *    neighbours' centroids; with two, the two-point gradient towards the one
+ *    neighbour; with three, none (a = b = 0: the kernel leaves its gradient
+ *    arrays untouched there, which for an isolated triangle means 0);
+ *  - vertex values c + a (x_i - x) + b (y_i - y) and edge values as midpoint
+ *    means (extrapolate_from_gradient, :172-224);
+ *  - limiter against the range of the centroid value and the true
+ *    neighbours' (limit_vertices_by_all_neighbours :5-75 on the vertex
+ *    values, limit_edges_by_all_neighbours :77-170 on the edge values, the
+ *    vertices then rebuilt as e_{i+1} + e_{i+2} - e_i): phi = min over the
+ *    three of min(r beta, 1), r = (qmax - qc) / dq or (qmin - qc) / dq.
+ * Edge values are left for the interpolation after balance. */
+void or_extrapolate_second_order_and_limit(or_domain *D, const double *cv, double *vv, double *ev, double beta,
+                                           int by_edge) {
+    const int64_t N = D->number_of_elements;
+    const double *cc = D->centroid_coordinates, *xv = D->vertex_coordinates;
+    for (int64_t k = 0; k < N; k++) {
+        const int64_t *sur = D->surrogate_neighbours + 3 * k;
+        const int64_t nob = D->number_of_boundaries[k];
+        double a = 0.0, b = 0.0;
+        if (nob < 2) {
+            const int64_t k0 = sur[0], k1 = sur[1], k2 = sur[2];
+            if (!(k0 == k1 || k1 == k2)) {
+                const double q0 = cv[k0], q1 = cv[k1], q2 = cv[k2];
+                const double x0 = cc[2 * k0], y0 = cc[2 * k0 + 1];
+                const double x1 = cc[2 * k1], y1 = cc[2 * k1 + 1];
+                const double x2 = cc[2 * k2], y2 = cc[2 * k2 + 1];
+                const double det = (y2 - y0) * (x1 - x0) - (y1 - y0) * (x2 - x0);
+                a = (y2 - y0) * (q1 - q0) - (y1 - y0) * (q2 - q0);
+                a /= det;
+                b = (x1 - x0) * (q2 - q0) - (x2 - x0) * (q1 - q0);
+                b /= det;
+            }
+        } else if (nob == 2) {
+            int64_t k0 = k;
+            for (int i = 0; i < 3 && k0 == k; i++) k0 = sur[i];
+            if (k0 != k) {
+                const double q0 = cv[k0], q1 = cv[k];
+                const double x0 = cc[2 * k0], y0 = cc[2 * k0 + 1];
+                const double x1 = cc[2 * k], y1 = cc[2 * k + 1];
+                const double det = (x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0);
+                a = (x1 - x0) * (q1 - q0) / det;
+                b = (y1 - y0) * (q1 - q0) / det;
+            }
+        }
+        const double x = cc[2 * k], y = cc[2 * k + 1], qc = cv[k];
+        double v[3], e[3];
+        for (int i = 0; i < 3; i++) v[i] = qc + a * (xv[6 * k + 2 * i] - x) + b * (xv[6 * k + 2 * i + 1] - y);
+        e[0] = 0.5 * (v[1] + v[2]);
+        e[1] = 0.5 * (v[2] + v[0]);
+        e[2] = 0.5 * (v[0] + v[1]);
+        double qmin = qc, qmax = qc;
+        for (int i = 0; i < 3; i++) {
+            const int64_t n = D->neighbours[3 * k + i];
+            if (n >= 0) {
+                qmin = omin(qmin, cv[n]);
+                qmax = omax(qmax, cv[n]);
+            }
+        }
+        const double *src = by_edge ? e : v;
+        double dqa[3], phi = 1.0;
+        for (int i = 0; i < 3; i++) {
+            double r = 1.0;
+            const double dq = src[i] - qc;
+            dqa[i] = dq;
+            if (dq > 0.0) r = (qmax - qc) / dq;
+            if (dq < 0.0) r = (qmin - qc) / dq;
+            phi = omin(omin(r * beta, 1.0), phi);
+        }
+        if (by_edge) {
+            for (int i = 0; i < 3; i++) e[i] = qc + phi * dqa[i];
+            v[0] = e[1] + e[2] - e[0];
+            v[1] = e[2] + e[0] - e[1];
+            v[2] = e[0] + e[1] - e[2];
+        } else {
+            for (int i = 0; i < 3; i++) v[i] = qc + phi * dqa[i];
+        }
+        for (int i = 0; i < 3; i++) {
+            vv[3 * k + i] = v[i];
+            ev[3 * k + i] = 0.5 * (v[(i + 1) % 3] + v[(i + 2) % 3]);
+        }
+    }
+}
+
 void or_extrapolate_first_order(int64_t N, const double *centroid, double *edge, double *vertex) {
     for (int64_t k = 0; k < N; k++) {
         int64_t k3 = 3 * k;
@@ -675,6 +763,14 @@ int or_distribute_to_vertices_and_edges(or_domain *D) {
                                    D->xmom_vertex_values);
         or_extrapolate_first_order(N, D->ymom_centroid_values, D->ymom_edge_values,
                                    D->ymom_vertex_values);
+    } else if (D->limiter == 1 || D->limiter == 2) {
+        const int by_edge = D->limiter == 2;
+        or_extrapolate_second_order_and_limit(D, D->stage_centroid_values, D->stage_vertex_values,
+                                              D->stage_edge_values, D->beta_stage, by_edge);
+        or_extrapolate_second_order_and_limit(D, D->xmom_centroid_values, D->xmom_vertex_values,
+                                              D->xmom_edge_values, D->beta_xmom, by_edge);
+        or_extrapolate_second_order_and_limit(D, D->ymom_centroid_values, D->ymom_vertex_values,
+                                              D->ymom_edge_values, D->beta_ymom, by_edge);
     } else {
         rc = or_extrapolate_second_order_sw(D);
         if (rc) return rc;

@@ -75,6 +75,12 @@ typedef struct or_domain {
      * of the conserved centroid values for the multi-stage steps */
     int64_t timestepping_method;
     double *stage_centroid_backup, *xmom_centroid_backup, *ymom_centroid_backup;
+    /* distribute path (gpu_domain_advanced.py:1814-1990): 0 the optimised
+     * vertex limiter (extrapolate_second_order_sw), 1 per-quantity limit by
+     * vertex (optimised_gradient_limiter False), 2 per-quantity limit by edge
+     * (use_edge_limiter); Quantity.beta of stage, xmomentum, ymomentum */
+    int64_t limiter;
+    double beta_stage, beta_xmom, beta_ymom;
 } or_domain;
 
 /* status codes */
@@ -98,6 +104,8 @@ int or_update(int64_t N, double timestep, double *centroid_values,
 int or_update_conserved_quantities(or_domain *D);
 void or_protect(or_domain *D);
 int or_extrapolate_second_order_sw(or_domain *D);
+void or_extrapolate_second_order_and_limit(or_domain *D, const double *centroid, double *vertex, double *edge,
+                                           double beta, int by_edge);
 void or_extrapolate_first_order(int64_t N, const double *centroid, double *edge, double *vertex);
 void or_balance_deep_and_shallow(or_domain *D);
 void or_interpolate_from_vertices_to_edges(int64_t N, const double *vertex, double *edge);

@@ -128,6 +128,26 @@ def test_runge_kutta_steps_match_oracle(method, fused):
     compare(d._handle.max_speed(), od.max_speed, "max_speed", TOL)
 
 
+@pytest.mark.parametrize("limiter", ["vertex", "edge"])
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "unfused"])
+def test_per_quantity_limiters_match_oracle(limiter, fused):
+    """The distribute paths of optimised_gradient_limiter=False (limit by
+    vertex) and use_edge_limiter=True (limit by edge), each quantity with its
+    own beta, in lockstep with the oracle."""
+    kw = {"use_edge_limiter": 1} if limiter == "edge" else {"optimised_gradient_limiter": 0}
+    if fused:
+        d = scenarios.delaunay_terrain(npoints=3000, order="random", extent=2000.0, mound_radius=200.0,
+                                       seed=7, terrain_seed=11, **kw)
+    else:
+        d = _dam(**kw)
+    d.quantities["xmomentum"].beta = 0.7
+    d.quantities["ymomentum"].beta = 1.3
+    assert d.limiter_path() == limiter
+    stats = {}
+    _, dts = _lockstep(d, 16 if fused else 6, fused=fused, stats=stats)
+    assert len(dts) == (16 if fused else 6)
+
+
 def test_backup_and_saxpy_operators_match_oracle():
     d = _dam()
     for _ in d.evolve(yieldstep=0.5, finaltime=1.0):

@@ -470,3 +470,79 @@ def test_timestepping_method_names():
     assert _oracle(d).d.timestepping_method == 3
     with pytest.raises(ValueError):
         d.set_timestepping_method("rk4")
+
+
+# ------------------------------------------------- per-quantity limiters
+def _limited(d, q, by_edge, beta):
+    """Run the oracle's per-quantity reconstruction of quantity q on its own."""
+    od = _oracle(d)
+    od.extrapolate_second_order_and_limit(q, by_edge, beta)
+    return od, od.q(q, "centroid_values"), od.q(q, "vertex_values"), od.q(q, "edge_values")
+
+
+def _field_domain(f, m=10, n=8):
+    points, tris, boundary = rectangular_cross(m, n, 100.0, 80.0)
+    d = Domain(mesh=Mesh(points, tris, boundary=boundary))
+    d.set_quantity("elevation", -10.0)
+    d.set_quantity("stage", f, location="centroids")
+    d.set_boundary({t: Reflective_boundary(d) for t in d.mesh.get_tag_names()})
+    return d
+
+
+@pytest.mark.parametrize("by_edge", [False, True], ids=["vertex", "edge"])
+def test_limited_reconstruction_reproduces_linear_fields(by_edge):
+    """Interior triangles of a linear field (generic gradient direction, so no
+    neighbour ties): the three-centroid gradient is exact and, with the
+    limiter opened up (large beta), the vertex values are the field's values
+    at the vertices; a constant field stays constant everywhere."""
+    f = lambda x, y: 0.37 * x - 0.61 * y + 2.0
+    d = _field_domain(f)
+    od, cv, vv, ev = _limited(d, "stage", by_edge, 1e30)
+    m = d.mesh
+    exact = f(m.vertex_coordinates[:, 0::2], m.vertex_coordinates[:, 1::2])
+    inner = m.number_of_boundaries == 0
+    assert inner.sum() > 100
+    np.testing.assert_allclose(vv[inner], exact[inner], rtol=0, atol=1e-12)
+    d = _field_domain(lambda x, y: 3.0 + 0 * x)
+    od, cv, vv, ev = _limited(d, "stage", by_edge, 1.0)
+    assert np.all(vv == 3.0)
+
+
+@pytest.mark.parametrize("by_edge", [False, True], ids=["vertex", "edge"])
+def test_limited_reconstruction_stays_within_neighbour_bounds(by_edge):
+    """With beta <= 1 the limited vertex (vertex limiter) or edge (edge
+    limiter) values lie within the range of the triangle's and its true
+    neighbours' centroid values; beta = 0 gives the first-order values."""
+    rng = np.random.default_rng(11)
+    d = _field_domain(lambda x, y: np.sin(x / 9.0) + np.cos(y / 7.0) + 0.3 * rng.normal(size=x.shape))
+    od, cv, vv, ev = _limited(d, "stage", by_edge, 1.0)
+    m = d.mesh
+    nb = m.neighbours
+    vals = np.where(nb >= 0, cv[np.maximum(nb, 0)], cv[:, None])
+    lo = np.minimum(vals.min(axis=1), cv)[:, None]
+    hi = np.maximum(vals.max(axis=1), cv)[:, None]
+    lim = ev if by_edge else vv  # the edge values are the limited midpoint values
+    tol = 1e-12 * np.abs(cv).max()
+    assert np.all(lim >= lo - tol) and np.all(lim <= hi + tol)
+    od, cv, vv, ev = _limited(d, "stage", by_edge, 0.0)
+    np.testing.assert_array_equal(vv, np.repeat(cv[:, None], 3, axis=1))
+
+
+@pytest.mark.parametrize("limiter", ["vertex", "edge"])
+def test_limiter_paths_keep_lake_at_rest_and_mass(limiter):
+    kw = {"use_edge_limiter": 1} if limiter == "edge" else {"optimised_gradient_limiter": 0}
+    d = scenarios.dam_break_channel(16, 16, high=2.0, low=2.0, bump=0.8, **kw)
+    assert d.limiter_path() == limiter
+    od = _run(_oracle(d), 30)
+    np.testing.assert_allclose(od.q("stage", "centroid_values"), 2.0, rtol=0, atol=1e-12)
+    assert np.max(np.abs(od.q("xmomentum", "centroid_values"))) < 1e-12
+    d = scenarios.dam_break_channel(16, 16, high=2.0, low=1.2, bump=0.3, **kw)
+    od = _oracle(d)
+    od.evolve_prepare(1e9, None)
+    area = d.mesh.areas
+    v0 = np.sum(area * od.q("stage", "centroid_values"))
+    for _ in range(40):
+        od.evolve_step()
+    v1 = np.sum(area * od.q("stage", "centroid_values"))
+    assert abs(v1 - v0) / v0 < 1e-13
+    assert od.d.time > 5.0
