@@ -319,15 +319,22 @@ void launch_update_b(sw_handle *h, int mode, double a, double b) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
     hipLaunchKernelGGL((k_update<true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp, h->st, mode, a, b);
 }
-template <int B>
-void launch_distribute_b(sw_handle *h, bool mixed) {
+template <int B, bool LIMITED>
+void launch_distribute_lb(sw_handle *h, bool mixed) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
     if (mixed)
-        hipLaunchKernelGGL((k_distribute<float, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp,
-                           h->st);
+        hipLaunchKernelGGL((k_distribute<float, true, B, LIMITED>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f,
+                           h->dp, h->st);
     else
-        hipLaunchKernelGGL((k_distribute<double, true, B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->dp,
-                           h->st);
+        hipLaunchKernelGGL((k_distribute<double, true, B, LIMITED>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f,
+                           h->dp, h->st);
+}
+template <int B>
+void launch_distribute_b(sw_handle *h, bool mixed) {
+    if (h->params.limiter != SW_LIMITER_OPTIMISED)
+        launch_distribute_lb<B, true>(h, mixed);
+    else
+        launch_distribute_lb<B, false>(h, mixed);
 }
 
 // Stages of a step: 1 (Euler), 2 (rk2) or 3 (rk3).
@@ -1247,12 +1254,22 @@ int sw_update_other_quantities(sw_handle *h) {
 int distribute_all(sw_handle *h) {
     const unsigned g = grid_for(h->N);
     hipLaunchKernelGGL(k_protect, dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
-    if (h->params.precision == SW_PRECISION_MIXED)
-        hipLaunchKernelGGL((k_distribute<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp,
-                           h->st);
-    else
-        hipLaunchKernelGGL((k_distribute<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp,
-                           h->st);
+    const bool limited = h->params.limiter != SW_LIMITER_OPTIMISED;
+    if (h->params.precision == SW_PRECISION_MIXED) {
+        if (limited)
+            hipLaunchKernelGGL((k_distribute<float, false, kBlock, true>), dim3(g), dim3(kBlock), 0, h->stream,
+                               h->mesh, h->f, h->dp, h->st);
+        else
+            hipLaunchKernelGGL((k_distribute<float, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f,
+                               h->dp, h->st);
+    } else {
+        if (limited)
+            hipLaunchKernelGGL((k_distribute<double, false, kBlock, true>), dim3(g), dim3(kBlock), 0, h->stream,
+                               h->mesh, h->f, h->dp, h->st);
+        else
+            hipLaunchKernelGGL((k_distribute<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f,
+                               h->dp, h->st);
+    }
     if (int rc = launch_check(h, "distribute")) return rc;
     std::vector<DevState> d;
     return sync_all(h, d);

@@ -1172,8 +1172,10 @@ __global__ __launch_bounds__(kBlock) void k_source_terms(DevMesh M, DevFields F,
 // balance_deep_and_shallow and vertex->edge interpolation in registers.  The
 // edge records are written whole; their bed value is recomputed from the bed
 // vertex values when the bed edges are known to be that interpolation
-// (st->bed_interp), else copied from the static bed edge array.
-template <typename R, bool FUSED, int BLOCK = kBlock>
+// (st->bed_interp), else copied from the static bed edge array.  LIMITED
+// selects the per-quantity limiters (P.limiter != SW_LIMITER_OPTIMISED) at
+// compile time, so the default path carries none of their code.
+template <typename R, bool FUSED, int BLOCK = kBlock, bool LIMITED = false>
 __global__ __launch_bounds__(BLOCK) void k_distribute(DevMesh M, DevFields F, DevParams P, DevState *st_all) {
     const Lane ln = lane_of<BLOCK>(M);
     DevState *st = st_all + ln.inst;
@@ -1190,7 +1192,7 @@ __global__ __launch_bounds__(BLOCK) void k_distribute(DevMesh M, DevFields F, De
         wv[0] = wv[1] = wv[2] = c.w;
         uv[0] = uv[1] = uv[2] = c.uh;
         vv[0] = vv[1] = vv[2] = c.vh;
-    } else if (P.limiter != 0) {
+    } else if (LIMITED) {
         reconstruct_limited_all(P, k, r, wv, uv, vv);
     } else {
         if (!extrapolate_one<R>(F, P, N, k, r, wv, uv, vv)) set_error(st, kErrNeighbour);
