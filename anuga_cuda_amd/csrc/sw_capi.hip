@@ -39,9 +39,11 @@ struct Inst {
 
 // Output points of one instance (sw_set_sww_points): point p averages the
 // vertex-value slots slot[start[p] .. start[p+1]); out holds the FP32 values.
+// Lane t handles point order[t]: the points sorted by the device index of
+// their first triangle, so neighbouring lanes gather neighbouring triangles.
 struct SwwPoints {
     int64_t P = 0;
-    int *start = nullptr, *slot = nullptr;
+    int *start = nullptr, *slot = nullptr, *order = nullptr;
     float *out = nullptr;
 };
 
@@ -1111,13 +1113,25 @@ int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *num
             flat[v] = (int)x;
         }
     }
+    // processing order: by the device index of each point's first triangle
+    std::vector<int> dev_of(I.n);
+    HIPCHECK(h, hipMemcpy(dev_of.data(), h->dev_of + I.poff, I.n * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<std::pair<int, int>> keyed(P);
+    for (int64_t p = 0; p < P; p++)
+        keyed[p] = {start[p] < start[p + 1] ? dev_of[flat[start[p]] / 3] : -1, (int)p};
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int> order(P);
+    for (int64_t p = 0; p < P; p++) order[p] = keyed[p].second;
     SwwPoints &S = h->sww[i];
     dfree(h, S.start);
     dfree(h, S.slot);
+    dfree(h, S.order);
     dfree(h, S.out);
     S = SwwPoints{};
-    if (dalloc(h, &S.start, P + 1) || dalloc(h, &S.slot, nv) || dalloc(h, &S.out, 4 * P)) return SW_ERR_HIP;
+    if (dalloc(h, &S.start, P + 1) || dalloc(h, &S.slot, nv) || dalloc(h, &S.order, P) || dalloc(h, &S.out, 4 * P))
+        return SW_ERR_HIP;
     S.P = P;
+    HIPCHECK(h, hipMemcpy(S.order, order.data(), P * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(h, hipMemcpy(S.start, start.data(), (P + 1) * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(h, hipMemcpy(S.slot, flat.data(), nv * sizeof(int), hipMemcpyHostToDevice));
     // caller vertex 3k+j -> device slot j*N + off + dev_of[k]
@@ -1136,8 +1150,8 @@ int sw_sww_values(sw_handle *h, double minimum_storable_height, float *out, int6
     if (S.P == 0) return fail(h, SW_ERR_STATE, "no output points (call sw_set_sww_points)");
     if (count != 4 * S.P) return fail(h, SW_ERR_ARG, "sww values: %lld floats for 4 x %lld points", (long long)count, (long long)S.P);
     const DevFields &F = h->f;
-    hipLaunchKernelGGL(k_sww_points, dim3(grid_for(S.P)), dim3(kBlock), 0, h->stream, (long long)S.P, S.start, S.slot,
-                       F.zv, F.wv, F.uhv, F.vhv, (float)minimum_storable_height, S.out);
+    hipLaunchKernelGGL(k_sww_points, dim3(grid_for(S.P)), dim3(kBlock), 0, h->stream, (long long)S.P, S.order, S.start,
+                       S.slot, F.zv, F.wv, F.uhv, F.vhv, (float)minimum_storable_height, S.out);
     if (int rc = launch_check(h, "k_sww_points")) return rc;
     HIPCHECK(h, hipMemcpyAsync(out, S.out, 4 * S.P * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));

@@ -1358,18 +1358,21 @@ __global__ __launch_bounds__(kBlock) void k_sww_slots(int nv, long long N, long 
 }
 
 // FP32 values of the SWW points out[4][P] (elevation, stage, xmomentum,
-// ymomentum).  Point p is the mean of the vertex values in slot[start[p] ..
+// ymomentum); lane t computes point order[t] (a permutation of 0 .. P-1).
+// Point p is the mean of the vertex values in slot[start[p] ..
 // start[p+1]), summed in that order and divided by the count
 // (ANUGA quantity_ext.c _average_vertex_values, the smoothing of
 // Quantity.get_vertex_values), then rounded to FP32; a unique-vertex point
 // lists one slot.  Where stage - elevation < minimum_storable_height (an
 // FP32 test, as the writer's) stage is stored as the elevation and the
 // momenta as 0.  A point with no triangles gets zeros.
-__global__ __launch_bounds__(kBlock) void k_sww_points(long long P, const int *start, const int *slot,
-                                                       const double *zv, const double *wv, const double *uhv,
-                                                       const double *vhv, float min_storable, float *out) {
-    const long long p = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= P) return;
+__global__ __launch_bounds__(kBlock) void k_sww_points(long long P, const int *order, const int *start,
+                                                       const int *slot, const double *zv, const double *wv,
+                                                       const double *uhv, const double *vhv, float min_storable,
+                                                       float *out) {
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= P) return;
+    const long long p = order[t];
     const int a = start[p], b = start[p + 1];
     double sz = 0.0, sw = 0.0, su = 0.0, sv = 0.0;
     for (int i = a; i < b; i++) {
