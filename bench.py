@@ -108,6 +108,10 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--locality", default="hilbert", choices=["hilbert", "keep"],
                     help="device numbering of the triangles (library renumbering or the input order)")
+    ap.add_argument("--timestepping", default="euler", choices=["euler", "rk2", "rk3"],
+                    help="fv: step of the timed loop (a step of rk2 / rk3 runs 2 / 3 stages)")
+    ap.add_argument("--limiter", default="optimised", choices=["optimised", "vertex", "edge"],
+                    help="fv: distribute path (vertex: optimised_gradient_limiter off; edge: use_edge_limiter)")
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=64)
@@ -143,8 +147,11 @@ def main():
         del a, b
 
     t0 = time.time()
+    limiter_kw = {"vertex": {"optimised_gradient_limiter": 0}, "edge": {"use_edge_limiter": 1}}.get(args.limiter, {})
     d = scenarios.delaunay_terrain(npoints=args.points, order=args.order, precision=args.precision,
-                                   locality_order=args.locality)
+                                   locality_order=args.locality, **limiter_kw)
+    d.set_timestepping_method(args.timestepping)
+    stages = {"euler": 1, "rk2": 2, "rk3": 3}[args.timestepping]
     N = d.number_of_elements
     log("mesh: %d triangles, %d boundary edges, built in %.1f s" % (N, d.number_of_boundary_elements,
                                                                   time.time() - t0))
@@ -181,7 +188,9 @@ def main():
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
     dominant = max(("flux", "update", "distribute"), key=lambda k: avg_us[k] or 0.0)
     kb = kernel_bytes(args.precision)
-    step_bytes = sum(kb.values())
+    # a Runge-Kutta step runs every kernel once per stage, plus the copy of the
+    # starting centroid records (32 B written, then read by each later stage)
+    step_bytes = sum(kb.values()) * stages + (32 * stages if stages > 1 else 0)
     bytes_launch = kb[dominant] * N
     achieved = bytes_launch / (avg_us[dominant] * 1e-6) / 1e9
     step_gbs = step_bytes * N * args.steps / elapsed / 1e9
@@ -192,7 +201,14 @@ def main():
     tfile = args.traffic_json or os.path.join(ROOT, "profiles", "%s.traffic.json" % workload_tag(args))
     if os.path.exists(tfile):
         kname = kernel_symbol(dominant, args.precision, launch)
-        rec = json.load(open(tfile)).get("kernels", {}).get(kname, {})
+        kernels = json.load(open(tfile)).get("kernels", {})
+        if dominant == "distribute":
+            # k_distribute<R, FUSED, BLOCK, LIMITED>: profiles name the limiter variant
+            limited = kname[:-1] + (", true>" if args.limiter != "optimised" else ", false>")
+            rec = kernels.get(limited) or (kernels.get(kname) if args.limiter == "optimised" else None)
+        else:
+            rec = kernels.get(kname)
+        rec = rec or {}
         traffic = rec.get("traffic_bytes")
 
     cpu = None
@@ -213,9 +229,10 @@ def main():
         except OSError:
             pass
         cpu = {"value": N * args.cpu_steps / c_el, "unit": "triangle-steps/s", "cores": 1, "kind": "port",
-               "sample": "%d Euler steps of the same %d-triangle workload, serial FP64 C oracle "
+               "sample": "%d %s steps of the same %d-triangle workload, serial FP64 C oracle "
                          "(restated reference C), %.2f s on %s (nproc %d)" % (
-                             args.cpu_steps, N, c_el, cpu_model, os.cpu_count() or 0)}
+                             args.cpu_steps, "Euler" if stages == 1 else args.timestepping, N, c_el, cpu_model,
+                             os.cpu_count() or 0)}
         log("cpu baseline: %.3e triangle-steps/s" % cpu["value"])
 
     out = {
@@ -233,7 +250,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain FV, seeded Delaunay %d points in 10 km x 10 km, %s order, fBm terrain "
                                "20 m, wet/dry mound, Manning 0.025, transmissive" % (args.points, args.order),
-                   "triangles": N, "precision": args.precision, "order": 2, "timestepping": "euler",
+                   "triangles": N, "precision": args.precision, "order": 2, "timestepping": args.timestepping,
+                   "limiter": args.limiter,
                    "device_numbering": args.locality, "workgroup_sizes": list(launch)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
