@@ -33,6 +33,8 @@ class Ensemble:
                 raise RuntimeError("ensemble members must not have device storage yet")
             if d._params != d0._params or d.precision != d0.precision:
                 raise ValueError("ensemble members must share their numerical parameters and precision")
+            if bytes(d._sw_params()) != bytes(d0._sw_params()):
+                raise ValueError("ensemble members must share their timestepping method and limiter settings")
         self.handle = _lib.Handle.ensemble([d.mesh for d in self.domains], d0._sw_params(), device)
         for i, d in enumerate(self.domains):
             d._attach(self.handle.view(i))
