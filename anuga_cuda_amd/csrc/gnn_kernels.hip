@@ -1049,9 +1049,31 @@ int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, cons
     return -1;
 }
 
-inline unsigned persist_blocks(int64_t rows) {
+// Workgroups of a persistent kernel that are resident on the whole device at
+// once (its occupancy per CU, from its registers and LDS, times the CUs).  A
+// persistent grid larger than this leaves its last workgroups waiting for a
+// second round while every wavefront owns the same number of tiles.
+extern "C++" template <typename Kernel>
+unsigned resident_blocks(Kernel kernel) {
+    constexpr int kMaxDevices = 64;
+    static unsigned cached[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    if (!cached[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kB, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 1;
+        cached[dev] = (unsigned)(per_cu * cus);
+    }
+    return cached[dev];
+}
+
+inline unsigned persist_blocks(int64_t rows, unsigned resident = kPersistMax) {
     const int64_t tiles = rows / 16, b = (tiles + kB / 64 - 1) / (kB / 64);
-    return (unsigned)(b < kPersistMax ? (b > 0 ? b : 1) : kPersistMax);
+    const int64_t cap = resident < (unsigned)kPersistMax ? (int64_t)resident : (int64_t)kPersistMax;
+    return (unsigned)(b < cap ? (b > 0 ? b : 1) : cap);
 }
 
 int gnn_feat_enc(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const float *nx,
@@ -1060,7 +1082,8 @@ int gnn_feat_enc(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const
                  const void *WT, const float *b, double g, double h_dry, float flux_scale, float *P, void *X0save,
                  void *Zsave, void *H1) {
     if (Epad % 64 || E > Epad || !(flux_scale > 0.0f)) return -1;
-    hipLaunchKernelGGL(k_feat_enc, dim3(persist_blocks(Epad / 4)), dim3(kB), 0, S(stream), E, Epad, ea, eb, nx, ny, len,
+    hipLaunchKernelGGL(k_feat_enc, dim3(persist_blocks(Epad / 4, resident_blocks(k_feat_enc))), dim3(kB), 0, S(stream),
+                       E, Epad, ea, eb, nx, ny, len,
                        closed, w, uh, vh, z, depth_scale, momentum_scale, length_scale, static_cast<const bf16 *>(WT),
                        b, g, h_dry, 1.0 / (double)flux_scale, P, static_cast<bf16 *>(X0save),
                        static_cast<bf16 *>(Zsave), static_cast<bf16 *>(H1));
@@ -1071,7 +1094,8 @@ int gnn_mp_direct(void *stream, int64_t E, int64_t Epad, const int32_t *nbr6, co
                   const float *b, void *Zsave, void *Hout, const void *WTh, const float *bh, const float *P,
                   float *F) {
     if (Epad % 16 || E > Epad || H == Hout || (!Hout && !WTh) || (WTh && (!bh || !F))) return -1;
-    hipLaunchKernelGGL(k_mp_direct, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), E, Epad, nbr6,
+    hipLaunchKernelGGL(k_mp_direct, dim3(persist_blocks(Epad, resident_blocks(k_mp_direct))), dim3(kB), 0, S(stream), E,
+                       Epad, nbr6,
                        static_cast<const bf16 *>(H), static_cast<const bf16 *>(WT), b, static_cast<bf16 *>(Zsave),
                        static_cast<bf16 *>(Hout), static_cast<const bf16 *>(WTh), bh, P, F);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1079,7 +1103,8 @@ int gnn_mp_direct(void *stream, int64_t E, int64_t Epad, const int32_t *nbr6, co
 
 int gnn_head(void *stream, int64_t Epad, const void *H4, const void *WT, const float *b, const float *P, float *F) {
     if (Epad % 16) return -1;
-    hipLaunchKernelGGL(k_head, dim3(persist_blocks(Epad)), dim3(kB), 0, S(stream), Epad, static_cast<const bf16 *>(H4),
+    hipLaunchKernelGGL(k_head, dim3(persist_blocks(Epad, resident_blocks(k_head))), dim3(kB), 0, S(stream), Epad,
+                       static_cast<const bf16 *>(H4),
                        static_cast<const bf16 *>(WT), b, P, F);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
