@@ -16,9 +16,11 @@ Usage::
 * ``--json``: writes {kernel short name: {avg_us, fetch_bytes, write_bytes,
   traffic_bytes}} for bench.py's roofline ``traffic`` field.
 * ``--bytes NAME=B`` (repeatable): algorithmic bytes per launch of the
-  kernels whose name contains NAME (mangled names included); prints their
-  achieved GB/s
-  (B / average duration) against the 8 TB/s HBM peak.
+  kernel whose base name (the function name without namespaces, template or
+  parameter lists; mangled names are parsed) is NAME; prints its achieved
+  GB/s (B / average duration) against the 8 TB/s HBM peak.  ``A+B=bytes``
+  names an operation of two kernels launched once each (their average
+  durations add).
 """
 from __future__ import annotations
 
@@ -35,6 +37,26 @@ def short_name(name):
     if n.startswith("void "):
         n = n[5:]
     return n.replace("sw::", "")
+
+
+def base_name(name):
+    """Function name of a kernel symbol without namespaces, template
+    arguments or parameters: '_ZN3gnn11k_mp_directE...' -> 'k_mp_direct',
+    'gnn::k_apply' -> 'k_apply', 'void k_flux<double, true, 256>(...)' ->
+    'k_flux'."""
+    if name.startswith("_ZN"):
+        i, parts = 3, []
+        while i < len(name) and name[i].isdigit():
+            j = i
+            while j < len(name) and name[j].isdigit():
+                j += 1
+            n = int(name[i:j])
+            parts.append(name[j:j + n])
+            i = j + n
+        if parts:
+            return parts[-1]
+    n = short_name(name).split("<")[0]
+    return n.split("::")[-1].strip()
 
 
 def kernel_stats(db):
@@ -102,13 +124,21 @@ def main(argv=None):
         print()
         print("%-44s %12s %10s %10s %8s" % ("kernel", "alg_bytes", "avg_us", "GB/s", "of_8TB/s"))
         for spec in a.bytes:
-            prefix, b = spec.split("=")
-            for name, (n, avg, mn, mx) in sorted(ks.items()):
-                if prefix in name:
-                    gbs = float(b) / (avg * 1e-6) / 1e9
-                    print("%-44s %12.4g %10.2f %10.1f %8.3f" % (short_name(name)[:44], float(b), avg, gbs,
-                                                                gbs / 8000.0))
-                    out.setdefault(short_name(name), {})["algorithmic_bytes"] = float(b)
+            names, b = spec.split("=")
+            parts = names.split("+")
+            found = {p: [(nm, v) for nm, v in ks.items() if base_name(nm) == p] for p in parts}
+            if not all(found.values()):
+                print("%-44s (no kernel named %s)" % (names[:44], [p for p in parts if not found[p]]))
+                continue
+            if len(parts) == 1:
+                rows = [(short_name(nm), v[1]) for nm, v in sorted(found[parts[0]])]
+            else:
+                # one launch of each kernel per operation: average durations add
+                rows = [(names, sum(sum(v[1] for _, v in found[p]) / len(found[p]) for p in parts))]
+            for label, avg in rows:
+                gbs = float(b) / (avg * 1e-6) / 1e9
+                print("%-44s %12.4g %10.2f %10.1f %8.3f" % (label[:44], float(b), avg, gbs, gbs / 8000.0))
+                out.setdefault(label, {})["algorithmic_bytes"] = float(b)
     if a.json:
         with open(a.json, "w") as fh:
             json.dump({"fetch_scale": sf if a.fetch else None, "write_scale": sw if a.write else None,
