@@ -740,11 +740,16 @@ __global__ __launch_bounds__(BLOCK) void k_update(DevMesh M, DevFields F, DevPar
     Q4 c = ldq(F.cq + k);
     if (FUSED && mode == kUpdateBackup) F.cq0[k] = c;
     double siw = ld_s(F.wsi + k), siu = ld_s(F.uhsi + k), siv = ld_s(F.vhsi + k);
+    const bool stage_si = siw != 0.0;
     bool ok = update_one(c.w, ld_s(F.weu + k), siw, dt);
     ok = update_one(c.uh, ld_s(F.uheu + k), siu, dt) && ok;
     ok = update_one(c.vh, ld_s(F.vheu + k), siv, dt) && ok;
     if (!ok) set_error(st, kErrUpdate);
     if (FUSED && mode == kUpdateSaxpy) saxpy_conserved(c, ldq(F.cq0 + k), a, b);
+    // a stage semi-implicit term is used once, as the update resets it
+    // (quantity.c:769-820); nothing in the loop sets one, so this store only
+    // happens after a caller uploaded one
+    if (FUSED && stage_si) F.wsi[k] = 0.0;
     if (FUSED) {
         protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
         // The centroid values are final for this step (distribute leaves them

@@ -352,6 +352,18 @@ def test_update_error_path_forced_from_host():
     assert exc.value.code == _lib.SW_ERR_UPDATE
 
 
+def test_uploaded_stage_semi_implicit_term_applies_once():
+    """A stage semi-implicit update set by the caller before a fused run is
+    applied by the first step and then reset, as the C update resets it
+    (quantity.c:769-820); the following steps match the oracle's."""
+    d = _dam()
+    stage = d.quantities["stage"].centroid_values.copy()
+    d.quantities["stage"].semi_implicit_update[:] = -0.05 * stage
+    stats = {}
+    _lockstep(d, 4, fused=True, stats=stats)
+    assert np.all(device_array(d, "stage", "semi_implicit_update") == 0.0)
+
+
 def test_device_renumbering_is_a_relabelling():
     """The library's Hilbert renumbering of the triangles changes only where they
     live on the device: every array seen through the ABI, and the timestep
