@@ -1168,8 +1168,9 @@ __global__ __launch_bounds__(kBlock) void k_source_terms(DevMesh M, DevFields F,
     F.grav[k] = gravity_gradient_term(v, wv, c, P.g);
     double fx, fy;
     friction_si(M, F, P, k, c, fx, fy);
-    F.uhsi[k] = fx;
-    F.vhsi[k] = fy;
+    // the forcing adds to what the caller may have set (zero otherwise)
+    F.uhsi[k] += fx;
+    F.vhsi[k] += fy;
 }
 
 // Fused distribute_to_vertices_and_edges after protect: reconstruction

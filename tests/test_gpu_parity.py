@@ -352,13 +352,15 @@ def test_update_error_path_forced_from_host():
     assert exc.value.code == _lib.SW_ERR_UPDATE
 
 
-def test_uploaded_stage_semi_implicit_term_applies_once():
-    """A stage semi-implicit update set by the caller before a fused run is
-    applied by the first step and then reset, as the C update resets it
-    (quantity.c:769-820); the following steps match the oracle's."""
+def test_uploaded_semi_implicit_terms_apply_once():
+    """Semi-implicit updates set by the caller before a fused run: the first
+    step's forcing adds Manning's term to them, the update applies them and
+    resets them, as the C update does (quantity.c:769-820); the following
+    steps match the oracle's."""
     d = _dam()
-    stage = d.quantities["stage"].centroid_values.copy()
-    d.quantities["stage"].semi_implicit_update[:] = -0.05 * stage
+    for q, f in (("stage", -0.05), ("xmomentum", -0.2), ("ymomentum", 0.1)):
+        c = d.quantities[q].centroid_values.copy()
+        d.quantities[q].semi_implicit_update[:] = f * c
     stats = {}
     _lockstep(d, 4, fused=True, stats=stats)
     assert np.all(device_array(d, "stage", "semi_implicit_update") == 0.0)
