@@ -546,3 +546,155 @@ def test_limiter_paths_keep_lake_at_rest_and_mass(limiter):
     v1 = np.sum(area * od.q("stage", "centroid_values"))
     assert abs(v1 - v0) / v0 < 1e-13
     assert od.d.time > 5.0
+
+
+# ------------------------------------------- protect, balance, boundaries
+def _random_state_domain(seed=3, **params):
+    points, tris, boundary = rectangular_cross(8, 6, 80.0, 60.0)
+    d = Domain(mesh=Mesh(points, tris, boundary=boundary), **params)
+    rng = np.random.default_rng(seed)
+    d.set_quantity("elevation", lambda x, y: 0.01 * x + rng.normal(scale=0.3, size=x.shape))
+    d.set_quantity("stage", 0.0)
+    from anuga_cuda_amd.domain import Dirichlet_boundary, Transmissive_boundary
+    d.set_boundary({"left": Reflective_boundary(d), "right": Dirichlet_boundary([1.5, 0.2, -0.1]),
+                    "top": Transmissive_boundary(d), "bottom": Reflective_boundary(d)})
+    od = _oracle(d)
+    N = d.number_of_elements
+    z = od.q("elevation", "centroid_values")
+    # depths from dry to deep, including h <= 0, 0 < h < minimum_allowed_height and h > H0
+    h = rng.choice([-0.01, 0.0, 2e-4, 5e-4, 0.3, 2.0], size=N) * rng.uniform(0.5, 1.5, size=N)
+    od.q("stage", "centroid_values")[:] = z + h
+    od.q("xmomentum", "centroid_values")[:] = rng.normal(scale=0.5, size=N)
+    od.q("ymomentum", "centroid_values")[:] = rng.normal(scale=0.5, size=N)
+    return d, od, h
+
+
+@pytest.mark.parametrize("max_speed", [0.0, 2.0])
+def test_protect_rules(max_speed):
+    """protect (shallow_water_ext.c:1638-1705): h <= 0 -> stage at the bed and
+    momentum 0; 0 < h < minimum_allowed_height -> momentum 0 (speed limit off)
+    or speeds capped at maximum_allowed_speed keeping their signs; deeper
+    cells untouched."""
+    d, od, h = _random_state_domain(maximum_allowed_speed=max_speed)
+    w0, u0, v0 = (od.q(q, "centroid_values").copy() for q in ("stage", "xmomentum", "ymomentum"))
+    z = od.q("elevation", "centroid_values")
+    od.protect()
+    w, u, v = (od.q(q, "centroid_values") for q in ("stage", "xmomentum", "ymomentum"))
+    hc = w0 - z
+    mah = d.minimum_allowed_height
+    dry = hc <= 0
+    shallow = (hc > 0) & (hc < mah)
+    deep = hc >= mah
+    assert dry.any() and shallow.any() and deep.any()
+    assert np.array_equal(w[dry], z[dry]) and np.all(u[dry] == 0) and np.all(v[dry] == 0)
+    assert np.array_equal(w[~dry], w0[~dry])
+    assert np.array_equal(u[deep], u0[deep]) and np.array_equal(v[deep], v0[deep])
+    if max_speed == 0.0:
+        assert np.all(u[shallow] == 0) and np.all(v[shallow] == 0)
+    else:
+        for new, old in ((u, u0), (v, v0)):
+            sp, sp0 = new[shallow] / hc[shallow], old[shallow] / hc[shallow]
+            assert np.all(np.abs(sp) <= max_speed * (1 + 1e-12))
+            assert np.all(np.sign(sp) == np.sign(sp0))
+            keep = np.abs(sp0) <= max_speed
+            assert np.array_equal(new[shallow][keep], old[shallow][keep])
+
+
+def test_balance_keeps_vertex_depths_above_H0_and_centroid_velocity():
+    """balance_deep_and_shallow with tight slope limiters
+    (shallow_water_ext.c:1462-1636): where a vertex depth falls below H0 and
+    the centroid is deeper than H0, the vertex depths become a convex blend
+    toward the centroid depth that leaves every vertex at least H0 deep, and
+    the vertex momenta carry the centroid velocity (use_centroid_velocities);
+    triangles whose vertices are all at least H0 deep are left alone."""
+    d, od, _ = _random_state_domain(seed=5)
+    rng = np.random.default_rng(6)
+    N = d.number_of_elements
+    zc = od.q("elevation", "centroid_values")
+    zv = od.q("elevation", "vertex_values")
+    hc = rng.uniform(0.0005, 0.5, size=N)
+    od.q("stage", "centroid_values")[:] = zc + hc
+    hv0 = hc[:, None] * rng.uniform(-0.5, 2.0, size=(N, 3))
+    od.q("stage", "vertex_values")[:] = zv + hv0
+    od.q("xmomentum", "vertex_values")[:] = rng.normal(size=(N, 3))
+    od.q("ymomentum", "vertex_values")[:] = rng.normal(size=(N, 3))
+    before = {q: od.q(q, "vertex_values").copy() for q in ("stage", "xmomentum", "ymomentum")}
+    od.balance_deep_and_shallow()
+    hv = od.q("stage", "vertex_values") - zv
+    H0 = d.H0
+    fix = (hv0.min(axis=1) < H0) & (hc > H0)
+    untouched = hv0.min(axis=1) >= H0
+    assert fix.sum() > 10 and untouched.sum() > 10
+    assert np.all(hv[fix] >= H0 * (1 - 1e-9))
+    lo = np.minimum(hv0, hc[:, None])[fix] - 1e-12
+    hi = np.maximum(hv0, hc[:, None])[fix] + 1e-12
+    assert np.all((hv[fix] >= lo) & (hv[fix] <= hi))  # between its old depth and the centroid's
+    uc = od.q("xmomentum", "centroid_values") / hc
+    changed = fix & np.any(hv != hv0, axis=1)
+    np.testing.assert_allclose(od.q("xmomentum", "vertex_values")[changed],
+                               uc[changed, None] * hv[changed], rtol=1e-12, atol=1e-15)
+    for q in before:
+        assert np.array_equal(od.q(q, "vertex_values")[untouched], before[q][untouched])
+
+
+def test_boundary_values_by_kind():
+    """update_boundary (evaluate_segment.cu:3-119): reflective edges negate the
+    normal momentum and keep the tangential one and the stage (velocities as
+    the reference's formula has them);
+    Dirichlet edges take the given stage and momenta; transmissive edges copy
+    the inside edge state."""
+    d, od, _ = _random_state_domain(seed=7)
+    od.update_other_quantities()
+    od.distribute_to_vertices_and_edges()
+    od.update_other_quantities()
+    od.update_boundary()
+    m = d.mesh
+    cells, edges = m.boundary_cells, m.boundary_edges
+    tags = m.boundary_tags
+    n = np.stack([m.normals[cells, 2 * edges], m.normals[cells, 2 * edges + 1]], axis=1)
+    t = np.stack([-n[:, 1], n[:, 0]], axis=1)
+
+    def inside(q):
+        return od.q(q, "edge_values")[cells, edges]
+
+    def bnd(q):
+        return od.q(q, "boundary_values")
+
+    refl = np.isin(tags, ["left", "bottom"])
+    # momentum: normal component negated, tangential kept; velocity: the
+    # reference's formula keeps the normal component and negates the
+    # tangential one (evaluate_segment.cu:71-76; the flux never reads it)
+    for (a, b), sn, st in ((("xmomentum", "ymomentum"), -1.0, 1.0), (("xvelocity", "yvelocity"), 1.0, -1.0)):
+        e = np.stack([inside(a), inside(b)], axis=1)
+        o = np.stack([bnd(a), bnd(b)], axis=1)
+        assert np.abs(np.sum(e * n, axis=1)[refl]).max() > 1e-3  # a non-trivial check
+        np.testing.assert_allclose(np.sum(o * n, axis=1)[refl], sn * np.sum(e * n, axis=1)[refl], atol=1e-13)
+        np.testing.assert_allclose(np.sum(o * t, axis=1)[refl], st * np.sum(e * t, axis=1)[refl], atol=1e-13)
+    assert np.array_equal(bnd("stage")[refl], inside("stage")[refl])
+    dirich = tags == "right"
+    assert np.all(bnd("stage")[dirich] == 1.5) and np.all(bnd("xmomentum")[dirich] == 0.2)
+    assert np.all(bnd("ymomentum")[dirich] == -0.1)
+    trans = tags == "top"
+    for q in ("stage", "xmomentum", "ymomentum", "height", "xvelocity", "yvelocity"):
+        assert np.array_equal(bnd(q)[trans], inside(q)[trans])
+
+
+def test_other_quantities_closed_form():
+    """update_other_quantities (update_centroids_of_velocities_and_height.cu:1-52):
+    height = max(w - z, 0); velocity = momentum h / (h^2 + 1e-8), i.e. the
+    momentum over the depth for depths well above 1e-4 and 0 for a dry cell;
+    edge and vertex values are the centroid values (first order)."""
+    d, od, h = _random_state_domain(seed=9)
+    od.update_other_quantities()
+    hc = od.q("height", "centroid_values")
+    w, z = od.q("stage", "centroid_values"), od.q("elevation", "centroid_values")
+    assert np.array_equal(hc, np.maximum(w - z, 0.0))
+    uh = od.q("xmomentum", "centroid_values")
+    u = od.q("xvelocity", "centroid_values")
+    deep = hc > 0.1
+    np.testing.assert_allclose(u[deep], uh[deep] / hc[deep], rtol=1e-6)
+    assert np.all(u[hc == 0] == 0)
+    for q in ("height", "xvelocity", "yvelocity"):
+        c = od.q(q, "centroid_values")
+        assert np.array_equal(od.q(q, "edge_values"), np.repeat(c[:, None], 3, axis=1))
+        assert np.array_equal(od.q(q, "vertex_values"), np.repeat(c[:, None], 3, axis=1))
