@@ -15,7 +15,11 @@ in numpy 2 and the file uses the Python-2 C API), so the oracle is pinned by:
     4-triangle known-answer case (src/anuga_cuda/anuga_gpu/mytest.py:17-31);
   * the rk2 / rk3 steps: the reference kernel's saxpy (saxpy_centroid_values.cu:11),
     the observed temporal order of accuracy (1, 2, 3 for Euler, rk2, rk3 on a
-    smooth wave at fixed timesteps), lake at rest and mass conservation.
+    smooth wave at fixed timesteps), lake at rest and mass conservation;
+  * the per-quantity limiters: exact linear reconstruction, neighbour bounds,
+    beta = 0, lake at rest and mass;
+  * protect's rules, the balance's H0 property, the other quantities' closed
+    form and the boundary values by kind.
 """
 import copy
 import math
