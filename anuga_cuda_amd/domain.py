@@ -709,7 +709,8 @@ class Domain:
                int(finaltime is not None))
         self._host_stale = True
         self._sync_scalars()
-        if self.store:
+        if self.store and self.writer is None:
+            # a later evolve() call continues the same file
             self.initialise_storage()
         if not skip_initial_step:
             if self.store:

@@ -165,3 +165,20 @@ def test_sww_records_match_oracle_at_yields(tmp_path, smooth):
     print("bit-identical records: %d of %d" % (exact, 3 * len(expected)))
     wet = got["vars"]["stage"][-1] > got["vars"]["elevation"]
     assert wet.any() and (~wet).any()  # a dry region stays, stored as the bed
+
+
+@pytest.mark.gpu
+def test_sww_file_continues_across_evolve_calls(tmp_path):
+    """A second evolve() continues the simulation and appends to the same
+    file (its first yield, at the time the first call ended, is stored
+    again, as ANUGA stores every yield)."""
+    d = scenarios.dam_break_channel(8, 8)
+    d.set_name("cont")
+    d.set_datadir(str(tmp_path))
+    d.set_store(True)
+    t1 = list(d.evolve(yieldstep=0.5, finaltime=1.0))
+    t2 = list(d.evolve(yieldstep=0.5, finaltime=2.0))
+    d.close()
+    got = _read(tmp_path / "cont.sww")
+    np.testing.assert_array_equal(got["vars"]["time"], t1 + t2)
+    assert got["vars"]["stage"].shape[0] == len(t1) + len(t2)
