@@ -36,6 +36,7 @@ __all__ = [
     "Dirichlet_boundary",
     "Transmissive_boundary",
     "DEFAULT_PARAMETERS",
+    "create_domain_from_file",
 ]
 
 DEFAULT_PARAMETERS = dict(
@@ -325,6 +326,8 @@ class Domain:
         self.smooth = True
         self.minimum_storable_height = 1.0e-5
         self.writer = None
+        # coordinates are relative to this origin (ANUGA's Geo_reference)
+        self.geo_reference = {"xllcorner": 0.0, "yllcorner": 0.0, "zone": -1}
         self.max_speed = np.zeros(self.number_of_elements)
 
     # ------------------------------------------------------------------ params
@@ -438,7 +441,8 @@ class Domain:
         if self.datadir and not os.path.isdir(self.datadir):
             os.makedirs(self.datadir, exist_ok=True)
         self.writer = SWWWriter(self, self._dev(), sww_path(self.datadir, self.name), smooth=self.smooth,
-                                minimum_storable_height=self.minimum_storable_height)
+                                minimum_storable_height=self.minimum_storable_height,
+                                georeference=self.geo_reference)
 
     def store_timestep(self):
         """Append the current state to the SWW file (gpu_domain_advanced.py:2167-2172)."""
@@ -755,3 +759,25 @@ class Domain:
             self._pull()
             self._handle.close()
             self._handle = None
+
+
+def create_domain_from_file(filename, DomainClass=None, build="host", **parameters):
+    """ANUGA's create_domain_from_file, which the reference's Merimbula
+    scenario calls (tests/CUDA/merimbula/merimbula.py:86): a domain on the
+    mesh of a ``.tsh`` or NetCDF ``.msh`` file (optionally gzip-compressed),
+    boundary edges tagged by the file's segments ('exterior' where untagged),
+    coordinates kept relative to the file's georeference, and each vertex
+    attribute column whose title names a quantity set as that quantity's
+    vertex values.  ``build`` ('host' or 'device') picks where the mesh
+    geometry is built; ``parameters`` go to the domain."""
+    from .mesh import read_mesh
+
+    points, triangles, segments, extra = read_mesh(filename, full=True)
+    mesh = Mesh(points, triangles, segments=segments, build=build)
+    domain = (DomainClass or Domain)(mesh=mesh, **parameters)
+    domain.geo_reference.update(extra.get("georeference") or {})
+    attributes = extra.get("vertex_attributes")
+    for i, title in enumerate(extra.get("vertex_attribute_titles") or []):
+        if title in domain.quantities and attributes is not None:
+            domain.set_quantity(title, attributes[:, i][mesh.triangles], location="vertices")
+    return domain

@@ -134,10 +134,12 @@ def read_tsh(path, full=False):
     vtx = np.loadtxt(io.StringIO("\n".join(lines[pos:pos + nv])), ndmin=2)
     pos += nv
     points = vtx[:, 1:3].astype(np.float64)
+    attributes = vtx[:, 3:3 + nattr].astype(np.float64) if nattr else None
     # vertex attribute titles
     while lines[pos].startswith("#"):
         pos += 1
-    pos += nattr if nattr else 0
+    titles = [lines[pos + i].strip() for i in range(nattr)]
+    pos += nattr
     fields = header()
     ntri = int(fields[0])
     tri_rows = [lines[pos + t].split() for t in range(ntri)]
@@ -153,7 +155,17 @@ def read_tsh(path, full=False):
         a, b = int(r[1]), int(r[2])
         segments[(min(a, b), max(a, b))] = tag
     if full:
-        return points, triangles, segments, {"triangle_neighbors": neighbors, "georeference": {}}
+        geo = {}
+        for i, line in enumerate(lines):
+            if line.strip().lower() == "#geo reference":
+                vals = [x.strip() for x in lines[i + 1:i + 4]]
+                try:
+                    geo = {"zone": int(vals[0]), "xllcorner": float(vals[1]), "yllcorner": float(vals[2])}
+                except (ValueError, IndexError):
+                    geo = {}
+                break
+        return points, triangles, segments, {"triangle_neighbors": neighbors, "georeference": geo,
+                                             "vertex_attributes": attributes, "vertex_attribute_titles": titles}
     return points, triangles, segments
 
 
@@ -187,7 +199,14 @@ def read_msh(path, full=False):
             for i, (a, b) in enumerate(segs.tolist()):
                 tag = b"".join(tags[i]).decode("utf-8", "replace").strip("\0 ") if tags is not None else ""
                 segments[(min(a, b), max(a, b))] = tag
-        extras = {"triangle_neighbors": np.array(v["triangle_neighbors"].data, dtype=np.int64)
+        attributes, titles = None, []
+        if "vertex_attributes" in v:
+            attributes = np.array(v["vertex_attributes"].data, dtype=np.float64).reshape(len(points), -1)
+            if "vertex_attribute_titles" in v:
+                titles = [b"".join(r).decode("utf-8", "replace").strip("\0 ")
+                          for r in v["vertex_attribute_titles"].data]
+        extras = {"vertex_attributes": attributes, "vertex_attribute_titles": titles,
+                  "triangle_neighbors": np.array(v["triangle_neighbors"].data, dtype=np.int64)
                   if "triangle_neighbors" in v else None,
                   "georeference": {k: (a.decode() if isinstance(a, bytes) else a.item() if hasattr(a, "item") else a)
                                    for k, a in f._attributes.items()

@@ -226,8 +226,9 @@ class SWWWriter:
                  "order": int(domain.default_order), "starttime": float(domain.starttime)}
         attrs.update(geo)
         self.nc = NetCDF3Writer(path, dims, variables, attrs)
-        self.nc.write("x", xy[:, 0] - geo["xllcorner"])
-        self.nc.write("y", xy[:, 1] - geo["yllcorner"])
+        # the domain's coordinates are already relative to its georeference
+        self.nc.write("x", xy[:, 0])
+        self.nc.write("y", xy[:, 1])
         self.nc.write("volumes", volumes)
         self.values = np.empty((4, P), dtype=np.float32)
         handle.sww_values(P, self.minimum_storable_height, self.values)

@@ -159,3 +159,29 @@ def test_read_msh_cairns_fixture():
     m = Mesh(p, t, segments=segs)
     tags, counts = np.unique(m.boundary_tags, return_counts=True)
     assert dict(zip(tags.tolist(), counts.tolist())) == {"bottom": 25, "ocean_east": 35, "onshore": 20, "top": 35}
+
+
+def test_create_domain_from_file_on_the_reference_meshes(tmp_path):
+    """create_domain_from_file (the reference's Merimbula scenario builds its
+    domain with it): the mesh, boundary tags and georeference of the file;
+    vertex attribute columns become quantities."""
+    from anuga_cuda_amd import create_domain_from_file, scenarios
+
+    d = create_domain_from_file(os.path.join(ROOT, "tests", "golden", "merimbula_43200.tsh.gz"))
+    ref = scenarios.merimbula()
+    assert np.array_equal(d.mesh.triangles, ref.mesh.triangles)
+    assert d.mesh.get_tag_names() == ref.mesh.get_tag_names()
+    assert d.geo_reference == {"xllcorner": 0.0, "yllcorner": 0.0, "zone": -1}
+    c = create_domain_from_file(os.path.join(ROOT, "tests", "golden", "cairns.msh.gz"))
+    assert c.number_of_elements == 17831 and c.mesh.get_tag_names() == ["bottom", "ocean_east", "onshore", "top"]
+    assert abs(c.geo_reference["xllcorner"] - 301278.51) < 1e-9 and c.geo_reference["zone"] == -1
+    # a small .tsh with an elevation attribute column
+    lines = ["4 1 # verts, attributes", "0 0.0 0.0 1.5", "1 1.0 0.0 2.5", "2 0.0 1.0 3.5", "3 1.0 1.0 4.5",
+             "# attribute column titles", "elevation", "2 # triangles", "0 0 1 2 -1 1 -1", "1 1 3 2 -1 0 -1",
+             "1 # segments", "0 0 1 bottom", "#geo reference", "56", "100.0", "200.0"]
+    f = tmp_path / "tiny.tsh"
+    f.write_text("\n".join(lines) + "\n")
+    t = create_domain_from_file(str(f))
+    np.testing.assert_array_equal(t.quantities["elevation"].vertex_values, [[1.5, 2.5, 3.5], [2.5, 4.5, 3.5]])
+    assert t.geo_reference == {"xllcorner": 100.0, "yllcorner": 200.0, "zone": 56}
+    assert sorted(t.mesh.get_tag_names()) == ["bottom", "exterior"]
