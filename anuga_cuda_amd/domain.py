@@ -326,6 +326,7 @@ class Domain:
         self.smooth = True
         self.minimum_storable_height = 1.0e-5
         self.writer = None
+        self.quantities_to_be_stored = {"elevation": 1, "stage": 2, "xmomentum": 2, "ymomentum": 2}
         # coordinates are relative to this origin (ANUGA's Geo_reference)
         self.geo_reference = {"xllcorner": 0.0, "yllcorner": 0.0, "zone": -1}
         self.max_speed = np.zeros(self.number_of_elements)
@@ -403,8 +404,21 @@ class Domain:
         if self._handle is not None:
             self._handle.set_params(self._sw_params())
 
-    def set_quantities_to_be_stored(self, *args, **kwargs):
-        return None
+    def set_quantities_to_be_stored(self, q):
+        """ANUGA's set_quantities_to_be_stored: None stops storing; otherwise a
+        dict of quantity name -> 1 (static, stored once) or 2 (every yield).
+        The SWW path stores elevation (static) and any of stage, xmomentum,
+        ymomentum (every yield); other quantities raise."""
+        if q is None:
+            self.store = False
+            return
+        allowed = {"elevation": (1,), "stage": (2,), "xmomentum": (2,), "ymomentum": (2,)}
+        for name, flag in dict(q).items():
+            if name not in allowed or flag not in allowed[name]:
+                raise NotImplementedError(
+                    "the SWW output stores elevation (1) and stage / xmomentum / ymomentum (2); got %r: %r"
+                    % (name, flag))
+        self.quantities_to_be_stored = dict(q)
 
     def set_name(self, name):
         self.name = name[:-4] if name.endswith(".sww") else name
@@ -440,9 +454,10 @@ class Domain:
             self.writer.close()
         if self.datadir and not os.path.isdir(self.datadir):
             os.makedirs(self.datadir, exist_ok=True)
+        dynamic = [n for n, f in self.quantities_to_be_stored.items() if f == 2]
         self.writer = SWWWriter(self, self._dev(), sww_path(self.datadir, self.name), smooth=self.smooth,
                                 minimum_storable_height=self.minimum_storable_height,
-                                georeference=self.geo_reference)
+                                georeference=self.geo_reference, quantities=dynamic)
 
     def store_timestep(self):
         """Append the current state to the SWW file (gpu_domain_advanced.py:2167-2172)."""

@@ -185,9 +185,14 @@ class SWWWriter:
     QUANTITIES = ("stage", "xmomentum", "ymomentum")
 
     def __init__(self, domain, handle, path, smooth=True, minimum_storable_height=1.0e-5,
-                 georeference=None):
+                 georeference=None, quantities=QUANTITIES):
         mesh = domain.mesh
         N = mesh.number_of_elements
+        unknown = set(quantities) - set(self.QUANTITIES)
+        if unknown:
+            raise ValueError("SWW dynamic quantities are stage, xmomentum, ymomentum; got %s" % sorted(unknown))
+        # stored every yield, in the file's order
+        self.dynamic = [q for q in self.QUANTITIES if q in quantities]
         self.handle = handle
         self.path = path
         self.smooth = bool(smooth)
@@ -214,10 +219,10 @@ class SWWWriter:
                      ("elevation", ("number_of_points",), np.float32, None),
                      ("elevation_range", ("numbers_in_range",), np.float32, None),
                      ("volumes", ("number_of_volumes", "number_of_vertices"), np.int32, None)]
-        for q in self.QUANTITIES:
+        for q in self.dynamic:
             variables.append((q + "_range", ("numbers_in_range",), np.float32, None))
         variables.append(("time", ("number_of_timesteps",), np.float64, None))
-        for q in self.QUANTITIES:
+        for q in self.dynamic:
             variables.append((q, ("number_of_timesteps", "number_of_points"), np.float32, None))
         attrs = {"institution": "Geoscience Australia",
                  "description": "Output from anuga.abstract_2d_finite_volumes suitable for plotting",
@@ -236,12 +241,14 @@ class SWWWriter:
         self.nc.write("elevation", z)
         self.nc.write("elevation_range", [z.min(), z.max()] if P else [0.0, 0.0])
         big = np.finfo(np.float32).max
-        self.ranges = {q: [big, -big] for q in self.QUANTITIES}
+        self.ranges = {q: [big, -big] for q in self.dynamic}
 
     def store_timestep(self, time):
         v = self.handle.sww_values(self.P, self.minimum_storable_height, self.values)
         rec = {"time": float(time)}
         for i, q in enumerate(self.QUANTITIES):
+            if q not in self.dynamic:
+                continue
             a = v[i + 1]
             rec[q] = a
             if a.size:

@@ -182,3 +182,28 @@ def test_sww_file_continues_across_evolve_calls(tmp_path):
     got = _read(tmp_path / "cont.sww")
     np.testing.assert_array_equal(got["vars"]["time"], t1 + t2)
     assert got["vars"]["stage"].shape[0] == len(t1) + len(t2)
+
+
+def test_quantities_to_be_stored_validation():
+    d = scenarios.dam_break_channel(4, 4)
+    d.set_store(True)
+    d.set_quantities_to_be_stored(None)
+    assert d.store is False
+    with pytest.raises(NotImplementedError):
+        d.set_quantities_to_be_stored({"friction": 1})
+    d.set_quantities_to_be_stored({"elevation": 1, "stage": 2})
+    assert d.quantities_to_be_stored == {"elevation": 1, "stage": 2}
+
+
+@pytest.mark.gpu
+def test_sww_stores_only_the_requested_quantities(tmp_path):
+    d = scenarios.dam_break_channel(8, 8)
+    d.set_name("stage_only")
+    d.set_datadir(str(tmp_path))
+    d.set_store(True)
+    d.set_quantities_to_be_stored({"elevation": 1, "stage": 2})
+    times = list(d.evolve(yieldstep=0.5, finaltime=1.0))
+    d.close()
+    got = _read(tmp_path / "stage_only.sww")
+    assert "stage" in got["vars"] and "xmomentum" not in got["vars"] and "ymomentum_range" not in got["vars"]
+    assert got["vars"]["stage"].shape[0] == len(times)
