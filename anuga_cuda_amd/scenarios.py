@@ -70,9 +70,11 @@ def fbm(x, y, seed=2024, octaves=5, amplitude=20.0, base_wavelength=5000.0, mode
 
 def delaunay_terrain(npoints=500000, order="morton", seed=12345, terrain_seed=2024,
                      extent=10000.0, friction=0.025, base_stage=5.0, mound=3.0,
-                     mound_radius=500.0, precision="fp64", **params):
+                     mound_radius=500.0, precision="fp64", build="host", **params):
+    """``build`` ('host' or 'device'): where the mesh geometry is built (the
+    same arrays either way)."""
     points, tris = delaunay_random(npoints, extent=(extent, extent), seed=seed)
-    mesh = Mesh(points, tris)
+    mesh = Mesh(points, tris, build=build)
     if order == "morton":
         mesh = mesh.morton_renumbered()
     elif order == "random":

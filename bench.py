@@ -112,6 +112,8 @@ def main():
                     help="fv: step of the timed loop (a step of rk2 / rk3 runs 2 / 3 stages)")
     ap.add_argument("--limiter", default="optimised", choices=["optimised", "vertex", "edge"],
                     help="fv: distribute path (vertex: optimised_gradient_limiter off; edge: use_edge_limiter)")
+    ap.add_argument("--mesh-build", default="host", choices=["host", "device"],
+                    help="fv: build the mesh geometry on the GPU (sw_build_mesh) or the host; same arrays")
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=64)
@@ -149,7 +151,7 @@ def main():
     t0 = time.time()
     limiter_kw = {"vertex": {"optimised_gradient_limiter": 0}, "edge": {"use_edge_limiter": 1}}.get(args.limiter, {})
     d = scenarios.delaunay_terrain(npoints=args.points, order=args.order, precision=args.precision,
-                                   locality_order=args.locality, **limiter_kw)
+                                   locality_order=args.locality, build=args.mesh_build, **limiter_kw)
     d.set_timestepping_method(args.timestepping)
     stages = {"euler": 1, "rk2": 2, "rk3": 3}[args.timestepping]
     N = d.number_of_elements
@@ -251,7 +253,7 @@ def main():
         "config": {"workload": "plain FV, seeded Delaunay %d points in 10 km x 10 km, %s order, fBm terrain "
                                "20 m, wet/dry mound, Manning 0.025, transmissive" % (args.points, args.order),
                    "triangles": N, "precision": args.precision, "order": 2, "timestepping": args.timestepping,
-                   "limiter": args.limiter,
+                   "limiter": args.limiter, "mesh_build": args.mesh_build,
                    "device_numbering": args.locality, "workgroup_sizes": list(launch)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
