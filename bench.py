@@ -123,7 +123,7 @@ def main():
                     help="also run a plain device copy of MIB MiB (scale for rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (each GPU box has one MI355X); "
+        print("bench.py: runs on one MI355X only (the multi-GPU sharding is a design, not built); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -2,7 +2,7 @@
 reader), ANUGA's inverted triangle structure and node smoothing, and (GPU) the
 stored records against the oracle's state at the same yields.
 
-Pins: the NetCDF layout is checked by a reader that is not this build's
+Pins: the NetCDF layout is checked by an independent reader
 (scipy.io.netcdf_file); the node structure by ANUGA's documented example on
 the reference's 4-triangle mesh (src/anuga_cuda/anuga_gpu/mytest.py:12-31);
 the smoothing by a linear field (every triangle holds the node's exact value)
