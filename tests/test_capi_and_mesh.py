@@ -185,3 +185,13 @@ def test_create_domain_from_file_on_the_reference_meshes(tmp_path):
     np.testing.assert_array_equal(t.quantities["elevation"].vertex_values, [[1.5, 2.5, 3.5], [2.5, 4.5, 3.5]])
     assert t.geo_reference == {"xllcorner": 100.0, "yllcorner": 200.0, "zone": 56}
     assert sorted(t.mesh.get_tag_names()) == ["bottom", "exterior"]
+
+
+def test_rocprof_summary_base_names():
+    from anuga_cuda_amd.rocprof_summary import base_name
+
+    assert base_name("_ZN3gnn11k_mp_directEllPKiPKDF16bS3_") == "k_mp_direct"
+    assert base_name("gnn::k_apply") == "k_apply"
+    assert base_name("gnn::k_apply_groups") == "k_apply_groups"
+    assert base_name("void sw::k_flux<double, true, 256>(sw::DevMesh, sw::DevFields)") == "k_flux"
+    assert base_name("(anonymous namespace)::k_mesh_pairs(long long, int const*)") == "k_mesh_pairs"
