@@ -75,7 +75,9 @@ typedef struct sw_params {
      * SW_LIMITER_VERTEX (per-quantity extrapolate_second_order_and_limit_by_vertex),
      * SW_LIMITER_EDGE (use_edge_limiter: ..._limit_by_edge) */
     int32_t limiter;
-    /* Quantity.beta of stage, xmomentum, ymomentum for the per-quantity limiters */
+    /* Quantity.beta of stage, xmomentum, ymomentum for the per-quantity
+     * limiters (ANUGA's default 1.0; 0 limits to first order); unused by
+     * SW_LIMITER_OPTIMISED */
     double quantity_beta[3];
 } sw_params;
 
