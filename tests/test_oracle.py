@@ -702,3 +702,36 @@ def test_other_quantities_closed_form():
         c = od.q(q, "centroid_values")
         assert np.array_equal(od.q(q, "edge_values"), np.repeat(c[:, None], 3, axis=1))
         assert np.array_equal(od.q(q, "vertex_values"), np.repeat(c[:, None], 3, axis=1))
+
+
+def test_balance_without_tight_slope_limiters():
+    """balance_deep_and_shallow's alpha_balance path (tight_slope_limiters
+    off, shallow_water_ext.c:1462-1636): alpha = min(1, alpha_balance * hmin /
+    dz) clamped at 0, dz the largest bed deviation of a vertex from the
+    centroid; every vertex depth becomes a blend between its old depth and the
+    centroid's; triangles on a flat bed or with hmin >= dz / alpha_balance are
+    unchanged."""
+    d, od, _ = _random_state_domain(seed=13, tight_slope_limiters=0)
+    rng = np.random.default_rng(14)
+    N = d.number_of_elements
+    zc = od.q("elevation", "centroid_values")
+    zv = od.q("elevation", "vertex_values")
+    hc = rng.uniform(0.01, 0.5, size=N)
+    od.q("stage", "centroid_values")[:] = zc + hc
+    hv0 = hc[:, None] * rng.uniform(0.0, 2.0, size=(N, 3))
+    od.q("stage", "vertex_values")[:] = zv + hv0
+    before = od.q("stage", "vertex_values").copy()
+    od.balance_deep_and_shallow()
+    hv = od.q("stage", "vertex_values") - zv
+    dz = np.max(np.abs(zv - zc[:, None]), axis=1)
+    hmin = hv0.min(axis=1)
+    keep = (dz == 0) | (d.alpha_balance * hmin >= dz)
+    assert keep.sum() > 5 and (~keep).sum() > 5
+    assert np.array_equal(od.q("stage", "vertex_values")[keep], before[keep])
+    lo = np.minimum(hv0, hc[:, None]) - 1e-12
+    hi = np.maximum(hv0, hc[:, None]) + 1e-12
+    assert np.all((hv >= lo) & (hv <= hi))
+    alpha = np.clip(d.alpha_balance * hmin / np.where(dz > 0, dz, 1.0), 0.0, 1.0)
+    moved = ~keep & (np.abs(hc[:, None] - hv0) > 1e-9).all(axis=1)
+    frac = (hv - hc[:, None]) / (hv0 - hc[:, None])  # = alpha for a blend toward the centroid
+    np.testing.assert_allclose(frac[moved], np.repeat(alpha[moved, None], 3, axis=1), rtol=1e-9)
