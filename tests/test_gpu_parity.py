@@ -391,16 +391,18 @@ def test_device_renumbering_is_a_relabelling():
     assert np.array_equal(a[2], b[2])
 
 
-@pytest.mark.parametrize("method", ["euler", "rk2"])
-def test_mixed_precision_error_against_oracle(method):
+@pytest.mark.parametrize("method,limiter", [("euler", "optimised"), ("rk2", "optimised"), ("euler", "edge")])
+def test_mixed_precision_error_against_oracle(method, limiter):
     """Mixed precision (FP32 flux and reconstruction arithmetic, FP32 edge
     state; FP64 accumulation, timestep and depth tests) on the 24 x 24 cell dam
     break, against the FP64 oracle at the same simulated time (t = 40 s, 28
     Euler steps).  Bounds: relative L2 error of the centroid values below 1e-6
     for stage and 1e-4 for the momenta (measured with Euler: 2.9e-8, 3.2e-7,
     3.8e-6); wet/dry masks differ in at most 1% of the cells (measured: none).
-    The same bounds hold for rk2 steps against the oracle's rk2."""
-    d = _dam(precision="mixed")
+    The same bounds hold for rk2 steps against the oracle's rk2, and for the
+    edge limiter (computed in FP64 inside the mixed loop) against the
+    oracle's."""
+    d = _dam(precision="mixed", **({"use_edge_limiter": 1} if limiter == "edge" else {}))
     d.set_timestepping_method(method)
     od = OracleDomain.from_domain(d)
     T = 40.0
