@@ -10,6 +10,10 @@
 // FP64.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include <cmath>
 
 #include "../../include/anuga_gnn.h"
@@ -1055,19 +1059,18 @@ int gnn_linear_fwd(void *stream, int64_t M, int K, int Nout, const void *X, cons
 // second round while every wavefront owns the same number of tiles.
 extern "C++" template <typename Kernel>
 unsigned resident_blocks(Kernel kernel) {
-    constexpr int kMaxDevices = 64;
-    static unsigned cached[kMaxDevices] = {};
+    static std::mutex lock;
+    static std::map<std::pair<const void *, int>, unsigned> cached;  // (kernel, device)
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
-    if (!cached[dev]) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kB, 0) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-            cus = 1;
-        cached[dev] = (unsigned)(per_cu * cus);
-    }
-    return cached[dev];
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const auto key = std::make_pair(reinterpret_cast<const void *>(kernel), dev);
+    std::lock_guard<std::mutex> guard(lock);
+    auto it = cached.find(key);
+    if (it != cached.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kB, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    return cached[key] = (unsigned)(per_cu * cus);
 }
 
 inline unsigned persist_blocks(int64_t rows, unsigned resident = kPersistMax) {
