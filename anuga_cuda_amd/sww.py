@@ -136,19 +136,22 @@ class NetCDF3Writer:
             out.append(struct.pack(">iiq", _NC_TYPE[v["dtype"]], v["vsize"], v["begin"]))
         return b"".join(out)
 
-    def _bytes(self, v, array):
+    def _write_var(self, v, array):
+        """Writes one variable's values (big-endian, padded to its vsize) at
+        the current file position, without an intermediate bytes copy."""
         a = np.ascontiguousarray(array, dtype=v["dtype"])
         if a.size * a.itemsize != v["nbytes"]:
             raise ValueError("%s: %d values for shape %s" % (v["name"], a.size, v["shape"]))
-        b = a.astype(v["dtype"].newbyteorder(">"), copy=False).tobytes()
-        return b + b"\0" * (v["vsize"] - len(b))
+        self.f.write(memoryview(a.astype(v["dtype"].newbyteorder(">"), copy=False)).cast("B"))
+        if v["vsize"] > v["nbytes"]:
+            self.f.write(b"\0" * (v["vsize"] - v["nbytes"]))
 
     def write(self, name, array):
         v = self.vars[name]
         if v["record"]:
             raise ValueError("%s is a record variable (append_record)" % name)
         self.f.seek(v["begin"])
-        self.f.write(self._bytes(v, array))
+        self._write_var(v, array)
 
     def append_record(self, values):
         rec = [v for v in self.vars.values() if v["record"]]
@@ -156,7 +159,8 @@ class NetCDF3Writer:
         if missing:
             raise ValueError("record without %s" % ", ".join(missing))
         self.f.seek(self.rec_begin + self.numrecs * self.recsize)
-        self.f.write(b"".join(self._bytes(v, values[v["name"]]) for v in rec))
+        for v in rec:
+            self._write_var(v, values[v["name"]])
         self.numrecs += 1
         self.f.seek(4)
         self.f.write(struct.pack(">i", self.numrecs))
