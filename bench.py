@@ -87,7 +87,7 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="fv", choices=["fv", "ensemble", "learned"],
+    ap.add_argument("--workload", default="fv", choices=["fv", "ensemble", "learned", "merimbula"],
                     help="fv: the 1M-triangle plain FV step (default); ensemble: the batched ensemble; "
                          "learned: train the edge-GNN surrogate on FV trajectories and roll it out at 1M")
     ap.add_argument("--delta", type=float, default=0.0625, help="learned: seconds per learned step")
@@ -135,6 +135,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "merimbula":
+        return bench_merimbula(args)
 
     if args.copy_calibration:
         import torch
@@ -265,6 +267,74 @@ def main():
     }
     print(json.dumps(out), flush=True)
     d.close()
+
+
+# The reference's only published timing of this path: its Merimbula scenario
+# (43,200 triangles, reflective walls) evolved to 500 s in 1,301 Euler steps in
+# 2.124 s on a GTX 480 (docs/profilling/CUDA/profile_on_GTX480_prepared_500fs_2.123959s,
+# the time in the file name), i.e. 43,200 x 1,301 / 2.124 s triangle-steps/s.
+REFERENCE_MERIMBULA_GTX480 = 43200 * 1301 / 2.123959
+
+
+def bench_merimbula(args):
+    """The reference's Merimbula scenario (tests/CUDA/merimbula/merimbula.py:50,
+    86-173: yields every 50 s) evolved to 500 s through Domain.evolve, wall
+    clock of the whole evolve (graph capture, yields and host syncs
+    included).  The first run warms the process; the second is reported."""
+    from anuga_cuda_amd import scenarios
+
+    cpu = None
+    if not args.no_cpu:
+        # the serial FP64 C oracle over the same run, on one host core
+        from oracle.oracle import OracleDomain
+
+        od = OracleDomain.from_domain(scenarios.merimbula())
+        c0 = time.perf_counter()
+        csteps = 0
+        for _ in od.evolve(50.0, 500.0):
+            csteps += od.d.number_of_steps
+        c_el = time.perf_counter() - c0
+        cpu = {"value": 43200 * csteps / c_el, "unit": "triangle-steps/s", "cores": 1, "kind": "port",
+               "sample": "the whole run (%d steps to 500 s) with the serial FP64 C oracle, %.2f s (nproc %d)" % (
+                   csteps, c_el, os.cpu_count() or 0)}
+        log("cpu baseline: %d steps in %.2f s (%.3e triangle-steps/s)" % (csteps, c_el, cpu["value"]))
+    runs = []
+    for rep in range(2):
+        d = scenarios.merimbula()
+        d._dev()
+        t0 = time.perf_counter()
+        steps = 0
+        for _ in d.evolve(yieldstep=50.0, finaltime=500.0):
+            steps += d.number_of_steps
+        elapsed = time.perf_counter() - t0
+        runs.append((elapsed, steps, d.get_time()))
+        log("merimbula run %d: %d steps to t = %.1f s in %.4f s (%.3e triangle-steps/s)" % (
+            rep, steps, d.get_time(), elapsed, d.number_of_elements * steps / elapsed))
+        N = d.number_of_elements
+        d.close()
+    elapsed, steps, t_end = runs[-1]
+    value = N * steps / elapsed
+    out = {
+        "metric": "triangle-steps/s on the reference's Merimbula scenario (43,200 triangles, to 500 s)",
+        "value": value,
+        "unit": "triangle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": 1000.0 * elapsed / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REFERENCE_MERIMBULA_GTX480,
+        "dtype": "f64",
+        "data": "the reference's own mesh fixture (merimbula_43200.tsh) and initial condition",
+        "config": {"workload": "Merimbula, %d triangles, reflective walls, yields every 50 s to t = %.0f s, "
+                               "whole Domain.evolve wall time; vs_baseline against the reference's 2.124 s "
+                               "(1,301 steps) on a GTX 480" % (N, t_end), "triangles": N,
+                   "evolve_seconds": elapsed, "reference_gtx480_seconds": 2.123959},
+        "roofline": None,  # 43,200 triangles sit in the 256 MB last-level cache: % of HBM is not meaningful
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
 
 
 def bench_ensemble(args):
