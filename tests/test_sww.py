@@ -207,3 +207,32 @@ def test_sww_stores_only_the_requested_quantities(tmp_path):
     got = _read(tmp_path / "stage_only.sww")
     assert "stage" in got["vars"] and "xmomentum" not in got["vars"] and "ymomentum_range" not in got["vars"]
     assert got["vars"]["stage"].shape[0] == len(times)
+
+
+@pytest.mark.gpu
+def test_reference_merimbula_script_end_to_end(tmp_path):
+    """The reference's Merimbula script (tests/CUDA/merimbula/merimbula.py:86-173)
+    on this package: create_domain_from_file, the stage bump, reflective
+    boundaries, SWW storage, yields every 50 s.  The steps to 50 s are the
+    reference's published 153, and the stored stage equals the oracle's."""
+    import os
+
+    from anuga_cuda_amd import GPU_domain, Reflective_boundary, create_domain_from_file
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    d = create_domain_from_file(os.path.join(root, "tests", "golden", "merimbula_43200.tsh.gz"), GPU_domain)
+    d.set_quantity("stage", lambda x, y: 2.0 * ((x > 756000.0) & (x < 756500.0)) + 1.0)
+    d.set_boundary({tag: Reflective_boundary(d) for tag in d.mesh.get_tag_names()})
+    d.set_datadir(str(tmp_path))
+    d.set_name("merimbula_new")
+    d.set_store(True)
+    expected = _oracle_records(d, 50.0, 100.0, True)
+    steps = []
+    for t in d.evolve(yieldstep=50, finaltime=100):
+        steps.append(d.number_of_steps)
+    d.close()
+    assert steps[1] == 153
+    got = _read(tmp_path / "merimbula_new.sww")
+    assert got["vars"]["stage"].shape == (3, len(d.mesh.points))
+    for i, (t, e) in enumerate(expected):
+        np.testing.assert_allclose(got["vars"]["stage"][i], e[1], rtol=1e-6, atol=1e-6)
