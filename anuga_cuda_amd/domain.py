@@ -359,10 +359,6 @@ class Domain:
     def set_CFL(self, cfl):
         self.CFL = float(cfl)
 
-    def set_beta(self, beta):
-        for n in ("beta_w", "beta_w_dry", "beta_uh", "beta_uh_dry", "beta_vh", "beta_vh_dry"):
-            setattr(self, n, float(beta))
-
     def set_minimum_allowed_height(self, h):
         self.minimum_allowed_height = float(h)
         self.H0 = float(h)
