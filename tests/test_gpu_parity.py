@@ -520,3 +520,47 @@ def test_autotuned_launch_shapes_do_not_change_results():
         for loc in ("centroid_values", "vertex_values", "edge_values"):
             assert np.array_equal(getattr(a.quantities[q], loc), getattr(b.quantities[q], loc)), (q, loc)
     assert a._handle.get_state().time == b._handle.get_state().time
+
+
+NON_DEFAULT = {
+    "sloped_mannings": dict(use_sloped_mannings=1),
+    "speed_cap": dict(maximum_allowed_speed=2.0),
+    "alpha_balance": dict(tight_slope_limiters=0),
+    "momentum_balance": dict(use_centroid_velocities=0),
+    "momentum_extrapolation": dict(extrapolate_velocity_second_order=0),
+    "no_dry_skip": dict(optimise_dry_cells=0),
+    "beta_dry": dict(beta_w_dry=0.2, beta_uh_dry=0.1, beta_vh_dry=0.3, beta_w=1.1, beta_uh=0.8, beta_vh=0.95),
+}
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "unfused"])
+@pytest.mark.parametrize("case", sorted(NON_DEFAULT))
+def test_non_default_parameter_branches_match_oracle(case, fused):
+    """The HIP branches behind ANUGA's non-default parameters, in lockstep
+    with the oracle for 16 steps on the 3,000-point wet/dry Delaunay workload
+    (the bar of the other lockstep tests: 1e-12 per step on centroid, vertex
+    and edge values, timesteps, wet/dry masks): sloped Manning friction
+    (shallow_water_ext.c:1272-1317), the speed cap of protect (:1638-1705),
+    the alpha_balance balance and the momentum blend without centroid
+    velocities (:1462-1636), momentum extrapolated directly, no dry-cell skip
+    (:2717-3324) and beta_*_dry different from beta_* (:2966-2974).  The
+    speed-cap case starts from momenta of 5 m/s where the water is shallow,
+    so the cap acts."""
+    d = _delaunay("random")
+    for k, v in NON_DEFAULT[case].items():
+        setattr(d, k, v)
+    if case == "speed_cap":
+        w, z = d.quantities["stage"].centroid_values, d.quantities["elevation"].centroid_values
+        h = w - z
+        shallow = (h > 0) & (h < 0.1)
+        uh = d.quantities["xmomentum"].centroid_values.copy()
+        uh[shallow] = 5.0 * h[shallow]
+        d.set_quantity("xmomentum", uh, location="centroids")
+        assert shallow.sum() > 20
+    stats = {}
+    od, dts = _lockstep(d, 16, fused=fused, stats=stats)
+    assert len(dts) == 16
+    assert all(getattr(od.d, k) == v for k, v in NON_DEFAULT[case].items() if hasattr(od.d, k))
+    compare(d._handle.max_speed(), od.max_speed, "max_speed", TOL)
+    print("%s (%s): bit-mismatching fields %s" % (case, "fused" if fused else "unfused",
+                                                  {k: v[0] for k, v in stats.items() if v[0]}))
