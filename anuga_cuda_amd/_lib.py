@@ -21,6 +21,7 @@ SW_PRECISION_FP64, SW_PRECISION_MIXED = 0, 1
 SW_ORDER_KEEP, SW_ORDER_HILBERT = 0, 1
 SW_LIMITER_OPTIMISED, SW_LIMITER_VERTEX, SW_LIMITER_EDGE = 0, 1, 2
 SW_EVENT_NONE, SW_EVENT_YIELD, SW_EVENT_FINAL = 0, 1, 2
+SW_STEP_AUTO, SW_STEP_FOUR, SW_STEP_TWO = 0, 1, 2
 
 SW_ERR_UPDATE = -1
 SW_ERR_SMALLSTEPS = -2
@@ -130,6 +131,8 @@ _SIGNATURES = {
     "sw_build_mesh": ([C.c_int, C.c_int64, _F64P, C.c_int64, _I64P, C.c_void_p], C.c_int),
     "sw_mesh_last_error": ([], C.c_char_p),
     "sw_kernel_times": ([C.c_void_p, _F64P, _I64P, C.c_int32], C.c_int),
+    "sw_set_step_path": ([C.c_void_p, C.c_int32], C.c_int),
+    "sw_get_step_path": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
 }
 
 
@@ -418,6 +421,16 @@ class Handle:
         chosen = (C.c_int32 * 3)()
         self.call("sw_autotune", int(steps), chosen)
         return tuple(chosen)
+
+    def set_step_path(self, path):
+        """SW_STEP_AUTO (two-kernel step where it applies) or SW_STEP_FOUR."""
+        self.call("sw_set_step_path", int(path))
+
+    def step_path(self):
+        """The step sw_evolve_steps would run now: SW_STEP_TWO or SW_STEP_FOUR."""
+        p = C.c_int32()
+        self.call("sw_get_step_path", C.byref(p))
+        return p.value
 
     def kernel_times(self, n=4):
         ms = np.zeros(n)

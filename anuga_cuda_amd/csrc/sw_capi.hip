@@ -19,6 +19,7 @@
 
 #include "../../include/anuga_sw.h"
 #include "sw_kernels.h"
+#include "sw_step2.h"
 
 using namespace sw;
 
@@ -74,6 +75,13 @@ struct sw_handle {
     int *dev_of = nullptr;      // [sum n] device index (within its instance) of caller triangle k
     int blk[3] = {kBlock, kBlock, kBlock};  // workgroup size of the fused flux, update, distribute kernels
     std::vector<SwwPoints> sww;             // [n_inst] output points
+    // two-kernel Euler step (sw_step2.h)
+    StepFields sf{};
+    int *nbw = nullptr;           // [3][N] neighbour words of k_flux_v
+    bool step2_geometry = false;  // caller's geometry equals tri_geometry and shared normals are negations
+    int step_path = SW_STEP_AUTO; // caller's choice
+    int vpar = 0;                 // parity of the next two-kernel step
+    int graph_path = 0;           // step path the captured graph holds
 };
 
 namespace {
@@ -405,6 +413,39 @@ int enqueue_fused_step(sw_handle *h, int slot) {
     return launch_check(h, "fused step");
 }
 
+// Whether sw_evolve_steps runs the two-kernel step (sw_step2.h) on the
+// states `d` of the instances.
+bool step2_applies(const sw_handle *h, const std::vector<DevState> &d) {
+    if (h->step_path == SW_STEP_FOUR || !h->step2_geometry) return false;
+    if (h->params.precision != SW_PRECISION_FP64 || rk_stages(h->params) != 1) return false;
+    if (h->params.limiter != SW_LIMITER_OPTIMISED) return false;
+    for (const auto &s : d)
+        if (!s.bed_interp) return false;
+    return true;
+}
+
+// Enqueues one two-kernel step at parity h->vpar; the timing slots are
+// flux (k_flux_v) and distribute (k_advance).
+int enqueue_step2(sw_handle *h, int slot) {
+    auto rec = [&](int idx) {
+        if (slot >= 0) (void)hipEventRecord(h->ev[slot * (kNumTimed + 1) + idx], h->stream);
+    };
+    const unsigned g = grid_for(h->N);
+    rec(0);
+    hipLaunchKernelGGL((k_flux_v<kBlock>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf, h->nbw,
+                       h->dp, h->vpar);
+    rec(1);
+    rec(2);
+    rec(3);
+    hipLaunchKernelGGL((k_advance<kBlock>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf, h->dp,
+                       h->vpar);
+    rec(4);
+    h->vpar ^= 1;
+    return launch_check(h, "two-kernel step");
+}
+
+int enqueue_step(sw_handle *h, bool two, int slot) { return two ? enqueue_step2(h, slot) : enqueue_fused_step(h, slot); }
+
 // Accumulates the kernel durations of event groups [0, nslots).  A kernel that
 // returned at once because the loop was halted still counts as a launch.
 int collect_timing(sw_handle *h, int nslots) {
@@ -422,11 +463,18 @@ int collect_timing(sw_handle *h, int nslots) {
     return SW_OK;
 }
 
-int build_graph(sw_handle *h) {
-    if (h->graph_exec) return SW_OK;
+void drop_graph(sw_handle *h);
+
+// Captures kGraphSteps steps of the given path (an even number, so a
+// two-kernel graph starts and ends at parity 0).
+int build_graph(sw_handle *h, bool two) {
+    const int path = two ? SW_STEP_TWO : SW_STEP_FOUR;
+    if (h->graph_exec && h->graph_path == path) return SW_OK;
+    drop_graph(h);
+    h->graph_path = path;
     HIPCHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = SW_OK;
-    for (int s = 0; s < kGraphSteps && rc == SW_OK; s++) rc = enqueue_fused_step(h, -1);
+    for (int s = 0; s < kGraphSteps && rc == SW_OK; s++) rc = enqueue_step(h, two, -1);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &graph);
     if (rc) return rc;
@@ -515,7 +563,8 @@ bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t 
                    std::vector<EdgeRec> &edgev, std::vector<EdgeRecF> &edgefv, std::vector<int> &flagsv,
                    std::vector<int> &surv, std::vector<double> &vcv, std::vector<double> &ccv,
                    std::vector<double> &rav, std::vector<int> &cal_of, std::vector<int> &dev_of,
-                   std::vector<int> &bcellv, std::vector<int> &bedgev, bool &own_normals) {
+                   std::vector<int> &bcellv, std::vector<int> &bedgev, bool &own_normals,
+                   std::vector<int> &nbwv, bool &geom_same) {
     const int64_t N = I.n, Nb = I.nb;
     auto bad = [&](const char *fmt, long long a, long long b = 0, long long c = 0) {
         char buf[256];
@@ -560,11 +609,13 @@ bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t 
                 if (b >= Nb || m->boundary_cells[b] != k || m->boundary_edges[b] != i)
                     return bad("neighbours[%lld] = %lld does not match boundary entry %lld", ki, n, b);
                 nbslot[i] = (int)(-(I.boff + b) - 1);
+                nbwv[i * Ntot + d] = nbslot[i];
             } else {
                 const int64_t e = m->neighbour_edges[ki];
                 if (n >= N || e < 0 || e > 2 || m->neighbours[3 * n + e] != k)
                     return bad("neighbour structure is not symmetric at edge %lld", ki);
                 nbslot[i] = (int)(e * Ntot + I.off + loc[n]);
+                nbwv[i * Ntot + d] = (int)((e << 29) | (I.off + loc[n]));
                 if (n < k) {
                     flags |= 1 << i;
                     computing = n;
@@ -608,6 +659,22 @@ bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t 
         }
         flagsv[d] = flags;
         for (int i = 0; i < 3; i++) edgev[i * Ntot + d].tflags = flags;
+        if (geom_same) {
+            // the two-kernel step recomputes the geometry from the vertex coordinates
+            double x[3], y[3];
+            for (int i = 0; i < 3; i++) {
+                x[i] = m->vertex_coordinates[6 * k + 2 * i];
+                y[i] = m->vertex_coordinates[6 * k + 2 * i + 1];
+            }
+            TriGeom g;
+            tri_geometry(x, y, g);
+            bool same = g.radius == m->radii[k] && g.area == m->areas[k] && g.cx == m->centroid_coordinates[2 * k] &&
+                        g.cy == m->centroid_coordinates[2 * k + 1];
+            for (int i = 0; i < 3; i++)
+                same = same && g.nx[i] == m->normals[6 * k + 2 * i] && g.ny[i] == m->normals[6 * k + 2 * i + 1] &&
+                       g.len[i] == m->edgelengths[3 * k + i];
+            geom_same = same;
+        }
         rav[2 * d] = m->radii[k];
         rav[2 * d + 1] = m->areas[k];
         ccv[2 * d] = m->centroid_coordinates[2 * k];
@@ -658,10 +725,11 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     std::vector<int> flagsv(Ntot, 0), surv(3 * Ntot, 0), cal_of(Ntot, 0), dev_of(Nsum, 0);
     std::vector<double> vcv(6 * Ntot, 0.0), ccv(2 * Ntot, 0.0), rav(2 * Ntot, 0.0);
     std::vector<int> bcellv(std::max<int64_t>(Nbtot, 1), 0), bedgev(std::max<int64_t>(Nbtot, 1), 0);
-    bool own_normals = true;
+    bool own_normals = true, geom_same = true;
+    std::vector<int> nbwv(3 * Ntot, -1);
     for (int i = 0; i < n_inst; i++)
         if (!pack_instance(&meshes[i], p, inst[i], Ntot, i, edgev, edgefv, flagsv, surv, vcv, ccv, rav, cal_of,
-                           dev_of, bcellv, bedgev, own_normals))
+                           dev_of, bcellv, bedgev, own_normals, nbwv, geom_same))
             return nullptr;
     const int64_t nwaves = (Ntot + kInstAlign - 1) / kInstAlign * (kInstAlign / 64);
     std::vector<int> wave_inst(nwaves, n_inst - 1), inst_off(n_inst), inst_end(n_inst);
@@ -747,6 +815,16 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         dalloc(h, &F.wave_min, (size_t)nwaves) || dalloc(h, &F.eqf, 3 * N) || dalloc(h, &h->staging, 3 * Nmax))
         return bail();
     if (dalloc(h, &h->st, n_inst)) return bail();
+    h->step2_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
+    {
+        StepFields &S = h->sf;
+        if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.cqb[1], N) ||
+            dalloc(h, &S.sir[0], N) || dalloc(h, &S.sir[1], N) || dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
+            dalloc(h, &S.errw, n_inst) || dalloc(h, &S.stv[1], n_inst))
+            return bail();
+        S.cqb[0] = F.cq;
+        S.stv[0] = h->st;
+    }
     std::vector<DevState> s0(n_inst);
     for (auto &d : s0) {
         d = DevState{};
@@ -1356,7 +1434,14 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     int64_t launched = 0;
     int rc = SW_OK;
     const bool mixed = h->params.precision == SW_PRECISION_MIXED;
-    if (max_steps > 0) {
+    const bool two = step2_applies(h, d0);
+    if (max_steps > 0 && two) {
+        h->vpar = 0;
+        hipLaunchKernelGGL(k_step_reset, dim3(h->n_inst), dim3(kSlots), 0, h->stream, h->sf, h->n_inst);
+        hipLaunchKernelGGL(k_step_begin, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf,
+                           h->dp);
+        rc = launch_check(h, "k_step_begin");
+    } else if (max_steps > 0) {
         if (mixed) {
             const unsigned g3 = grid_for(3 * h->N);
             hipLaunchKernelGGL(k_check_bed, dim3(g3), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->st);
@@ -1373,18 +1458,26 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         int64_t this_batch = max_steps - launched < batch ? max_steps - launched : batch;
         int64_t done_here = 0;
         if (h->timing) {
-            for (; done_here < this_batch && rc == SW_OK; done_here++)
-                rc = enqueue_fused_step(h, (int)done_here);
+            for (; done_here < this_batch && rc == SW_OK; done_here++) rc = enqueue_step(h, two, (int)done_here);
             if (rc == SW_OK) rc = collect_timing(h, (int)done_here);
         } else {
-            rc = build_graph(h);
-            for (; rc == SW_OK && done_here + kGraphSteps <= this_batch; done_here += kGraphSteps) {
+            // a two-kernel graph is captured and replayed at parity 0
+            if (two && h->vpar && this_batch > kGraphSteps) rc = enqueue_step(h, two, -1), done_here++;
+            const bool graphs = !two || h->vpar == 0;
+            if (rc == SW_OK && graphs) rc = build_graph(h, two);
+            for (; rc == SW_OK && graphs && done_here + kGraphSteps <= this_batch; done_here += kGraphSteps) {
                 hipError_t e = hipGraphLaunch(h->graph_exec, h->stream);
                 if (e != hipSuccess) rc = fail(h, SW_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
             }
-            for (; rc == SW_OK && done_here < this_batch; done_here++) rc = enqueue_fused_step(h, -1);
+            for (; rc == SW_OK && done_here < this_batch; done_here++) rc = enqueue_step(h, two, -1);
         }
         launched += done_here;
+        if (two) {
+            // the next step's state, errors merged, is also the handle's state
+            hipLaunchKernelGGL(k_step_settle, dim3((h->n_inst + 63) / 64), dim3(64), 0, h->stream, h->sf, h->n_inst,
+                               h->vpar);
+            if (rc == SW_OK) rc = launch_check(h, "k_step_settle");
+        }
         if (rc) break;
         rc = sync_all(h, d);
         if (rc) break;
@@ -1396,6 +1489,15 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     if (mixed && launched > 0) {
         // the mixed loop keeps FP32 edge state: rebuild the FP64 edge records
         hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
+    }
+    if (two && max_steps > 0) {
+        // the two-kernel step keeps vertex, update and double-buffered centroid
+        // records: back into the arrays of the operator API
+        hipLaunchKernelGGL(k_step_end, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf,
+                           h->vpar);
+        h->vpar = 0;
+        if (int rc4 = launch_check(h, "k_step_end"))
+            if (rc == SW_OK) rc = rc4;
     }
     int rc2 = epilogue(h);
     int rc3 = sync_all(h, d);
@@ -1504,6 +1606,25 @@ int sw_autotune(sw_handle *h, int32_t steps, int32_t *chosen) {
     if (chosen)
         for (int k = 0; k < 3; k++) chosen[k] = h->blk[k];
     return rc;
+}
+
+int sw_set_step_path(sw_handle *h, int32_t path) {
+    if (!h) return SW_ERR_ARG;
+    if (path != SW_STEP_AUTO && path != SW_STEP_FOUR)
+        return fail(h, SW_ERR_ARG, "step path %d (SW_STEP_AUTO or SW_STEP_FOUR)", path);
+    h->step_path = path;
+    drop_graph(h);
+    return SW_OK;
+}
+
+int sw_get_step_path(sw_handle *h, int32_t *path) {
+    if (!h || !path) return SW_ERR_ARG;
+    std::vector<DevState> d;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    d.resize(h->n_inst);
+    HIPCHECK(h, hipMemcpy(d.data(), h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost));
+    *path = step2_applies(h, d) ? SW_STEP_TWO : SW_STEP_FOUR;
+    return SW_OK;
 }
 
 int sw_synchronize(sw_handle *h) {

@@ -302,8 +302,15 @@ SW_DEV void extrap_one(R dq1, R dx2, R dy2, const R dxv[3], const R dyv[3], R be
 }
 
 // quantity.c:769-820 (_update) for one triangle.  Returns false when the
-// semi-implicit denominator is <= 0 (the C returns -1 there).
+// semi-implicit denominator is <= 0 (the C returns -1 there).  With si = 0 the
+// C's steps reduce to q + dt eu exactly (si / q = +-0, denominator 1 - dt * +-0
+// = 1, q / 1 = q), so that case skips the two divisions; si is left for the
+// caller to reset, as every caller does.
 SW_DEV bool update_one(double &q, double eu, double &si, double dt) {
+    if (si == 0.0) {
+        q += dt * eu;
+        return true;
+    }
     double x = q;
     if (x == 0.0)
         si = 0.0;
@@ -311,6 +318,21 @@ SW_DEV bool update_one(double &q, double eu, double &si, double dt) {
         si /= x;
     q += dt * eu;
     double denominator = 1.0 - dt * si;
+    if (denominator <= 0.0) return false;
+    q /= denominator;
+    return true;
+}
+
+// The semi-implicit term divided by the centroid value, the first loop of
+// quantity.c:769-820 (0 for a zero centroid value).
+SW_DEV double si_ratio(double si, double x) { return (x == 0.0) ? 0.0 : si / x; }
+
+// update_one with that division already done (sr = si_ratio(si, q)): the same
+// operations, in the same order.
+SW_DEV bool update_ratio(double &q, double eu, double sr, double dt) {
+    q += dt * eu;
+    if (sr == 0.0) return true;  // denominator 1 - dt * +-0 = 1
+    const double denominator = 1.0 - dt * sr;
     if (denominator <= 0.0) return false;
     q /= denominator;
     return true;

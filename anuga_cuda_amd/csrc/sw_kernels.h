@@ -123,6 +123,9 @@ struct DevState {
     long long has_finaltime, event, error;
     // device-only
     long long halt, total_steps, bed_interp;
+    // two-kernel step (sw_step2.h): whether a caller-set stage semi-implicit
+    // term is still to be applied; steps it ran in the current evolve call
+    long long wsi_live, call_steps;
 };
 
 enum : long long { kErrNone = 0, kErrUpdate = 1, kErrSmallsteps = 2, kErrNeighbour = 4, kErrDiscontinuous = 8 };
@@ -836,6 +839,17 @@ SW_DEV void load_rec(const DevMesh &M, const DevFields &F, int k, bool second_or
     }
 }
 
+// Previous vertex values of triangle k from the vertex arrays.
+struct VertexArrays {
+    const DevFields &F;
+    int N, k;
+    SW_DEV void operator()(int i, double &w, double &u, double &v) const {
+        w = F.wv[i * N + k];
+        u = F.uhv[i * N + k];
+        v = F.vhv[i * N + k];
+    }
+};
+
 // Second-order reconstruction of one conserved quantity of triangle k with the
 // per-quantity limiters (extrapolate_second_order_and_limit_by_vertex /
 // _by_edge, extrapolate_second_order_and_limit_by_vertex_or_edge.cu:694-800):
@@ -924,9 +938,11 @@ SW_DEV void reconstruct_limited_all(const DevParams &P, int k, const RecIn &r, d
 
 // extrapolate_second_order_sw for triangle k (shallow_water_ext.c:2717-3324),
 // result in wv/uv/vv (momentum).  Returns false on the missing-neighbour error.
-template <typename R>
-SW_DEV bool extrapolate_one(const DevFields &F, const DevParams &P, int N, int k, const RecIn &r, double wv[3],
-                            double uv[3], double vv[3]) {
+// stale(i, w, u, v) gives the previous vertex values of vertex i (a dry
+// triangle keeps them).
+template <typename R, typename Stale>
+SW_DEV bool extrapolate_one(const DevParams &P, int k, const RecIn &r, double wv[3], double uv[3], double vv[3],
+                            Stale stale) {
     const Q4 &ck = r.c;
     const int nob = nob_of(r.flags);
     double wc, uc, vc;
@@ -967,11 +983,7 @@ SW_DEV bool extrapolate_one(const DevFields &F, const DevParams &P, int N, int k
                 if (P.optimise_dry_cells && hmax < P.epsilon) {
                     // dry: vertex values keep their previous contents
 #pragma unroll
-                    for (int i = 0; i < 3; i++) {
-                        wv[i] = F.wv[i * N + k];
-                        uv[i] = F.uhv[i * N + k];
-                        vv[i] = F.vhv[i * N + k];
-                    }
+                    for (int i = 0; i < 3; i++) stale(i, wv[i], uv[i], vv[i]);
                 } else {
                     const R inv_area2 = R(1.0) / R(area2);
                     R d[3];
@@ -1053,7 +1065,7 @@ __global__ __launch_bounds__(kBlock) void k_extrapolate_second_order(DevMesh M, 
     RecIn r;
     load_rec(M, F, k, true, r);
     double wv[3], uv[3], vv[3];
-    if (!extrapolate_one<R>(F, P, N, k, r, wv, uv, vv)) set_error(st, kErrNeighbour);
+    if (!extrapolate_one<R>(P, k, r, wv, uv, vv, VertexArrays{F, N, k})) set_error(st, kErrNeighbour);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         F.wv[i * N + k] = wv[i];
@@ -1201,7 +1213,7 @@ __global__ __launch_bounds__(BLOCK) void k_distribute(DevMesh M, DevFields F, De
     } else if (LIMITED) {
         reconstruct_limited_all(P, k, r, wv, uv, vv);
     } else {
-        if (!extrapolate_one<R>(F, P, N, k, r, wv, uv, vv)) set_error(st, kErrNeighbour);
+        if (!extrapolate_one<R>(P, k, r, wv, uv, vv, VertexArrays{F, N, k})) set_error(st, kErrNeighbour);
     }
     BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
     balance_one(c.w, c.z, wv, r.zv, c.uh, c.vh, uv, vv, bp);

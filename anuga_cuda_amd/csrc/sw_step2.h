@@ -1,0 +1,546 @@
+// Two-kernel Euler step of the FP64 loop (extrapolate_second_order_sw path).
+//
+// Per step:
+//   k_flux_v    compute_fluxes + boundary values on the fly + gravity_wb + the
+//               CFL minimum (one non-returning atomic per workgroup into one of
+//               kSlots slots of the instance);
+//   k_advance   update_timestep and the time bookkeeping (every wavefront
+//               reduces the slots itself: no separate launch), then
+//               update_conserved_quantities + protect of the triangle and of
+//               its three surrogate neighbours (recomputed from the same
+//               inputs, so bitwise what the neighbours compute for
+//               themselves), extrapolate_second_order_sw +
+//               balance_deep_and_shallow, the next step's Manning friction and
+//               stage-gradient gravity term.
+//
+// What goes through HBM per triangle-step (FP64):
+//   k_flux_v   reads vertex coordinates 48, neighbour words 12, flags 4, the
+//              triangle's three vertex records 96, gravity term 16; writes the
+//              update record 32 (208 B);
+//   k_advance  reads centroid record 32, update record 32, semi-implicit pair
+//              16, Manning n 8, vertex coordinates 48, surrogates 12, flags 4,
+//              bed vertex values 24; writes centroid record 32, semi-implicit
+//              pair 16, vertex records 96, gravity term 16 (336 B).
+// Neighbour gathers (vertex records in k_flux_v; centroid, update and
+// semi-implicit records and centroids in k_advance) come from the L2 / last-level
+// cache: the locality order keeps neighbours in nearby workgroups.
+//
+// Differences from the four-kernel step of sw_kernels.h, none of which changes
+// a result bit:
+//   * no edge records: k_flux_v forms edge values from the vertex records
+//     (e_i = (v_{i+1} + v_{i+2}) / 2, the interpolation of quantity.c:687-707),
+//     so the edge state is never written; the edge bed is the interpolation of
+//     the vertex bed (required: bed_interp);
+//   * normals, edge lengths, inscribed radius, area and centroid are
+//     recomputed from the vertex coordinates by tri_geometry, the formulas the
+//     mesh is built with; sw_create checks on the host that they reproduce the
+//     caller's arrays bit for bit and otherwise keeps the four-kernel step;
+//   * centroid records, semi-implicit pairs and the scalar state are read
+//     from buffer p and written to buffer p ^ 1, p the step parity (a kernel
+//     argument: step i of a captured graph uses parity i % 2), so a workgroup
+//     never reads a neighbour's value that another workgroup has already
+//     advanced; a halted step copies them through.  Every load is requested
+//     before the step's state is examined (a halted step computes and stores
+//     nothing), so a wavefront waits for two memory round trips.  Errors found
+//     by lanes go to a separate word per instance (errw) that is merged into
+//     the state at the end of each batch of steps.
+#pragma once
+
+#include "sw_kernels.h"
+
+namespace sw {
+
+// CFL slots per instance and parity, one 128-byte line apart.
+constexpr int kSlots = 64;
+constexpr int kSlotStride = 16;  // unsigned long longs between slots
+constexpr int kSlotBlock = kSlots * kSlotStride;
+
+struct StepFields {
+    Q4 *vr;                   // [3][N] vertex records {w, uh, vh, z}
+    Q4 *eur;                  // [N] {stage, xmom, ymom explicit update, max_speed}
+    Q4 *cqb[2];               // [N] centroid records; cqb[0] is DevFields::cq
+    double2 *sir[2];          // [N] {xmom, ymom semi-implicit update} / centroid value (si_ratio)
+    unsigned long long *slot; // [2][n_inst][kSlotBlock] CFL minima (bits of positive doubles)
+    long long *errw;          // [n_inst] errors found by lanes
+    DevState *stv[2];         // [n_inst] scalar state; stv[0] is the handle's state
+};
+
+// Triangle geometry from its vertex coordinates, by the formulas of the mesh
+// builder: centroid ((x0 + x1) + x2) / 3; area |(x1 y0 - x0 y1) + (x2 y1 - x1
+// y2) + (x0 y2 - x2 y0)| / 2; edge i from vertex i+1 to vertex i+2 with length
+// l = sqrt(ex^2 + ey^2) and outward normal (ey / l, -(ex / l)); radius the
+// smallest centroid-to-edge-midpoint distance.  Host and device evaluate the
+// same operations (no FMA contraction), so sw_create can check the caller's
+// arrays against it.
+struct TriGeom {
+    double nx[3], ny[3], len[3];
+    double radius, area, cx, cy;
+};
+__host__ __device__ inline double geo_min(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ inline void tri_geometry(const double x[3], const double y[3], TriGeom &g) {
+    g.cx = ((x[0] + x[1]) + x[2]) / 3.0;
+    g.cy = ((y[0] + y[1]) + y[2]) / 3.0;
+    g.area = fabs(((x[1] * y[0] - x[0] * y[1]) + (x[2] * y[1] - x[1] * y[2])) + (x[0] * y[2] - x[2] * y[0])) / 2.0;
+    double d[3];
+    for (int i = 0; i < 3; i++) {
+        const int a = (i + 1) % 3, b = (i + 2) % 3;
+        const double ex = x[b] - x[a], ey = y[b] - y[a];
+        const double l = sqrt(ex * ex + ey * ey);
+        g.len[i] = l;
+        g.nx[i] = ey / l;
+        g.ny[i] = -(ex / l);
+        const double mx = (x[a] + x[b]) / 2.0, my = (y[a] + y[b]) / 2.0;
+        const double dx = g.cx - mx, dy = g.cy - my;
+        d[i] = sqrt(dx * dx + dy * dy);
+    }
+    g.radius = geo_min(geo_min(d[0], d[1]), d[2]);
+}
+
+SW_DEV unsigned long long wave_min_u64(unsigned long long b) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(b, off, 64);
+        b = (o < b) ? o : b;
+    }
+    return b;
+}
+
+// Slot ks of instance inst at parity p.
+SW_DEV unsigned long long *cfl_slot(const StepFields &S, int n_inst, int p, int inst, int ks) {
+    return S.slot + ((size_t)(p * n_inst + inst) * kSlotBlock + (size_t)ks * kSlotStride);
+}
+
+// The instance of workgroup (logical block) b and the bounds of its triangles.
+struct BlockInst {
+    int inst, off, end;
+};
+template <int BLOCK>
+SW_DEV BlockInst block_inst(const DevMesh &M, int b) {
+    const int k0 = b * BLOCK;
+    if (M.n_inst == 1) return BlockInst{0, 0, M.N};
+    const int inst = M.wave_inst[k0 >> 6];
+    return BlockInst{inst, M.inst_off[inst], M.inst_end[inst]};
+}
+
+// Edge state {w, uh, vh, z} of the edge between two vertex records.
+SW_DEV Q4 edge_of(const Q4 &a, const Q4 &b) {
+    return Q4{0.5 * (a.w + b.w), 0.5 * (a.uh + b.uh), 0.5 * (a.vh + b.vh), 0.5 * (a.z + b.z)};
+}
+SW_DEV int next3(int i) { return i == 2 ? 0 : i + 1; }
+SW_DEV int prev3(int i) { return i == 0 ? 2 : i - 1; }
+
+// compute_fluxes (shallow_water_ext.c:4519-4696) with boundary values and
+// gravity_wb, as k_flux<double, true>, from vertex records.  nbw[i*N + k]
+// encodes the neighbour across edge i: (m << 29) | n for edge m of triangle n
+// (N < 2^29), or -(b+1) for boundary edge b.
+#ifndef SW_FLUX_MINWAVES
+#define SW_FLUX_MINWAVES 1
+#endif
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, DevFields F, StepFields S, const int *nbw,
+                                                  DevParams P, int p) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BlockInst bi = block_inst<BLOCK>(M, b);
+    const DevState *st = S.stv[p] + bi.inst;
+    // a halted step (k_advance carries the state over) stores nothing
+    const bool halted = st->event || st->error || st->halt || S.errw[bi.inst];
+    const int N = M.N;
+    const int k = b * BLOCK + (int)threadIdx.x;
+    double dtk = P.evolve_max_timestep;
+    if (k < bi.end) {
+        double x[3], y[3];
+        Q4 vq[3], na[3], nc[3];
+        int nb[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double2 v = ld_s(M.vc + i * N + k);
+            x[i] = v.x;
+            y[i] = v.y;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) vq[i] = ldq(S.vr + i * N + k);
+#pragma unroll
+        for (int i = 0; i < 3; i++) nb[i] = ld_s(nbw + i * N + k);
+        const int flags = ld_s(M.flags + k);
+        const double2 gr = ld_s(F.grav + k);
+        // the two vertex records of the neighbour's matching edge (a boundary
+        // edge reads its own, which are not used)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            int n = k, m = i;
+            if (nb[i] >= 0) {
+                n = nb[i] & 0x1FFFFFFF;
+                m = nb[i] >> 29;
+            }
+            na[i] = ldq(S.vr + next3(m) * N + n);
+            nc[i] = ldq(S.vr + prev3(m) * N + n);
+        }
+        TriGeom gm;
+        tri_geometry(x, y, gm);
+        const double eps = P.epsilon, g = P.g;
+        const int last_edge = (flags >> 10) & 3;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0;
+        bool have_last = false;
+        double sidex = 0.0, sidey = 0.0, cfl_speed = 0.0, max_speed_total = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const Q4 own = edge_of(vq[next3(i)], vq[prev3(i)]);
+            const bool lower = (flags >> i) & 1;
+            const double lk = gm.len[i];
+            {
+                const double hh = own.w - own.z;
+                const double fact = -0.5 * g * hh * hh * lk;
+                sidex = sidex + fact * gm.nx[i];
+                sidey = sidey + fact * gm.ny[i];
+            }
+            double n1 = gm.nx[i], n2 = gm.ny[i];
+            double ql[3], qr[3], zl, zr;
+            if (nb[i] >= 0) {
+                const Q4 nbq = edge_of(na[i], nc[i]);
+                if (lower) {
+                    // the neighbour evaluates this edge in the C: its state on
+                    // the left, its normal (the negation of ours), its length
+                    ql[0] = nbq.w; ql[1] = nbq.uh; ql[2] = nbq.vh; zl = nbq.z;
+                    qr[0] = own.w; qr[1] = own.uh; qr[2] = own.vh; zr = own.z;
+                    n1 = -n1;
+                    n2 = -n2;
+                } else {
+                    ql[0] = own.w; ql[1] = own.uh; ql[2] = own.vh; zl = own.z;
+                    qr[0] = nbq.w; qr[1] = nbq.uh; qr[2] = nbq.vh; zr = nbq.z;
+                }
+            } else {
+                const int m = -nb[i] - 1;
+                ql[0] = own.w; ql[1] = own.uh; ql[2] = own.vh; zl = own.z;
+                zr = zl;
+                const int kind = M.bkind[m];
+                double b0 = ql[0], b1 = ql[1], b2 = ql[2];  // transmissive: inside edge state
+                if (kind == 0) {  // reflective (evaluate_segment.cu:36-63)
+                    reflect_momentum(ql[1], ql[2], n1, n2, b1, b2);
+                } else if (kind == 1) {  // Dirichlet constants
+                    b0 = M.bdir[3 * m];
+                    b1 = M.bdir[3 * m + 1];
+                    b2 = M.bdir[3 * m + 2];
+                }
+                qr[0] = b0; qr[1] = b1; qr[2] = b2;
+            }
+            if (P.optimise_dry_cells && fabs(ql[0] - zl) < eps && fabs(qr[0] - zr) < eps) continue;
+            if (fabs(zl - zr) > 1.0e-10 && !halted)
+                atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrDiscontinuous);
+            double efd[3], ms;
+            flux_central<double>(ql, qr, zl, zr, n1, n2, eps, P.h0, P.limiting_threshold, g, efd, ms);
+            double f0 = efd[0] * lk, f1 = efd[1] * lk, f2 = efd[2] * lk;
+            if (!lower) {
+                f0 = -f0;
+                f1 = -f1;
+                f2 = -f2;
+            }
+            if (i == last_edge) {
+                l0 = f0;
+                l1 = f1;
+                l2 = f2;
+                have_last = true;
+            } else {
+                s0 = s0 + f0;
+                s1 = s1 + f1;
+                s2 = s2 + f2;
+            }
+            if (((flags >> (3 + i)) & 1) && ms > eps) cfl_speed = omax<double>(cfl_speed, ms);
+            if (!lower) max_speed_total = omax<double>(max_speed_total, ms);
+        }
+        if (have_last) {
+            s0 = s0 + l0;
+            s1 = s1 + l1;
+            s2 = s2 + l2;
+        }
+        if (cfl_speed > 0.0) dtk = omin<double>(dtk, gm.radius / cfl_speed);
+        const double area = gm.area;
+        const double inv_area = 1.0 / area;
+        s0 *= inv_area;
+        s1 *= inv_area;
+        s2 *= inv_area;
+        s1 += gr.x;
+        s2 += gr.y;
+        s1 += -sidex / area;
+        s2 += -sidey / area;
+        if (!halted) S.eur[k] = Q4{s0, s1, s2, max_speed_total};
+    }
+    // workgroup minimum of the CFL candidates, one atomic into a slot
+    __shared__ unsigned long long red[BLOCK / 64];
+    const unsigned long long wm = wave_min_bits(dtk);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0 && !halted) {
+        unsigned long long m = red[0];
+#pragma unroll
+        for (int w = 1; w < BLOCK / 64; w++) m = (red[w] < m) ? red[w] : m;
+        atomicMin(cfl_slot(S, M.n_inst, p, bi.inst, b & (kSlots - 1)), m);
+    }
+}
+
+// The three explicit updates of a triangle's update record.
+struct E3 {
+    double w, uh, vh;
+};
+SW_DEV E3 ld_e3(const Q4 *p) { return E3{p->w, p->uh, p->vh}; }
+
+// update_one + protect_one of one triangle's centroid record (quantity.c:769-820,
+// shallow_water_ext.c:1638-1705); the momenta's semi-implicit terms come as
+// si_ratio.  Returns false on the semi-implicit error.
+SW_DEV bool advance_centroid(Q4 &c, const E3 &eu, double siw, const double2 &sr, double dt, const DevParams &P) {
+    double sw = siw;
+    bool ok = update_one(c.w, eu.w, sw, dt);
+    ok = update_ratio(c.uh, eu.uh, sr.x, dt) && ok;
+    ok = update_ratio(c.vh, eu.vh, sr.y, dt) && ok;
+    protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
+    return ok;
+}
+
+// Manning friction of the next step (shallow_water_ext.c:1238-1317) from the
+// final centroid record, with the geometry in registers; the operations of
+// friction_si.
+SW_DEV double2 friction_next(const DevParams &P, double eta, const double x[3], const double y[3],
+                             const double zv[3], const Q4 &c) {
+    double si_x = 0.0, si_y = 0.0;
+    const double mah = P.minimum_allowed_height, g = P.g;
+    if (eta > mah) {
+        const double uh = c.uh, vh = c.vh;
+        const double zbar = (zv[0] + zv[1] + zv[2]) / 3.0;
+        const double h = c.w - zbar;
+        if (h >= mah) {
+            double S;
+            if (P.use_sloped_mannings) {
+                double zx, zy;
+                gradient3(x[0], y[0], x[1], y[1], x[2], y[2], zv[0], zv[1], zv[2], zx, zy);
+                const double zs = sqrt(1.0 + zx * zx + zy * zy);
+                S = -g * eta * eta * zs * sqrt((uh * uh + vh * vh));
+            } else {
+                S = -g * eta * eta * sqrt((uh * uh + vh * vh));
+            }
+            S /= pow(h, 7.0 / 3);
+            si_x += S * uh;
+            si_y += S * vh;
+        }
+    }
+    return make_double2(si_x, si_y);
+}
+
+#ifndef SW_ADV_MINWAVES
+#define SW_ADV_MINWAVES 1
+#endif
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, DevFields F, StepFields S, DevParams P,
+                                                                    int p) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BlockInst bi = block_inst<BLOCK>(M, b);
+    const int lane = (int)threadIdx.x & 63;
+    const bool lead_wave = (b * BLOCK == bi.off) && threadIdx.x < 64;  // first wavefront of the instance
+    const int N = M.N;
+    const int k = b * BLOCK + (int)threadIdx.x;
+    const bool valid = k < bi.end;
+    const int kk = valid ? k : bi.off;  // padding lanes load a real triangle's data and store nothing
+    const Q4 *cin = S.cqb[p];
+    const double2 *sin = S.sir[p];
+
+    // Loads: the CFL slots, the triangle's inputs, then its surrogate
+    // neighbours' inputs, all before the state decides anything.
+    const unsigned long long slotv = *cfl_slot(S, M.n_inst, p, bi.inst, lane);
+    Q4 c = ldq(cin + kk);
+    const E3 eu = ld_e3(S.eur + kk);
+    const double2 si = sin[kk];  // cached: the neighbours gather the same lines
+    double x[3], y[3], zv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double2 v = ld_s(M.vc + i * N + kk);
+        x[i] = v.x;
+        y[i] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + kk);
+    const int flags = ld_s(M.flags + kk);
+    const double eta = ld_s(F.friction + kk);
+    RecIn r;
+    r.xy = M.cc[kk];
+#pragma unroll
+    for (int i = 0; i < 3; i++) r.sk[i] = ld_s(M.sur + i * N + kk);
+    Q4 cn[3];
+    E3 eun[3];
+    double2 sn[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int n = r.sk[i];
+        cn[i] = ldq(cin + n);
+        eun[i] = ld_e3(S.eur + n);
+        sn[i] = sin[n];
+        r.xs[i] = M.cc[n];
+    }
+
+    // the next step's slots start at +inf (k_flux_v of the next step takes
+    // its minima there after this launch has ended)
+    if (lead_wave) *cfl_slot(S, M.n_inst, p ^ 1, bi.inst, lane) = ~0ull;
+    // update_timestep and the time advance (k_timestep<true>, Euler step),
+    // evaluated identically by every wavefront
+    const DevState s = S.stv[p][bi.inst];
+    const bool halted = s.event || s.error || s.halt || S.errw[bi.inst];
+    DevState s2 = s;
+    bool stop = halted;
+    if (halted) {
+        s2.halt = 1;
+    } else {
+        s2.flux_timestep = bitsd(wave_min_u64(slotv));
+        timestep_logic(&s2, P);
+        stop = (s2.error & kErrSmallsteps) != 0;
+        if (!stop) {
+            s2.time = s2.time + s2.timestep;
+            finish_step(&s2);
+            s2.wsi_live = 0;
+            s2.call_steps += 1;
+        }
+    }
+    if (lead_wave && lane == 0) S.stv[p ^ 1][bi.inst] = s2;
+    const double dt = s2.timestep;
+    const bool second_order = s2.order != 1;
+    double siw = 0.0, swn[3] = {0.0, 0.0, 0.0};
+    if (s.wsi_live && !stop) {  // a caller-set stage term (first step of a call)
+        siw = F.wsi[kk];
+#pragma unroll
+        for (int i = 0; i < 3; i++) swn[i] = F.wsi[r.sk[i]];
+    }
+
+    // the step (a stopped one computes too, on whatever it loaded, and stores nothing)
+    const bool ok = advance_centroid(c, eu, siw, si, dt, P);
+    r.c = c;
+    r.flags = flags;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        r.v[i] = make_double2(x[i], y[i]);
+        r.zv[i] = zv[i];
+    }
+    double wv[3], uv[3], vv[3];
+    bool ok_nb = true;
+    if (!second_order) {
+        wv[0] = wv[1] = wv[2] = c.w;
+        uv[0] = uv[1] = uv[2] = c.uh;
+        vv[0] = vv[1] = vv[2] = c.vh;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Q4 t = cn[i];
+            advance_centroid(t, eun[i], swn[i], sn[i], dt, P);  // its own lane reports errors
+            r.cs[i] = t;
+        }
+        // a dry triangle keeps its previous vertex values
+        auto stale = [&](int i, double &w, double &u, double &v) {
+            const Q4 o = ldq(S.vr + i * N + kk);
+            w = o.w;
+            u = o.uh;
+            v = o.vh;
+        };
+        ok_nb = extrapolate_one<double>(P, kk, r, wv, uv, vv, stale);
+    }
+    BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
+    balance_one(c.w, c.z, wv, zv, c.uh, c.vh, uv, vv, bp);
+    const double2 grav = gravity_gradient_term(r.v, wv, c, P.g);
+    const double2 fr = friction_next(P, eta, x, y, zv, c);
+    if (!valid) return;
+    if (stop) {
+        // nothing advances: the state is carried over to the next parity
+        S.cqb[p ^ 1][k] = ldq(cin + k);
+        S.sir[p ^ 1][k] = ld_s(sin + k);
+        return;
+    }
+    if (!ok) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrUpdate);
+    if (!ok_nb) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrNeighbour);
+#pragma unroll
+    for (int i = 0; i < 3; i++) S.vr[i * N + k] = Q4{wv[i], uv[i], vv[i], zv[i]};
+    st_s(F.grav + k, grav);
+    S.cqb[p ^ 1][k] = c;
+    st_s(S.sir[p ^ 1] + k, make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh)));
+}
+
+// Start of an evolve call on the two-kernel step: the first step's source
+// terms (as k_source_terms: gravity term, Manning friction added to the
+// caller's semi-implicit momenta, into sir[0]), the vertex records from the
+// vertex arrays, and whether a caller-set stage semi-implicit term is live.
+__global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, StepFields S, DevParams P) {
+    const int N = M.N;
+    const Lane ln = lane_of(M);
+    const int k = ln.k;
+    if (!ln.valid) return;
+    const Q4 c = ldq(F.cq + k);
+    double x[3], y[3], zv[3], wv[3];
+    for (int i = 0; i < 3; i++) {
+        const double2 v = M.vc[i * N + k];
+        x[i] = v.x;
+        y[i] = v.y;
+        zv[i] = F.zv[i * N + k];
+        wv[i] = F.wv[i * N + k];
+        S.vr[i * N + k] = Q4{wv[i], F.uhv[i * N + k], F.vhv[i * N + k], zv[i]};
+    }
+    const double2 v3[3] = {make_double2(x[0], y[0]), make_double2(x[1], y[1]), make_double2(x[2], y[2])};
+    F.grav[k] = gravity_gradient_term(v3, wv, c, P.g);
+    const double2 f = friction_next(P, F.friction[k], x, y, zv, c);
+    S.sir[0][k] = make_double2(si_ratio(F.uhsi[k] + f.x, c.uh), si_ratio(F.vhsi[k] + f.y, c.vh));
+    const bool live = F.wsi[k] != 0.0;
+    if (__ballot(live) != 0 && (threadIdx.x & 63) == 0)
+        atomicOr((unsigned long long *)&S.stv[0][ln.inst].wsi_live, 1ull);
+}
+
+// Per instance: both parities' slots at +inf, the stage semi-implicit flag
+// cleared (k_step_begin sets it again), no step run yet in this call.
+__global__ void k_step_reset(StepFields S, int n_inst) {
+    const int inst = blockIdx.x;
+    for (int p = 0; p < 2; p++) *cfl_slot(S, n_inst, p, inst, threadIdx.x) = ~0ull;
+    if (threadIdx.x == 0) {
+        S.stv[0][inst].wsi_live = 0;
+        S.stv[0][inst].call_steps = 0;
+    }
+}
+
+// End of a batch of steps: the lanes' errors are merged into the state of
+// parity p (the next step's), and the handle's state (stv[0]) is a copy of it.
+__global__ void k_step_settle(StepFields S, int n_inst, int p) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= n_inst) return;
+    DevState s = S.stv[p][inst];
+    s.error |= S.errw[inst];
+    S.errw[inst] = 0;
+    S.stv[p][inst] = s;
+    S.stv[0][inst] = s;
+}
+
+// End of an evolve call whose next step has parity p: the current centroid
+// records into buffer 0, and the vertex records and update records into the
+// arrays of the operator API (vertex values; edge values as their
+// interpolation, quantity.c:687-707; explicit updates; max_speed).
+__global__ __launch_bounds__(kBlock) void k_step_end(DevMesh M, DevFields F, StepFields S, int p) {
+    const int N = M.N;
+    const Lane ln = lane_of(M);
+    const int k = ln.k;
+    if (!ln.valid) return;
+    const DevState &st = S.stv[0][ln.inst];
+    if (p) F.cq[k] = S.cqb[1][k];
+    Q4 v[3];
+    for (int i = 0; i < 3; i++) {
+        v[i] = S.vr[i * N + k];
+        F.wv[i * N + k] = v[i].w;
+        F.uhv[i * N + k] = v[i].uh;
+        F.vhv[i * N + k] = v[i].vh;
+    }
+    for (int i = 0; i < 3; i++) {
+        const Q4 e = edge_of(v[next3(i)], v[prev3(i)]);
+        Q4 o = F.eq[i * N + k];
+        o.w = e.w;
+        o.uh = e.uh;
+        o.vh = e.vh;
+        F.eq[i * N + k] = o;
+    }
+    if (st.call_steps == 0) return;  // no step ran: the update records are stale
+    const Q4 u = S.eur[k];
+    F.weu[k] = u.w;
+    F.uheu[k] = u.uh;
+    F.vheu[k] = u.vh;
+    F.max_speed[k] = u.z;
+    F.wsi[k] = 0.0;  // a caller-set stage term is used by one step (quantity.c:769-820)
+}
+
+}  // namespace sw

@@ -564,3 +564,44 @@ def test_non_default_parameter_branches_match_oracle(case, fused):
     compare(d._handle.max_speed(), od.max_speed, "max_speed", TOL)
     print("%s (%s): bit-mismatching fields %s" % (case, "fused" if fused else "unfused",
                                                   {k: v[0] for k, v in stats.items() if v[0]}))
+
+
+def test_two_kernel_step_equals_four_kernel_step():
+    """The two-kernel Euler step (flux; then timestep, update and distribute of
+    each triangle with its surrogate neighbours' updates recomputed) and the
+    four-kernel step give the same bits: timesteps, yields and every array of
+    the operator API, over yields that stop a captured 16-step graph part way."""
+    runs = {}
+    for path in (_lib.SW_STEP_AUTO, _lib.SW_STEP_FOUR):
+        d = _delaunay("random")
+        d._dev().set_step_path(path)
+        assert d._handle.step_path() == (_lib.SW_STEP_TWO if path == _lib.SW_STEP_AUTO else _lib.SW_STEP_FOUR)
+        dts, steps = [], []
+        for _ in d.evolve(yieldstep=0.37, finaltime=3.0):
+            dts.append(d._handle.get_state().timestep)
+            steps.append(d.number_of_steps)
+        arrays = {(q, loc): getattr(d.quantities[q], loc).copy()
+                  for q in ("stage", "xmomentum", "ymomentum", "height", "xvelocity", "yvelocity")
+                  for loc in ("centroid_values", "vertex_values", "edge_values", "boundary_values")}
+        for q in ("stage", "xmomentum", "ymomentum"):
+            arrays[(q, "explicit_update")] = device_array(d, q, "explicit_update")
+        runs[path] = (dts, steps, arrays, d.max_speed.copy(), d.get_time())
+        d.close()
+    a, b = runs[_lib.SW_STEP_AUTO], runs[_lib.SW_STEP_FOUR]
+    assert a[0] == b[0] and a[1] == b[1] and sum(a[1]) > 40 and a[4] == b[4]
+    for key in a[2]:
+        assert np.array_equal(a[2][key], b[2][key]), key
+    assert np.array_equal(a[3], b[3])
+
+
+def test_two_kernel_step_needs_recomputable_geometry():
+    """A mesh whose geometry arrays are not what the vertex coordinates give
+    (here a radius changed in its last bit) keeps the four-kernel step."""
+    d = _delaunay("morton")
+    d.mesh.radii = d.mesh.radii.copy()
+    d.mesh.radii[10] = np.nextafter(d.mesh.radii[10], 0.0)
+    assert d._dev().step_path() == _lib.SW_STEP_FOUR
+    d2 = _delaunay("morton")
+    assert d2._dev().step_path() == _lib.SW_STEP_TWO
+    d2.set_timestepping_method("rk2")
+    assert d2._handle.step_path() == _lib.SW_STEP_FOUR
