@@ -59,6 +59,34 @@ KERNEL_BYTES_PER_TRIANGLE_MIXED = {
 # bytes per 64 triangles).
 STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
 
+# The two-kernel Euler step (the default FP64 path; sw_step2.h).  The step
+# timing reports k_flux_v as "flux" and k_advance as "distribute".
+KERNEL_BYTES_PER_TRIANGLE_TWO = {
+    # reads: vertex xy 48, neighbour words 12, flags 4, the triangle's vertex
+    # records 3x32 96, gravity term 16 | writes: update record 32
+    "flux": 48 + 12 + 4 + 96 + 16 + 32,
+    # reads: centroid record 32, update record 32, semi-implicit pair 16,
+    # Manning n 8, vertex xy 48, surrogates 12, flags 4, bed vertex 24 |
+    # writes: centroid record 32, semi-implicit pair 16, vertex records 3x32 96,
+    # gravity term 16
+    "distribute": 32 + 32 + 16 + 8 + 48 + 12 + 4 + 24 + 32 + 16 + 96 + 16,
+}
+# The unique footprint of one FP64 Euler step (each array read once, each
+# mutated array written once, the edge-value exchange counted as one write
+# plus one read, neighbour gathers as cache hits): 604 bytes per triangle,
+# split by the kernel of the two-kernel step that touches each array.
+FOOTPRINT_BYTES_PER_TRIANGLE_STEP = 604
+FOOTPRINT_SHARE_TWO = {
+    # neighbours 12, neighbour edges 12, full flag 4, normals 48, edge lengths
+    # 24, areas 8, radii 8, bed edge 24, edge values read 72, max_speed 8
+    "flux": 12 + 12 + 4 + 48 + 24 + 8 + 8 + 24 + 72 + 8,
+    # surrogates 12, boundary counts 4, centroid xy 16, vertex xy 48, bed
+    # centroid 8, bed vertex 24, friction 8, centroid state read+write 48,
+    # vertex state read+write 144, edge values written 72
+    "distribute": 12 + 4 + 16 + 48 + 8 + 24 + 8 + 48 + 144 + 72,
+}
+assert sum(FOOTPRINT_SHARE_TWO.values()) == FOOTPRINT_BYTES_PER_TRIANGLE_STEP
+
 
 def kernel_bytes(precision):
     return KERNEL_BYTES_PER_TRIANGLE if precision == "fp64" else KERNEL_BYTES_PER_TRIANGLE_MIXED
@@ -70,9 +98,11 @@ def workload_tag(args):
     return tag if args.locality == "hilbert" else tag + "_keep_order"
 
 
-def kernel_symbol(kernel, precision, launch):
+def kernel_symbol(kernel, precision, launch, two=False):
     """Name of a fused kernel as rocprofv3 reports it; ``launch`` = workgroup
     sizes of (flux, update, distribute)."""
+    if two:
+        return {"flux": "k_flux_v<256>", "distribute": "k_advance<256>"}[kernel]
     r = "double" if precision == "fp64" else "float"
     fb, ub, db = launch
     flux = "k_flux<%s, true, %d>" % (r, fb) if precision == "fp64" else "k_flux_mixed<%d>" % fb
@@ -82,6 +112,28 @@ def kernel_symbol(kernel, precision, launch):
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def copy_peak_gbs(mib=1024, reps=20):
+    """Bandwidth of a plain device-to-device copy of ``mib`` MiB (bytes read
+    plus bytes written per second), timed with HIP events: the measured
+    ceiling beside the vendor's 8 TB/s."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / sec / 1e9
 
 
 def main():
@@ -114,7 +166,9 @@ def main():
                     help="fv: distribute path (vertex: optimised_gradient_limiter off; edge: use_edge_limiter)")
     ap.add_argument("--mesh-build", default="host", choices=["host", "device"],
                     help="fv: build the mesh geometry on the GPU (sw_build_mesh) or the host; same arrays")
-    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--step-path", default="auto", choices=["auto", "four"],
+                    help="FV: two-kernel step where it applies (auto) or the four-kernel step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=64)
     ap.add_argument("--traffic-json", default=None,
@@ -165,8 +219,12 @@ def main():
         cpu_snapshot = OracleDomain.from_domain(d)
 
     h = d._dev()
+    from anuga_cuda_amd import _lib
+    if args.step_path == "four":
+        h.set_step_path(_lib.SW_STEP_FOUR)
     h.call("sw_evolve_prepare", 1e9, 0.0, 0)
-    log("device prepared")
+    two = h.step_path() == _lib.SW_STEP_TWO
+    log("device prepared (%s-kernel step)" % ("two" if two else "four"))
     done, ev = h.evolve_steps(args.warmup)
     launch = h.autotune(3)
     h.call("sw_synchronize")
@@ -190,29 +248,42 @@ def main():
     ms, cnt = h.kernel_times(4)
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
-    dominant = max(("flux", "update", "distribute"), key=lambda k: avg_us[k] or 0.0)
-    kb = kernel_bytes(args.precision)
+    kb = KERNEL_BYTES_PER_TRIANGLE_TWO if two else kernel_bytes(args.precision)
+    dominant = max(kb, key=lambda k: avg_us[k] or 0.0)
     # a Runge-Kutta step runs every kernel once per stage, plus the copy of the
     # starting centroid records (32 B written, then read by each later stage)
-    step_bytes = sum(kb.values()) * stages + (32 * stages if stages > 1 else 0)
-    bytes_launch = kb[dominant] * N
-    achieved = bytes_launch / (avg_us[dominant] * 1e-6) / 1e9
+    design_bytes = sum(kb.values()) * stages + (32 * stages if stages > 1 else 0)
+    # roofline bytes: the step's unique footprint (604 B per triangle and
+    # stage) and, for the dominant kernel of the two-kernel step, its share
+    # of it; the four-kernel step's kernels are counted by their design bytes
+    step_bytes = FOOTPRINT_BYTES_PER_TRIANGLE_STEP * stages
+    kernel_roof_bytes = (FOOTPRINT_SHARE_TWO if two else kb)[dominant]
+    achieved = kernel_roof_bytes * N / (avg_us[dominant] * 1e-6) / 1e9
     step_gbs = step_bytes * N * args.steps / elapsed / 1e9
+    copy_gbs = copy_peak_gbs()
+    log("device copy: %.0f GB/s" % copy_gbs)
     # HBM bytes per launch of the dominant kernel, measured by rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated; written by
     # anuga_cuda_amd.rocprof_summary), if such a profile is present.
     traffic = None
     tfile = args.traffic_json or os.path.join(ROOT, "profiles", "%s.traffic.json" % workload_tag(args))
+    step_traffic = None
     if os.path.exists(tfile):
-        kname = kernel_symbol(dominant, args.precision, launch)
         kernels = json.load(open(tfile)).get("kernels", {})
-        if dominant == "distribute":
-            # k_distribute<R, FUSED, BLOCK, LIMITED>: profiles name the limiter variant
-            limited = kname[:-1] + (", true>" if args.limiter != "optimised" else ", false>")
-            rec = kernels.get(limited) or (kernels.get(kname) if args.limiter == "optimised" else None)
+        if two:
+            recs = {k: kernels.get(kernel_symbol(k, args.precision, launch, True)) for k in kb}
+            rec = recs[dominant] or {}
+            if all(recs.values()):
+                step_traffic = sum(r["traffic_bytes"] for r in recs.values())
         else:
-            rec = kernels.get(kname)
-        rec = rec or {}
+            kname = kernel_symbol(dominant, args.precision, launch)
+            if dominant == "distribute":
+                # k_distribute<R, FUSED, BLOCK, LIMITED>: profiles name the limiter variant
+                limited = kname[:-1] + (", true>" if args.limiter != "optimised" else ", false>")
+                rec = kernels.get(limited) or (kernels.get(kname) if args.limiter == "optimised" else None)
+            else:
+                rec = kernels.get(kname)
+            rec = rec or {}
         traffic = rec.get("traffic_bytes")
 
     cpu = None
@@ -257,12 +328,19 @@ def main():
                    "triangles": N, "precision": args.precision, "order": 2, "timestepping": args.timestepping,
                    "limiter": args.limiter, "mesh_build": args.mesh_build,
                    "device_numbering": args.locality, "workgroup_sizes": list(launch)},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kernel_symbol(dominant, args.precision, launch, two),
+                     "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_triangle": kb[dominant],
-                     "avg_launch_us": avg_us[dominant]},
+                     "bytes_per_triangle": kernel_roof_bytes,
+                     "design_bytes_per_triangle": kb[dominant],
+                     "avg_launch_us": avg_us[dominant],
+                     "measured_copy_peak": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "step_roofline": {"bytes_per_triangle_step": step_bytes, "achieved": step_gbs,
-                          "frac": step_gbs / HBM_PEAK_GBS, "kernel_avg_us": avg_us},
+                          "frac": step_gbs / HBM_PEAK_GBS, "frac_of_copy": step_gbs / copy_gbs,
+                          "design_bytes_per_triangle_step": design_bytes,
+                          "traffic_per_step": step_traffic,
+                          "step_kernels": "two (k_flux_v, k_advance)" if two else "four",
+                          "kernel_avg_us": avg_us},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
