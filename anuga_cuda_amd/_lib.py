@@ -26,6 +26,7 @@ SW_STEP_AUTO, SW_STEP_FOUR, SW_STEP_TWO = 0, 1, 2
 SW_ERR_UPDATE = -1
 SW_ERR_SMALLSTEPS = -2
 SW_ERR_NEIGHBOUR = -3
+SW_ERR_ARG, SW_ERR_HIP, SW_ERR_STATE = -10, -11, -12
 
 
 class SwParams(C.Structure):
@@ -130,6 +131,7 @@ _SIGNATURES = {
     "sw_enable_timing": ([C.c_void_p, C.c_int32], C.c_int),
     "sw_build_mesh": ([C.c_int, C.c_int64, _F64P, C.c_int64, _I64P, C.c_void_p], C.c_int),
     "sw_mesh_last_error": ([], C.c_char_p),
+    "sw_build_mesh_noalloc": ([C.c_int, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
     "sw_kernel_times": ([C.c_void_p, _F64P, _I64P, C.c_int32], C.c_int),
     "sw_set_step_path": ([C.c_void_p, C.c_int32], C.c_int),
     "sw_get_step_path": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
@@ -163,7 +165,7 @@ def build_mesh_geometry(points, triangles, device=0):
     rc = lib.sw_build_mesh(int(device), len(pts), f64p(pts), N, i64p(tris), C.byref(g))
     if rc != 0:
         msg = (lib.sw_mesh_last_error() or b"").decode()
-        raise (ValueError if rc == -10 else SwError)(msg or "sw_build_mesh failed (%d)" % rc)
+        raise (ValueError if rc == SW_ERR_ARG else SwError)(msg or "sw_build_mesh failed (%d)" % rc)
     return out
 
 

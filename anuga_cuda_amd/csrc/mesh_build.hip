@@ -16,6 +16,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -109,6 +110,8 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_fill(long long n, long long
     if (i < n) a[i] = v;
 }
 
+__global__ void k_mesh_clear_flag(int *err) { *err = 0; }
+
 // Sorted edge keys: the first edge of each run of equal keys pairs itself
 // with the second; a run longer than two is an edge shared by more than two
 // triangles.
@@ -148,6 +151,45 @@ thread_local std::string g_mesh_error;
 int mesh_fail(const char *msg) {
     g_mesh_error = msg;
     return SW_ERR_ARG;
+}
+
+// Scratch of sw_build_mesh_noalloc, one copy per GPU for the life of the
+// process: the edge keys and flat edge indices before and after the radix
+// sort, the error flags and the sort's temporaries, for meshes of up to
+// SW_NOALLOC_MAX_TRIANGLES triangles.  Host code reaches it only through the
+// address hipGetSymbolAddress returns on each device (the array's name is a
+// host-side stand-in).
+constexpr long long kNaEdges = 3ll * SW_NOALLOC_MAX_TRIANGLES;
+constexpr size_t kNaKeyBytes = (size_t)kNaEdges * sizeof(unsigned long long);
+constexpr size_t kNaFlatBytes = (size_t)kNaEdges * sizeof(int);
+constexpr size_t kNaErrBytes = 256;
+constexpr size_t kNaSortBytes = (size_t)8 << 20;
+constexpr size_t kNaBytes = 2 * kNaKeyBytes + 2 * kNaFlatBytes + kNaErrBytes + kNaSortBytes;
+static_assert(kNaKeyBytes % 256 == 0 && kNaFlatBytes % 256 == 0, "scratch sections stay 256-byte aligned");
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch (the device is current), looked up
+// once per device.
+int scratch_address(int device, unsigned char **out) {
+    if (device < 0 || device >= kMaxDevices) return mesh_fail("sw_build_mesh_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "sw_build_mesh_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            g_mesh_error = buf;
+            return SW_ERR_HIP;
+        }
+        g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return SW_OK;
 }
 
 }  // namespace
@@ -232,6 +274,85 @@ int sw_build_mesh(int device, int64_t number_of_points, const double *points, in
     MCHECK(hipMemcpy(out->radii, rad, N * sizeof(double), hipMemcpyDeviceToHost));
     MCHECK(hipMemcpy(out->neighbours, nb, E * sizeof(long long), hipMemcpyDeviceToHost));
     MCHECK(hipMemcpy(out->neighbour_edges, ne, E * sizeof(long long), hipMemcpyDeviceToHost));
+#undef MCHECK
+    return SW_OK;
+}
+
+int sw_build_mesh_noalloc(int device, int64_t number_of_points, const double *points, int64_t number_of_elements,
+                          const int64_t *triangles, const sw_mesh_geometry *out) {
+    const long long P = number_of_points, N = number_of_elements, E = 3 * N;
+    if (!points || !triangles || !out || P < 3 || N < 1)
+        return mesh_fail("sw_build_mesh_noalloc: empty mesh or NULL array");
+    if (!out->triangles || !out->vertex_coordinates || !out->centroid_coordinates || !out->areas || !out->normals ||
+        !out->edgelengths || !out->edge_midpoint_coordinates || !out->radii || !out->neighbours ||
+        !out->neighbour_edges)
+        return mesh_fail("sw_build_mesh_noalloc: NULL output array");
+    if (N > SW_NOALLOC_MAX_TRIANGLES) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "sw_build_mesh_noalloc: %lld triangles exceed the scratch limit of %d", N,
+                 SW_NOALLOC_MAX_TRIANGLES);
+        g_mesh_error = buf;
+        return SW_ERR_ARG;
+    }
+    if (P >= (1ll << 31)) return mesh_fail("sw_build_mesh_noalloc: too many points for 32-bit edge keys");
+    char buf[256];
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        snprintf(buf, sizeof buf, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
+        g_mesh_error = buf;
+        return SW_ERR_HIP;
+    }
+    unsigned char *base = nullptr;
+    if (int rc = scratch_address(device, &base)) return rc;
+    unsigned long long *key = reinterpret_cast<unsigned long long *>(base);
+    unsigned long long *key_s = reinterpret_cast<unsigned long long *>(base + kNaKeyBytes);
+    int *flat = reinterpret_cast<int *>(base + 2 * kNaKeyBytes);
+    int *flat_s = reinterpret_cast<int *>(base + 2 * kNaKeyBytes + kNaFlatBytes);
+    int *err = reinterpret_cast<int *>(base + 2 * kNaKeyBytes + 2 * kNaFlatBytes);
+    void *sort_tmp = base + 2 * kNaKeyBytes + 2 * kNaFlatBytes + kNaErrBytes;
+    hipStream_t s = nullptr;
+#define MCHECK(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            snprintf(buf, sizeof buf, "sw_build_mesh_noalloc: %s: %s", #call, hipGetErrorString(e_)); \
+            g_mesh_error = buf;                                                                       \
+            return SW_ERR_HIP;                                                                        \
+        }                                                                                             \
+    } while (0)
+    // keys are below P^2: sort only the bits that can be set
+    const unsigned long long kmax = (unsigned long long)P * (unsigned long long)P - 1;
+    const int end_bit = 64 - __builtin_clzll(kmax);
+    // the sort passes alternate between the two key / index arrays of the
+    // scratch, so its temporaries are only its histograms
+    hipcub::DoubleBuffer<unsigned long long> keys(key, key_s);
+    hipcub::DoubleBuffer<int> flats(flat, flat_s);
+    size_t temp = 0;
+    MCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, flats, (int)E, 0, end_bit, s));
+    if (temp > kNaSortBytes) return mesh_fail("sw_build_mesh_noalloc: the radix sort needs more scratch than reserved");
+    // the flags start at zero on every call (the array keeps the last call's contents)
+    hipLaunchKernelGGL(k_mesh_clear_flag, dim3(1), dim3(1), 0, s, err);
+    hipLaunchKernelGGL(k_mesh_triangles, dim3(grid_of(N)), dim3(kMeshBlock), 0, s, N, P, points,
+                       reinterpret_cast<const long long *>(triangles), reinterpret_cast<long long *>(out->triangles),
+                       out->vertex_coordinates, out->centroid_coordinates, out->areas, out->normals, out->edgelengths,
+                       out->edge_midpoint_coordinates, out->radii, key, flat, err);
+    hipLaunchKernelGGL(k_mesh_fill, dim3(grid_of(E)), dim3(kMeshBlock), 0, s, E,
+                       reinterpret_cast<long long *>(out->neighbours), -1ll);
+    hipLaunchKernelGGL(k_mesh_fill, dim3(grid_of(E)), dim3(kMeshBlock), 0, s, E,
+                       reinterpret_cast<long long *>(out->neighbour_edges), -1ll);
+    MCHECK(hipGetLastError());
+    int herr = 0;
+    MCHECK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr & kBadIndex) return mesh_fail("triangle vertex index out of range");
+    MCHECK(hipcub::DeviceRadixSort::SortPairs(sort_tmp, temp, keys, flats, (int)E, 0, end_bit, s));
+    hipLaunchKernelGGL(k_mesh_pairs, dim3(grid_of(E)), dim3(kMeshBlock), 0, s, E, keys.Current(), flats.Current(),
+                       reinterpret_cast<long long *>(out->neighbours),
+                       reinterpret_cast<long long *>(out->neighbour_edges), err);
+    MCHECK(hipGetLastError());
+    MCHECK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr & kDegenerate) return mesh_fail("degenerate triangle (zero area)");
+    if (herr & kZeroEdge) return mesh_fail("zero-length edge");
+    if (herr & kNonManifold) return mesh_fail("non-manifold mesh: an edge is shared by more than two triangles");
 #undef MCHECK
     return SW_OK;
 }

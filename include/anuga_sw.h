@@ -163,6 +163,21 @@ typedef struct {
 } sw_mesh_geometry;
 int sw_build_mesh(int device, int64_t number_of_points, const double *points, int64_t number_of_elements,
                   const int64_t *triangles, sw_mesh_geometry *out);
+
+/* sw_build_mesh on device memory, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * points, triangles and every array of *out are device pointers on GPU
+ * `device` (out itself is a host struct); the layouts, results and errors are
+ * those of sw_build_mesh.  The edge keys, their sort and the error flags use
+ * a scratch array the library holds on each GPU for the life of the process
+ * (about 84 MB), so meshes are limited to SW_NOALLOC_MAX_TRIANGLES triangles
+ * (larger ones return SW_ERR_ARG).  Calls that overlap on one GPU race on
+ * that scratch: serialise them.  The call returns when the results are in
+ * *out (it synchronises the null stream). */
+#define SW_NOALLOC_MAX_TRIANGLES 1048576
+int sw_build_mesh_noalloc(int device, int64_t number_of_points, const double *points, int64_t number_of_elements,
+                          const int64_t *triangles, const sw_mesh_geometry *out);
 const char *sw_mesh_last_error(void);
 int sw_set_params(sw_handle *h, const sw_params *params);
 int sw_get_params(const sw_handle *h, sw_params *params);
