@@ -3,16 +3,19 @@
 // Two families share the numerics in sw_device.h:
 //   * operator kernels, one per overridden ANUGA operator (the drop-in API:
 //     each mirrors one reference kernel / C routine and its array contract);
-//   * fused step kernels used by evolve(); per Euler step
-//       flux       = compute_fluxes + boundary evaluation + gravity_wb +
-//                    CFL minimum (wavefront shuffles, one atomic per
-//                    workgroup) + update_timestep and the time bookkeeping,
-//                    done by the workgroup that finishes last;
-//       update     = update_conserved_quantities + protect;
+//   * the four-kernel fused step used by evolve() (rk2 / rk3, mixed
+//     precision, the per-quantity limiters, and meshes the two-kernel step of
+//     sw_step2.h does not take); per stage
+//       flux       = compute_fluxes + boundary evaluation + gravity_wb; each
+//                    wavefront stores the minimum of its CFL candidates
+//                    (wavefront shuffles, a plain store);
+//       timestep   = one workgroup per instance reduces those minima and does
+//                    update_timestep and the time bookkeeping (k_timestep);
+//       update     = update_conserved_quantities + protect + the next step's
+//                    Manning friction;
 //       distribute = extrapolate_second_order_sw + balance_deep_and_shallow +
 //                    interpolate_from_vertices_to_edges + the next step's
-//                    state-only source terms (stage-gradient gravity term,
-//                    Manning friction);
+//                    stage-gradient gravity term;
 //     the explicit / semi-implicit updates are the only intermediates that go
 //     through HBM and the host is not involved between steps.
 //

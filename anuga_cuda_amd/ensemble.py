@@ -53,7 +53,13 @@ class Ensemble:
             d._dev()
 
     def prepare(self, yieldstep=None, finaltime=None):
-        """The evolve preamble of every instance (distribute, boundary values)."""
+        """The evolve preamble of every instance (distribute, boundary values).
+        The members share one yield schedule, so they must share a start
+        time."""
+        t0 = self.domains[0].starttime
+        if any(d.starttime != t0 for d in self.domains):
+            raise ValueError("ensemble members must share their start time (got %s)"
+                             % sorted({d.starttime for d in self.domains}))
         self._push_all()
         if yieldstep is None:
             yieldstep = self.domains[0].evolve_max_timestep

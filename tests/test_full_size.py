@@ -1,0 +1,156 @@
+"""Parity at the sizes of the benchmark configurations (GPU).
+
+* the 1,002,826-triangle Delaunay workload (500k seeded points in 10 km x
+  10 km, fBm terrain, wet/dry mound, transmissive boundaries) in random and in
+  Morton order: 10 steps in lockstep with the FP64 oracle (centroid values,
+  timestep, time and wet/dry mask every step), then 16 steps in one call (a
+  captured graph) and the whole state compared again;
+* the 4,000,000-triangle dam break in mixed precision against the FP64 oracle
+  at the time the oracle reaches in 20 steps, error stated per quantity;
+* the 64-member ensemble (8.67M triangles): 5 steps, each member's timesteps
+  and state against its own oracle;
+* the device mesh build of the 1M-triangle mesh against the host build.
+
+The bar is the other lockstep tests': 1e-12 normwise relative per quantity
+(in practice bitwise, but for the last bit of Manning's pow()), identical
+timesteps and wet/dry masks.
+"""
+import numpy as np
+import pytest
+
+from anuga_cuda_amd import Ensemble, _lib, scenarios
+from anuga_cuda_amd.mesh import Mesh, delaunay_random
+from oracle.oracle import OracleDomain
+from tests.helpers import compare, compare_state, device_array, wet_mask
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+CONSERVED = ("stage", "xmomentum", "ymomentum")
+
+
+def _centroids(domain, od, tag, stats):
+    for q in CONSERVED:
+        compare(device_array(domain, q, "centroid_values"), od.q(q, "centroid_values"), tag + q, TOL, stats)
+
+
+@pytest.mark.parametrize("order", ["random", "morton"])
+def test_config2_million_triangles_lockstep(order):
+    d = scenarios.delaunay_terrain(npoints=500000, order=order)
+    assert d.number_of_elements == 1002826
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    od.evolve_prepare(1e9, None)
+    assert h.step_path() == _lib.SW_STEP_TWO
+    bed = od.q("elevation", "centroid_values")
+    eps = d.minimum_allowed_height
+    stats = {}
+    for step in range(10):
+        done, ev = h.evolve_steps(1)
+        od.evolve_step()
+        s = h.get_state()
+        assert done == 1 and ev == _lib.SW_EVENT_NONE
+        compare([s.timestep, s.time], [od.d.timestep, od.d.time], "timestep/time", TOL, stats)
+        _centroids(d, od, "step%d." % step, stats)
+        w = device_array(d, "stage", "centroid_values")
+        assert np.array_equal(wet_mask(w, bed, eps), wet_mask(od.q("stage", "centroid_values"), bed, eps)), step
+    # 16 steps in one call: the captured graph of the fused step
+    done, ev = h.evolve_steps(16)
+    assert done == 16
+    for _ in range(16):
+        od.evolve_step()
+    s = h.get_state()
+    compare([s.time], [od.d.time], "time after 26 steps", TOL)
+    assert s.number_of_steps == od.d.number_of_steps == 26
+    compare([s.recorded_min_timestep, s.recorded_max_timestep],
+            [od.d.recorded_min_timestep, od.d.recorded_max_timestep], "recorded timesteps", TOL)
+    compare_state(d, od, TOL, stats=stats, tag="after 26 steps.")
+    for q in CONSERVED:
+        compare(device_array(d, q, "explicit_update"), od.q(q, "explicit_update"), q + ".explicit_update", TOL)
+    compare(d._handle.max_speed(), od.max_speed, "max_speed", TOL)
+    wet = np.count_nonzero(wet_mask(od.q("stage", "centroid_values"), bed, eps))
+    print("1M %s: %d triangles, %d wet, t = %.6f s after 26 steps; bit-mismatching fields %s" % (
+        order, d.number_of_elements, wet, s.time, {k: v[0] for k, v in stats.items() if v[0]}))
+    d.close()
+
+
+def test_config3_mixed_precision_4M_against_oracle():
+    """Mixed precision at 4,000,000 triangles (rectangular_cross 1000 x 1000,
+    the dam break of the CPU configuration) against the FP64 oracle after the
+    oracle's first 20 steps: L2 error of the stage relative to the stage norm
+    below 1e-6, of each momentum component relative to the norm of the whole
+    momentum field below 1e-4 (the y-momentum of a dam break along x is nearly
+    zero, so its error relative to its own norm is also printed), wet/dry masks
+    differing in at most 0.1% of the cells."""
+    d = scenarios.dam_break_channel(1000, 1000, precision="mixed")
+    assert d.number_of_elements == 4000000
+    od = OracleDomain.from_domain(d)
+    od.evolve_prepare(1e9, None)
+    for _ in range(20):
+        od.evolve_step()
+    T = od.d.time
+    for _ in d.evolve(yieldstep=T, finaltime=T):
+        pass
+    assert abs(d.get_time() - T) <= 1e-12 * T
+    errs, own, maxabs = {}, {}, {}
+    mom = np.hypot(np.linalg.norm(od.q("xmomentum", "centroid_values")),
+                   np.linalg.norm(od.q("ymomentum", "centroid_values")))
+    for q in CONSERVED:
+        a = d.quantities[q].centroid_values
+        b = od.q(q, "centroid_values")
+        own[q] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        errs[q] = own[q] if q == "stage" else float(np.linalg.norm(a - b) / mom)
+        maxabs[q] = float(np.max(np.abs(a - b)))
+    bed = od.q("elevation", "centroid_values")
+    eps = d.minimum_allowed_height
+    mism = int(np.count_nonzero(wet_mask(d.quantities["stage"].centroid_values, bed, eps)
+                                != wet_mask(od.q("stage", "centroid_values"), bed, eps)))
+    print("4M mixed vs FP64 oracle at t = %.6f s (oracle 20 steps, mixed %d steps): L2 error relative to the "
+          "stage / momentum norm %s, relative to each quantity's own norm %s, max abs %s, wet/dry mismatches %d"
+          % (T, d.number_of_steps, errs, own, maxabs, mism))
+    assert errs["stage"] < 1e-6
+    assert errs["xmomentum"] < 1e-4 and errs["ymomentum"] < 1e-4
+    assert mism <= 0.001 * d.number_of_elements
+    d.close()
+
+
+def test_config4_ensemble_of_64_members_against_their_oracles():
+    """The 64 seeded members of the ensemble workload (triangle counts
+    log-uniform in [20k, 500k], 8.67M in all; Manning n, dam height and
+    inflow varied) in one handle: 5 steps, each member's timestep sequence
+    against its own oracle's and its centroid values within the FP64 bar
+    (the pow() of Manning friction may differ in its last bit, which is not
+    re-synchronised: 1e-12 times the steps taken)."""
+    domains, specs = scenarios.ensemble_scenarios(count=64)
+    assert len(domains) == 64 and sum(d.number_of_elements for d in domains) > 8_000_000
+    oracles = [OracleDomain.from_domain(d) for d in domains]
+    ens = Ensemble(domains)
+    ens.prepare(1e9, None)
+    for od in oracles:
+        od.evolve_prepare(1e9, None)
+    for step in range(5):
+        ens.evolve_steps(1)
+        for i, (d, od) in enumerate(zip(domains, oracles)):
+            od.evolve_step()
+            s = d._handle.get_state()
+            compare([s.timestep, s.time], [od.d.timestep, od.d.time], "member %d step %d timestep" % (i, step),
+                    TOL)
+    for i, (d, od) in enumerate(zip(domains, oracles)):
+        for q in CONSERVED:
+            compare(d.quantities[q].centroid_values, od.q(q, "centroid_values"), "member %d %s" % (i, q), 5 * TOL)
+    dts = [od.d.timestep for od in oracles]
+    print("64 members, %d triangles, member timesteps %.3g .. %.3g s" % (
+        sum(d.number_of_elements for d in domains), min(dts), max(dts)))
+    ens.close()
+
+
+def test_device_mesh_build_at_one_million_triangles():
+    points, tris = delaunay_random(500000, extent=(10000.0, 10000.0), seed=12345)
+    host = Mesh(points, tris)
+    dev = Mesh(points, tris, build="device")
+    assert host.number_of_elements == 1002826
+    for n in ("triangles", "vertex_coordinates", "centroid_coordinates", "areas", "normals", "edgelengths",
+              "edge_midpoint_coordinates", "radii", "neighbours", "neighbour_edges", "surrogate_neighbours",
+              "number_of_boundaries", "boundary_cells", "boundary_edges"):
+        assert np.array_equal(getattr(host, n), getattr(dev, n)), n

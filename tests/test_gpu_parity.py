@@ -605,3 +605,14 @@ def test_two_kernel_step_needs_recomputable_geometry():
     assert d2._dev().step_path() == _lib.SW_STEP_TWO
     d2.set_timestepping_method("rk2")
     assert d2._handle.step_path() == _lib.SW_STEP_FOUR
+
+
+def test_ensemble_members_must_share_their_start_time():
+    specs = [dict(triangles=900, seed=11, manning=0.012, dam_height=1.5, inflow=0.2),
+             dict(triangles=900, seed=12, manning=0.05, dam_height=3.7, inflow=0.9)]
+    members = [scenarios.ensemble_member(extent=300.0, **sp) for sp in specs]
+    members[1].set_starttime(2.0)
+    ens = Ensemble(members)
+    with pytest.raises(ValueError):
+        ens.prepare(1.0, 4.0)
+    ens.close()
