@@ -146,9 +146,13 @@ def main():
     ap.add_argument("--merge-radius", type=float, default=2.5,
                     help="learned: inscribed radius (m) below which triangles are merged for the learned update")
     ap.add_argument("--train-iters", type=int, default=3000, help="learned: training iterations")
-    ap.add_argument("--train-meshes", type=int, default=40, help="learned: training meshes (50 trajectories each)")
+    ap.add_argument("--train-meshes", type=int, default=40,
+                    help="learned: training windows taken at a time (50 trajectories each)")
+    ap.add_argument("--train-trajectories", type=int, default=2000,
+                    help="learned: take training windows until this many trajectories are kept")
     ap.add_argument("--save-net", default=None, help="learned: write the trained network (.npz)")
-    ap.add_argument("--load-net", default=None, help="learned: use this trained network (.npz) instead of training")
+    ap.add_argument("--load-net", default=None,
+                    help="learned: use this trained network (.npz); without --save-net the committed --network is used")
     ap.add_argument("--judge-precision", default="mixed", choices=["mixed", "fp64"],
                     help="learned: precision of the FV judge's step (the reference and fallbacks are FP64; the "
                          "other choice is reported beside it)")
@@ -170,6 +174,10 @@ def main():
     ap.add_argument("--step-path", default="auto", choices=["auto", "four"],
                     help="FV: two-kernel step where it applies (auto) or the four-kernel step")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="FV: skip the learned rollout that the default line carries")
+    ap.add_argument("--network", default=os.path.join(ROOT, "anuga_cuda_amd", "networks", "edge_gnn_cfg5.npz"),
+                    help="FV / learned: the trained network of the learned rollout")
     ap.add_argument("--timing-steps", type=int, default=64)
     ap.add_argument("--traffic-json", default=None,
                     help="rocprof traffic summary (default: profiles/<workload>.traffic.json)")
@@ -343,8 +351,35 @@ def main():
                           "kernel_avg_us": avg_us},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     d.close()
+    if not args.no_learned and args.precision == "fp64" and args.points == 500000 and args.order == "morton":
+        out["learned"] = learned_rollout(args)
+    print(json.dumps(out), flush=True)
+
+
+def learned_rollout(args):
+    """The learned form of the same workload: the committed network (trained
+    by this package's kernels; provenance in anuga_cuda_amd/networks/)
+    rolled out for 100 learned steps of 0.0625 s from the workload's initial
+    state, the FV judge (mixed-precision step) every 8 steps with the FP64 FV
+    fallback above 1% stage L2 error, against the FP64 FV path over the same
+    6.25 s.  speedup = FV wall time / judged rollout wall time."""
+    from anuga_cuda_amd import learned, scenarios
+
+    net = learned.EdgeGNN.load(args.network)
+    dom = scenarios.delaunay_terrain(npoints=args.points, order=args.order)
+    res = learned.evaluate_rollout(net, dom, 100, 0.0625, judge_every=8, threshold=0.01, judge_precision="mixed",
+                                   log=log)
+    dom.close()
+    keys = ("speedup", "learned_rate", "fv_rate", "steps", "delta", "fv_steps", "final_stage_rel_l2",
+            "max_stage_rel_l2", "mass_balance_rel", "mass_change_vs_fv", "fallback_windows", "judge_precision",
+            "learned_step_ms", "fv_seconds", "learned_seconds")
+    r = {k: res[k] for k in keys}
+    r.update(unit="FV-equivalent triangle-steps/s (triangles x FV steps for the same simulated time / s)",
+             windows_judged=len(res["judged"]), network=os.path.relpath(args.network, ROOT),
+             stage_error="area-weighted L2 of the stage error over the depth norm, against FP64 FV",
+             mass_balance="(volume change + open-boundary outflow) / volume over one window of 8 learned steps")
+    return r
 
 
 # The reference's only published timing of this path: its Merimbula scenario
@@ -492,8 +527,12 @@ def bench_learned(args):
     simulated time, per second)."""
     from anuga_cuda_amd import learned
 
-    res = learned.run_learned_workload(train_meshes=args.train_meshes, delta=args.delta, iters=args.train_iters,
-                                       eval_points=args.points, save_net=args.save_net, load_net=args.load_net,
+    load_net = args.load_net
+    if load_net is None and args.save_net is None:
+        load_net = args.network  # the committed network; --save-net trains a new one
+    res = learned.run_learned_workload(train_meshes=args.train_meshes, train_trajectories=args.train_trajectories,
+                                       delta=args.delta, iters=args.train_iters,
+                                       eval_points=args.points, save_net=args.save_net, load_net=load_net,
                                        judge_precision=args.judge_precision, merge_radius=args.merge_radius, log=log)
     out = {
         "metric": "triangle-steps/s at 1M tris, FV vs learned at stated error; % HBM peak",
