@@ -311,7 +311,10 @@ bool locate(sw_handle *h, int inst, int q, int loc, Loc *out) {
     return true;
 }
 
-__global__ void k_set_bed_interp(DevState *st, long long v) { st->bed_interp = v; }
+__global__ void k_set_bed_interp(DevState *st, long long v) {
+    st->bed_interp = v;
+    st->bed_cont = v;
+}
 
 // Fused-kernel launches at one of the workgroup sizes the autotuner chooses
 // from (128, 256, 512 lanes).
@@ -420,7 +423,7 @@ bool step2_applies(const sw_handle *h, const std::vector<DevState> &d) {
     if (h->params.precision != SW_PRECISION_FP64 || rk_stages(h->params) != 1) return false;
     if (h->params.limiter != SW_LIMITER_OPTIMISED) return false;
     for (const auto &s : d)
-        if (!s.bed_interp) return false;
+        if (!s.bed_interp || !s.bed_cont) return false;
     return true;
 }
 
@@ -818,7 +821,8 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     h->step2_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
     {
         StepFields &S = h->sf;
-        if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.cqb[1], N) ||
+        if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.hc, N) ||
+            dalloc(h, &S.cqb[1], N) ||
             dalloc(h, &S.sir[0], N) || dalloc(h, &S.sir[1], N) || dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
             dalloc(h, &S.errw, n_inst) || dalloc(h, &S.stv[1], n_inst))
             return bail();
@@ -830,6 +834,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         d = DevState{};
         d.order = p->default_order;
         d.bed_interp = 1;
+        d.bed_cont = 1;
     }
     if (hipMemcpy(h->st, s0.data(), n_inst * sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) {
         fail(h, SW_ERR_HIP, "state upload failed");
@@ -940,7 +945,8 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
     if (quantity == SW_ELEVATION && (location == SW_VERTEX || location == SW_EDGE)) {
         hipLaunchKernelGGL(k_set_bed_interp, dim3(1), dim3(1), 0, h->stream, h->st + i, 1ll);
         hipLaunchKernelGGL(k_bed_derived, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (int)h->N, (int)I.off,
-                           (int)I.n, h->f.zv, h->f.ze, h->f.zbar, h->st + i);
+                           (int)I.n, h->f.zv, h->f.ze, h->f.zbar, h->step2_geometry ? h->nbw : nullptr,
+                           h->st + i);
         if (int rc = launch_check(h, "k_bed_derived")) return rc;
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));

@@ -127,8 +127,10 @@ struct DevState {
     // device-only
     long long halt, total_steps, bed_interp;
     // two-kernel step (sw_step2.h): whether a caller-set stage semi-implicit
-    // term is still to be applied; steps it ran in the current evolve call
-    long long wsi_live, call_steps;
+    // term is still to be applied; steps it ran in the current evolve call;
+    // whether the bed edge values of every shared edge are bitwise equal on
+    // both sides (then its flux reads only its own side's bed)
+    long long wsi_live, call_steps, bed_cont;
 };
 
 enum : long long { kErrNone = 0, kErrUpdate = 1, kErrSmallsteps = 2, kErrNeighbour = 4, kErrDiscontinuous = 8 };
@@ -709,12 +711,15 @@ SW_DEV void friction_si(const DevMesh &M, const DevFields &F, const DevParams &P
 }
 
 // The stage-gradient part of gravity_wb (shallow_water_ext.c:1942-1990:
-// -g * w_x * avg_h, w_x from the vertex stage), from the vertex stage values.
-SW_DEV double2 gravity_gradient_term(const double2 v[3], const double wv[3], const Q4 &c, double g) {
+// -g * w_x * avg_h, w_x from the vertex stage), from the vertex stage values
+// and the centroid depth avg_h = w_c - z_c.
+SW_DEV double2 gravity_gradient(const double2 v[3], const double wv[3], double avg_h, double g) {
     double wx, wy;
     gradient3(v[0].x, v[0].y, v[1].x, v[1].y, v[2].x, v[2].y, wv[0], wv[1], wv[2], wx, wy);
-    const double avg_h = c.w - c.z;
     return make_double2(-g * wx * avg_h, -g * wy * avg_h);
+}
+SW_DEV double2 gravity_gradient_term(const double2 v[3], const double wv[3], const Q4 &c, double g) {
+    return gravity_gradient(v, wv, c.w - c.z, g);
 }
 
 // saxpy_centroid_values (saxpy_centroid_values.cu:1-12) on the conserved
@@ -1418,12 +1423,14 @@ __global__ __launch_bounds__(kBlock) void k_sww_points(long long P, const int *o
     out[3 * P + p] = storable ? v : 0.f;
 }
 
-// Mean bed and the check that the bed edge values are the interpolation of the
-// bed vertex values, over device triangles [off, off + n) of one instance.
+// Mean bed, the check that the bed edge values are the interpolation of the
+// bed vertex values and, given the neighbour words of the two-kernel step
+// (nbw, else null), the check that both sides of every shared edge hold the
+// same bed edge value, over device triangles [off, off + n) of one instance.
 __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, const double *zv, const double *ze,
-                                                        double *zbar, DevState *st) {
+                                                        double *zbar, const int *nbw, DevState *st) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    bool bad = false;
+    bool bad = false, disc = false;
     if (t < n) {
         const int k = off + t;
         const double v[3] = {zv[k], zv[N + k], zv[2 * N + k]};
@@ -1431,9 +1438,16 @@ __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, c
         double e[3];
         interpolate_one(v, e);
         bad = dbits(e[0]) != dbits(ze[k]) || dbits(e[1]) != dbits(ze[N + k]) || dbits(e[2]) != dbits(ze[2 * N + k]);
+        if (nbw) {
+            for (int i = 0; i < 3; i++) {
+                const int w = nbw[i * N + k];  // (m << 29) | n, or negative on a boundary edge
+                if (w >= 0) disc = disc || dbits(ze[i * N + k]) != dbits(ze[(w >> 29) * N + (w & 0x1FFFFFFF)]);
+            }
+        }
     }
     // one atomic per wavefront that found a mismatch
     if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicAnd((unsigned long long *)&st->bed_interp, 0ull);
+    if (__ballot(disc) != 0 && (threadIdx.x & 63) == 0) atomicAnd((unsigned long long *)&st->bed_cont, 0ull);
 }
 
 }  // namespace sw

@@ -1,8 +1,9 @@
 // Two-kernel Euler step of the FP64 loop (extrapolate_second_order_sw path).
 //
 // Per step:
-//   k_flux_v    compute_fluxes + boundary values on the fly + gravity_wb + the
-//               CFL minimum (one non-returning atomic per workgroup into one of
+//   k_flux_v    compute_fluxes + boundary values on the fly + gravity_wb (its
+//               stage-gradient term from the vertex records) + the CFL
+//               minimum (one non-returning atomic per workgroup into one of
 //               kSlots slots of the instance);
 //   k_advance   update_timestep and the time bookkeeping (every wavefront
 //               reduces the slots itself: no separate launch), then
@@ -10,17 +11,18 @@
 //               its three surrogate neighbours (recomputed from the same
 //               inputs, so bitwise what the neighbours compute for
 //               themselves), extrapolate_second_order_sw +
-//               balance_deep_and_shallow, the next step's Manning friction and
-//               stage-gradient gravity term.
+//               balance_deep_and_shallow and the next step's Manning friction.
 //
 // What goes through HBM per triangle-step (FP64):
 //   k_flux_v   reads vertex coordinates 48, neighbour words 12, flags 4, the
-//              triangle's three vertex records 96, gravity term 16; writes the
-//              update record 32 (208 B);
-//   k_advance  reads centroid record 32, update record 32, semi-implicit pair
-//              16, Manning n 8, vertex coordinates 48, surrogates 12, flags 4,
-//              bed vertex values 24; writes centroid record 32, semi-implicit
-//              pair 16, vertex records 96, gravity term 16 (336 B).
+//              triangle's three vertex records 72, bed vertex values 24,
+//              centroid depth 8; writes the update record 24 and max_speed 8
+//              (200 B);
+//   k_advance  reads centroid record 32, update record 24, semi-implicit pair
+//              16, Manning n 8, vertex coordinates 48, centroid 16,
+//              surrogates 12, flags 4, bed vertex values 24; writes centroid
+//              record 32, semi-implicit pair 16, vertex records 72, centroid
+//              depth 8 (312 B).
 // Neighbour gathers (vertex records in k_flux_v; centroid, update and
 // semi-implicit records and centroids in k_advance) come from the L2 / last-level
 // cache: the locality order keeps neighbours in nearby workgroups.
@@ -30,11 +32,16 @@
 //   * no edge records: k_flux_v forms edge values from the vertex records
 //     (e_i = (v_{i+1} + v_{i+2}) / 2, the interpolation of quantity.c:687-707),
 //     so the edge state is never written; the edge bed is the interpolation of
-//     the vertex bed (required: bed_interp);
+//     the bed vertex values (required: bed_interp) and the same on both sides
+//     of every shared edge (required: bed_cont), so a flux reads only its own
+//     side's bed;
 //   * normals, edge lengths, inscribed radius, area and centroid are
 //     recomputed from the vertex coordinates by tri_geometry, the formulas the
 //     mesh is built with; sw_create checks on the host that they reproduce the
 //     caller's arrays bit for bit and otherwise keeps the four-kernel step;
+//   * the stage-gradient gravity term is formed by the flux from the vertex
+//     stage and the centroid depth the previous k_advance stored (the same
+//     operations on the same values as forming it there);
 //   * centroid records, semi-implicit pairs and the scalar state are read
 //     from buffer p and written to buffer p ^ 1, p the step parity (a kernel
 //     argument: step i of a captured graph uses parity i % 2), so a workgroup
@@ -55,9 +62,19 @@ constexpr int kSlots = 64;
 constexpr int kSlotStride = 16;  // unsigned long longs between slots
 constexpr int kSlotBlock = kSlots * kSlotStride;
 
+// Vertex record {w, uh, vh} of one vertex and update record {stage, xmom,
+// ymom explicit update} of one triangle (24 bytes each).
+struct V3 {
+    double w, uh, vh;
+};
+struct E3 {
+    double w, uh, vh;
+};
+
 struct StepFields {
-    Q4 *vr;                   // [3][N] vertex records {w, uh, vh, z}
-    Q4 *eur;                  // [N] {stage, xmom, ymom explicit update, max_speed}
+    V3 *vr;                   // [3][N] vertex records
+    E3 *eur;                  // [N] explicit updates of the step (max_speed goes to DevFields::max_speed)
+    double *hc;               // [N] centroid depth w - z after the step (for the next step's gravity term)
     Q4 *cqb[2];               // [N] centroid records; cqb[0] is DevFields::cq
     double2 *sir[2];          // [N] {xmom, ymom semi-implicit update} / centroid value (si_ratio)
     unsigned long long *slot; // [2][n_inst][kSlotBlock] CFL minima (bits of positive doubles)
@@ -122,9 +139,10 @@ SW_DEV BlockInst block_inst(const DevMesh &M, int b) {
     return BlockInst{inst, M.inst_off[inst], M.inst_end[inst]};
 }
 
-// Edge state {w, uh, vh, z} of the edge between two vertex records.
-SW_DEV Q4 edge_of(const Q4 &a, const Q4 &b) {
-    return Q4{0.5 * (a.w + b.w), 0.5 * (a.uh + b.uh), 0.5 * (a.vh + b.vh), 0.5 * (a.z + b.z)};
+// Edge state {w, uh, vh} of the edge between two vertex records, and the bed
+// of that edge from the two bed vertex values.
+SW_DEV Q4 edge_of(const V3 &a, const V3 &b, double za, double zb) {
+    return Q4{0.5 * (a.w + b.w), 0.5 * (a.uh + b.uh), 0.5 * (a.vh + b.vh), 0.5 * (za + zb)};
 }
 SW_DEV int next3(int i) { return i == 2 ? 0 : i + 1; }
 SW_DEV int prev3(int i) { return i == 0 ? 2 : i - 1; }
@@ -148,8 +166,8 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
     const int k = b * BLOCK + (int)threadIdx.x;
     double dtk = P.evolve_max_timestep;
     if (k < bi.end) {
-        double x[3], y[3];
-        Q4 vq[3], na[3], nc[3];
+        double x[3], y[3], zv[3];
+        V3 vq[3], na[3], nc[3];
         int nb[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -158,11 +176,13 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
             y[i] = v.y;
         }
 #pragma unroll
-        for (int i = 0; i < 3; i++) vq[i] = ldq(S.vr + i * N + k);
+        for (int i = 0; i < 3; i++) vq[i] = S.vr[i * N + k];
+#pragma unroll
+        for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + k);
 #pragma unroll
         for (int i = 0; i < 3; i++) nb[i] = ld_s(nbw + i * N + k);
         const int flags = ld_s(M.flags + k);
-        const double2 gr = ld_s(F.grav + k);
+        const double avg_h = ld_s(S.hc + k);
         // the two vertex records of the neighbour's matching edge (a boundary
         // edge reads its own, which are not used)
 #pragma unroll
@@ -172,11 +192,15 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
                 n = nb[i] & 0x1FFFFFFF;
                 m = nb[i] >> 29;
             }
-            na[i] = ldq(S.vr + next3(m) * N + n);
-            nc[i] = ldq(S.vr + prev3(m) * N + n);
+            na[i] = S.vr[next3(m) * N + n];
+            nc[i] = S.vr[prev3(m) * N + n];
         }
         TriGeom gm;
         tri_geometry(x, y, gm);
+        // gravity_wb's stage-gradient term from this step's vertex stage
+        const double2 v3[3] = {make_double2(x[0], y[0]), make_double2(x[1], y[1]), make_double2(x[2], y[2])};
+        const double wv3[3] = {vq[0].w, vq[1].w, vq[2].w};
+        const double2 gr = gravity_gradient(v3, wv3, avg_h, P.g);
         const double eps = P.epsilon, g = P.g;
         const int last_edge = (flags >> 10) & 3;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, l0 = 0.0, l1 = 0.0, l2 = 0.0;
@@ -184,7 +208,7 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
         double sidex = 0.0, sidey = 0.0, cfl_speed = 0.0, max_speed_total = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const Q4 own = edge_of(vq[next3(i)], vq[prev3(i)]);
+            const Q4 own = edge_of(vq[next3(i)], vq[prev3(i)], zv[next3(i)], zv[prev3(i)]);
             const bool lower = (flags >> i) & 1;
             const double lk = gm.len[i];
             {
@@ -196,7 +220,8 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
             double n1 = gm.nx[i], n2 = gm.ny[i];
             double ql[3], qr[3], zl, zr;
             if (nb[i] >= 0) {
-                const Q4 nbq = edge_of(na[i], nc[i]);
+                // both sides hold the same bed value on a shared edge (bed_cont)
+                const Q4 nbq = edge_of(na[i], nc[i], zv[next3(i)], zv[prev3(i)]);
                 if (lower) {
                     // the neighbour evaluates this edge in the C: its state on
                     // the left, its normal (the negation of ours), its length
@@ -262,7 +287,10 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
         s2 += gr.y;
         s1 += -sidex / area;
         s2 += -sidey / area;
-        if (!halted) S.eur[k] = Q4{s0, s1, s2, max_speed_total};
+        if (!halted) {
+            S.eur[k] = E3{s0, s1, s2};
+            st_s(F.max_speed + k, max_speed_total);
+        }
     }
     // workgroup minimum of the CFL candidates, one atomic into a slot
     __shared__ unsigned long long red[BLOCK / 64];
@@ -277,11 +305,6 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
     }
 }
 
-// The three explicit updates of a triangle's update record.
-struct E3 {
-    double w, uh, vh;
-};
-SW_DEV E3 ld_e3(const Q4 *p) { return E3{p->w, p->uh, p->vh}; }
 
 // update_one + protect_one of one triangle's centroid record (quantity.c:769-820,
 // shallow_water_ext.c:1638-1705); the momenta's semi-implicit terms come as
@@ -345,7 +368,7 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
     // neighbours' inputs, all before the state decides anything.
     const unsigned long long slotv = *cfl_slot(S, M.n_inst, p, bi.inst, lane);
     Q4 c = ldq(cin + kk);
-    const E3 eu = ld_e3(S.eur + kk);
+    const E3 eu = S.eur[kk];
     const double2 si = sin[kk];  // cached: the neighbours gather the same lines
     double x[3], y[3], zv[3];
 #pragma unroll
@@ -369,7 +392,7 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
     for (int i = 0; i < 3; i++) {
         const int n = r.sk[i];
         cn[i] = ldq(cin + n);
-        eun[i] = ld_e3(S.eur + n);
+        eun[i] = S.eur[n];
         sn[i] = sin[n];
         r.xs[i] = M.cc[n];
     }
@@ -430,7 +453,7 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
         }
         // a dry triangle keeps its previous vertex values
         auto stale = [&](int i, double &w, double &u, double &v) {
-            const Q4 o = ldq(S.vr + i * N + kk);
+            const V3 o = S.vr[i * N + kk];
             w = o.w;
             u = o.uh;
             v = o.vh;
@@ -439,7 +462,6 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
     }
     BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
     balance_one(c.w, c.z, wv, zv, c.uh, c.vh, uv, vv, bp);
-    const double2 grav = gravity_gradient_term(r.v, wv, c, P.g);
     const double2 fr = friction_next(P, eta, x, y, zv, c);
     if (!valid) return;
     if (stop) {
@@ -451,8 +473,8 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
     if (!ok) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrUpdate);
     if (!ok_nb) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrNeighbour);
 #pragma unroll
-    for (int i = 0; i < 3; i++) S.vr[i * N + k] = Q4{wv[i], uv[i], vv[i], zv[i]};
-    st_s(F.grav + k, grav);
+    for (int i = 0; i < 3; i++) S.vr[i * N + k] = V3{wv[i], uv[i], vv[i]};
+    st_s(S.hc + k, c.w - c.z);
     S.cqb[p ^ 1][k] = c;
     st_s(S.sir[p ^ 1] + k, make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh)));
 }
@@ -474,10 +496,9 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, S
         y[i] = v.y;
         zv[i] = F.zv[i * N + k];
         wv[i] = F.wv[i * N + k];
-        S.vr[i * N + k] = Q4{wv[i], F.uhv[i * N + k], F.vhv[i * N + k], zv[i]};
+        S.vr[i * N + k] = V3{wv[i], F.uhv[i * N + k], F.vhv[i * N + k]};
     }
-    const double2 v3[3] = {make_double2(x[0], y[0]), make_double2(x[1], y[1]), make_double2(x[2], y[2])};
-    F.grav[k] = gravity_gradient_term(v3, wv, c, P.g);
+    S.hc[k] = c.w - c.z;
     const double2 f = friction_next(P, F.friction[k], x, y, zv, c);
     S.sir[0][k] = make_double2(si_ratio(F.uhsi[k] + f.x, c.uh), si_ratio(F.vhsi[k] + f.y, c.vh));
     const bool live = F.wsi[k] != 0.0;
@@ -519,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_step_end(DevMesh M, DevFields F, Ste
     if (!ln.valid) return;
     const DevState &st = S.stv[0][ln.inst];
     if (p) F.cq[k] = S.cqb[1][k];
-    Q4 v[3];
+    V3 v[3];
     for (int i = 0; i < 3; i++) {
         v[i] = S.vr[i * N + k];
         F.wv[i * N + k] = v[i].w;
@@ -527,20 +548,19 @@ __global__ __launch_bounds__(kBlock) void k_step_end(DevMesh M, DevFields F, Ste
         F.vhv[i * N + k] = v[i].vh;
     }
     for (int i = 0; i < 3; i++) {
-        const Q4 e = edge_of(v[next3(i)], v[prev3(i)]);
+        const Q4 e = edge_of(v[next3(i)], v[prev3(i)], 0.0, 0.0);  // the bed edge value is kept
         Q4 o = F.eq[i * N + k];
         o.w = e.w;
         o.uh = e.uh;
         o.vh = e.vh;
         F.eq[i * N + k] = o;
     }
-    if (st.call_steps == 0) return;  // no step ran: the update records are stale
-    const Q4 u = S.eur[k];
+    // the first step's flux always runs (a call starts with no pending event)
+    const E3 u = S.eur[k];
     F.weu[k] = u.w;
     F.uheu[k] = u.uh;
     F.vheu[k] = u.vh;
-    F.max_speed[k] = u.z;
-    F.wsi[k] = 0.0;  // a caller-set stage term is used by one step (quantity.c:769-820)
+    if (st.call_steps > 0) F.wsi[k] = 0.0;  // a caller-set stage term is used by one step (quantity.c:769-820)
 }
 
 }  // namespace sw
