@@ -427,21 +427,39 @@ bool step2_applies(const sw_handle *h, const std::vector<DevState> &d) {
     return true;
 }
 
+// Two-kernel step launches at one of the workgroup sizes the autotuner
+// chooses from (the flux size for k_flux_v, the distribute size for k_advance).
+template <int B>
+void launch_flux_v(sw_handle *h) {
+    const unsigned g = (unsigned)((h->N + B - 1) / B);
+    hipLaunchKernelGGL((k_flux_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->nbw, h->dp, h->vpar);
+}
+template <int B>
+void launch_advance(sw_handle *h) {
+    const unsigned g = (unsigned)((h->N + B - 1) / B);
+    hipLaunchKernelGGL((k_advance<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
+}
+
 // Enqueues one two-kernel step at parity h->vpar; the timing slots are
 // flux (k_flux_v) and distribute (k_advance).
 int enqueue_step2(sw_handle *h, int slot) {
     auto rec = [&](int idx) {
         if (slot >= 0) (void)hipEventRecord(h->ev[slot * (kNumTimed + 1) + idx], h->stream);
     };
-    const unsigned g = grid_for(h->N);
     rec(0);
-    hipLaunchKernelGGL((k_flux_v<kBlock>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf, h->nbw,
-                       h->dp, h->vpar);
+    switch (h->blk[0]) {
+        case 128: launch_flux_v<128>(h); break;
+        case 512: launch_flux_v<512>(h); break;
+        default: launch_flux_v<256>(h); break;
+    }
     rec(1);
     rec(2);
     rec(3);
-    hipLaunchKernelGGL((k_advance<kBlock>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf, h->dp,
-                       h->vpar);
+    switch (h->blk[2]) {
+        case 128: launch_advance<128>(h); break;
+        case 512: launch_advance<512>(h); break;
+        default: launch_advance<256>(h); break;
+    }
     rec(4);
     h->vpar ^= 1;
     return launch_check(h, "two-kernel step");
