@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_sw.so")
+# ORACLE_LIB: another build of the same source (the sanitized build of the
+# tests' sanitizer run)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle_sw.so")
 
 _I64P = C.POINTER(C.c_int64)
 _F64P = C.POINTER(C.c_double)
