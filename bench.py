@@ -63,13 +63,14 @@ STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
 # timing reports k_flux_v as "flux" and k_advance as "distribute".
 KERNEL_BYTES_PER_TRIANGLE_TWO = {
     # reads: vertex xy 48, neighbour words 12, flags 4, the triangle's vertex
-    # records 3x32 96, gravity term 16 | writes: update record 32
-    "flux": 48 + 12 + 4 + 96 + 16 + 32,
-    # reads: centroid record 32, update record 32, semi-implicit pair 16,
-    # Manning n 8, vertex xy 48, surrogates 12, flags 4, bed vertex 24 |
-    # writes: centroid record 32, semi-implicit pair 16, vertex records 3x32 96,
-    # gravity term 16
-    "distribute": 32 + 32 + 16 + 8 + 48 + 12 + 4 + 24 + 32 + 16 + 96 + 16,
+    # records 3x24 72, bed vertex values 24, centroid depth 8 | writes: update
+    # record 24, max_speed 8
+    "flux": 48 + 12 + 4 + 72 + 24 + 8 + 24 + 8,
+    # reads: centroid record 32, update record 24, semi-implicit pair 16,
+    # Manning n 8, vertex xy 48, centroid 16, surrogates 12, flags 4, bed
+    # vertex values 24 | writes: centroid record 32, semi-implicit pair 16,
+    # vertex records 3x24 72, centroid depth 8
+    "distribute": 32 + 24 + 16 + 8 + 48 + 16 + 12 + 4 + 24 + 32 + 16 + 72 + 8,
 }
 # The unique footprint of one FP64 Euler step (each array read once, each
 # mutated array written once, the edge-value exchange counted as one write
@@ -102,7 +103,7 @@ def kernel_symbol(kernel, precision, launch, two=False):
     """Name of a fused kernel as rocprofv3 reports it; ``launch`` = workgroup
     sizes of (flux, update, distribute)."""
     if two:
-        return {"flux": "k_flux_v<256>", "distribute": "k_advance<256>"}[kernel]
+        return {"flux": "k_flux_v<%d>" % launch[0], "distribute": "k_advance<%d>" % launch[2]}[kernel]
     r = "double" if precision == "fp64" else "float"
     fb, ub, db = launch
     flux = "k_flux<%s, true, %d>" % (r, fb) if precision == "fp64" else "k_flux_mixed<%d>" % fb
