@@ -725,9 +725,10 @@ __global__ __launch_bounds__(kB) void k_base_flux(int64_t E, const int32_t *ea, 
 // ---------------------------------------------------------------------------
 // State update from the head's edge outputs (FP64 accumulation), loss and its
 // gradient with respect to the state change.
-// Packed edge record (32 bytes, int32[8]) read by the per-triangle kernels:
-// ea, eb, bits of nx, ny and len (FP32), closed flag, two zero words.  One
-// edge is two 16-byte loads instead of six scattered ones.
+// Packed edge record (32 bytes, int32[8]) read by the per-edge backward of
+// the update (k_apply_bwd): ea, eb, bits of nx, ny and len (FP32), closed
+// flag, two zero words.  One edge is two 16-byte loads instead of six
+// scattered ones.
 struct EdgeR {
     int a, b;
     float nx, ny, len;
@@ -739,36 +740,51 @@ GNN_DEV EdgeR load_edge(const int32_t *rec, int64_t e) {
     return EdgeR{r0.x, r0.y, __int_as_float(r0.z), __int_as_float(r0.w), __int_as_float(r1.x), r1.y != 0};
 }
 
+// Slot records of the per-triangle kernels, streamed in the order those
+// kernels consume them (instead of a gather of the edge's record): slot q of
+// triangle m (its tri_edges order) at 3m + q holds {the edge, the triangle
+// across it (-1: boundary), FP32 bits of the edge's length and nx}, sny[3m +
+// q] its ny; sflags[m] holds per slot bits 3q (an edge), 3q + 1 (closed),
+// 3q + 2 (m is the edge's side a).  The head's outputs stay per edge (F).
+struct SlotR {
+    int e, other;
+    float len, nx;
+};
+GNN_DEV SlotR load_slot(const int32_t *slots, int64_t k) {
+    const int4 r = reinterpret_cast<const int4 *>(slots)[k];
+    return SlotR{r.x, r.y, __int_as_float(r.z), __int_as_float(r.w)};
+}
+
 // Contributions of the three edges of triangle m to its integrated change:
 // mass (side-signed edge flux), x and y momentum (rotated side outputs).
 // With a group, the mass flux of an edge whose other side is in the group is
 // left out (it would cancel in the group's sum).
 // theta (may be NULL) scales the mass flux of each edge by the limiter factor
 // of the cell it drains (k_limit).
-__device__ __forceinline__ void edge_sums(int m, const int32_t *tri_edges, const int32_t *rec, const float *F,
+__device__ __forceinline__ void edge_sums(int m, const int32_t *slots, const float *sny, int sf, const float *F,
                                           float fs, const double *theta, const int *mem, double &sh, double &sx,
                                           double &sy) {
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const int e = tri_edges[(int64_t)m * 3 + q];
-        if (e < 0) continue;
-        const EdgeR ed = load_edge(rec, e);
-        const float4 f03 = *reinterpret_cast<const float4 *>(F + (int64_t)e * kFOut);
-        const float f4 = F[(int64_t)e * kFOut + 4];
-        const bool side_a = ed.a == m;
-        const double L = (double)ed.len * (double)fs;
-        const double n1 = ed.nx, n2 = ed.ny;
+        if (!((sf >> (3 * q)) & 1)) continue;
+        const int64_t k = (int64_t)m * 3 + q;
+        const SlotR sr = load_slot(slots, k);
+        const bool closed = (sf >> (3 * q + 1)) & 1, side_a = (sf >> (3 * q + 2)) & 1;
+        const float4 f03 = *reinterpret_cast<const float4 *>(F + (int64_t)sr.e * kFOut);
+        const float f0 = f03.x, f1 = f03.y, f2 = f03.z, f3 = f03.w, f4 = F[(int64_t)sr.e * kFOut + 4];
+        const double L = (double)sr.len * (double)fs;
+        const double n1 = sr.nx, n2 = sny[k];
         // the mass flux of an edge: one value, added to one side, subtracted from the other
-        const int other = side_a ? ed.b : ed.a;
+        const int other = sr.other;
         const bool internal = mem && other >= 0 &&
                               (other == mem[0] || other == mem[1] || other == mem[2] || other == mem[3]);
-        double c = (ed.closed || internal) ? 0.0 : (double)f03.x * L;
+        double c = (closed || internal) ? 0.0 : (double)f0 * L;
         if (theta) {
-            const int donor = c > 0.0 ? ed.a : ed.b;
+            const int donor = (c > 0.0) == side_a ? m : other;  // side a drains into b when c > 0
             if (donor >= 0) c *= theta[donor];
         }
         sh += side_a ? -c : c;
-        const double mn = side_a ? f03.y : f03.w, mt = side_a ? f03.z : f4;
+        const double mn = side_a ? f1 : f3, mt = side_a ? f2 : f4;
         sx += (mn * n1 - mt * n2) * L;
         sy += (mn * n2 + mt * n1) * L;
     }
@@ -808,8 +824,8 @@ __device__ __forceinline__ void finish_cell(int64_t N, int64_t i, double dh, dou
 // other side is in its group left out); a lone triangle takes its update at
 // once, a grouped one leaves its sums in M.  Then every member of a group
 // adds up its members' sums in member order and takes the group's change.
-__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edges, const int32_t *rec,
-                                              const double *area, const int32_t *group,
+__global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *slots, const float *sny,
+                                              const int32_t *sflags, const double *area, const int32_t *group,
                                               const double *group_area, const double *delta, const int32_t *graph_of,
                                               const float *F, float fs, const double *theta, const double *w0,
                                               const double *uh0,
@@ -822,13 +838,14 @@ __global__ __launch_bounds__(kB) void k_apply(int64_t N, const int32_t *tri_edge
     if (i >= N) return;
     double sh = 0.0, sx = 0.0, sy = 0.0;
     const int4 grow = group ? reinterpret_cast<const int4 *>(group)[i] : make_int4((int)i, -1, -1, -1);
+    const int sf = sflags[i];
     if (grow.y >= 0) {
         const int mem[4] = {grow.x, grow.y, grow.z, grow.w};
-        edge_sums((int)i, tri_edges, rec, F, fs, theta, mem, sh, sx, sy);
+        edge_sums((int)i, slots, sny, sf, F, fs, theta, mem, sh, sx, sy);
         M[i] = make_double4(sh, sx, sy, 0.0);
         return;
     }
-    edge_sums((int)i, tri_edges, rec, F, fs, theta, nullptr, sh, sx, sy);
+    edge_sums((int)i, slots, sny, sf, F, fs, theta, nullptr, sh, sx, sy);
     const double s = delta[graph_of[i]] / area[i];
     finish_cell(N, i, s * sh, s * sx, s * sy, area, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma, area_ref,
                 gq, loss_terms);
@@ -866,7 +883,7 @@ __global__ __launch_bounds__(kB) void k_apply_groups(int64_t N, const double *ar
 // outgoing mass fluxes would take more, all of them are scaled by
 // theta = volume / outflow.  Each edge takes the factor of the cell it drains,
 // on both sides, so mass stays conserved and no cell's volume goes below 0.
-__global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edges, const int32_t *rec,
+__global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *slots, const int32_t *sflags,
                                               const double *area, const int32_t *group, const double *delta,
                                               const int32_t *graph_of, const float *F, float fs, const double *w,
                                               const double *z, double *theta, double4 *M) {
@@ -876,17 +893,17 @@ __global__ __launch_bounds__(kB) void k_limit(int64_t N, const int32_t *tri_edge
     const bool grouped = grow.y >= 0;
     const double vol = area[i] * fmax(w[i] - z[i], 0.0);
     double out = 0.0;
+    const int sf = sflags[i];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const int e = tri_edges[i * 3 + q];
-        if (e < 0) continue;
-        const EdgeR ed = load_edge(rec, e);
-        if (ed.closed) continue;
-        const bool side_a = ed.a == i;
-        const int other = side_a ? ed.b : ed.a;
+        if (!((sf >> (3 * q)) & 1) || ((sf >> (3 * q + 1)) & 1)) continue;  // no edge, or a closed one
+        const int64_t k = i * 3 + q;
+        const SlotR sr = load_slot(slots, k);
+        const bool side_a = (sf >> (3 * q + 2)) & 1;
+        const int other = sr.other;
         if (grouped && other >= 0 && (other == grow.x || other == grow.y || other == grow.z || other == grow.w))
             continue;
-        const double c = (double)F[(int64_t)e * kFOut] * (double)ed.len * (double)fs;
+        const double c = (double)F[(int64_t)sr.e * kFOut] * (double)sr.len * (double)fs;
         out += side_a ? fmax(c, 0.0) : fmax(-c, 0.0);
     }
     if (grouped) {  // the group's factor is formed by k_limit_groups
@@ -1107,8 +1124,7 @@ int gnn_mp_direct(void *stream, int64_t E, int64_t Epad, const int32_t *nbr6, co
 int gnn_head(void *stream, int64_t Epad, const void *H4, const void *WT, const float *b, const float *P, float *F) {
     if (Epad % 16) return -1;
     hipLaunchKernelGGL(k_head, dim3(persist_blocks(Epad, resident_blocks(k_head))), dim3(kB), 0, S(stream), Epad,
-                       static_cast<const bf16 *>(H4),
-                       static_cast<const bf16 *>(WT), b, P, F);
+                       static_cast<const bf16 *>(H4), static_cast<const bf16 *>(WT), b, P, F);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1170,16 +1186,16 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
-              const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
-              const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
+int gnn_apply(void *stream, int64_t N, const int32_t *slots, const float *sny, const int32_t *sflags,
+              const double *area, const int32_t *group, const double *group_area, const double *delta,
+              const int32_t *graph_of, const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
               const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
               const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms,
               double *cell_sums) {
     if (group && (!group_area || !cell_sums)) return -1;
     double4 *M = reinterpret_cast<double4 *>(cell_sums);
-    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, rec, area, group, group_area,
-                       delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma,
+    hipLaunchKernelGGL(k_apply, dim3(blocks(N)), dim3(kB), 0, S(stream), N, slots, sny, sflags, area, group,
+                       group_area, delta, graph_of, F, flux_scale, theta, w0, uh0, vh0, w1, uh1, vh1, dw_t, duh_t, dvh_t, sigma,
                        area_ref, gq, loss_terms, M);
     if (group)
         hipLaunchKernelGGL(k_apply_groups, dim3(blocks(N)), dim3(kB), 0, S(stream), N, area, group, group_area, delta,
@@ -1197,12 +1213,12 @@ int gnn_base_flux(void *stream, int64_t E, const int32_t *ea, const int32_t *eb,
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
+int gnn_limit(void *stream, int64_t N, const int32_t *slots, const int32_t *sflags, const double *area,
               const int32_t *group, const double *delta, const int32_t *graph_of, const float *F, float flux_scale,
               const double *w, const double *z, double *theta, double *cell_sums) {
     if (group && !cell_sums) return -1;
     double4 *M = reinterpret_cast<double4 *>(cell_sums);
-    hipLaunchKernelGGL(k_limit, dim3(blocks(N)), dim3(kB), 0, S(stream), N, tri_edges, rec, area, group, delta,
+    hipLaunchKernelGGL(k_limit, dim3(blocks(N)), dim3(kB), 0, S(stream), N, slots, sflags, area, group, delta,
                        graph_of, F, flux_scale, w, z, theta, M);
     if (group)
         hipLaunchKernelGGL(k_limit_groups, dim3(blocks(N)), dim3(kB), 0, S(stream), N, group, delta, graph_of, M,

@@ -77,7 +77,7 @@ def _load():
         _gnn.gnn_concat.argtypes = [p, i64, i64, p, p, p, p, p]
         _gnn.gnn_node_grad.argtypes = [p, i64, p, p, p]
         _gnn.gnn_edge_grad.argtypes = [p, i64, i64, p, p, p, p, p, p]
-        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 8 + [f] + [p] * 11 + [C.c_double, p, p, p]
+        _gnn.gnn_apply.argtypes = [p, i64] + [p] * 9 + [f] + [p] * 11 + [C.c_double, p, p, p]
         _gnn.gnn_limit.argtypes = [p, i64] + [p] * 7 + [f, p, p, p, p]
         _gnn.gnn_protect.argtypes = [p, i64, p, p, p, p, C.c_double]
         _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 6 + [f, p, p]
@@ -234,6 +234,33 @@ class EdgeGraph:
         rec[:, 4] = np.tile(ln, B).astype(np.float32).view(np.int32)
         rec[:, 5] = np.tile(closed, B)
         self.rec = torch.tensor(rec, device=dev)
+        # per triangle slot (tri_edges order) the edge's data in the order the
+        # limiter and the update stream it: {edge, triangle across (-1:
+        # boundary), FP32 bits of len, nx}, ny apart, flags per triangle
+        # (bits 3q: an edge, 3q+1: closed, 3q+2: the triangle is side a)
+        NB = B * N
+        tri = np.repeat(np.arange(NB), 3)
+        e_s = te.ravel()
+        has = e_s >= 0
+        es = np.where(has, e_s, 0)
+        side_a = has & (ea_all[es] == tri)
+        other = np.where(side_a, eb_all[es], ea_all[es])
+        slots = np.zeros((NB * 3, 4), dtype=np.int32)
+        slots[:, 0] = np.where(has, e_s, -1)
+        slots[:, 1] = np.where(has, other, -1)
+        ln_all = np.tile(ln, B).astype(np.float32)
+        nx_all = np.tile(nrm[:, 0], B).astype(np.float32)
+        ny_all = np.tile(nrm[:, 1], B).astype(np.float32)
+        slots[:, 2] = np.where(has, ln_all[es].view(np.int32), 0)
+        slots[:, 3] = np.where(has, nx_all[es].view(np.int32), 0)
+        sny = np.where(has, ny_all[es], np.float32(0.0)).astype(np.float32)
+        closed_all = np.tile(closed, B)
+        bits = (has.astype(np.int32) | ((has & (closed_all[es] != 0)).astype(np.int32) << 1)
+                | (side_a.astype(np.int32) << 2)).reshape(NB, 3)
+        sflags = (bits[:, 0] | (bits[:, 1] << 3) | (bits[:, 2] << 6)).astype(np.int32)
+        self.slots = torch.tensor(slots, device=dev)
+        self.sny = torch.tensor(sny, device=dev)
+        self.sflags = torch.tensor(sflags, device=dev)
         # per edge, the edges of its two triangles in tri_edges order (ea's,
         # then eb's; -1 for a missing triangle and the padding rows): the
         # inference layers form the node sums from these rows
@@ -413,8 +440,8 @@ def apply(net, graph, ws, delta, w0, uh0, vh0, w1, uh1, vh1, targets=None, theta
     the positivity limiter's factors (limit())."""
     L, s, g = net.lib, net.stream(), graph
     dw, duh, dvh = targets if targets is not None else (None, None, None)
-    _ck(L.gnn_apply(s, g.N, _ptr(g.tri_edges), _ptr(g.rec), _ptr(g.area), _ptr(g.group), _ptr(g.group_area),
-                    _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(theta), _ptr(w0), _ptr(uh0),
+    _ck(L.gnn_apply(s, g.N, _ptr(g.slots), _ptr(g.sny), _ptr(g.sflags), _ptr(g.area), _ptr(g.group),
+                    _ptr(g.group_area), _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(theta), _ptr(w0), _ptr(uh0),
                     _ptr(vh0), _ptr(w1), _ptr(uh1), _ptr(vh1), _ptr(dw), _ptr(duh), _ptr(dvh), _ptr(net.sigma),
                     graph.mean_area, _ptr(ws.gq), _ptr(ws.loss_terms), _ptr(ws.cells)), "apply")
 
@@ -668,7 +695,7 @@ def torch_index(a):
 def limit(net, graph, ws, delta, w, z):
     """Positivity limiter factors of the current edge outputs (ws.theta)."""
     g = graph
-    _ck(net.lib.gnn_limit(net.stream(), g.N, _ptr(g.tri_edges), _ptr(g.rec), _ptr(g.area), _ptr(g.group),
+    _ck(net.lib.gnn_limit(net.stream(), g.N, _ptr(g.slots), _ptr(g.sflags), _ptr(g.area), _ptr(g.group),
                           _ptr(delta), _ptr(g.graph_of), _ptr(ws.F), net.scales["flux"], _ptr(w), _ptr(z),
                           _ptr(ws.theta), _ptr(ws.cells)), "limit")
     return ws.theta
@@ -1071,12 +1098,13 @@ def learned_kernel_bytes(N, E):
         # own row 128 + neighbour list 32, writes the new row 128; third layer:
         # + physical part 32, writes F 32 instead of the row
         "k_mp_direct": E * (2 * (128 + 32 + 128) + (128 + 32 + 32 + 32)) // 3,
-        # tri_edges 12, group row 16, area, w, z 24, theta 8, graph_of 4;
-        # per edge ea, eb, len, closed 16 and the mass flux 4
-        "k_limit": N * 64 + E * 20,
-        # tri_edges 12, group 16, areas 16, graph_of 4, theta 8, state in
-        # 24 and out 24; per edge the record 32 and five outputs 20
-        "k_apply": N * 104 + E * 52,
+        # slot flags 4, per slot the edge, the other triangle and the length
+        # 3x12, group row 16, area, w, z 24, theta 8, graph_of 4; per edge the
+        # mass flux 4
+        "k_limit": N * (4 + 36 + 16 + 24 + 8 + 4) + E * 4,
+        # slot flags 4, slot records 3x16 and ny 3x4, group 16, areas 16,
+        # graph_of 4, theta 8, state in 24 and out 24; per edge five outputs 20
+        "k_apply": N * (4 + 48 + 12 + 16 + 16 + 4 + 8 + 24 + 24) + E * 20,
         # w and z read, both momenta written (at most: only where dry)
         "k_protect_dry": N * 32,
     }

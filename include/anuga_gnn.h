@@ -102,14 +102,17 @@ int gnn_node_grad(void *stream, int64_t N, const int32_t *tri_edges, const void 
 int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, const int32_t *eb, const void *dHnext,
                   const void *dC, const void *dS, void *dH);
 
-/* The per-triangle kernels below read each edge from rec[E][8] (int32; 32
- * bytes): ea, eb, the FP32 bits of nx, ny and len, closed, 0, 0.
+/* The per-triangle kernels below read triangle i's edges per slot q (its
+ * tri_edges order, s = 3i + q), streamed: slots[s][4] (int32: the edge, the
+ * triangle across it, -1 on a boundary, the FP32 bits of its length and nx),
+ * sny[s] (its ny, FP32; gnn_apply only), sflags[i] (bits 3q: an edge in the
+ * slot, 3q + 1: closed, 3q + 2: i is its side a).
  *
  * Apply the head's outputs F[E][8] (FP32; columns 0: mass flux, 1-2: side-a
  * momentum (normal, tangential), 3-4: side-b momentum) over a step delta:
  * dq[i] = delta / area[i] * sum over edges of side-signed, rotated
- * contributions * len * flux_scale; closed[e] != 0 zeroes the mass flux of a
- * wall.  group (may be NULL) merges small cells: group[i][4] lists the
+ * contributions * len * flux_scale; a closed edge (a wall) carries no mass
+ * flux.  group (may be NULL) merges small cells: group[i][4] lists the
  * members of triangle i's group (-1 padded; a lone triangle lists itself
  * only), group_area[i] the group's area; every member then takes
  * delta / group_area * (sum of the members' contributions, the mass flux of
@@ -123,9 +126,9 @@ int gnn_edge_grad(void *stream, int64_t E, int64_t Epad, const int32_t *ea, cons
  * each grouped triangle's own sums between the two passes: first every
  * triangle sums its own edges, then a group's members add them up in member
  * order. */
-int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
-              const int32_t *group, const double *group_area, const double *delta, const int32_t *graph_of,
-              const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
+int gnn_apply(void *stream, int64_t N, const int32_t *slots, const float *sny, const int32_t *sflags,
+              const double *area, const int32_t *group, const double *group_area, const double *delta,
+              const int32_t *graph_of, const float *F, float flux_scale, const double *theta, const double *w0, const double *uh0,
               const double *vh0, double *w1, double *uh1, double *vh1, const double *dw_t, const double *duh_t,
               const double *dvh_t, const float *sigma, double area_ref, float *gq, float *loss_terms,
               double *cell_sums);
@@ -137,7 +140,7 @@ int gnn_apply(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *
  * open edges.  Used at inference; training treats theta as 1.  With groups,
  * cell_sums (4 N doubles of scratch) holds each grouped triangle's volume and
  * outflow between the two passes. */
-int gnn_limit(void *stream, int64_t N, const int32_t *tri_edges, const int32_t *rec, const double *area,
+int gnn_limit(void *stream, int64_t N, const int32_t *slots, const int32_t *sflags, const double *area,
               const int32_t *group, const double *delta, const int32_t *graph_of, const float *F, float flux_scale,
               const double *w, const double *z, double *theta, double *cell_sums);
 
