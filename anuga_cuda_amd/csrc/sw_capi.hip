@@ -82,6 +82,14 @@ struct sw_handle {
     int step_path = SW_STEP_AUTO; // caller's choice
     int vpar = 0;                 // parity of the next two-kernel step
     int graph_path = 0;           // step path the captured graph holds
+    // After a two-kernel evolve call the step's records (vertex records,
+    // centroid records and semi-implicit pairs at parity vpar, centroid
+    // depths) hold the state.  records_live: they are what k_step_begin would
+    // build from the operator API's arrays, so the next call continues from
+    // them.  api_pending: those arrays are behind the records (k_step_end and
+    // the epilogue run when something reads or writes them: sync_api).
+    bool records_live = false;
+    bool api_pending = false;
 };
 
 namespace {
@@ -525,6 +533,26 @@ int epilogue(sw_handle *h) {
     return launch_check(h, "evolve epilogue");
 }
 
+// Brings the operator API's arrays up to the two-kernel step's records
+// (k_step_end, then the epilogue of the evolve call), if they are behind.
+// Entry points that read those arrays call it first.
+int sync_api(sw_handle *h) {
+    if (!h->api_pending) return SW_OK;
+    h->api_pending = false;
+    hipLaunchKernelGGL(k_step_end, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf, h->vpar);
+    if (int rc = launch_check(h, "k_step_end")) return rc;
+    return epilogue(h);
+}
+
+// Entry points that may write the operator API's arrays or change what the
+// step reads: the arrays are brought up to date first, and the next evolve
+// call rebuilds the step's records from them.
+int touch_api(sw_handle *h) {
+    const int rc = sync_api(h);
+    h->records_live = false;
+    return rc;
+}
+
 // Position of cell (x, y) along the Hilbert curve filling a 2^16 x 2^16 grid
 // (the iterative xy -> d conversion with quadrant rotation).
 uint64_t hilbert_index(uint32_t x, uint32_t y) {
@@ -900,6 +928,7 @@ int sw_destroy(sw_handle *h) {
 
 int sw_set_params(sw_handle *h, const sw_params *p) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = check_params(h, p)) return rc;
     h->params = *p;
     h->dp = to_dev(*p);
@@ -915,6 +944,7 @@ int sw_get_params(const sw_handle *h, sw_params *p) {
 
 int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const Inst &I = h->inst[i];
@@ -937,6 +967,7 @@ int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet) 
 
 int sw_upload(sw_handle *h, int quantity, int location, const double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const Inst &I = h->inst[i];
@@ -973,6 +1004,7 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
 
 int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const Inst &I = h->inst[i];
@@ -1037,6 +1069,7 @@ static int download_async(sw_handle *h, int i, int quantity, int location, doubl
 
 int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsrc, int64_t count) {
     if (!h || !dsrc) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     if (int rc = upload_async(h, i, quantity, location, dsrc, count)) return rc;
@@ -1046,6 +1079,7 @@ int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsr
 
 int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, int64_t count) {
     if (!h || !ddst) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     if (int rc = download_async(h, i, quantity, location, ddst, count)) return rc;
@@ -1056,6 +1090,7 @@ int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, i
 int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, const double *ymom, double time,
                       double yieldstep) {
     if (!h || !stage || !xmom || !ymom) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (h->n_inst != 1) return fail(h, SW_ERR_STATE, "sw_restart_device restarts a single domain, not an ensemble");
     const int64_t n = h->inst[0].n;
     if (int rc = upload_async(h, 0, SW_STAGE, SW_CENTROID, stage, n)) return rc;
@@ -1075,6 +1110,7 @@ int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, con
 
 int sw_download_conserved_device(sw_handle *h, double *stage, double *xmom, double *ymom) {
     if (!h || !stage || !xmom || !ymom) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const int64_t n = h->inst[i].n;
@@ -1087,6 +1123,7 @@ int sw_download_conserved_device(sw_handle *h, double *stage, double *xmom, doub
 
 int sw_download_max_speed(sw_handle *h, double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const Inst &I = h->inst[i];
@@ -1124,6 +1161,7 @@ int sw_set_state(sw_handle *h, const sw_state *s) {
 
 int sw_compute_fluxes(sw_handle *h, double *flux_timestep) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "compute_fluxes")) return rc;
     const unsigned g = grid_for(h->N);
     if (h->params.precision == SW_PRECISION_MIXED)
@@ -1143,6 +1181,7 @@ int sw_compute_fluxes(sw_handle *h, double *flux_timestep) {
 
 int sw_gravity_wb(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "gravity_wb")) return rc;
     hipLaunchKernelGGL(k_gravity_wb, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "gravity_wb");
@@ -1150,6 +1189,7 @@ int sw_gravity_wb(sw_handle *h) {
 
 int sw_manning_friction(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "manning_friction")) return rc;
     hipLaunchKernelGGL(k_manning, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "manning_friction");
@@ -1157,6 +1197,7 @@ int sw_manning_friction(sw_handle *h) {
 
 int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "update_timestep")) return rc;
     (void)yieldstep;
     DevState d;
@@ -1171,6 +1212,7 @@ int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t
 
 int sw_update_conserved_quantities(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "update_conserved_quantities")) return rc;
     hipLaunchKernelGGL((k_update<false>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st,
                        kUpdateEuler, 0.0, 0.0);
@@ -1182,6 +1224,7 @@ int sw_update_conserved_quantities(sw_handle *h) {
 int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *number_of_triangles_per_node,
                       const int64_t *vertex_value_indices) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const Inst &I = h->inst[i];
@@ -1246,6 +1289,7 @@ int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *num
 
 int sw_sww_values(sw_handle *h, double minimum_storable_height, float *out, int64_t count) {
     if (!h || !out) return SW_ERR_ARG;
+    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const SwwPoints &S = h->sww[i];
@@ -1262,6 +1306,7 @@ int sw_sww_values(sw_handle *h, double minimum_storable_height, float *out, int6
 
 int sw_backup_conserved_quantities(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "backup_conserved_quantities")) return rc;
     hipLaunchKernelGGL(k_backup, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "backup_conserved_quantities");
@@ -1269,6 +1314,7 @@ int sw_backup_conserved_quantities(sw_handle *h) {
 
 int sw_saxpy_conserved_quantities(sw_handle *h, double a, double b) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "saxpy_conserved_quantities")) return rc;
     hipLaunchKernelGGL(k_saxpy, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, a, b);
     return launch_check(h, "saxpy_conserved_quantities");
@@ -1276,6 +1322,7 @@ int sw_saxpy_conserved_quantities(sw_handle *h, double a, double b) {
 
 int sw_protect(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "protect")) return rc;
     hipLaunchKernelGGL(k_protect, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "protect");
@@ -1283,6 +1330,7 @@ int sw_protect(sw_handle *h) {
 
 int sw_extrapolate_second_order_sw(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "extrapolate_second_order_sw")) return rc;
     if (h->params.precision == SW_PRECISION_MIXED)
         hipLaunchKernelGGL((k_extrapolate_second_order<float>), dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp, h->st);
@@ -1295,6 +1343,7 @@ int sw_extrapolate_second_order_sw(sw_handle *h) {
 
 static int extrapolate_limited(sw_handle *h, int limiter, const char *what) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, what)) return rc;
     DevParams dp = h->dp;
     dp.limiter = limiter;
@@ -1312,6 +1361,7 @@ int sw_extrapolate_second_order_and_limit_by_edge(sw_handle *h) {
 
 int sw_extrapolate_first_order(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "extrapolate_first_order")) return rc;
     hipLaunchKernelGGL(k_first_order_conserved, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "extrapolate_first_order");
@@ -1319,6 +1369,7 @@ int sw_extrapolate_first_order(sw_handle *h) {
 
 int sw_balance_deep_and_shallow(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "balance_deep_and_shallow")) return rc;
     hipLaunchKernelGGL(k_balance, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     return launch_check(h, "balance_deep_and_shallow");
@@ -1326,6 +1377,7 @@ int sw_balance_deep_and_shallow(sw_handle *h) {
 
 int sw_interpolate_from_vertices_to_edges(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "interpolate_from_vertices_to_edges")) return rc;
     hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "interpolate_from_vertices_to_edges");
@@ -1333,6 +1385,7 @@ int sw_interpolate_from_vertices_to_edges(sw_handle *h) {
 
 int sw_distribute_to_vertices_and_edges(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "distribute_to_vertices_and_edges")) return rc;
     DevState d;
     if (int rc = sync_state(h, 0, &d)) return rc;
@@ -1352,6 +1405,7 @@ int sw_distribute_to_vertices_and_edges(sw_handle *h) {
 
 int sw_update_boundary(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (h->Nb == 0) return SW_OK;
     hipLaunchKernelGGL(k_update_boundary, dim3(grid_for(h->Nb)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "update_boundary");
@@ -1359,6 +1413,7 @@ int sw_update_boundary(sw_handle *h) {
 
 int sw_update_other_quantities(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     const int64_t n = h->Nb > h->N ? h->Nb : h->N;
     hipLaunchKernelGGL(k_other_quantities, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     return launch_check(h, "update_other_quantities");
@@ -1393,6 +1448,7 @@ int distribute_all(sw_handle *h) {
 
 int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (h->n_inst > 1 && h->cur >= 0)
         return fail(h, SW_ERR_STATE, "an ensemble is prepared as a whole (select instance -1)");
     std::vector<DevState> d;
@@ -1458,13 +1514,24 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     int64_t launched = 0;
     int rc = SW_OK;
     const bool mixed = h->params.precision == SW_PRECISION_MIXED;
-    const bool two = step2_applies(h, d0);
-    if (max_steps > 0 && two) {
-        h->vpar = 0;
-        hipLaunchKernelGGL(k_step_reset, dim3(h->n_inst), dim3(kSlots), 0, h->stream, h->sf, h->n_inst);
-        hipLaunchKernelGGL(k_step_begin, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf,
-                           h->dp);
+    const bool two = step2_applies(h, d0) && max_steps > 0;
+    // the other paths step the operator API's arrays themselves
+    if (!two)
+        if (int rc0 = touch_api(h)) return rc0;
+    if (two) {
+        // continue from the records of the previous call, or build them
+        const bool fresh = !h->records_live;
+        if (fresh) h->vpar = 0;
+        hipLaunchKernelGGL(k_step_reset, dim3(h->n_inst), dim3(kSlots), 0, h->stream, h->sf, h->n_inst, h->vpar,
+                           fresh ? 1 : 0);
+        if (fresh)
+            hipLaunchKernelGGL(k_step_begin, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf,
+                               h->dp);
         rc = launch_check(h, "k_step_begin");
+        // from here on the records hold the state; the operator API's arrays
+        // follow when something reads them (sync_api)
+        h->records_live = true;
+        h->api_pending = true;
     } else if (max_steps > 0) {
         if (mixed) {
             const unsigned g3 = grid_for(3 * h->N);
@@ -1514,17 +1581,12 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         // the mixed loop keeps FP32 edge state: rebuild the FP64 edge records
         hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     }
-    if (two && max_steps > 0) {
-        // the two-kernel step keeps vertex, update and double-buffered centroid
-        // records: back into the arrays of the operator API
-        hipLaunchKernelGGL(k_step_end, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf,
-                           h->vpar);
-        h->vpar = 0;
-        if (int rc4 = launch_check(h, "k_step_end"))
-            if (rc == SW_OK) rc = rc4;
-    }
-    int rc2 = epilogue(h);
-    int rc3 = sync_all(h, d);
+    // The two-kernel step's records stay authoritative (k_step_end and the
+    // epilogue are deferred to sync_api); the states read after the last
+    // batch are current unless the loop stopped on a launch error.
+    int rc2 = SW_OK, rc3 = SW_OK;
+    if (!two) rc2 = epilogue(h);
+    if (!two || rc != SW_OK || launched == 0) rc3 = sync_all(h, d);
     int64_t steps = 0;
     int32_t ev = SW_EVENT_NONE;
     for (int i = 0; i < h->n_inst && i < (int)d.size(); i++) {
@@ -1559,6 +1621,7 @@ int sw_advance_yield(sw_handle *h) {
 
 int sw_evolve_one_step_unfused(sw_handle *h, int32_t *event) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (int rc = single(h, "evolve_one_step_unfused")) return rc;
     DevState d;
     if (int rc = sync_state(h, 0, &d)) return rc;
@@ -1634,6 +1697,7 @@ int sw_autotune(sw_handle *h, int32_t steps, int32_t *chosen) {
 
 int sw_set_step_path(sw_handle *h, int32_t path) {
     if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
     if (path != SW_STEP_AUTO && path != SW_STEP_FOUR)
         return fail(h, SW_ERR_ARG, "step path %d (SW_STEP_AUTO or SW_STEP_FOUR)", path);
     h->step_path = path;

@@ -506,14 +506,22 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, S
         atomicOr((unsigned long long *)&S.stv[0][ln.inst].wsi_live, 1ull);
 }
 
-// Per instance: both parities' slots at +inf, the stage semi-implicit flag
-// cleared (k_step_begin sets it again), no step run yet in this call.
-__global__ void k_step_reset(StepFields S, int n_inst) {
+// Start of an evolve call whose first step has parity p, per instance: both
+// parities' slots at +inf.  A fresh start (k_step_begin follows, p = 0)
+// clears the stage semi-implicit flag (k_step_begin sets it again) and the
+// count of steps run since the records were built; a call that continues
+// from the records of the previous one keeps both, and its first step reads
+// the handle's state (stv[0], which the caller may have changed between the
+// calls) at parity p.
+__global__ void k_step_reset(StepFields S, int n_inst, int p, int fresh) {
     const int inst = blockIdx.x;
-    for (int p = 0; p < 2; p++) *cfl_slot(S, n_inst, p, inst, threadIdx.x) = ~0ull;
+    for (int q = 0; q < 2; q++) *cfl_slot(S, n_inst, q, inst, threadIdx.x) = ~0ull;
     if (threadIdx.x == 0) {
-        S.stv[0][inst].wsi_live = 0;
-        S.stv[0][inst].call_steps = 0;
+        if (fresh) {
+            S.stv[0][inst].wsi_live = 0;
+            S.stv[0][inst].call_steps = 0;
+        }
+        if (p) S.stv[1][inst] = S.stv[0][inst];
     }
 }
 
@@ -529,10 +537,13 @@ __global__ void k_step_settle(StepFields S, int n_inst, int p) {
     S.stv[0][inst] = s;
 }
 
-// End of an evolve call whose next step has parity p: the current centroid
-// records into buffer 0, and the vertex records and update records into the
-// arrays of the operator API (vertex values; edge values as their
-// interpolation, quantity.c:687-707; explicit updates; max_speed).
+// The operator API's arrays from the records of the two-kernel step, whose
+// next step has parity p (run when something reads or writes those arrays
+// after one or more evolve calls): the current centroid records into buffer
+// 0, and the vertex records and update records into the arrays of the
+// operator API (vertex values; edge values as their interpolation,
+// quantity.c:687-707; explicit updates; max_speed).  The records stay valid,
+// so a later call continues from them at parity p.
 __global__ __launch_bounds__(kBlock) void k_step_end(DevMesh M, DevFields F, StepFields S, int p) {
     const int N = M.N;
     const Lane ln = lane_of(M);
@@ -560,7 +571,9 @@ __global__ __launch_bounds__(kBlock) void k_step_end(DevMesh M, DevFields F, Ste
     F.weu[k] = u.w;
     F.uheu[k] = u.uh;
     F.vheu[k] = u.vh;
-    if (st.call_steps > 0) F.wsi[k] = 0.0;  // a caller-set stage term is used by one step (quantity.c:769-820)
+    // a caller-set stage term is used by one step (quantity.c:769-820);
+    // call_steps counts the steps run since k_step_begin built the records
+    if (st.call_steps > 0) F.wsi[k] = 0.0;
 }
 
 }  // namespace sw

@@ -594,6 +594,78 @@ def test_two_kernel_step_equals_four_kernel_step():
     assert np.array_equal(a[3], b[3])
 
 
+def _api_arrays(d):
+    out = {}
+    for q in ("stage", "xmomentum", "ymomentum", "height", "xvelocity", "yvelocity"):
+        for loc in ("centroid_values", "vertex_values", "edge_values", "boundary_values"):
+            out[(q, loc)] = device_array(d, q, loc)
+    for q in ("stage", "xmomentum", "ymomentum"):
+        for loc in ("explicit_update", "semi_implicit_update"):
+            out[(q, loc)] = device_array(d, q, loc)
+    out["max_speed"] = d._handle.max_speed()
+    return out
+
+
+def test_chained_calls_continue_from_the_step_records():
+    """Consecutive two-kernel evolve calls continue from the step's records
+    at whatever parity the previous call ended on, and the operator API's
+    arrays are brought up to date only when read: 40 steps as one call, as
+    calls of 1, 2, 17, 3, 16 and 1 steps with nothing read in between, and as
+    the same calls with every array read (and the stage re-uploaded
+    unchanged, which rebuilds the records) after each, give the same bits."""
+    chunks = [1, 2, 17, 3, 16, 1]
+    runs = []
+    for mode in ("one", "chained", "read"):
+        d = _delaunay("morton")
+        h = d._dev()
+        assert h.step_path() == _lib.SW_STEP_TWO
+        h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+        states = []
+        for n in ([sum(chunks)] if mode == "one" else chunks):
+            done, ev = h.evolve_steps(n)
+            assert (done, ev) == (n, _lib.SW_EVENT_NONE)
+            s = h.get_state()
+            states.append((s.time, s.timestep, s.number_of_steps))
+            if mode == "read":
+                _api_arrays(d)
+                w = device_array(d, "stage", "centroid_values")
+                h.upload(_lib.SW_STAGE, _lib.SW_CENTROID, w)
+        runs.append((states[-1], _api_arrays(d)))
+        d.close()
+    (s0, a0) = runs[0]
+    for s, a in runs[1:]:
+        assert s == s0
+        for key in a0:
+            assert np.array_equal(a[key], a0[key]), key
+
+
+def test_upload_between_chained_calls_matches_oracle():
+    """A caller's upload between two evolve calls reaches the next step (the
+    step's records are rebuilt from the operator API's arrays), as in the
+    oracle given the same values."""
+    d = _delaunay("morton")
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    od.evolve_prepare(1e9, None)
+    assert h.evolve_steps(5)[0] == 5
+    for _ in range(5):
+        od.evolve_step()
+    w = device_array(d, "stage", "centroid_values")
+    bed = od.q("elevation", "centroid_values")
+    w = np.where(w - bed > 0.05, w + 0.01, w)
+    h.upload(_lib.SW_STAGE, _lib.SW_CENTROID, w)
+    od.q("stage", "centroid_values")[:] = w
+    stats = {}
+    for n in (2, 3):
+        assert h.evolve_steps(n)[0] == n
+        for _ in range(n):
+            od.evolve_step()
+    s = h.get_state()
+    compare([s.time], [od.d.time], "time", TOL, stats)
+    compare_state(d, od, TOL, stats=stats, tag="after upload.")
+
+
 def test_two_kernel_step_needs_recomputable_geometry():
     """A mesh whose geometry arrays are not what the vertex coordinates give
     (here a radius changed in its last bit) keeps the four-kernel step."""
