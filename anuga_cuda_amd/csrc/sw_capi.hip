@@ -738,6 +738,54 @@ bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t 
     return true;
 }
 
+// Mesh nodes of the two-kernel step: the vertex slots of each instance whose
+// coordinates are bitwise equal share a node.  Nodes are numbered in the
+// order the device numbering first reaches them, so the node arrays follow
+// the triangles' locality order.  tri_node[i * Ntot + d] is the node of
+// vertex i of device triangle d (padding triangles: node 0); node_xy holds
+// the coordinates.  Open addressing over the coordinate bits.
+void build_nodes(const std::vector<double> &vcv, int64_t Ntot, const std::vector<Inst> &inst,
+                 std::vector<int> &tri_node, std::vector<double> &node_xy) {
+    tri_node.assign(3 * Ntot, 0);
+    node_xy.clear();
+    for (const Inst &I : inst) {
+        size_t cap = 16;
+        while (cap < (size_t)(3 * I.n) * 2) cap <<= 1;
+        std::vector<int> table(cap, -1);  // node id, or -1
+        for (int64_t d = I.off; d < I.off + I.n; d++) {
+            for (int i = 0; i < 3; i++) {
+                const double x = vcv[2 * (i * Ntot + d)], y = vcv[2 * (i * Ntot + d) + 1];
+                uint64_t bx, by;
+                std::memcpy(&bx, &x, 8);
+                std::memcpy(&by, &y, 8);
+                uint64_t hsh = (bx * 0x9E3779B97F4A7C15ull) ^ (by + 0x632BE59BD9B4E019ull + (bx << 6) + (bx >> 2));
+                hsh ^= hsh >> 29;
+                size_t s = (size_t)(hsh * 0xBF58476D1CE4E5B9ull) & (cap - 1);
+                int node = -1;
+                while (table[s] >= 0) {
+                    const int c = table[s];
+                    uint64_t cx, cy;
+                    std::memcpy(&cx, &node_xy[2 * c], 8);
+                    std::memcpy(&cy, &node_xy[2 * c + 1], 8);
+                    if (cx == bx && cy == by) {
+                        node = c;
+                        break;
+                    }
+                    s = (s + 1) & (cap - 1);
+                }
+                if (node < 0) {
+                    node = (int)(node_xy.size() / 2);
+                    table[s] = node;
+                    node_xy.push_back(x);
+                    node_xy.push_back(y);
+                }
+                tri_node[i * Ntot + d] = node;
+            }
+        }
+    }
+    if (node_xy.empty()) node_xy.assign(2, 0.0);
+}
+
 sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, int device) {
     g_create_error.clear();
     if (!meshes || n_inst < 1) {
@@ -866,6 +914,16 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     if (dalloc(h, &h->st, n_inst)) return bail();
     h->step2_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
     {
+        std::vector<int> tri_node;
+        std::vector<double> node_xy;
+        build_nodes(vcv, Ntot, inst, tri_node, node_xy);
+        int *dtn;
+        double *dnxy;
+        if (put(&dtn, tri_node) || put(&dnxy, node_xy) || dalloc(h, &F.node_z, node_xy.size() / 2)) return bail();
+        M.tri_node = dtn;
+        M.node_xy = reinterpret_cast<const double2 *>(dnxy);
+    }
+    {
         StepFields &S = h->sf;
         if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.hc, N) ||
             dalloc(h, &S.cqb[1], N) ||
@@ -993,9 +1051,11 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
     }
     if (quantity == SW_ELEVATION && (location == SW_VERTEX || location == SW_EDGE)) {
         hipLaunchKernelGGL(k_set_bed_interp, dim3(1), dim3(1), 0, h->stream, h->st + i, 1ll);
+        hipLaunchKernelGGL(k_node_bed, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (int)h->N, (int)I.off,
+                           (int)I.n, (const double *)h->f.zv, h->mesh.tri_node, h->f.node_z);
         hipLaunchKernelGGL(k_bed_derived, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (int)h->N, (int)I.off,
                            (int)I.n, h->f.zv, h->f.ze, h->f.zbar, h->step2_geometry ? h->nbw : nullptr,
-                           h->st + i);
+                           h->mesh.tri_node, (const double *)h->f.node_z, h->st + i);
         if (int rc = launch_check(h, "k_bed_derived")) return rc;
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));

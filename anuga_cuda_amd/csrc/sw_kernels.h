@@ -89,6 +89,10 @@ struct DevMesh {
     const double2 *vc;    // [3][N] vertex coordinates
     const double2 *cc;    // [N]    centroid coordinates
     const double2 *ra;    // [N]    {radius, area}
+    // mesh nodes (two-kernel step): node of vertex i of triangle k at
+    // tri_node[i*N + k], node coordinates node_xy[n_nodes]
+    const int *tri_node;
+    const double2 *node_xy;
     const int *bcell, *bedge, *bkind;  // [Nb]
     const double *bdir;                // [Nb][3]
     // Ensembles: instance i owns device triangles [inst_off[i], inst_end[i]);
@@ -112,6 +116,7 @@ struct DevFields {
     double *uc, *ue, *uv, *ub;            // xvelocity
     double *vc, *ve, *vv, *vb;            // yvelocity
     double *friction, *zbar, *max_speed;  // [N]
+    double *node_z;                       // [mesh nodes] bed at the node (two-kernel step)
     double2 *grav;                        // [N] -g*(wx, wy)*avg_h of the next step
     unsigned long long *wave_min;         // [kBlock/64 * blocks] CFL minimum of each wavefront (bits)
     EdgeF *eqf;                           // [3][N] mixed-precision edge state
@@ -1428,7 +1433,8 @@ __global__ __launch_bounds__(kBlock) void k_sww_points(long long P, const int *o
 // (nbw, else null), the check that both sides of every shared edge hold the
 // same bed edge value, over device triangles [off, off + n) of one instance.
 __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, const double *zv, const double *ze,
-                                                        double *zbar, const int *nbw, DevState *st) {
+                                                        double *zbar, const int *nbw, const int *tri_node,
+                                                        const double *node_z, DevState *st) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     bool bad = false, disc = false;
     if (t < n) {
@@ -1443,11 +1449,25 @@ __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, c
                 const int w = nbw[i * N + k];  // (m << 29) | n, or negative on a boundary edge
                 if (w >= 0) disc = disc || dbits(ze[i * N + k]) != dbits(ze[(w >> 29) * N + (w & 0x1FFFFFFF)]);
             }
+            // every vertex slot of a node holds the node's value (k_node_bed
+            // stored one of them): then the two-kernel step reads the bed at
+            // the nodes
+            for (int i = 0; i < 3; i++) disc = disc || dbits(v[i]) != dbits(node_z[tri_node[i * N + k]]);
         }
     }
     // one atomic per wavefront that found a mismatch
     if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicAnd((unsigned long long *)&st->bed_interp, 0ull);
     if (__ballot(disc) != 0 && (threadIdx.x & 63) == 0) atomicAnd((unsigned long long *)&st->bed_cont, 0ull);
+}
+
+// The bed value of every node from one of its vertex slots (k_bed_derived
+// then checks that all slots agree).
+__global__ __launch_bounds__(kBlock) void k_node_bed(int N, int off, int n, const double *zv, const int *tri_node,
+                                                     double *node_z) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const int k = off + t;
+    for (int i = 0; i < 3; i++) node_z[tri_node[i * N + k]] = zv[i * N + k];
 }
 
 }  // namespace sw

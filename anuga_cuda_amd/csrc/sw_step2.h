@@ -168,21 +168,24 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
     if (k < bi.end) {
         double x[3], y[3], zv[3];
         V3 vq[3], na[3], nc[3];
-        int nb[3];
+        int nb[3], nd[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const double2 v = ld_s(M.vc + i * N + k);
-            x[i] = v.x;
-            y[i] = v.y;
-        }
+        for (int i = 0; i < 3; i++) nd[i] = ld_s(M.tri_node + i * N + k);
 #pragma unroll
         for (int i = 0; i < 3; i++) vq[i] = S.vr[i * N + k];
-#pragma unroll
-        for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + k);
 #pragma unroll
         for (int i = 0; i < 3; i++) nb[i] = ld_s(nbw + i * N + k);
         const int flags = ld_s(M.flags + k);
         const double avg_h = ld_s(S.hc + k);
+        // the coordinates and bed of the triangle's nodes (every vertex slot
+        // of a node holds its value, checked when the bed is uploaded)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double2 v = M.node_xy[nd[i]];
+            x[i] = v.x;
+            y[i] = v.y;
+            zv[i] = F.node_z[nd[i]];
+        }
         // the two vertex records of the neighbour's matching edge (a boundary
         // edge reads its own, which are not used)
 #pragma unroll
@@ -370,15 +373,9 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
     Q4 c = ldq(cin + kk);
     const E3 eu = S.eur[kk];
     const double2 si = sin[kk];  // cached: the neighbours gather the same lines
-    double x[3], y[3], zv[3];
+    int nd[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const double2 v = ld_s(M.vc + i * N + kk);
-        x[i] = v.x;
-        y[i] = v.y;
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + kk);
+    for (int i = 0; i < 3; i++) nd[i] = ld_s(M.tri_node + i * N + kk);
     const int flags = ld_s(M.flags + kk);
     const double eta = ld_s(F.friction + kk);
     RecIn r;
@@ -395,6 +392,14 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
         eun[i] = S.eur[n];
         sn[i] = sin[n];
         r.xs[i] = M.cc[n];
+    }
+    double x[3], y[3], zv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double2 v = M.node_xy[nd[i]];
+        x[i] = v.x;
+        y[i] = v.y;
+        zv[i] = F.node_z[nd[i]];
     }
 
     // the next step's slots start at +inf (k_flux_v of the next step takes
