@@ -90,6 +90,11 @@ struct sw_handle {
     // the epilogue run when something reads or writes them: sync_api).
     bool records_live = false;
     bool api_pending = false;
+    // host copy of the instances' states (valid while st_host_ok) and the
+    // pinned buffer they are read through
+    std::vector<DevState> st_host;
+    bool st_host_ok = false;
+    DevState *st_pinned = nullptr;
 };
 
 namespace {
@@ -221,21 +226,39 @@ int sync_state(sw_handle *h, int i, DevState *out) {
     return state_error(h, out);
 }
 
-// Reads the states of all instances; reports the first instance's error.
-int sync_all(sw_handle *h, std::vector<DevState> &out) {
-    out.resize(h->n_inst);
-    HIPCHECK(h, hipMemcpyAsync(out.data(), h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
+// Reports the first instance's latched error among the states `d`.
+int states_error(sw_handle *h, std::vector<DevState> &d) {
     for (int i = 0; i < h->n_inst; i++)
-        if (out[i].error) {
-            int rc = state_error(h, &out[i]);
+        if (d[i].error) {
+            int rc = state_error(h, &d[i]);
             if (h->n_inst > 1) h->err = "instance " + std::to_string(i) + ": " + h->err;
             return rc;
         }
     return SW_OK;
 }
 
+// Reads the states of all instances (through a pinned buffer); reports the
+// first instance's error.
+int sync_all(sw_handle *h, std::vector<DevState> &out) {
+    out.resize(h->n_inst);
+    HIPCHECK(h, hipMemcpyAsync(h->st_pinned, h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    std::memcpy(out.data(), h->st_pinned, h->n_inst * sizeof(DevState));
+    return states_error(h, out);
+}
+
+// The states of all instances: the host's copy if it is current (kept by
+// sw_evolve_steps and sw_advance_yield, which know what they leave on the
+// device; dropped by every entry point that writes a state or runs device
+// work on it), else read.
+int current_states(sw_handle *h, std::vector<DevState> &out) {
+    if (!h->st_host_ok) return sync_all(h, out);
+    out = h->st_host;
+    return states_error(h, out);
+}
+
 int put_state(sw_handle *h, int i, const DevState &d) {
+    h->st_host_ok = false;
     HIPCHECK(h, hipMemcpy(h->st + i, &d, sizeof d, hipMemcpyHostToDevice));
     return SW_OK;
 }
@@ -510,6 +533,8 @@ int build_graph(sw_handle *h, bool two) {
     if (e != hipSuccess) return fail(h, SW_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     h->graph = graph;
     HIPCHECK(h, hipGraphInstantiate(&h->graph_exec, graph, nullptr, nullptr, 0));
+    // the graph's device-side set-up happens now, not at its first launch
+    HIPCHECK(h, hipGraphUpload(h->graph_exec, h->stream));
     return SW_OK;
 }
 
@@ -550,6 +575,7 @@ int sync_api(sw_handle *h) {
 int touch_api(sw_handle *h) {
     const int rc = sync_api(h);
     h->records_live = false;
+    h->st_host_ok = false;
     return rc;
 }
 
@@ -912,6 +938,11 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         dalloc(h, &F.wave_min, (size_t)nwaves) || dalloc(h, &F.eqf, 3 * N) || dalloc(h, &h->staging, 3 * Nmax))
         return bail();
     if (dalloc(h, &h->st, n_inst)) return bail();
+    if (hipHostMalloc((void **)&h->st_pinned, n_inst * sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+        h->st_pinned = nullptr;
+        fail(h, SW_ERR_HIP, "hipHostMalloc of the state buffer failed");
+        return bail();
+    }
     h->step2_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
     {
         std::vector<int> tri_node;
@@ -979,6 +1010,7 @@ int sw_destroy(sw_handle *h) {
     for (auto e : h->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *p : h->allocations) (void)hipFree(p);
+    if (h->st_pinned) (void)hipHostFree(h->st_pinned);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SW_OK;
@@ -1567,7 +1599,8 @@ static int64_t batch_estimate(const sw_handle *h, const std::vector<DevState> &d
 int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_t *event) {
     if (!h || max_steps < 0) return SW_ERR_ARG;
     std::vector<DevState> d0, d;
-    if (int rc = sync_all(h, d0)) return rc;
+    if (int rc = current_states(h, d0)) return rc;
+    h->st_host_ok = false;
     for (const auto &s : d0)
         if (s.event != SW_EVENT_NONE || s.halt)
             return fail(h, SW_ERR_STATE, "evolve_steps called while a yield is pending (call sw_advance_yield)");
@@ -1647,6 +1680,11 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     int rc2 = SW_OK, rc3 = SW_OK;
     if (!two) rc2 = epilogue(h);
     if (!two || rc != SW_OK || launched == 0) rc3 = sync_all(h, d);
+    // nothing runs on the states after this read: the host's copy is current
+    if (rc == SW_OK && rc2 == SW_OK && rc3 == SW_OK) {
+        h->st_host = d;
+        h->st_host_ok = true;
+    }
     int64_t steps = 0;
     int32_t ev = SW_EVENT_NONE;
     for (int i = 0; i < h->n_inst && i < (int)d.size(); i++) {
@@ -1663,7 +1701,7 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
 int sw_advance_yield(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
     std::vector<DevState> d;
-    if (int rc = sync_all(h, d)) return rc;
+    if (int rc = current_states(h, d)) return rc;
     for (int i = 0; i < h->n_inst; i++) {
         if (h->cur >= 0 && i != h->cur) continue;
         DevState &s = d[i];
@@ -1675,7 +1713,10 @@ int sw_advance_yield(sw_handle *h) {
         s.event = 0;
         s.halt = 0;
     }
+    h->st_host_ok = false;
     HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    h->st_host = d;
+    h->st_host_ok = true;
     return SW_OK;
 }
 

@@ -234,8 +234,11 @@ def main():
     h.call("sw_evolve_prepare", 1e9, 0.0, 0)
     two = h.step_path() == _lib.SW_STEP_TWO
     log("device prepared (%s-kernel step)" % ("two" if two else "four"))
+    # the autotuner first (10 timed steps per candidate workgroup size): it
+    # times launches one by one and drops the captured step graph, which the
+    # warmup then captures again
+    launch = h.autotune(10)
     done, ev = h.evolve_steps(args.warmup)
-    launch = h.autotune(3)
     h.call("sw_synchronize")
     log("warmup %d steps done; autotuned workgroup sizes (flux, update, distribute): %s" % (done, launch))
 
@@ -467,8 +470,8 @@ def bench_ensemble(args):
         len(domains), N, min(d.number_of_elements for d in domains), max(d.number_of_elements for d in domains),
         time.time() - t0))
     ens.prepare(1e9, None)
+    launch = ens.autotune(10)
     ens.evolve_steps(args.warmup)
-    launch = ens.autotune(3)
     log("autotuned workgroup sizes (flux, update, distribute): %s" % (launch,))
     h = ens.handle
     h.call("sw_synchronize")
