@@ -21,7 +21,7 @@ SW_PRECISION_FP64, SW_PRECISION_MIXED = 0, 1
 SW_ORDER_KEEP, SW_ORDER_HILBERT = 0, 1
 SW_LIMITER_OPTIMISED, SW_LIMITER_VERTEX, SW_LIMITER_EDGE = 0, 1, 2
 SW_EVENT_NONE, SW_EVENT_YIELD, SW_EVENT_FINAL = 0, 1, 2
-SW_STEP_AUTO, SW_STEP_FOUR, SW_STEP_TWO = 0, 1, 2
+SW_STEP_AUTO, SW_STEP_EDGE, SW_STEP_VERTEX = 0, 1, 2
 
 SW_ERR_UPDATE = -1
 SW_ERR_SMALLSTEPS = -2
@@ -425,11 +425,11 @@ class Handle:
         return tuple(chosen)
 
     def set_step_path(self, path):
-        """SW_STEP_AUTO (two-kernel step where it applies) or SW_STEP_FOUR."""
+        """SW_STEP_AUTO (vertex-record step where it applies) or SW_STEP_EDGE."""
         self.call("sw_set_step_path", int(path))
 
     def step_path(self):
-        """The step sw_evolve_steps would run now: SW_STEP_TWO or SW_STEP_FOUR."""
+        """The step sw_evolve_steps would run now: SW_STEP_VERTEX or SW_STEP_EDGE."""
         p = C.c_int32()
         self.call("sw_get_step_path", C.byref(p))
         return p.value

@@ -19,7 +19,7 @@
 
 #include "../../include/anuga_sw.h"
 #include "sw_kernels.h"
-#include "sw_step2.h"
+#include "sw_step_vertex.h"
 
 using namespace sw;
 
@@ -75,16 +75,16 @@ struct sw_handle {
     int *dev_of = nullptr;      // [sum n] device index (within its instance) of caller triangle k
     int blk[3] = {kBlock, kBlock, kBlock};  // workgroup size of the fused flux, update, distribute kernels
     std::vector<SwwPoints> sww;             // [n_inst] output points
-    // two-kernel Euler step (sw_step2.h)
+    // vertex-record Euler step (sw_step_vertex.h)
     StepFields sf{};
     int *nbw = nullptr;           // [3][N] neighbour words of k_flux_v
-    bool step2_geometry = false;  // caller's geometry equals tri_geometry and shared normals are negations
+    bool vstep_geometry = false;  // caller's geometry equals tri_geometry and shared normals are negations
     int step_path = SW_STEP_AUTO; // caller's choice
-    int vpar = 0;                 // parity of the next two-kernel step
+    int vpar = 0;                 // parity of the next vertex-record step
     int graph_path = 0;           // step path the captured graph holds
-    // After a two-kernel evolve call the step's records (vertex records,
-    // centroid records and semi-implicit pairs at parity vpar, centroid
-    // depths) hold the state.  records_live: they are what k_step_begin would
+    // After a vertex-record evolve call the step's records (vertex records,
+    // semi-implicit ratios, centroid depths; the states at parity vpar) hold
+    // the state.  records_live: they are what k_step_begin would
     // build from the operator API's arrays, so the next call continues from
     // them.  api_pending: those arrays are behind the records (k_step_end and
     // the epilogue run when something reads or writes them: sync_api).
@@ -447,10 +447,10 @@ int enqueue_fused_step(sw_handle *h, int slot) {
     return launch_check(h, "fused step");
 }
 
-// Whether sw_evolve_steps runs the two-kernel step (sw_step2.h) on the
+// Whether sw_evolve_steps runs the vertex-record step (sw_step_vertex.h) on the
 // states `d` of the instances.
-bool step2_applies(const sw_handle *h, const std::vector<DevState> &d) {
-    if (h->step_path == SW_STEP_FOUR || !h->step2_geometry) return false;
+bool vstep_applies(const sw_handle *h, const std::vector<DevState> &d) {
+    if (h->step_path == SW_STEP_EDGE || !h->vstep_geometry) return false;
     if (h->params.precision != SW_PRECISION_FP64 || rk_stages(h->params) != 1) return false;
     if (h->params.limiter != SW_LIMITER_OPTIMISED) return false;
     for (const auto &s : d)
@@ -458,22 +458,28 @@ bool step2_applies(const sw_handle *h, const std::vector<DevState> &d) {
     return true;
 }
 
-// Two-kernel step launches at one of the workgroup sizes the autotuner
-// chooses from (the flux size for k_flux_v, the distribute size for k_advance).
+// Vertex-record step launches at one of the workgroup sizes the autotuner
+// chooses from (the flux size for k_flux_v, the update size for k_update_v, the
+// distribute size for k_extrap_v).
 template <int B>
 void launch_flux_v(sw_handle *h) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
     hipLaunchKernelGGL((k_flux_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->nbw, h->dp, h->vpar);
 }
 template <int B>
-void launch_advance(sw_handle *h) {
+void launch_update_v(sw_handle *h) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
-    hipLaunchKernelGGL((k_advance<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
+    hipLaunchKernelGGL((k_update_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
+}
+template <int B>
+void launch_extrap_v(sw_handle *h) {
+    const unsigned g = (unsigned)((h->N + B - 1) / B);
+    hipLaunchKernelGGL((k_extrap_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
 }
 
-// Enqueues one two-kernel step at parity h->vpar; the timing slots are
-// flux (k_flux_v) and distribute (k_advance).
-int enqueue_step2(sw_handle *h, int slot) {
+// Enqueues one vertex-record step at parity h->vpar; the timing slots are
+// flux (k_flux_v), update (k_update_v) and distribute (k_extrap_v).
+int enqueue_vstep(sw_handle *h, int slot) {
     auto rec = [&](int idx) {
         if (slot >= 0) (void)hipEventRecord(h->ev[slot * (kNumTimed + 1) + idx], h->stream);
     };
@@ -485,18 +491,23 @@ int enqueue_step2(sw_handle *h, int slot) {
     }
     rec(1);
     rec(2);
+    switch (h->blk[1]) {
+        case 128: launch_update_v<128>(h); break;
+        case 512: launch_update_v<512>(h); break;
+        default: launch_update_v<256>(h); break;
+    }
     rec(3);
     switch (h->blk[2]) {
-        case 128: launch_advance<128>(h); break;
-        case 512: launch_advance<512>(h); break;
-        default: launch_advance<256>(h); break;
+        case 128: launch_extrap_v<128>(h); break;
+        case 512: launch_extrap_v<512>(h); break;
+        default: launch_extrap_v<256>(h); break;
     }
     rec(4);
     h->vpar ^= 1;
-    return launch_check(h, "two-kernel step");
+    return launch_check(h, "vertex-record step");
 }
 
-int enqueue_step(sw_handle *h, bool two, int slot) { return two ? enqueue_step2(h, slot) : enqueue_fused_step(h, slot); }
+int enqueue_step(sw_handle *h, bool vstep, int slot) { return vstep ? enqueue_vstep(h, slot) : enqueue_fused_step(h, slot); }
 
 // Accumulates the kernel durations of event groups [0, nslots).  A kernel that
 // returned at once because the loop was halted still counts as a launch.
@@ -518,15 +529,15 @@ int collect_timing(sw_handle *h, int nslots) {
 void drop_graph(sw_handle *h);
 
 // Captures kGraphSteps steps of the given path (an even number, so a
-// two-kernel graph starts and ends at parity 0).
-int build_graph(sw_handle *h, bool two) {
-    const int path = two ? SW_STEP_TWO : SW_STEP_FOUR;
+// vertex-record graph starts and ends at parity 0).
+int build_graph(sw_handle *h, bool vstep) {
+    const int path = vstep ? SW_STEP_VERTEX : SW_STEP_EDGE;
     if (h->graph_exec && h->graph_path == path) return SW_OK;
     drop_graph(h);
     h->graph_path = path;
     HIPCHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = SW_OK;
-    for (int s = 0; s < kGraphSteps && rc == SW_OK; s++) rc = enqueue_step(h, two, -1);
+    for (int s = 0; s < kGraphSteps && rc == SW_OK; s++) rc = enqueue_step(h, vstep, -1);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(h->stream, &graph);
     if (rc) return rc;
@@ -558,7 +569,7 @@ int epilogue(sw_handle *h) {
     return launch_check(h, "evolve epilogue");
 }
 
-// Brings the operator API's arrays up to the two-kernel step's records
+// Brings the operator API's arrays up to the vertex-record step's records
 // (k_step_end, then the epilogue of the evolve call), if they are behind.
 // Entry points that read those arrays call it first.
 int sync_api(sw_handle *h) {
@@ -735,7 +746,7 @@ bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t 
         flagsv[d] = flags;
         for (int i = 0; i < 3; i++) edgev[i * Ntot + d].tflags = flags;
         if (geom_same) {
-            // the two-kernel step recomputes the geometry from the vertex coordinates
+            // the vertex-record step recomputes the geometry from the vertex coordinates
             double x[3], y[3];
             for (int i = 0; i < 3; i++) {
                 x[i] = m->vertex_coordinates[6 * k + 2 * i];
@@ -764,7 +775,7 @@ bool pack_instance(const sw_mesh *m, const sw_params *p, const Inst &I, int64_t 
     return true;
 }
 
-// Mesh nodes of the two-kernel step: the vertex slots of each instance whose
+// Mesh nodes of the vertex-record step: the vertex slots of each instance whose
 // coordinates are bitwise equal share a node.  Nodes are numbered in the
 // order the device numbering first reaches them, so the node arrays follow
 // the triangles' locality order.  tri_node[i * Ntot + d] is the node of
@@ -943,7 +954,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         fail(h, SW_ERR_HIP, "hipHostMalloc of the state buffer failed");
         return bail();
     }
-    h->step2_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
+    h->vstep_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
     {
         std::vector<int> tri_node;
         std::vector<double> node_xy;
@@ -957,11 +968,9 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     {
         StepFields &S = h->sf;
         if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.hc, N) ||
-            dalloc(h, &S.cqb[1], N) ||
-            dalloc(h, &S.sir[0], N) || dalloc(h, &S.sir[1], N) || dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
+            dalloc(h, &S.sir, N) || dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
             dalloc(h, &S.errw, n_inst) || dalloc(h, &S.stv[1], n_inst))
             return bail();
-        S.cqb[0] = F.cq;
         S.stv[0] = h->st;
     }
     std::vector<DevState> s0(n_inst);
@@ -1086,7 +1095,7 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
         hipLaunchKernelGGL(k_node_bed, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (int)h->N, (int)I.off,
                            (int)I.n, (const double *)h->f.zv, h->mesh.tri_node, h->f.node_z);
         hipLaunchKernelGGL(k_bed_derived, dim3(grid_for(I.n)), dim3(kBlock), 0, h->stream, (int)h->N, (int)I.off,
-                           (int)I.n, h->f.zv, h->f.ze, h->f.zbar, h->step2_geometry ? h->nbw : nullptr,
+                           (int)I.n, h->f.zv, h->f.ze, h->f.zbar, h->vstep_geometry ? h->nbw : nullptr,
                            h->mesh.tri_node, (const double *)h->f.node_z, h->st + i);
         if (int rc = launch_check(h, "k_bed_derived")) return rc;
     }
@@ -1607,11 +1616,11 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
     int64_t launched = 0;
     int rc = SW_OK;
     const bool mixed = h->params.precision == SW_PRECISION_MIXED;
-    const bool two = step2_applies(h, d0) && max_steps > 0;
+    const bool vstep = vstep_applies(h, d0) && max_steps > 0;
     // the other paths step the operator API's arrays themselves
-    if (!two)
+    if (!vstep)
         if (int rc0 = touch_api(h)) return rc0;
-    if (two) {
+    if (vstep) {
         // continue from the records of the previous call, or build them
         const bool fresh = !h->records_live;
         if (fresh) h->vpar = 0;
@@ -1642,21 +1651,21 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         int64_t this_batch = max_steps - launched < batch ? max_steps - launched : batch;
         int64_t done_here = 0;
         if (h->timing) {
-            for (; done_here < this_batch && rc == SW_OK; done_here++) rc = enqueue_step(h, two, (int)done_here);
+            for (; done_here < this_batch && rc == SW_OK; done_here++) rc = enqueue_step(h, vstep, (int)done_here);
             if (rc == SW_OK) rc = collect_timing(h, (int)done_here);
         } else {
-            // a two-kernel graph is captured and replayed at parity 0
-            if (two && h->vpar && this_batch > kGraphSteps) rc = enqueue_step(h, two, -1), done_here++;
-            const bool graphs = !two || h->vpar == 0;
-            if (rc == SW_OK && graphs) rc = build_graph(h, two);
+            // a vertex-record graph is captured and replayed at parity 0
+            if (vstep && h->vpar && this_batch > kGraphSteps) rc = enqueue_step(h, vstep, -1), done_here++;
+            const bool graphs = !vstep || h->vpar == 0;
+            if (rc == SW_OK && graphs) rc = build_graph(h, vstep);
             for (; rc == SW_OK && graphs && done_here + kGraphSteps <= this_batch; done_here += kGraphSteps) {
                 hipError_t e = hipGraphLaunch(h->graph_exec, h->stream);
                 if (e != hipSuccess) rc = fail(h, SW_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
             }
-            for (; rc == SW_OK && done_here < this_batch; done_here++) rc = enqueue_step(h, two, -1);
+            for (; rc == SW_OK && done_here < this_batch; done_here++) rc = enqueue_step(h, vstep, -1);
         }
         launched += done_here;
-        if (two) {
+        if (vstep) {
             // the next step's state, errors merged, is also the handle's state
             hipLaunchKernelGGL(k_step_settle, dim3((h->n_inst + 63) / 64), dim3(64), 0, h->stream, h->sf, h->n_inst,
                                h->vpar);
@@ -1674,12 +1683,12 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         // the mixed loop keeps FP32 edge state: rebuild the FP64 edge records
         hipLaunchKernelGGL(k_interpolate, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     }
-    // The two-kernel step's records stay authoritative (k_step_end and the
+    // The vertex-record step's records stay authoritative (k_step_end and the
     // epilogue are deferred to sync_api); the states read after the last
     // batch are current unless the loop stopped on a launch error.
     int rc2 = SW_OK, rc3 = SW_OK;
-    if (!two) rc2 = epilogue(h);
-    if (!two || rc != SW_OK || launched == 0) rc3 = sync_all(h, d);
+    if (!vstep) rc2 = epilogue(h);
+    if (!vstep || rc != SW_OK || launched == 0) rc3 = sync_all(h, d);
     // nothing runs on the states after this read: the host's copy is current
     if (rc == SW_OK && rc2 == SW_OK && rc3 == SW_OK) {
         h->st_host = d;
@@ -1799,8 +1808,8 @@ int sw_autotune(sw_handle *h, int32_t steps, int32_t *chosen) {
 int sw_set_step_path(sw_handle *h, int32_t path) {
     if (!h) return SW_ERR_ARG;
     if (int rc = touch_api(h)) return rc;
-    if (path != SW_STEP_AUTO && path != SW_STEP_FOUR)
-        return fail(h, SW_ERR_ARG, "step path %d (SW_STEP_AUTO or SW_STEP_FOUR)", path);
+    if (path != SW_STEP_AUTO && path != SW_STEP_EDGE)
+        return fail(h, SW_ERR_ARG, "step path %d (SW_STEP_AUTO or SW_STEP_EDGE)", path);
     h->step_path = path;
     drop_graph(h);
     return SW_OK;
@@ -1812,7 +1821,7 @@ int sw_get_step_path(sw_handle *h, int32_t *path) {
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     d.resize(h->n_inst);
     HIPCHECK(h, hipMemcpy(d.data(), h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost));
-    *path = step2_applies(h, d) ? SW_STEP_TWO : SW_STEP_FOUR;
+    *path = vstep_applies(h, d) ? SW_STEP_VERTEX : SW_STEP_EDGE;
     return SW_OK;
 }
 

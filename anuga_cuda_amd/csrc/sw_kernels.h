@@ -3,9 +3,9 @@
 // Two families share the numerics in sw_device.h:
 //   * operator kernels, one per overridden ANUGA operator (the drop-in API:
 //     each mirrors one reference kernel / C routine and its array contract);
-//   * the four-kernel fused step used by evolve() (rk2 / rk3, mixed
-//     precision, the per-quantity limiters, and meshes the two-kernel step of
-//     sw_step2.h does not take); per stage
+//   * the edge-record fused step used by evolve() (rk2 / rk3, mixed
+//     precision, the per-quantity limiters, and meshes the vertex-record step of
+//     sw_step_vertex.h does not take); per stage
 //       flux       = compute_fluxes + boundary evaluation + gravity_wb; each
 //                    wavefront stores the minimum of its CFL candidates
 //                    (wavefront shuffles, a plain store);
@@ -89,7 +89,7 @@ struct DevMesh {
     const double2 *vc;    // [3][N] vertex coordinates
     const double2 *cc;    // [N]    centroid coordinates
     const double2 *ra;    // [N]    {radius, area}
-    // mesh nodes (two-kernel step): node of vertex i of triangle k at
+    // mesh nodes (vertex-record step): node of vertex i of triangle k at
     // tri_node[i*N + k], node coordinates node_xy[n_nodes]
     const int *tri_node;
     const double2 *node_xy;
@@ -116,7 +116,7 @@ struct DevFields {
     double *uc, *ue, *uv, *ub;            // xvelocity
     double *vc, *ve, *vv, *vb;            // yvelocity
     double *friction, *zbar, *max_speed;  // [N]
-    double *node_z;                       // [mesh nodes] bed at the node (two-kernel step)
+    double *node_z;                       // [mesh nodes] bed at the node (vertex-record step)
     double2 *grav;                        // [N] -g*(wx, wy)*avg_h of the next step
     unsigned long long *wave_min;         // [kBlock/64 * blocks] CFL minimum of each wavefront (bits)
     EdgeF *eqf;                           // [3][N] mixed-precision edge state
@@ -131,10 +131,11 @@ struct DevState {
     long long has_finaltime, event, error;
     // device-only
     long long halt, total_steps, bed_interp;
-    // two-kernel step (sw_step2.h): whether a caller-set stage semi-implicit
-    // term is still to be applied; steps it ran in the current evolve call;
-    // whether the bed edge values of every shared edge are bitwise equal on
-    // both sides (then its flux reads only its own side's bed)
+    // vertex-record step (sw_step_vertex.h): whether a caller-set stage
+    // semi-implicit term is still to be applied; steps run since the step's
+    // records were built; whether the bed edge values of every shared edge
+    // are bitwise equal on both sides and every vertex slot of a mesh node
+    // holds the same bed value (then the bed is read per node)
     long long wsi_live, call_steps, bed_cont;
 };
 
@@ -1429,7 +1430,7 @@ __global__ __launch_bounds__(kBlock) void k_sww_points(long long P, const int *o
 }
 
 // Mean bed, the check that the bed edge values are the interpolation of the
-// bed vertex values and, given the neighbour words of the two-kernel step
+// bed vertex values and, given the neighbour words of the vertex-record step
 // (nbw, else null), the check that both sides of every shared edge hold the
 // same bed edge value, over device triangles [off, off + n) of one instance.
 __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, const double *zv, const double *ze,
@@ -1450,7 +1451,7 @@ __global__ __launch_bounds__(kBlock) void k_bed_derived(int N, int off, int n, c
                 if (w >= 0) disc = disc || dbits(ze[i * N + k]) != dbits(ze[(w >> 29) * N + (w & 0x1FFFFFFF)]);
             }
             // every vertex slot of a node holds the node's value (k_node_bed
-            // stored one of them): then the two-kernel step reads the bed at
+            // stored one of them): then the vertex-record step reads the bed at
             // the nodes
             for (int i = 0; i < 3; i++) disc = disc || dbits(v[i]) != dbits(node_z[tri_node[i * N + k]]);
         }

@@ -1,56 +1,55 @@
-// Two-kernel Euler step of the FP64 loop (extrapolate_second_order_sw path).
+// Vertex-record Euler step of the FP64 loop (extrapolate_second_order_sw path).
 //
-// Per step:
+// Per step, three kernels:
 //   k_flux_v    compute_fluxes + boundary values on the fly + gravity_wb (its
 //               stage-gradient term from the vertex records) + the CFL
 //               minimum (one non-returning atomic per workgroup into one of
 //               kSlots slots of the instance);
-//   k_advance   update_timestep and the time bookkeeping (every wavefront
+//   k_update_v  update_timestep and the time bookkeeping (every wavefront
 //               reduces the slots itself: no separate launch), then
-//               update_conserved_quantities + protect of the triangle and of
-//               its three surrogate neighbours (recomputed from the same
-//               inputs, so bitwise what the neighbours compute for
-//               themselves), extrapolate_second_order_sw +
-//               balance_deep_and_shallow and the next step's Manning friction.
+//               update_conserved_quantities + protect of the centroid values,
+//               in place, and the next step's Manning friction;
+//   k_extrap_v  extrapolate_second_order_sw + balance_deep_and_shallow of the
+//               updated centroid values into the vertex records, and the
+//               centroid depth of the next step's gravity term.
 //
-// What goes through HBM per triangle-step (FP64):
-//   k_flux_v   reads vertex coordinates 48, neighbour words 12, flags 4, the
-//              triangle's three vertex records 72, bed vertex values 24,
-//              centroid depth 8; writes the update record 24 and max_speed 8
-//              (200 B);
-//   k_advance  reads centroid record 32, update record 24, semi-implicit pair
-//              16, Manning n 8, vertex coordinates 48, centroid 16,
-//              surrogates 12, flags 4, bed vertex values 24; writes centroid
-//              record 32, semi-implicit pair 16, vertex records 72, centroid
-//              depth 8 (312 B).
-// Neighbour gathers (vertex records in k_flux_v; centroid, update and
-// semi-implicit records and centroids in k_advance) come from the L2 / last-level
-// cache: the locality order keeps neighbours in nearby workgroups.
+// What goes through HBM per triangle-step (FP64; node arrays gathered through
+// the node indices, about half a node per triangle):
+//   k_flux_v    reads node indices 12, node coordinates and bed ~12,
+//               neighbour words 12, flags 4, the triangle's three vertex
+//               records 72, centroid depth 8; writes the update record 24 and
+//               max_speed 8 (152 B);
+//   k_update_v  reads centroid record 32, update record 24, semi-implicit
+//               ratios 16, Manning n 8, mean bed 8; writes centroid record 32,
+//               semi-implicit ratios 16 (136 B);
+//   k_extrap_v  reads centroid record 32, flags 4, centroid 16, node indices
+//               12, surrogates 12, node coordinates and bed ~12; writes vertex
+//               records 72, centroid depth 8 (168 B).
+// Neighbour gathers (vertex records in k_flux_v; centroid records and
+// centroids in k_extrap_v; node arrays in both) come from the L2 / last-level
+// cache: the locality order keeps neighbours in nearby workgroups and the
+// nodes are numbered in the order the triangles first reach them.
 //
-// Differences from the four-kernel step of sw_kernels.h, none of which changes
+// Differences from the edge-record step of sw_kernels.h, none of which changes
 // a result bit:
 //   * no edge records: k_flux_v forms edge values from the vertex records
 //     (e_i = (v_{i+1} + v_{i+2}) / 2, the interpolation of quantity.c:687-707),
 //     so the edge state is never written; the edge bed is the interpolation of
-//     the bed vertex values (required: bed_interp) and the same on both sides
-//     of every shared edge (required: bed_cont), so a flux reads only its own
-//     side's bed;
+//     the bed vertex values (required: bed_interp), and every vertex slot of a
+//     mesh node holds the node's bed value (required: bed_cont), so both sides
+//     of a shared edge see the same bed and the bed is stored per node;
 //   * normals, edge lengths, inscribed radius, area and centroid are
 //     recomputed from the vertex coordinates by tri_geometry, the formulas the
 //     mesh is built with; sw_create checks on the host that they reproduce the
-//     caller's arrays bit for bit and otherwise keeps the four-kernel step;
+//     caller's arrays bit for bit and otherwise keeps the edge-record step;
 //   * the stage-gradient gravity term is formed by the flux from the vertex
-//     stage and the centroid depth the previous k_advance stored (the same
+//     stage and the centroid depth the previous k_extrap_v stored (the same
 //     operations on the same values as forming it there);
-//   * centroid records, semi-implicit pairs and the scalar state are read
-//     from buffer p and written to buffer p ^ 1, p the step parity (a kernel
-//     argument: step i of a captured graph uses parity i % 2), so a workgroup
-//     never reads a neighbour's value that another workgroup has already
-//     advanced; a halted step copies them through.  Every load is requested
-//     before the step's state is examined (a halted step computes and stores
-//     nothing), so a wavefront waits for two memory round trips.  Errors found
-//     by lanes go to a separate word per instance (errw) that is merged into
-//     the state at the end of each batch of steps.
+//   * the scalar state and the CFL slots are read at parity p and written at
+//     p ^ 1, p the step parity (a kernel argument: step i of a captured graph
+//     uses parity i % 2); a halted step stores nothing.  Errors found by lanes
+//     go to a separate word per instance (errw) that is merged into the state
+//     at the end of each batch of steps.
 #pragma once
 
 #include "sw_kernels.h"
@@ -75,8 +74,7 @@ struct StepFields {
     V3 *vr;                   // [3][N] vertex records
     E3 *eur;                  // [N] explicit updates of the step (max_speed goes to DevFields::max_speed)
     double *hc;               // [N] centroid depth w - z after the step (for the next step's gravity term)
-    Q4 *cqb[2];               // [N] centroid records; cqb[0] is DevFields::cq
-    double2 *sir[2];          // [N] {xmom, ymom semi-implicit update} / centroid value (si_ratio)
+    double2 *sir;             // [N] {xmom, ymom semi-implicit update} / centroid value (si_ratio)
     unsigned long long *slot; // [2][n_inst][kSlotBlock] CFL minima (bits of positive doubles)
     long long *errw;          // [n_inst] errors found by lanes
     DevState *stv[2];         // [n_inst] scalar state; stv[0] is the handle's state
@@ -160,7 +158,7 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const BlockInst bi = block_inst<BLOCK>(M, b);
     const DevState *st = S.stv[p] + bi.inst;
-    // a halted step (k_advance carries the state over) stores nothing
+    // a halted step stores nothing
     const bool halted = st->event || st->error || st->halt || S.errw[bi.inst];
     const int N = M.N;
     const int k = b * BLOCK + (int)threadIdx.x;
@@ -350,48 +348,141 @@ SW_DEV double2 friction_next(const DevParams &P, double eta, const double x[3], 
     return make_double2(si_x, si_y);
 }
 
-#ifndef SW_ADV_MINWAVES
-#define SW_ADV_MINWAVES 1
+// friction_next on a flat bed (use_sloped_mannings off) with the mean bed
+// value zbar = (z0 + z1 + z2) / 3 precomputed: the same operations.
+SW_DEV double2 friction_flat(const DevParams &P, double eta, double zbar, const Q4 &c) {
+    double si_x = 0.0, si_y = 0.0;
+    const double mah = P.minimum_allowed_height, g = P.g;
+    if (eta > mah) {
+        const double uh = c.uh, vh = c.vh;
+        const double h = c.w - zbar;
+        if (h >= mah) {
+            double S = -g * eta * eta * sqrt((uh * uh + vh * vh));
+            S /= pow(h, 7.0 / 3);
+            si_x += S * uh;
+            si_y += S * vh;
+        }
+    }
+    return make_double2(si_x, si_y);
+}
+
+// update_timestep and the time advance (k_timestep<true>, Euler step) from
+// the CFL slots of the step, evaluated identically by every wavefront; the
+// state after the step goes to parity p ^ 1.  stop: nothing advances (a
+// halted step, or the too-small-timestep error).
+struct StepStart {
+    DevState s2;
+    bool stop;
+};
+SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevState &s, long long errw,
+                            unsigned long long slotv) {
+    const bool halted = s.event || s.error || s.halt || errw;
+    StepStart r{s, halted};
+    if (halted) {
+        r.s2.halt = 1;
+    } else {
+        r.s2.flux_timestep = bitsd(wave_min_u64(slotv));
+        timestep_logic(&r.s2, P);
+        r.stop = (r.s2.error & kErrSmallsteps) != 0;
+        if (!r.stop) {
+            r.s2.time = r.s2.time + r.s2.timestep;
+            finish_step(&r.s2);
+            r.s2.wsi_live = 0;
+            r.s2.call_steps += 1;
+        }
+    }
+    return r;
+}
+
+// The step of the centroid values, in place: update_timestep and the time
+// bookkeeping (every wavefront reduces the CFL slots itself), then
+// update_conserved_quantities + protect (quantity.c:769-820,
+// shallow_water_ext.c:1638-1705) and the next step's Manning friction
+// (shallow_water_ext.c:1238-1317) as the ratio of each momentum's
+// semi-implicit term to its centroid value.
+#ifndef SW_UPD_MINWAVES
+#define SW_UPD_MINWAVES 1
 #endif
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, DevFields F, StepFields S, DevParams P,
-                                                                    int p) {
+__global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, DevFields F, StepFields S,
+                                                                     DevParams P, int p) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const BlockInst bi = block_inst<BLOCK>(M, b);
     const int lane = (int)threadIdx.x & 63;
     const bool lead_wave = (b * BLOCK == bi.off) && threadIdx.x < 64;  // first wavefront of the instance
-    const int N = M.N;
     const int k = b * BLOCK + (int)threadIdx.x;
     const bool valid = k < bi.end;
     const int kk = valid ? k : bi.off;  // padding lanes load a real triangle's data and store nothing
-    const Q4 *cin = S.cqb[p];
-    const double2 *sin = S.sir[p];
 
-    // Loads: the CFL slots, the triangle's inputs, then its surrogate
-    // neighbours' inputs, all before the state decides anything.
     const unsigned long long slotv = *cfl_slot(S, M.n_inst, p, bi.inst, lane);
-    Q4 c = ldq(cin + kk);
+    Q4 c = ldq(F.cq + kk);
     const E3 eu = S.eur[kk];
-    const double2 si = sin[kk];  // cached: the neighbours gather the same lines
+    const double2 si = ld_s(S.sir + kk);
+    const double eta = ld_s(F.friction + kk);
+    const double zbar = ld_s(F.zbar + kk);
+
+    // the next step's slots start at +inf (k_flux_v of the next step takes
+    // its minima there after this launch has ended)
+    if (lead_wave) *cfl_slot(S, M.n_inst, p ^ 1, bi.inst, lane) = ~0ull;
+    const DevState s = S.stv[p][bi.inst];
+    const StepStart st = step_start(S, P, s, S.errw[bi.inst], slotv);
+    if (lead_wave && lane == 0) S.stv[p ^ 1][bi.inst] = st.s2;
+    if (!valid || st.stop) return;
+    const double dt = st.s2.timestep;
+    const double siw = s.wsi_live ? F.wsi[k] : 0.0;  // a caller-set stage term (first step of a call)
+    const bool ok = advance_centroid(c, eu, siw, si, dt, P);
+    double2 fr;
+    if (P.use_sloped_mannings) {
+        // the bed's slope from the node coordinates and bed values
+        double x[3], y[3], zv[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int nd = M.tri_node[i * M.N + k];
+            const double2 v = M.node_xy[nd];
+            x[i] = v.x;
+            y[i] = v.y;
+            zv[i] = F.node_z[nd];
+        }
+        fr = friction_next(P, eta, x, y, zv, c);
+    } else {
+        fr = friction_flat(P, eta, zbar, c);
+    }
+    if (!ok) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrUpdate);
+    F.cq[k] = c;
+    st_s(S.sir + k, make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh)));
+}
+
+// extrapolate_second_order_sw + balance_deep_and_shallow of the updated
+// centroid values (shallow_water_ext.c:2717-3324, :1462-1636): the vertex
+// records of the next step's flux and the centroid depth of its gravity term.
+#ifndef SW_EXT_MINWAVES
+#define SW_EXT_MINWAVES 1
+#endif
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK, SW_EXT_MINWAVES) void k_extrap_v(DevMesh M, DevFields F, StepFields S,
+                                                                     DevParams P, int p) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BlockInst bi = block_inst<BLOCK>(M, b);
+    const int N = M.N;
+    const int k = b * BLOCK + (int)threadIdx.x;
+    const bool valid = k < bi.end;
+    const int kk = valid ? k : bi.off;
+
+    // own inputs, then the surrogate neighbours' updated centroid records and
+    // centroids and the nodes' coordinates and bed, before anything is used
+    RecIn r;
+    r.c = ldq(F.cq + kk);
+    r.flags = ld_s(M.flags + kk);
+    r.xy = M.cc[kk];
     int nd[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) nd[i] = ld_s(M.tri_node + i * N + kk);
-    const int flags = ld_s(M.flags + kk);
-    const double eta = ld_s(F.friction + kk);
-    RecIn r;
-    r.xy = M.cc[kk];
 #pragma unroll
     for (int i = 0; i < 3; i++) r.sk[i] = ld_s(M.sur + i * N + kk);
-    Q4 cn[3];
-    E3 eun[3];
-    double2 sn[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const int n = r.sk[i];
-        cn[i] = ldq(cin + n);
-        eun[i] = S.eur[n];
-        sn[i] = sin[n];
-        r.xs[i] = M.cc[n];
+        r.cs[i] = ldq(F.cq + r.sk[i]);
+        r.xs[i] = M.cc[r.sk[i]];
     }
     double x[3], y[3], zv[3];
 #pragma unroll
@@ -401,43 +492,9 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
         y[i] = v.y;
         zv[i] = F.node_z[nd[i]];
     }
-
-    // the next step's slots start at +inf (k_flux_v of the next step takes
-    // its minima there after this launch has ended)
-    if (lead_wave) *cfl_slot(S, M.n_inst, p ^ 1, bi.inst, lane) = ~0ull;
-    // update_timestep and the time advance (k_timestep<true>, Euler step),
-    // evaluated identically by every wavefront
-    const DevState s = S.stv[p][bi.inst];
-    const bool halted = s.event || s.error || s.halt || S.errw[bi.inst];
-    DevState s2 = s;
-    bool stop = halted;
-    if (halted) {
-        s2.halt = 1;
-    } else {
-        s2.flux_timestep = bitsd(wave_min_u64(slotv));
-        timestep_logic(&s2, P);
-        stop = (s2.error & kErrSmallsteps) != 0;
-        if (!stop) {
-            s2.time = s2.time + s2.timestep;
-            finish_step(&s2);
-            s2.wsi_live = 0;
-            s2.call_steps += 1;
-        }
-    }
-    if (lead_wave && lane == 0) S.stv[p ^ 1][bi.inst] = s2;
-    const double dt = s2.timestep;
-    const bool second_order = s2.order != 1;
-    double siw = 0.0, swn[3] = {0.0, 0.0, 0.0};
-    if (s.wsi_live && !stop) {  // a caller-set stage term (first step of a call)
-        siw = F.wsi[kk];
-#pragma unroll
-        for (int i = 0; i < 3; i++) swn[i] = F.wsi[r.sk[i]];
-    }
-
-    // the step (a stopped one computes too, on whatever it loaded, and stores nothing)
-    const bool ok = advance_centroid(c, eu, siw, si, dt, P);
-    r.c = c;
-    r.flags = flags;
+    const DevState &s2 = S.stv[p ^ 1][bi.inst];  // the state after this step's update
+    if (!valid || s2.halt || s2.error) return;
+    const Q4 c = r.c;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         r.v[i] = make_double2(x[i], y[i]);
@@ -445,46 +502,29 @@ __global__ __launch_bounds__(BLOCK, SW_ADV_MINWAVES) void k_advance(DevMesh M, D
     }
     double wv[3], uv[3], vv[3];
     bool ok_nb = true;
-    if (!second_order) {
+    if (s2.order == 1) {
         wv[0] = wv[1] = wv[2] = c.w;
         uv[0] = uv[1] = uv[2] = c.uh;
         vv[0] = vv[1] = vv[2] = c.vh;
     } else {
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            Q4 t = cn[i];
-            advance_centroid(t, eun[i], swn[i], sn[i], dt, P);  // its own lane reports errors
-            r.cs[i] = t;
-        }
         // a dry triangle keeps its previous vertex values
         auto stale = [&](int i, double &w, double &u, double &v) {
-            const V3 o = S.vr[i * N + kk];
+            const V3 o = S.vr[i * N + k];
             w = o.w;
             u = o.uh;
             v = o.vh;
         };
-        ok_nb = extrapolate_one<double>(P, kk, r, wv, uv, vv, stale);
+        ok_nb = extrapolate_one<double>(P, k, r, wv, uv, vv, stale);
     }
     BalanceParams bp{P.H0, P.alpha_balance, P.tight_slope_limiters, P.use_centroid_velocities};
     balance_one(c.w, c.z, wv, zv, c.uh, c.vh, uv, vv, bp);
-    const double2 fr = friction_next(P, eta, x, y, zv, c);
-    if (!valid) return;
-    if (stop) {
-        // nothing advances: the state is carried over to the next parity
-        S.cqb[p ^ 1][k] = ldq(cin + k);
-        S.sir[p ^ 1][k] = ld_s(sin + k);
-        return;
-    }
-    if (!ok) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrUpdate);
     if (!ok_nb) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrNeighbour);
 #pragma unroll
     for (int i = 0; i < 3; i++) S.vr[i * N + k] = V3{wv[i], uv[i], vv[i]};
     st_s(S.hc + k, c.w - c.z);
-    S.cqb[p ^ 1][k] = c;
-    st_s(S.sir[p ^ 1] + k, make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh)));
 }
 
-// Start of an evolve call on the two-kernel step: the first step's source
+// Start of an evolve call on the vertex-record step: the first step's source
 // terms (as k_source_terms: gravity term, Manning friction added to the
 // caller's semi-implicit momenta, into sir[0]), the vertex records from the
 // vertex arrays, and whether a caller-set stage semi-implicit term is live.
@@ -505,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, S
     }
     S.hc[k] = c.w - c.z;
     const double2 f = friction_next(P, F.friction[k], x, y, zv, c);
-    S.sir[0][k] = make_double2(si_ratio(F.uhsi[k] + f.x, c.uh), si_ratio(F.vhsi[k] + f.y, c.vh));
+    S.sir[k] = make_double2(si_ratio(F.uhsi[k] + f.x, c.uh), si_ratio(F.vhsi[k] + f.y, c.vh));
     const bool live = F.wsi[k] != 0.0;
     if (__ballot(live) != 0 && (threadIdx.x & 63) == 0)
         atomicOr((unsigned long long *)&S.stv[0][ln.inst].wsi_live, 1ull);
@@ -542,7 +582,7 @@ __global__ void k_step_settle(StepFields S, int n_inst, int p) {
     S.stv[0][inst] = s;
 }
 
-// The operator API's arrays from the records of the two-kernel step, whose
+// The operator API's arrays from the records of the vertex-record step, whose
 // next step has parity p (run when something reads or writes those arrays
 // after one or more evolve calls): the current centroid records into buffer
 // 0, and the vertex records and update records into the arrays of the
@@ -555,7 +595,6 @@ __global__ __launch_bounds__(kBlock) void k_step_end(DevMesh M, DevFields F, Ste
     const int k = ln.k;
     if (!ln.valid) return;
     const DevState &st = S.stv[0][ln.inst];
-    if (p) F.cq[k] = S.cqb[1][k];
     V3 v[3];
     for (int i = 0; i < 3; i++) {
         v[i] = S.vr[i * N + k];

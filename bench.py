@@ -59,34 +59,41 @@ KERNEL_BYTES_PER_TRIANGLE_MIXED = {
 # bytes per 64 triangles).
 STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
 
-# The two-kernel Euler step (the default FP64 path; sw_step2.h).  The step
-# timing reports k_flux_v as "flux" and k_advance as "distribute".
-KERNEL_BYTES_PER_TRIANGLE_TWO = {
-    # reads: vertex xy 48, neighbour words 12, flags 4, the triangle's vertex
-    # records 3x24 72, bed vertex values 24, centroid depth 8 | writes: update
-    # record 24, max_speed 8
-    "flux": 48 + 12 + 4 + 72 + 24 + 8 + 24 + 8,
-    # reads: centroid record 32, update record 24, semi-implicit pair 16,
-    # Manning n 8, vertex xy 48, centroid 16, surrogates 12, flags 4, bed
-    # vertex values 24 | writes: centroid record 32, semi-implicit pair 16,
-    # vertex records 3x24 72, centroid depth 8
-    "distribute": 32 + 24 + 16 + 8 + 48 + 16 + 12 + 4 + 24 + 32 + 16 + 72 + 8,
+# The vertex-record Euler step (the default FP64 path; sw_step_vertex.h):
+# k_flux_v ("flux"), k_update_v ("update"), k_extrap_v ("distribute").  Node
+# arrays (coordinates 16 B and bed 8 B per node, about half a node per
+# triangle) are gathered through the node indices and counted once.
+KERNEL_BYTES_PER_TRIANGLE_VERTEX = {
+    # reads: node indices 12, node coordinates and bed 12, neighbour words 12,
+    # flags 4, the triangle's vertex records 3x24 72, centroid depth 8 |
+    # writes: update record 24, max_speed 8
+    "flux": 12 + 12 + 12 + 4 + 72 + 8 + 24 + 8,
+    # reads: centroid record 32, update record 24, semi-implicit ratios 16,
+    # Manning n 8, mean bed 8 | writes: centroid record 32, semi-implicit
+    # ratios 16
+    "update": 32 + 24 + 16 + 8 + 8 + 32 + 16,
+    # reads: centroid record 32, flags 4, centroid 16, node indices 12,
+    # surrogates 12, node coordinates and bed 12 | writes: vertex records 3x24
+    # 72, centroid depth 8
+    "distribute": 32 + 4 + 16 + 12 + 12 + 12 + 72 + 8,
 }
 # The unique footprint of one FP64 Euler step (each array read once, each
 # mutated array written once, the edge-value exchange counted as one write
 # plus one read, neighbour gathers as cache hits): 604 bytes per triangle,
-# split by the kernel of the two-kernel step that touches each array.
+# split by the kernel of the vertex-record step that touches each array.
 FOOTPRINT_BYTES_PER_TRIANGLE_STEP = 604
-FOOTPRINT_SHARE_TWO = {
+FOOTPRINT_SHARE_VERTEX = {
     # neighbours 12, neighbour edges 12, full flag 4, normals 48, edge lengths
     # 24, areas 8, radii 8, bed edge 24, edge values read 72, max_speed 8
     "flux": 12 + 12 + 4 + 48 + 24 + 8 + 8 + 24 + 72 + 8,
+    # centroid state read+write 48, friction 8, bed centroid 8
+    "update": 48 + 8 + 8,
     # surrogates 12, boundary counts 4, centroid xy 16, vertex xy 48, bed
-    # centroid 8, bed vertex 24, friction 8, centroid state read+write 48,
-    # vertex state read+write 144, edge values written 72
-    "distribute": 12 + 4 + 16 + 48 + 8 + 24 + 8 + 48 + 144 + 72,
+    # vertex 24, vertex state read+write 144, edge values written 72
+    "distribute": 12 + 4 + 16 + 48 + 24 + 144 + 72,
 }
-assert sum(FOOTPRINT_SHARE_TWO.values()) == FOOTPRINT_BYTES_PER_TRIANGLE_STEP
+assert sum(FOOTPRINT_SHARE_VERTEX.values()) == FOOTPRINT_BYTES_PER_TRIANGLE_STEP
+VERTEX_STEP_KERNELS = "vertex-record (k_flux_v, k_update_v, k_extrap_v)"
 
 
 def kernel_bytes(precision):
@@ -99,11 +106,12 @@ def workload_tag(args):
     return tag if args.locality == "hilbert" else tag + "_keep_order"
 
 
-def kernel_symbol(kernel, precision, launch, two=False):
+def kernel_symbol(kernel, precision, launch, vstep=False):
     """Name of a fused kernel as rocprofv3 reports it; ``launch`` = workgroup
     sizes of (flux, update, distribute)."""
-    if two:
-        return {"flux": "k_flux_v<%d>" % launch[0], "distribute": "k_advance<%d>" % launch[2]}[kernel]
+    if vstep:
+        return {"flux": "k_flux_v<%d>" % launch[0], "update": "k_update_v<%d>" % launch[1],
+                "distribute": "k_extrap_v<%d>" % launch[2]}[kernel]
     r = "double" if precision == "fp64" else "float"
     fb, ub, db = launch
     flux = "k_flux<%s, true, %d>" % (r, fb) if precision == "fp64" else "k_flux_mixed<%d>" % fb
@@ -172,8 +180,8 @@ def main():
     ap.add_argument("--mesh-build", default="host", choices=["host", "device"],
                     help="fv: build the mesh geometry on the GPU (sw_build_mesh) or the host; same arrays")
     ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps timed for the CPU baseline")
-    ap.add_argument("--step-path", default="auto", choices=["auto", "four"],
-                    help="FV: two-kernel step where it applies (auto) or the four-kernel step")
+    ap.add_argument("--step-path", default="auto", choices=["auto", "edge"],
+                    help="FV: vertex-record step where it applies (auto) or the edge-record step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="FV: skip the learned rollout that the default line carries")
@@ -229,11 +237,11 @@ def main():
 
     h = d._dev()
     from anuga_cuda_amd import _lib
-    if args.step_path == "four":
-        h.set_step_path(_lib.SW_STEP_FOUR)
+    if args.step_path == "edge":
+        h.set_step_path(_lib.SW_STEP_EDGE)
     h.call("sw_evolve_prepare", 1e9, 0.0, 0)
-    two = h.step_path() == _lib.SW_STEP_TWO
-    log("device prepared (%s-kernel step)" % ("two" if two else "four"))
+    vstep = h.step_path() == _lib.SW_STEP_VERTEX
+    log("device prepared (%s-record step)" % ("vertex" if vstep else "edge"))
     # the autotuner first (10 timed steps per candidate workgroup size): it
     # times launches one by one and drops the captured step graph, which the
     # warmup then captures again
@@ -259,17 +267,19 @@ def main():
     h.call("sw_enable_timing", 0)
     ms, cnt = h.kernel_times(4)
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
+    if vstep:
+        avg_us["timestep"] = None  # k_update_v reduces the timestep; the interval holds no kernel
     log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
-    kb = KERNEL_BYTES_PER_TRIANGLE_TWO if two else kernel_bytes(args.precision)
+    kb = KERNEL_BYTES_PER_TRIANGLE_VERTEX if vstep else kernel_bytes(args.precision)
     dominant = max(kb, key=lambda k: avg_us[k] or 0.0)
     # a Runge-Kutta step runs every kernel once per stage, plus the copy of the
     # starting centroid records (32 B written, then read by each later stage)
     design_bytes = sum(kb.values()) * stages + (32 * stages if stages > 1 else 0)
     # roofline bytes: the step's unique footprint (604 B per triangle and
-    # stage) and, for the dominant kernel of the two-kernel step, its share
-    # of it; the four-kernel step's kernels are counted by their design bytes
+    # stage) and, for the dominant kernel of the vertex-record step, its share
+    # of it; the edge-record step's kernels are counted by their design bytes
     step_bytes = FOOTPRINT_BYTES_PER_TRIANGLE_STEP * stages
-    kernel_roof_bytes = (FOOTPRINT_SHARE_TWO if two else kb)[dominant]
+    kernel_roof_bytes = (FOOTPRINT_SHARE_VERTEX if vstep else kb)[dominant]
     achieved = kernel_roof_bytes * N / (avg_us[dominant] * 1e-6) / 1e9
     step_gbs = step_bytes * N * args.steps / elapsed / 1e9
     copy_gbs = copy_peak_gbs()
@@ -282,7 +292,7 @@ def main():
     step_traffic = None
     if os.path.exists(tfile):
         kernels = json.load(open(tfile)).get("kernels", {})
-        if two:
+        if vstep:
             recs = {k: kernels.get(kernel_symbol(k, args.precision, launch, True)) for k in kb}
             rec = recs[dominant] or {}
             if all(recs.values()):
@@ -340,7 +350,7 @@ def main():
                    "triangles": N, "precision": args.precision, "order": 2, "timestepping": args.timestepping,
                    "limiter": args.limiter, "mesh_build": args.mesh_build,
                    "device_numbering": args.locality, "workgroup_sizes": list(launch)},
-        "roofline": {"bound": "hbm", "kernel": kernel_symbol(dominant, args.precision, launch, two),
+        "roofline": {"bound": "hbm", "kernel": kernel_symbol(dominant, args.precision, launch, vstep),
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_triangle": kernel_roof_bytes,
@@ -351,7 +361,7 @@ def main():
                           "frac": step_gbs / HBM_PEAK_GBS, "frac_of_copy": step_gbs / copy_gbs,
                           "design_bytes_per_triangle_step": design_bytes,
                           "traffic_per_step": step_traffic,
-                          "step_kernels": "two (k_flux_v, k_advance)" if two else "four",
+                          "step_kernels": VERTEX_STEP_KERNELS if vstep else "edge-record (four kernels)",
                           "kernel_avg_us": avg_us},
         "cpu_baseline": cpu,
     }
@@ -492,13 +502,15 @@ def bench_ensemble(args):
     h.call("sw_enable_timing", 0)
     ms, cnt = h.kernel_times(4)
     avg_us = {k: (1000.0 * ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(KERNELS)}
-    log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
     from anuga_cuda_amd import _lib
-    two = h.step_path() == _lib.SW_STEP_TWO
+    vstep = h.step_path() == _lib.SW_STEP_VERTEX
+    if vstep:
+        avg_us["timestep"] = None  # k_update_v reduces the timestep; the interval holds no kernel
+    log("kernel avg us: %s" % {k: round(v, 2) if v else v for k, v in avg_us.items()})
     # the step's unique footprint (604 B per triangle; the same figure as the
-    # single-domain line) when the two-kernel step runs, and the design bytes
+    # single-domain line) when the vertex-record step runs, and the design bytes
     # of the kernels that run beside it
-    design = sum((KERNEL_BYTES_PER_TRIANGLE_TWO if two else kernel_bytes(args.precision)).values())
+    design = sum((KERNEL_BYTES_PER_TRIANGLE_VERTEX if vstep else kernel_bytes(args.precision)).values())
     step_bytes = FOOTPRINT_BYTES_PER_TRIANGLE_STEP if args.precision == "fp64" else design
     out = {
         "metric": "triangle-steps/s at 1M tris, FV vs learned at stated error; % HBM peak",
@@ -522,7 +534,7 @@ def bench_ensemble(args):
                           "achieved": step_bytes * N * args.steps / elapsed / 1e9,
                           "frac": step_bytes * N * args.steps / elapsed / 1e9 / HBM_PEAK_GBS,
                           "design_bytes_per_triangle_step": design,
-                          "step_kernels": "two (k_flux_v, k_advance)" if two else "four",
+                          "step_kernels": VERTEX_STEP_KERNELS if vstep else "edge-record (four kernels)",
                           "kernel_avg_us": avg_us},
         "cpu_baseline": None,
     }

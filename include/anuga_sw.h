@@ -57,9 +57,9 @@ enum sw_locality_order { SW_ORDER_KEEP = 0, SW_ORDER_HILBERT = 1 };
 enum sw_limiter { SW_LIMITER_OPTIMISED = 0, SW_LIMITER_VERTEX = 1, SW_LIMITER_EDGE = 2 };
 enum sw_event { SW_EVENT_NONE = 0, SW_EVENT_YIELD = 1, SW_EVENT_FINAL = 2 };
 /* Kernels of the fused Euler step (sw_set_step_path): SW_STEP_AUTO picks the
- * two-kernel step wherever it applies, SW_STEP_FOUR keeps the four-kernel one;
- * sw_get_step_path reports SW_STEP_TWO or SW_STEP_FOUR. */
-enum sw_step_path { SW_STEP_AUTO = 0, SW_STEP_FOUR = 1, SW_STEP_TWO = 2 };
+ * vertex-record step wherever it applies, SW_STEP_EDGE keeps the edge-record one;
+ * sw_get_step_path reports SW_STEP_VERTEX or SW_STEP_EDGE. */
+enum sw_step_path { SW_STEP_AUTO = 0, SW_STEP_EDGE = 1, SW_STEP_VERTEX = 2 };
 
 /* Domain parameters; names follow struct domain (src/anuga_HMPP/sw_domain.h:16-60)
  * and the attributes the reference driver passes to its kernels. */
@@ -326,16 +326,18 @@ int sw_enable_timing(sw_handle *h, int32_t enable);
 int sw_kernel_times(sw_handle *h, double *ms, int64_t *launches, int32_t nkernels);
 
 /* Kernels of the fused step: no counterpart in the reference (its drivers
- * launch one kernel per operator).  The two-kernel step (flux; then timestep,
- * update, protect and distribute of a triangle and its surrogate neighbours)
- * applies to FP64 Euler steps with the optimised limiter on meshes whose
- * geometry arrays equal those recomputed from the vertex coordinates and whose
- * bed edge values are the interpolation of the bed vertex values; it gives the
- * same bits as the four-kernel step (flux, timestep, update, distribute).
- * In timing mode the two-kernel step's second kernel is reported as
- * "distribute" and the timestep / update entries stay at zero. */
+ * launch one kernel per operator).  The vertex-record step (flux from the
+ * vertex records; timestep, update, protect and the next step's friction;
+ * extrapolation and balance into the vertex records) applies to FP64 Euler
+ * steps with the optimised limiter on meshes whose geometry arrays equal those
+ * recomputed from the vertex coordinates, whose bed edge values are the
+ * interpolation of the bed vertex values and whose bed is single-valued at
+ * every mesh node; it gives the same bits as the edge-record step (flux,
+ * timestep, update, distribute).  In timing mode the vertex-record step's
+ * kernels are reported as flux, update and distribute; the timestep entry
+ * counts an empty interval (the update kernel reduces the timestep). */
 int sw_set_step_path(sw_handle *h, int32_t path);
-/* The step sw_evolve_steps would run now (SW_STEP_TWO or SW_STEP_FOUR). */
+/* The step sw_evolve_steps would run now (SW_STEP_VERTEX or SW_STEP_EDGE). */
 int sw_get_step_path(sw_handle *h, int32_t *path);
 
 #ifdef __cplusplus

@@ -18,7 +18,7 @@ $(OUT)/liboracle_sw.so: $(ROOT)/oracle/sw_oracle.c $(ROOT)/oracle/sw_oracle.h
 	    -fsanitize=address,undefined -fno-sanitize-recover=undefined -o $@ $(ROOT)/oracle/sw_oracle.c -lm
 
 $(OUT)/libanuga_sw.so: $(CSRC)/sw_capi.hip $(CSRC)/mesh_build.hip $(CSRC)/sw_kernels.h $(CSRC)/sw_device.h \
-                       $(CSRC)/sw_step2.h $(ROOT)/include/anuga_sw.h
+                       $(CSRC)/sw_step_vertex.h $(ROOT)/include/anuga_sw.h
 	mkdir -p $(OUT)
 	$(HIPCC) --offload-arch=gfx950 -O1 -g -std=c++17 -fPIC -shared -ffp-contract=off -fno-fast-math \
 	    -Xclang -target-feature -Xclang -packed-fp32-ops \

@@ -1,7 +1,7 @@
 """Driver of the sanitized host-code run (tests/test_sanitized.py): loads the
 ASan + UBSan build of the library given on the command line and runs its
 host-side mesh code -- validation, device numbering (Hilbert order), packing
-of the edge, flag, surrogate and neighbour-word arrays, the two-kernel step's
+of the edge, flag, surrogate and neighbour-word arrays, the vertex-record step's
 geometry check -- on meshes of several kinds, single and as an ensemble, plus
 malformed meshes for the error paths.  Without a GPU, sw_create stops at
 hipSetDevice after all of that; with one it creates and destroys the handle.

@@ -42,7 +42,7 @@ def test_config2_million_triangles_lockstep(order):
     h = d._dev()
     h.call("sw_evolve_prepare", 1e9, 0.0, 0)
     od.evolve_prepare(1e9, None)
-    assert h.step_path() == _lib.SW_STEP_TWO
+    assert h.step_path() == _lib.SW_STEP_VERTEX
     bed = od.q("elevation", "centroid_values")
     eps = d.minimum_allowed_height
     stats = {}

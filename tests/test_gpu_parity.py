@@ -567,15 +567,15 @@ def test_non_default_parameter_branches_match_oracle(case, fused):
 
 
 def test_two_kernel_step_equals_four_kernel_step():
-    """The two-kernel Euler step (flux; then timestep, update and distribute of
+    """The vertex-record Euler step (flux; then timestep, update and distribute of
     each triangle with its surrogate neighbours' updates recomputed) and the
-    four-kernel step give the same bits: timesteps, yields and every array of
+    edge-record step give the same bits: timesteps, yields and every array of
     the operator API, over yields that stop a captured 16-step graph part way."""
     runs = {}
-    for path in (_lib.SW_STEP_AUTO, _lib.SW_STEP_FOUR):
+    for path in (_lib.SW_STEP_AUTO, _lib.SW_STEP_EDGE):
         d = _delaunay("random")
         d._dev().set_step_path(path)
-        assert d._handle.step_path() == (_lib.SW_STEP_TWO if path == _lib.SW_STEP_AUTO else _lib.SW_STEP_FOUR)
+        assert d._handle.step_path() == (_lib.SW_STEP_VERTEX if path == _lib.SW_STEP_AUTO else _lib.SW_STEP_EDGE)
         dts, steps = [], []
         for _ in d.evolve(yieldstep=0.37, finaltime=3.0):
             dts.append(d._handle.get_state().timestep)
@@ -587,7 +587,7 @@ def test_two_kernel_step_equals_four_kernel_step():
             arrays[(q, "explicit_update")] = device_array(d, q, "explicit_update")
         runs[path] = (dts, steps, arrays, d.max_speed.copy(), d.get_time())
         d.close()
-    a, b = runs[_lib.SW_STEP_AUTO], runs[_lib.SW_STEP_FOUR]
+    a, b = runs[_lib.SW_STEP_AUTO], runs[_lib.SW_STEP_EDGE]
     assert a[0] == b[0] and a[1] == b[1] and sum(a[1]) > 40 and a[4] == b[4]
     for key in a[2]:
         assert np.array_equal(a[2][key], b[2][key]), key
@@ -607,7 +607,7 @@ def _api_arrays(d):
 
 
 def test_chained_calls_continue_from_the_step_records():
-    """Consecutive two-kernel evolve calls continue from the step's records
+    """Consecutive vertex-record evolve calls continue from the step's records
     at whatever parity the previous call ended on, and the operator API's
     arrays are brought up to date only when read: 40 steps as one call, as
     calls of 1, 2, 17, 3, 16 and 1 steps with nothing read in between, and as
@@ -618,7 +618,7 @@ def test_chained_calls_continue_from_the_step_records():
     for mode in ("one", "chained", "read"):
         d = _delaunay("morton")
         h = d._dev()
-        assert h.step_path() == _lib.SW_STEP_TWO
+        assert h.step_path() == _lib.SW_STEP_VERTEX
         h.call("sw_evolve_prepare", 1e9, 0.0, 0)
         states = []
         for n in ([sum(chunks)] if mode == "one" else chunks):
@@ -668,15 +668,15 @@ def test_upload_between_chained_calls_matches_oracle():
 
 def test_two_kernel_step_needs_recomputable_geometry():
     """A mesh whose geometry arrays are not what the vertex coordinates give
-    (here a radius changed in its last bit) keeps the four-kernel step."""
+    (here a radius changed in its last bit) keeps the edge-record step."""
     d = _delaunay("morton")
     d.mesh.radii = d.mesh.radii.copy()
     d.mesh.radii[10] = np.nextafter(d.mesh.radii[10], 0.0)
-    assert d._dev().step_path() == _lib.SW_STEP_FOUR
+    assert d._dev().step_path() == _lib.SW_STEP_EDGE
     d2 = _delaunay("morton")
-    assert d2._dev().step_path() == _lib.SW_STEP_TWO
+    assert d2._dev().step_path() == _lib.SW_STEP_VERTEX
     d2.set_timestepping_method("rk2")
-    assert d2._handle.step_path() == _lib.SW_STEP_FOUR
+    assert d2._handle.step_path() == _lib.SW_STEP_EDGE
 
 
 def test_ensemble_members_must_share_their_start_time():
