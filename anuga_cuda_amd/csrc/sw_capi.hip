@@ -83,8 +83,7 @@ struct sw_handle {
     int vpar = 0;                 // parity of the next vertex-record step
     int graph_path = 0;           // step path the captured graph holds
     // After a vertex-record evolve call the step's records (vertex records,
-    // semi-implicit ratios, centroid depths; the states at parity vpar) hold
-    // the state.  records_live: they are what k_step_begin would
+    // centroid depths; the states at parity vpar) hold the state.  records_live: they are what k_step_begin would
     // build from the operator API's arrays, so the next call continues from
     // them.  api_pending: those arrays are behind the records (k_step_end and
     // the epilogue run when something reads or writes them: sync_api).
@@ -968,7 +967,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     {
         StepFields &S = h->sf;
         if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.hc, N) ||
-            dalloc(h, &S.sir, N) || dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
+            dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
             dalloc(h, &S.errw, n_inst) || dalloc(h, &S.stv[1], n_inst))
             return bail();
         S.stv[0] = h->st;
@@ -1627,8 +1626,7 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         hipLaunchKernelGGL(k_step_reset, dim3(h->n_inst), dim3(kSlots), 0, h->stream, h->sf, h->n_inst, h->vpar,
                            fresh ? 1 : 0);
         if (fresh)
-            hipLaunchKernelGGL(k_step_begin, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf,
-                               h->dp);
+            hipLaunchKernelGGL(k_step_begin, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf);
         rc = launch_check(h, "k_step_begin");
         // from here on the records hold the state; the operator API's arrays
         // follow when something reads them (sync_api)

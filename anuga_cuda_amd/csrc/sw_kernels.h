@@ -131,12 +131,12 @@ struct DevState {
     long long has_finaltime, event, error;
     // device-only
     long long halt, total_steps, bed_interp;
-    // vertex-record step (sw_step_vertex.h): whether a caller-set stage
-    // semi-implicit term is still to be applied; steps run since the step's
-    // records were built; whether the bed edge values of every shared edge
+    // vertex-record step (sw_step_vertex.h): whether caller-set semi-implicit
+    // terms are still to be applied; steps run since the step's records were
+    // built; whether the bed edge values of every shared edge
     // are bitwise equal on both sides and every vertex slot of a mesh node
     // holds the same bed value (then the bed is read per node)
-    long long wsi_live, call_steps, bed_cont;
+    long long csi_live, call_steps, bed_cont;
 };
 
 enum : long long { kErrNone = 0, kErrUpdate = 1, kErrSmallsteps = 2, kErrNeighbour = 4, kErrDiscontinuous = 8 };

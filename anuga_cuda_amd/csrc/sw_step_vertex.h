@@ -6,9 +6,9 @@
 //               minimum (one non-returning atomic per workgroup into one of
 //               kSlots slots of the instance);
 //   k_update_v  update_timestep and the time bookkeeping (every wavefront
-//               reduces the slots itself: no separate launch), then
-//               update_conserved_quantities + protect of the centroid values,
-//               in place, and the next step's Manning friction;
+//               reduces the slots itself: no separate launch), Manning
+//               friction of the centroid values the step starts from, then
+//               update_conserved_quantities + protect, in place;
 //   k_extrap_v  extrapolate_second_order_sw + balance_deep_and_shallow of the
 //               updated centroid values into the vertex records, and the
 //               centroid depth of the next step's gravity term.
@@ -19,9 +19,8 @@
 //               neighbour words 12, flags 4, the triangle's three vertex
 //               records 72, centroid depth 8; writes the update record 24 and
 //               max_speed 8 (152 B);
-//   k_update_v  reads centroid record 32, update record 24, semi-implicit
-//               ratios 16, Manning n 8, mean bed 8; writes centroid record 32,
-//               semi-implicit ratios 16 (136 B);
+//   k_update_v  reads centroid record 32, update record 24, Manning n 8,
+//               mean bed 8; writes centroid record 32 (104 B);
 //   k_extrap_v  reads centroid record 32, flags 4, centroid 16, node indices
 //               12, surrogates 12, node coordinates and bed ~12; writes vertex
 //               records 72, centroid depth 8 (168 B).
@@ -74,7 +73,6 @@ struct StepFields {
     V3 *vr;                   // [3][N] vertex records
     E3 *eur;                  // [N] explicit updates of the step (max_speed goes to DevFields::max_speed)
     double *hc;               // [N] centroid depth w - z after the step (for the next step's gravity term)
-    double2 *sir;             // [N] {xmom, ymom semi-implicit update} / centroid value (si_ratio)
     unsigned long long *slot; // [2][n_inst][kSlotBlock] CFL minima (bits of positive doubles)
     long long *errw;          // [n_inst] errors found by lanes
     DevState *stv[2];         // [n_inst] scalar state; stv[0] is the handle's state
@@ -387,7 +385,7 @@ SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevSt
         if (!r.stop) {
             r.s2.time = r.s2.time + r.s2.timestep;
             finish_step(&r.s2);
-            r.s2.wsi_live = 0;
+            r.s2.csi_live = 0;
             r.s2.call_steps += 1;
         }
     }
@@ -395,11 +393,13 @@ SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevSt
 }
 
 // The step of the centroid values, in place: update_timestep and the time
-// bookkeeping (every wavefront reduces the CFL slots itself), then
-// update_conserved_quantities + protect (quantity.c:769-820,
-// shallow_water_ext.c:1638-1705) and the next step's Manning friction
-// (shallow_water_ext.c:1238-1317) as the ratio of each momentum's
-// semi-implicit term to its centroid value.
+// bookkeeping (every wavefront reduces the CFL slots itself), Manning friction
+// (shallow_water_ext.c:1238-1317) from the centroid values the step starts
+// from (what the previous step's update left; nothing changes them in
+// between), then update_conserved_quantities + protect (quantity.c:769-820,
+// shallow_water_ext.c:1638-1705).  On the first step of a call that starts
+// from caller-set semi-implicit terms (csi_live) those are added, as the
+// forcing adds friction to them.
 #ifndef SW_UPD_MINWAVES
 #define SW_UPD_MINWAVES 1
 #endif
@@ -417,7 +417,6 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
     const unsigned long long slotv = *cfl_slot(S, M.n_inst, p, bi.inst, lane);
     Q4 c = ldq(F.cq + kk);
     const E3 eu = S.eur[kk];
-    const double2 si = ld_s(S.sir + kk);
     const double eta = ld_s(F.friction + kk);
     const double zbar = ld_s(F.zbar + kk);
 
@@ -429,8 +428,6 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
     if (lead_wave && lane == 0) S.stv[p ^ 1][bi.inst] = st.s2;
     if (!valid || st.stop) return;
     const double dt = st.s2.timestep;
-    const double siw = s.wsi_live ? F.wsi[k] : 0.0;  // a caller-set stage term (first step of a call)
-    const bool ok = advance_centroid(c, eu, siw, si, dt, P);
     double2 fr;
     if (P.use_sloped_mannings) {
         // the bed's slope from the node coordinates and bed values
@@ -447,9 +444,16 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
     } else {
         fr = friction_flat(P, eta, zbar, c);
     }
+    double siw = 0.0;
+    if (s.csi_live) {
+        siw = F.wsi[k];
+        fr.x = F.uhsi[k] + fr.x;
+        fr.y = F.vhsi[k] + fr.y;
+    }
+    const double2 sr = make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh));
+    const bool ok = advance_centroid(c, eu, siw, sr, dt, P);
     if (!ok) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrUpdate);
     F.cq[k] = c;
-    st_s(S.sir + k, make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh)));
 }
 
 // extrapolate_second_order_sw + balance_deep_and_shallow of the updated
@@ -524,36 +528,25 @@ __global__ __launch_bounds__(BLOCK, SW_EXT_MINWAVES) void k_extrap_v(DevMesh M, 
     st_s(S.hc + k, c.w - c.z);
 }
 
-// Start of an evolve call on the vertex-record step: the first step's source
-// terms (as k_source_terms: gravity term, Manning friction added to the
-// caller's semi-implicit momenta, into sir[0]), the vertex records from the
-// vertex arrays, and whether a caller-set stage semi-implicit term is live.
-__global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, StepFields S, DevParams P) {
+// Start of an evolve call on the vertex-record step: the vertex records from
+// the vertex arrays, the centroid depths, and whether caller-set semi-implicit
+// terms are live (the first step adds them).
+__global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, StepFields S) {
     const int N = M.N;
     const Lane ln = lane_of(M);
     const int k = ln.k;
     if (!ln.valid) return;
     const Q4 c = ldq(F.cq + k);
-    double x[3], y[3], zv[3], wv[3];
-    for (int i = 0; i < 3; i++) {
-        const double2 v = M.vc[i * N + k];
-        x[i] = v.x;
-        y[i] = v.y;
-        zv[i] = F.zv[i * N + k];
-        wv[i] = F.wv[i * N + k];
-        S.vr[i * N + k] = V3{wv[i], F.uhv[i * N + k], F.vhv[i * N + k]};
-    }
+    for (int i = 0; i < 3; i++) S.vr[i * N + k] = V3{F.wv[i * N + k], F.uhv[i * N + k], F.vhv[i * N + k]};
     S.hc[k] = c.w - c.z;
-    const double2 f = friction_next(P, F.friction[k], x, y, zv, c);
-    S.sir[k] = make_double2(si_ratio(F.uhsi[k] + f.x, c.uh), si_ratio(F.vhsi[k] + f.y, c.vh));
-    const bool live = F.wsi[k] != 0.0;
+    const bool live = F.wsi[k] != 0.0 || F.uhsi[k] != 0.0 || F.vhsi[k] != 0.0;
     if (__ballot(live) != 0 && (threadIdx.x & 63) == 0)
-        atomicOr((unsigned long long *)&S.stv[0][ln.inst].wsi_live, 1ull);
+        atomicOr((unsigned long long *)&S.stv[0][ln.inst].csi_live, 1ull);
 }
 
 // Start of an evolve call whose first step has parity p, per instance: both
 // parities' slots at +inf.  A fresh start (k_step_begin follows, p = 0)
-// clears the stage semi-implicit flag (k_step_begin sets it again) and the
+// clears the caller semi-implicit flag (k_step_begin sets it again) and the
 // count of steps run since the records were built; a call that continues
 // from the records of the previous one keeps both, and its first step reads
 // the handle's state (stv[0], which the caller may have changed between the
@@ -563,7 +556,7 @@ __global__ void k_step_reset(StepFields S, int n_inst, int p, int fresh) {
     for (int q = 0; q < 2; q++) *cfl_slot(S, n_inst, q, inst, threadIdx.x) = ~0ull;
     if (threadIdx.x == 0) {
         if (fresh) {
-            S.stv[0][inst].wsi_live = 0;
+            S.stv[0][inst].csi_live = 0;
             S.stv[0][inst].call_steps = 0;
         }
         if (p) S.stv[1][inst] = S.stv[0][inst];

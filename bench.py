@@ -68,10 +68,9 @@ KERNEL_BYTES_PER_TRIANGLE_VERTEX = {
     # flags 4, the triangle's vertex records 3x24 72, centroid depth 8 |
     # writes: update record 24, max_speed 8
     "flux": 12 + 12 + 12 + 4 + 72 + 8 + 24 + 8,
-    # reads: centroid record 32, update record 24, semi-implicit ratios 16,
-    # Manning n 8, mean bed 8 | writes: centroid record 32, semi-implicit
-    # ratios 16
-    "update": 32 + 24 + 16 + 8 + 8 + 32 + 16,
+    # reads: centroid record 32, update record 24, Manning n 8, mean bed 8 |
+    # writes: centroid record 32
+    "update": 32 + 24 + 8 + 8 + 32,
     # reads: centroid record 32, flags 4, centroid 16, node indices 12,
     # surrogates 12, node coordinates and bed 12 | writes: vertex records 3x24
     # 72, centroid depth 8
