@@ -89,6 +89,7 @@ struct sw_handle {
     // the epilogue run when something reads or writes them: sync_api).
     bool records_live = false;
     bool api_pending = false;
+    unsigned uni = 0;  // kUniFriction / kUniMeanBed bits that hold for every instance (k_update_v argument)
     // host copy of the instances' states (valid while st_host_ok) and the
     // pinned buffer they are read through
     std::vector<DevState> st_host;
@@ -468,7 +469,8 @@ void launch_flux_v(sw_handle *h) {
 template <int B>
 void launch_update_v(sw_handle *h) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
-    hipLaunchKernelGGL((k_update_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
+    hipLaunchKernelGGL((k_update_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar,
+                       h->uni);
 }
 template <int B>
 void launch_extrap_v(sw_handle *h) {
@@ -968,7 +970,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         StepFields &S = h->sf;
         if (put(&h->nbw, nbwv) || dalloc(h, &S.vr, 3 * N) || dalloc(h, &S.eur, N) || dalloc(h, &S.hc, N) ||
             dalloc(h, &S.slot, (size_t)2 * n_inst * kSlotBlock) ||
-            dalloc(h, &S.errw, n_inst) || dalloc(h, &S.stv[1], n_inst))
+            dalloc(h, &S.errw, n_inst) || dalloc(h, &S.stv[1], n_inst) || dalloc(h, &S.uni, n_inst))
             return bail();
         S.stv[0] = h->st;
     }
@@ -1625,9 +1627,25 @@ int sw_evolve_steps(sw_handle *h, int64_t max_steps, int64_t *steps_done, int32_
         if (fresh) h->vpar = 0;
         hipLaunchKernelGGL(k_step_reset, dim3(h->n_inst), dim3(kSlots), 0, h->stream, h->sf, h->n_inst, h->vpar,
                            fresh ? 1 : 0);
-        if (fresh)
+        if (fresh) {
             hipLaunchKernelGGL(k_step_begin, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->sf);
-        rc = launch_check(h, "k_step_begin");
+            rc = launch_check(h, "k_step_begin");
+            // which loads k_update_v can skip: a launch argument, so a change
+            // means a new step graph
+            std::vector<unsigned long long> u(h->n_inst);
+            if (rc == SW_OK) {
+                HIPCHECK(h, hipMemcpyAsync(u.data(), h->sf.uni, h->n_inst * sizeof(u[0]), hipMemcpyDeviceToHost,
+                                           h->stream));
+                HIPCHECK(h, hipStreamSynchronize(h->stream));
+                unsigned all = kUniFriction | kUniMeanBed;
+                for (auto x : u) all &= (unsigned)x;
+                if (all != h->uni) {
+                    h->uni = all;
+                    drop_graph(h);
+                }
+            }
+        }
+        if (rc == SW_OK) rc = launch_check(h, "k_step_reset");
         // from here on the records hold the state; the operator API's arrays
         // follow when something reads them (sync_api)
         h->records_live = true;

@@ -55,6 +55,12 @@
 
 namespace sw {
 
+// Properties of an instance's static fields that let k_update_v skip loads
+// (checked bitwise by k_step_begin, passed to the kernel as a launch
+// argument): every triangle has the first triangle's Manning n; every
+// triangle's centroid bed equals the mean of its bed vertex values.
+constexpr unsigned kUniFriction = 1, kUniMeanBed = 2;
+
 // CFL slots per instance and parity, one 128-byte line apart.
 constexpr int kSlots = 64;
 constexpr int kSlotStride = 16;  // unsigned long longs between slots
@@ -75,6 +81,7 @@ struct StepFields {
     double *hc;               // [N] centroid depth w - z after the step (for the next step's gravity term)
     unsigned long long *slot; // [2][n_inst][kSlotBlock] CFL minima (bits of positive doubles)
     long long *errw;          // [n_inst] errors found by lanes
+    unsigned long long *uni;  // [n_inst] kUniFriction | kUniMeanBed where they hold (k_step_begin)
     DevState *stv[2];         // [n_inst] scalar state; stv[0] is the handle's state
 };
 
@@ -405,7 +412,7 @@ SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevSt
 #endif
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, DevFields F, StepFields S,
-                                                                     DevParams P, int p) {
+                                                                     DevParams P, int p, unsigned uni) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const BlockInst bi = block_inst<BLOCK>(M, b);
     const int lane = (int)threadIdx.x & 63;
@@ -417,8 +424,8 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
     const unsigned long long slotv = *cfl_slot(S, M.n_inst, p, bi.inst, lane);
     Q4 c = ldq(F.cq + kk);
     const E3 eu = S.eur[kk];
-    const double eta = ld_s(F.friction + kk);
-    const double zbar = ld_s(F.zbar + kk);
+    const double eta = (uni & kUniFriction) ? F.friction[bi.off] : ld_s(F.friction + kk);
+    const double zbar = (uni & kUniMeanBed) ? c.z : ld_s(F.zbar + kk);
 
     // the next step's slots start at +inf (k_flux_v of the next step takes
     // its minima there after this launch has ended)
@@ -542,6 +549,13 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(DevMesh M, DevFields F, S
     const bool live = F.wsi[k] != 0.0 || F.uhsi[k] != 0.0 || F.vhsi[k] != 0.0;
     if (__ballot(live) != 0 && (threadIdx.x & 63) == 0)
         atomicOr((unsigned long long *)&S.stv[0][ln.inst].csi_live, 1ull);
+    const int off = (M.n_inst == 1) ? 0 : M.inst_off[ln.inst];
+    const bool eta_differs = dbits(F.friction[k]) != dbits(F.friction[off]);
+    const bool bed_differs = dbits(F.zbar[k]) != dbits(c.z);
+    unsigned long long clear = 0;
+    if (__ballot(eta_differs) != 0) clear |= kUniFriction;
+    if (__ballot(bed_differs) != 0) clear |= kUniMeanBed;
+    if (clear && (threadIdx.x & 63) == 0) atomicAnd(S.uni + ln.inst, ~clear);
 }
 
 // Start of an evolve call whose first step has parity p, per instance: both
@@ -558,6 +572,7 @@ __global__ void k_step_reset(StepFields S, int n_inst, int p, int fresh) {
         if (fresh) {
             S.stv[0][inst].csi_live = 0;
             S.stv[0][inst].call_steps = 0;
+            S.uni[inst] = kUniFriction | kUniMeanBed;  // k_step_begin clears what does not hold
         }
         if (p) S.stv[1][inst] = S.stv[0][inst];
     }

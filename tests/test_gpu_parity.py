@@ -666,6 +666,25 @@ def test_upload_between_chained_calls_matches_oracle():
     compare_state(d, od, TOL, stats=stats, tag="after upload.")
 
 
+def test_vertex_step_with_varying_friction_and_centroid_bed_matches_oracle():
+    """The vertex-record step reads Manning n and the mean bed per triangle
+    when they are not uniform / not the centroid bed (its update skips those
+    loads only where k_step_begin found them so): spatially varying n and a
+    centroid bed raised by 1 mm on every third triangle, in lockstep with the
+    oracle for 16 steps."""
+    d = _delaunay("morton")
+    x = d.mesh.centroid_coordinates[:, 0]
+    d.set_quantity("friction", 0.015 + 0.02 * (0.5 + 0.5 * np.sin(x / 150.0)), location="centroids")
+    # only the centroid values (set_quantity would extrapolate them to the
+    # vertices, making the bed discontinuous)
+    d.quantities["elevation"].centroid_values[::3] += 1e-3
+    d._all_dirty = True
+    assert d._dev().step_path() == _lib.SW_STEP_VERTEX
+    stats = {}
+    od, dts = _lockstep(d, 16, fused=True, stats=stats)
+    assert len(dts) == 16
+
+
 def test_two_kernel_step_needs_recomputable_geometry():
     """A mesh whose geometry arrays are not what the vertex coordinates give
     (here a radius changed in its last bit) keeps the edge-record step."""
