@@ -47,18 +47,29 @@ struct FluxParams {
     double epsilon, h0, limiting_threshold, g;
 };
 
-// shallow_water_ext.c:214-240 (_compute_speed)
+// _compute_speed (shallow_water_ext.c:214-240) for uh, then for vh on the
+// depth the first call left (the C's two calls, shallow_water_ext.c:360-379):
+// below limiting_threshold the speed is limited as the ANUGA manual has it,
+// u = uh / (h + h0 / h), and the momentum made consistent, uh = u h; when
+// h < epsilon the first call sets h = 0 and the second gives vh = 0 * 0;
+// otherwise both divide by the same h + h0 / h, formed once here; above
+// limiting_threshold u = uh / h, the second call's velocity is discarded and
+// vh is unchanged.
 template <typename R>
-SW_DEV R compute_speed(R &uh, R &h, R epsilon, R h0, R limiting_threshold) {
+SW_DEV R compute_speed2(R &uh, R &vh, R &h, R epsilon, R h0, R limiting_threshold) {
     R u;
     if (h < limiting_threshold) {
         if (h < epsilon) {
             h = R(0);
             u = R(0);
+            uh = u * h;
+            vh = u * h;
         } else {
-            u = uh / (h + h0 / h);
+            const R d = h + h0 / h;
+            u = uh / d;
+            uh = u * h;
+            vh = (vh / d) * h;
         }
-        uh = u * h;
     } else {
         u = uh / h;
     }
@@ -81,14 +92,12 @@ SW_DEV void flux_central(const R ql_in[3], const R qr_in[3], R z_left, R z_right
 
     R h_left = ql0 - z;
     R uh_left = ql1;
-    R u_left = compute_speed<R>(uh_left, h_left, epsilon, h0, limiting_threshold);
+    R vh_left = ql2;
+    R u_left = compute_speed2<R>(uh_left, vh_left, h_left, epsilon, h0, limiting_threshold);
     R h_right = qr0 - z;
     R uh_right = qr1;
-    R u_right = compute_speed<R>(uh_right, h_right, epsilon, h0, limiting_threshold);
-    R vh_left = ql2;
     R vh_right = qr2;
-    compute_speed<R>(vh_left, h_left, epsilon, h0, limiting_threshold);
-    compute_speed<R>(vh_right, h_right, epsilon, h0, limiting_threshold);
+    R u_right = compute_speed2<R>(uh_right, vh_right, h_right, epsilon, h0, limiting_threshold);
 
     R soundspeed_left = sqrt(g * h_left);
     R soundspeed_right = sqrt(g * h_right);
