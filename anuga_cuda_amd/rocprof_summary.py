@@ -15,6 +15,13 @@ Usage::
   read requests of the memory fabric at a smaller size, so its factor is not 1.
 * ``--json``: writes {kernel short name: {avg_us, fetch_bytes, write_bytes,
   traffic_bytes}} for bench.py's roofline ``traffic`` field.
+* ``--pmc DB`` (repeatable): databases of other ``rocprofv3 --pmc`` runs;
+  prints per kernel, averaged over its dispatches, what the SQ / TCC counters
+  present in them give: VALU instructions, FP64 transcendental instructions
+  and vector memory reads per wavefront (SQ_INSTS_* / SQ_WAVES), the share
+  of wavefront cycles spent waiting (SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES) and
+  issuing VALU work (SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES), and the L2 hit
+  rate (TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum)).
 * ``--bytes NAME=B`` (repeatable): algorithmic bytes per launch of the
   kernel whose base name (the function name without namespaces, template or
   parameter lists; mangled names are parsed) is NAME; prints its achieved
@@ -75,6 +82,39 @@ def counter(db, name):
     return {r[0]: (r[1], r[2], r[3]) for r in rows}
 
 
+def counters_by_dispatch(db):
+    """{kernel: {counter: mean over dispatches of the counter's sum over
+    its instances (shader engines, channels)}}."""
+    con = sqlite3.connect(db)
+    per = {}
+    for name, cname, disp, val in con.execute(
+            "select name, counter_name, dispatch_id, sum(counter_value) from pmc_events "
+            "group by name, counter_name, dispatch_id"):
+        per.setdefault(name, {}).setdefault(cname, []).append(val)
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in per.items()}
+
+
+def pmc_figures(c):
+    """Derived figures of one kernel's counters (those present)."""
+    out = {}
+    w = c.get("SQ_WAVES")
+    for key, label in (("SQ_INSTS_VALU", "valu_per_wave"), ("SQ_INSTS_VALU_TRANS_F64", "trans_f64_per_wave"),
+                       ("SQ_INSTS_VMEM_RD", "vmem_rd_per_wave")):
+        if w and key in c:
+            out[label] = c[key] / w
+    cyc = c.get("SQ_WAVE_CYCLES")
+    if cyc:
+        if "SQ_WAIT_INST_ANY" in c:
+            out["wait_frac"] = c["SQ_WAIT_INST_ANY"] / cyc
+        if "SQ_ACTIVE_INST_VALU" in c:
+            out["valu_active_frac"] = c["SQ_ACTIVE_INST_VALU"] / cyc
+    if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
+        tot = c["TCC_HIT_sum"] + c["TCC_MISS_sum"]
+        if tot:
+            out["l2_hit"] = c["TCC_HIT_sum"] / tot
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--trace", required=True)
@@ -84,6 +124,7 @@ def main(argv=None):
     ap.add_argument("--triangles", type=int, default=0, help="triangles per launch, for bytes per triangle")
     ap.add_argument("--json")
     ap.add_argument("--bytes", action="append", default=[], metavar="NAME=B")
+    ap.add_argument("--pmc", action="append", default=[], metavar="DB")
     a = ap.parse_args(argv)
 
     ks = kernel_stats(a.trace)
@@ -139,6 +180,20 @@ def main(argv=None):
                 gbs = float(b) / (avg * 1e-6) / 1e9
                 print("%-44s %12.4g %10.2f %10.1f %8.3f" % (label[:44], float(b), avg, gbs, gbs / 8000.0))
                 out.setdefault(label, {})["algorithmic_bytes"] = float(b)
+    if a.pmc:
+        figs = {}
+        for db in a.pmc:
+            for name, c in counters_by_dispatch(db).items():
+                figs.setdefault(short_name(name), {}).update(pmc_figures(c))
+        cols = ("valu_per_wave", "trans_f64_per_wave", "vmem_rd_per_wave", "wait_frac", "valu_active_frac", "l2_hit")
+        print()
+        print("%-44s" % "kernel" + "".join(" %12s" % c[:12] for c in cols))
+        for name in sorted(figs):
+            if name.startswith("__amd") or not figs[name]:
+                continue
+            print("%-44s" % name[:44] + "".join(
+                " %12.3f" % figs[name][c] if c in figs[name] else " %12s" % "-" for c in cols))
+            out.setdefault(name, {})["counters"] = figs[name]
     if a.json:
         with open(a.json, "w") as fh:
             json.dump({"fetch_scale": sf if a.fetch else None, "write_scale": sw if a.write else None,
