@@ -956,7 +956,9 @@ def evaluate_rollout(net, domain, steps, delta, judge_every=8, threshold=0.01, j
                 d, b = state[0] - last[0], last[0] - z
                 num, den = torch.stack([torch.sum(area * d * d), torch.sum(area * b * b)]).tolist()
                 err = float(np.sqrt(num) / np.sqrt(den))
-                if done > health_steps:
+                # a damaged state: spurious speeds (too many FV steps) or a
+                # non-finite value anywhere (NaN compares false with the bar)
+                if done > health_steps or not np.isfinite(err):
                     err = float("inf")
             except _lib.SwError:
                 done, err = -1, float("inf")

@@ -293,6 +293,48 @@ def test_rollout_judge_and_fallback():
     print("free rollout stage rel L2 %.3g, speed %.2fx FV" % (free["final_stage_rel_l2"], free["speedup"]))
 
 
+def test_judge_falls_back_on_a_non_finite_learned_state(monkeypatch):
+    """A learned step that leaves a NaN in the state (planted here in the
+    second learned step of the rollout's first window) makes that window's
+    judge error non-finite: the window must be recomputed by the FV loop (NaN
+    compares false with the threshold, so a plain comparison would accept
+    it), and the rollout ends finite and close to the FV reference."""
+    import torch
+
+    from anuga_cuda_amd import learned as L, scenarios
+
+    meshes = L.training_meshes(1, extent=400.0, seed=4)
+    data = L.fv_trajectories(meshes, per_mesh=8, samples=4, delta=0.05, horizon=1.0, extent=400.0, seed=2)
+    net = L.EdgeGNN(seed=2)
+    L.fit_scales(net, data)
+    dom = scenarios.delaunay_terrain(npoints=3000, extent=2000.0, mound_radius=200.0, seed=7, terrain_seed=11)
+    calls = {"rollout": False, "n": 0}
+    real_step, real_balance = L.learned_step, L.mass_balance
+
+    def balance(*a, **k):
+        r = real_balance(*a, **k)
+        calls["rollout"] = True  # the rollout's learned steps come next
+        return r
+
+    def step(net_, graph, ws, dl, w, uh, vh, z, *a, **k):
+        r = real_step(net_, graph, ws, dl, w, uh, vh, z, *a, **k)
+        if calls["rollout"]:
+            calls["n"] += 1
+            if calls["n"] == 2:
+                w[0] = float("nan")
+        return r
+
+    monkeypatch.setattr(L, "learned_step", step)
+    monkeypatch.setattr(L, "mass_balance", balance)
+    res = L.evaluate_rollout(net, dom, 8, 0.05, judge_every=4, threshold=0.01)
+    dom.close()
+    assert calls["n"] >= 2
+    assert not np.isfinite(res["judged"][0][1])  # the window's judge saw the damaged state
+    assert res["fallback_windows"] >= 1
+    assert np.isfinite(res["final_stage_rel_l2"]) and res["final_stage_rel_l2"] < 1e-3
+    assert torch.cuda.is_available()
+
+
 def test_mass_balance_with_open_boundaries():
     """On a configuration-2-style domain with transmissive (open) boundaries,
     each learned inference step (random network on top of the physical part,
