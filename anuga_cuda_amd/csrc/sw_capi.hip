@@ -581,6 +581,15 @@ int sync_api(sw_handle *h) {
     return epilogue(h);
 }
 
+// Whether the operator API's array of (quantity, location) is current even
+// while the others are behind the vertex-record step's records: the centroid
+// values of the conserved quantities and of the bed live in the centroid
+// records, which that step updates in place.
+bool record_current(int quantity, int location) {
+    return location == SW_CENTROID && (quantity == SW_STAGE || quantity == SW_XMOM || quantity == SW_YMOM ||
+                                       quantity == SW_ELEVATION || quantity == SW_FRICTION);
+}
+
 // Entry points that may write the operator API's arrays or change what the
 // step reads: the arrays are brought up to date first, and the next evolve
 // call rebuilds the step's records from them.
@@ -1106,7 +1115,8 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
 
 int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t count) {
     if (!h || !host) return SW_ERR_ARG;
-    if (int rc = sync_api(h)) return rc;
+    if (!record_current(quantity, location))
+        if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const Inst &I = h->inst[i];
@@ -1181,7 +1191,8 @@ int sw_upload_device(sw_handle *h, int quantity, int location, const double *dsr
 
 int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, int64_t count) {
     if (!h || !ddst) return SW_ERR_ARG;
-    if (int rc = sync_api(h)) return rc;
+    if (!record_current(quantity, location))
+        if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     if (int rc = download_async(h, i, quantity, location, ddst, count)) return rc;
@@ -1212,7 +1223,6 @@ int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, con
 
 int sw_download_conserved_device(sw_handle *h, double *stage, double *xmom, double *ymom) {
     if (!h || !stage || !xmom || !ymom) return SW_ERR_ARG;
-    if (int rc = sync_api(h)) return rc;
     const int i = selected(h);
     if (i < 0) return i;
     const int64_t n = h->inst[i].n;
@@ -1524,7 +1534,10 @@ int sw_update_other_quantities(sw_handle *h) {
 // distribute_to_vertices_and_edges of every instance (protect, then the
 // reconstruction kernel with each instance's order), used by the evolve
 // preamble.
-int distribute_all(sw_handle *h) {
+// protect, then distribute_to_vertices_and_edges of the edge-record path on
+// every instance (and the boundary values if `boundary`); d receives the
+// states after it (one read).
+int distribute_states(sw_handle *h, std::vector<DevState> &d, bool boundary) {
     const unsigned g = grid_for(h->N);
     hipLaunchKernelGGL(k_protect, dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     const bool limited = h->params.limiter != SW_LIMITER_OPTIMISED;
@@ -1543,9 +1556,15 @@ int distribute_all(sw_handle *h) {
             hipLaunchKernelGGL((k_distribute<double, false>), dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f,
                                h->dp, h->st);
     }
+    if (boundary && h->Nb > 0)
+        hipLaunchKernelGGL(k_update_boundary, dim3(grid_for(h->Nb)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     if (int rc = launch_check(h, "distribute")) return rc;
-    std::vector<DevState> d;
     return sync_all(h, d);
+}
+
+int distribute_all(sw_handle *h) {
+    std::vector<DevState> d;
+    return distribute_states(h, d, false);
 }
 
 int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
@@ -1557,8 +1576,7 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
     if (int rc = sync_all(h, d)) return rc;
     for (auto &s : d) s.order = h->params.default_order;
     HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
-    if (int rc = distribute_all(h)) return rc;
-    if (int rc = sync_all(h, d)) return rc;
+    if (int rc = distribute_states(h, d, false)) return rc;
     for (auto &s : d) {
         s.time = s.starttime;
         s.yieldstep = yieldstep;
@@ -1576,12 +1594,7 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
         s.halt = 0;
     }
     HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
-    if (int rc = distribute_all(h)) return rc;
-    if (h->Nb > 0) {
-        hipLaunchKernelGGL(k_update_boundary, dim3(grid_for(h->Nb)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
-        if (int rc = launch_check(h, "update_boundary")) return rc;
-    }
-    return sync_all(h, d);
+    return distribute_states(h, d, true);
 }
 
 // Steps to launch before the next host check: the estimated number of steps
