@@ -16,9 +16,9 @@
 // What goes through HBM per triangle-step (FP64; node arrays gathered through
 // the node indices, about half a node per triangle):
 //   k_flux_v    reads node indices 12, node coordinates and bed ~12,
-//               neighbour words 12, flags 4, the triangle's three vertex
-//               records 72, centroid depth 8; writes the update record 24 and
-//               max_speed 8 (152 B);
+//               neighbour words 12, flags 4, {inscribed radius, area} 16, the
+//               triangle's three vertex records 72, centroid depth 8; writes
+//               the update record 24 and max_speed 8 (168 B);
 //   k_update_v  reads centroid record 32, update record 24, Manning n 8,
 //               mean bed 8; writes centroid record 32 (104 B);
 //   k_extrap_v  reads centroid record 32, flags 4, centroid 16, node indices
@@ -37,10 +37,11 @@
 //     the bed vertex values (required: bed_interp), and every vertex slot of a
 //     mesh node holds the node's bed value (required: bed_cont), so both sides
 //     of a shared edge see the same bed and the bed is stored per node;
-//   * normals, edge lengths, inscribed radius, area and centroid are
-//     recomputed from the vertex coordinates by tri_geometry, the formulas the
-//     mesh is built with; sw_create checks on the host that they reproduce the
-//     caller's arrays bit for bit and otherwise keeps the edge-record step;
+//   * normals and edge lengths are recomputed from the vertex coordinates by
+//     tri_geometry, the formulas the mesh is built with (inscribed radius and
+//     area are read: they cost more to form than to load); sw_create checks
+//     on the host that tri_geometry reproduces the caller's arrays bit for
+//     bit and otherwise keeps the edge-record step;
 //   * the stage-gradient gravity term is formed by the flux from the vertex
 //     stage and the centroid depth the previous k_extrap_v stored (the same
 //     operations on the same values as forming it there);
@@ -180,6 +181,10 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
         for (int i = 0; i < 3; i++) nb[i] = ld_s(nbw + i * N + k);
         const int flags = ld_s(M.flags + k);
         const double avg_h = ld_s(S.hc + k);
+        // inscribed radius and area (the caller's arrays, which sw_create
+        // checked tri_geometry reproduces: reading them is cheaper than the
+        // three square roots and two divisions that form them)
+        const double2 ra = ld_s(M.ra + k);
         // the coordinates and bed of the triangle's nodes (every vertex slot
         // of a node holds its value, checked when the bed is uploaded)
 #pragma unroll
@@ -283,8 +288,8 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
             s1 = s1 + l1;
             s2 = s2 + l2;
         }
-        if (cfl_speed > 0.0) dtk = omin<double>(dtk, gm.radius / cfl_speed);
-        const double area = gm.area;
+        if (cfl_speed > 0.0) dtk = omin<double>(dtk, ra.x / cfl_speed);
+        const double area = ra.y;
         const double inv_area = 1.0 / area;
         s0 *= inv_area;
         s1 *= inv_area;

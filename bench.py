@@ -65,9 +65,9 @@ STEP_BYTES_PER_TRIANGLE = sum(KERNEL_BYTES_PER_TRIANGLE.values())
 # triangle) are gathered through the node indices and counted once.
 KERNEL_BYTES_PER_TRIANGLE_VERTEX = {
     # reads: node indices 12, node coordinates and bed 12, neighbour words 12,
-    # flags 4, the triangle's vertex records 3x24 72, centroid depth 8 |
-    # writes: update record 24, max_speed 8
-    "flux": 12 + 12 + 12 + 4 + 72 + 8 + 24 + 8,
+    # flags 4, {radius, area} 16, the triangle's vertex records 3x24 72,
+    # centroid depth 8 | writes: update record 24, max_speed 8
+    "flux": 12 + 12 + 12 + 4 + 16 + 72 + 8 + 24 + 8,
     # reads: centroid record 32, update record 24, Manning n 8, mean bed 8 |
     # writes: centroid record 32
     "update": 32 + 24 + 8 + 8 + 32,
