@@ -99,7 +99,8 @@ def pmc_figures(c):
     out = {}
     w = c.get("SQ_WAVES")
     for key, label in (("SQ_INSTS_VALU", "valu_per_wave"), ("SQ_INSTS_VALU_TRANS_F64", "trans_f64_per_wave"),
-                       ("SQ_INSTS_VMEM_RD", "vmem_rd_per_wave")):
+                       ("SQ_INSTS_VMEM_RD", "vmem_rd_per_wave"), ("SQ_INSTS_MFMA", "mfma_per_wave"),
+                       ("SQ_INSTS_LDS", "lds_per_wave")):
         if w and key in c:
             out[label] = c[key] / w
     cyc = c.get("SQ_WAVE_CYCLES")
@@ -108,6 +109,10 @@ def pmc_figures(c):
             out["wait_frac"] = c["SQ_WAIT_INST_ANY"] / cyc
         if "SQ_ACTIVE_INST_VALU" in c:
             out["valu_active_frac"] = c["SQ_ACTIVE_INST_VALU"] / cyc
+        if "SQ_WAIT_ANY" in c:
+            out["wait_any_frac"] = c["SQ_WAIT_ANY"] / cyc
+    if "SQ_LDS_BANK_CONFLICT" in c and c.get("SQ_ACTIVE_INST_LDS"):
+        out["lds_conflict_frac"] = c["SQ_LDS_BANK_CONFLICT"] / c["SQ_ACTIVE_INST_LDS"]
     if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
         tot = c["TCC_HIT_sum"] + c["TCC_MISS_sum"]
         if tot:
@@ -181,11 +186,14 @@ def main(argv=None):
                 print("%-44s %12.4g %10.2f %10.1f %8.3f" % (label[:44], float(b), avg, gbs, gbs / 8000.0))
                 out.setdefault(label, {})["algorithmic_bytes"] = float(b)
     if a.pmc:
-        figs = {}
+        merged = {}
         for db in a.pmc:
             for name, c in counters_by_dispatch(db).items():
-                figs.setdefault(short_name(name), {}).update(pmc_figures(c))
-        cols = ("valu_per_wave", "trans_f64_per_wave", "vmem_rd_per_wave", "wait_frac", "valu_active_frac", "l2_hit")
+                merged.setdefault(short_name(name), {}).update(c)
+        figs = {name: pmc_figures(c) for name, c in merged.items()}
+        cols = [c for c in ("valu_per_wave", "trans_f64_per_wave", "vmem_rd_per_wave", "mfma_per_wave",
+                            "lds_per_wave", "wait_frac", "wait_any_frac", "valu_active_frac", "lds_conflict_frac",
+                            "l2_hit") if any(c in f for f in figs.values())]
         print()
         print("%-44s" % "kernel" + "".join(" %12s" % c[:12] for c in cols))
         for name in sorted(figs):
