@@ -451,7 +451,7 @@ int enqueue_fused_step(sw_handle *h, int slot) {
 // states `d` of the instances.
 bool vstep_applies(const sw_handle *h, const std::vector<DevState> &d) {
     if (h->step_path == SW_STEP_EDGE || !h->vstep_geometry) return false;
-    if (h->params.precision != SW_PRECISION_FP64 || rk_stages(h->params) != 1) return false;
+    if (h->params.precision != SW_PRECISION_FP64) return false;
     if (h->params.limiter != SW_LIMITER_OPTIMISED) return false;
     for (const auto &s : d)
         if (!s.bed_interp || !s.bed_cont) return false;
@@ -467,10 +467,10 @@ void launch_flux_v(sw_handle *h) {
     hipLaunchKernelGGL((k_flux_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->nbw, h->dp, h->vpar);
 }
 template <int B>
-void launch_update_v(sw_handle *h) {
+void launch_update_v(sw_handle *h, int mode, double a, double b) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
     hipLaunchKernelGGL((k_update_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar,
-                       h->uni);
+                       h->uni, mode, a, b);
 }
 template <int B>
 void launch_extrap_v(sw_handle *h) {
@@ -478,33 +478,44 @@ void launch_extrap_v(sw_handle *h) {
     hipLaunchKernelGGL((k_extrap_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
 }
 
-// Enqueues one vertex-record step at parity h->vpar; the timing slots are
-// flux (k_flux_v), update (k_update_v) and distribute (k_extrap_v).
+// Enqueues one vertex-record step (its Runge-Kutta stages, one parity each)
+// starting at parity h->vpar; the timing slots are the first stage's flux
+// (k_flux_v), update (k_update_v) and distribute (k_extrap_v).
 int enqueue_vstep(sw_handle *h, int slot) {
+    const int stages = rk_stages(h->params);
     auto rec = [&](int idx) {
         if (slot >= 0) (void)hipEventRecord(h->ev[slot * (kNumTimed + 1) + idx], h->stream);
     };
-    rec(0);
-    switch (h->blk[0]) {
-        case 128: launch_flux_v<128>(h); break;
-        case 512: launch_flux_v<512>(h); break;
-        default: launch_flux_v<256>(h); break;
+    for (int stage = 1; stage <= stages; stage++) {
+        const bool first = stage == 1;
+        int mode = kStepEuler;
+        double a = 0.0, b = 0.0;
+        if (stages > 1) {
+            mode = first ? kStageFirst : stage == stages ? kStageLast : kStageMid;
+            if (!first) rk_weights(stages, stage, &a, &b);
+        }
+        if (first) rec(0);
+        switch (h->blk[0]) {
+            case 128: launch_flux_v<128>(h); break;
+            case 512: launch_flux_v<512>(h); break;
+            default: launch_flux_v<256>(h); break;
+        }
+        if (first) rec(1);
+        if (first) rec(2);
+        switch (h->blk[1]) {
+            case 128: launch_update_v<128>(h, mode, a, b); break;
+            case 512: launch_update_v<512>(h, mode, a, b); break;
+            default: launch_update_v<256>(h, mode, a, b); break;
+        }
+        if (first) rec(3);
+        switch (h->blk[2]) {
+            case 128: launch_extrap_v<128>(h); break;
+            case 512: launch_extrap_v<512>(h); break;
+            default: launch_extrap_v<256>(h); break;
+        }
+        if (first) rec(4);
+        h->vpar ^= 1;
     }
-    rec(1);
-    rec(2);
-    switch (h->blk[1]) {
-        case 128: launch_update_v<128>(h); break;
-        case 512: launch_update_v<512>(h); break;
-        default: launch_update_v<256>(h); break;
-    }
-    rec(3);
-    switch (h->blk[2]) {
-        case 128: launch_extrap_v<128>(h); break;
-        case 512: launch_extrap_v<512>(h); break;
-        default: launch_extrap_v<256>(h); break;
-    }
-    rec(4);
-    h->vpar ^= 1;
     return launch_check(h, "vertex-record step");
 }
 

@@ -249,7 +249,9 @@ SW_DEV bool stage_skipped(DevState *st, int first) {
 // stage uses the first stage's timestep, gpu_domain_advanced.py:2566-2573
 // with ANUGA's evolve_one_rk2_step / evolve_one_rk3_step); the last stage of
 // one (its flux_timestep and the end-of-step bookkeeping).
-enum : int { kStepEuler = 0, kStageFirst = 1, kStageLast = 2 };
+// kStageMid: a Runge-Kutta stage between the first and the last (rk3's
+// second), which neither takes nor reports a timestep (vertex-record step).
+enum : int { kStepEuler = 0, kStageFirst = 1, kStageLast = 2, kStageMid = 3 };
 
 // ---------------------------------------------------------------------------
 // compute_fluxes (shallow_water_ext.c:4519-4696).

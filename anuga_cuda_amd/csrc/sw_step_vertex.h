@@ -329,6 +329,20 @@ SW_DEV bool advance_centroid(Q4 &c, const E3 &eu, double siw, const double2 &sr,
     return ok;
 }
 
+// A later Runge-Kutta stage: the update, then q = a q + b q0 with the values
+// the step started from (saxpy_conserved_quantities), then protect -- the
+// edge-record step's order (k_update, kUpdateSaxpy).
+SW_DEV bool advance_centroid_saxpy(Q4 &c, const Q4 &c0, const E3 &eu, double siw, const double2 &sr, double dt,
+                                   double a, double b, const DevParams &P) {
+    double sw = siw;
+    bool ok = update_one(c.w, eu.w, sw, dt);
+    ok = update_ratio(c.uh, eu.uh, sr.x, dt) && ok;
+    ok = update_ratio(c.vh, eu.vh, sr.y, dt) && ok;
+    saxpy_conserved(c, c0, a, b);
+    protect_one(c.w, c.z, c.uh, c.vh, P.minimum_allowed_height, P.maximum_allowed_speed, P.epsilon);
+    return ok;
+}
+
 // Manning friction of the next step (shallow_water_ext.c:1238-1317) from the
 // final centroid record, with the geometry in registers; the operations of
 // friction_si.
@@ -385,22 +399,29 @@ struct StepStart {
     bool stop;
 };
 SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevState &s, long long errw,
-                            unsigned long long slotv) {
-    const bool halted = s.event || s.error || s.halt || errw;
+                            unsigned long long slotv, int mode) {
+    // a step starts unless a yield, final time or error is pending; a later
+    // stage runs unless the step halted or failed (stage_skipped)
+    const bool halted = (mode == kStepEuler || mode == kStageFirst) ? (s.event || s.error || s.halt || errw)
+                                                                     : (s.halt || s.error || errw);
     StepStart r{s, halted};
     if (halted) {
         r.s2.halt = 1;
-    } else {
-        r.s2.flux_timestep = bitsd(wave_min_u64(slotv));
+        return r;
+    }
+    if (mode == kStageMid) return r;
+    // the first stage takes the timestep and advances the time; the last
+    // reports its flux timestep and ends the step (k_timestep's modes)
+    r.s2.flux_timestep = bitsd(wave_min_u64(slotv));
+    if (mode != kStageLast) {
         timestep_logic(&r.s2, P);
         r.stop = (r.s2.error & kErrSmallsteps) != 0;
-        if (!r.stop) {
-            r.s2.time = r.s2.time + r.s2.timestep;
-            finish_step(&r.s2);
-            r.s2.csi_live = 0;
-            r.s2.call_steps += 1;
-        }
+        if (r.stop) return r;
+        r.s2.time = r.s2.time + r.s2.timestep;
+        r.s2.csi_live = 0;
+        r.s2.call_steps += 1;
     }
+    if (mode != kStageFirst) finish_step(&r.s2);
     return r;
 }
 
@@ -411,13 +432,17 @@ SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevSt
 // between), then update_conserved_quantities + protect (quantity.c:769-820,
 // shallow_water_ext.c:1638-1705).  On the first step of a call that starts
 // from caller-set semi-implicit terms (csi_live) those are added, as the
-// forcing adds friction to them.
+// forcing adds friction to them.  Runge-Kutta stages (mode): the first keeps
+// the values it starts from in cq0; later ones combine their update with them
+// (q = a q + b q0) before protect; only the first takes a timestep and only
+// the last ends the step.
 #ifndef SW_UPD_MINWAVES
 #define SW_UPD_MINWAVES 1
 #endif
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, DevFields F, StepFields S,
-                                                                     DevParams P, int p, unsigned uni) {
+                                                                     DevParams P, int p, unsigned uni, int mode,
+                                                                     double sa, double sb) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const BlockInst bi = block_inst<BLOCK>(M, b);
     const int lane = (int)threadIdx.x & 63;
@@ -436,7 +461,7 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
     // its minima there after this launch has ended)
     if (lead_wave) *cfl_slot(S, M.n_inst, p ^ 1, bi.inst, lane) = ~0ull;
     const DevState s = S.stv[p][bi.inst];
-    const StepStart st = step_start(S, P, s, S.errw[bi.inst], slotv);
+    const StepStart st = step_start(S, P, s, S.errw[bi.inst], slotv, mode);
     if (lead_wave && lane == 0) S.stv[p ^ 1][bi.inst] = st.s2;
     if (!valid || st.stop) return;
     const double dt = st.s2.timestep;
@@ -463,7 +488,13 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
         fr.y = F.vhsi[k] + fr.y;
     }
     const double2 sr = make_double2(si_ratio(fr.x, c.uh), si_ratio(fr.y, c.vh));
-    const bool ok = advance_centroid(c, eu, siw, sr, dt, P);
+    bool ok;
+    if (mode == kStageMid || mode == kStageLast) {
+        ok = advance_centroid_saxpy(c, ldq(F.cq0 + k), eu, siw, sr, dt, sa, sb, P);
+    } else {
+        if (mode == kStageFirst) F.cq0[k] = c;  // backup_conserved_quantities
+        ok = advance_centroid(c, eu, siw, sr, dt, P);
+    }
     if (!ok) atomicOr((unsigned long long *)(S.errw + bi.inst), (unsigned long long)kErrUpdate);
     F.cq[k] = c;
 }

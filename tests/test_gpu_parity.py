@@ -566,14 +566,17 @@ def test_non_default_parameter_branches_match_oracle(case, fused):
                                                   {k: v[0] for k, v in stats.items() if v[0]}))
 
 
-def test_two_kernel_step_equals_four_kernel_step():
-    """The vertex-record Euler step (flux; then timestep, update and distribute of
-    each triangle with its surrogate neighbours' updates recomputed) and the
-    edge-record step give the same bits: timesteps, yields and every array of
-    the operator API, over yields that stop a captured 16-step graph part way."""
+@pytest.mark.parametrize("method", ["euler", "rk2", "rk3"])
+def test_vertex_record_step_equals_edge_record_step(method):
+    """The vertex-record step (flux from the vertex records; timestep, update
+    and friction; extrapolation and balance), with Euler steps or the stages
+    of rk2 / rk3, and the edge-record step give the same bits: timesteps,
+    yields and every array of the operator API, over yields that stop a
+    captured 16-step graph part way."""
     runs = {}
     for path in (_lib.SW_STEP_AUTO, _lib.SW_STEP_EDGE):
         d = _delaunay("random")
+        d.set_timestepping_method(method)
         d._dev().set_step_path(path)
         assert d._handle.step_path() == (_lib.SW_STEP_VERTEX if path == _lib.SW_STEP_AUTO else _lib.SW_STEP_EDGE)
         dts, steps = [], []
@@ -687,7 +690,9 @@ def test_vertex_step_with_varying_friction_and_centroid_bed_matches_oracle():
 
 def test_two_kernel_step_needs_recomputable_geometry():
     """A mesh whose geometry arrays are not what the vertex coordinates give
-    (here a radius changed in its last bit) keeps the edge-record step."""
+    (here a radius changed in its last bit) keeps the edge-record step, and
+    so does the mixed-precision loop; rk2 / rk3 run on the vertex-record
+    step."""
     d = _delaunay("morton")
     d.mesh.radii = d.mesh.radii.copy()
     d.mesh.radii[10] = np.nextafter(d.mesh.radii[10], 0.0)
@@ -695,7 +700,10 @@ def test_two_kernel_step_needs_recomputable_geometry():
     d2 = _delaunay("morton")
     assert d2._dev().step_path() == _lib.SW_STEP_VERTEX
     d2.set_timestepping_method("rk2")
-    assert d2._handle.step_path() == _lib.SW_STEP_EDGE
+    assert d2._handle.step_path() == _lib.SW_STEP_VERTEX
+    d3 = scenarios.delaunay_terrain(npoints=3000, order="morton", extent=2000.0, mound_radius=200.0, seed=7,
+                                    terrain_seed=11, precision="mixed")
+    assert d3._dev().step_path() == _lib.SW_STEP_EDGE
 
 
 def test_ensemble_members_must_share_their_start_time():
