@@ -328,8 +328,8 @@ int sw_kernel_times(sw_handle *h, double *ms, int64_t *launches, int32_t nkernel
 /* Kernels of the fused step: no counterpart in the reference (its drivers
  * launch one kernel per operator).  The vertex-record step (flux from the
  * vertex records; timestep, update, protect and the next step's friction;
- * extrapolation and balance into the vertex records) applies to FP64 Euler
- * steps with the optimised limiter on meshes whose geometry arrays equal those
+ * extrapolation and balance into the vertex records) applies to FP64 Euler,
+ * rk2 and rk3 steps with the optimised limiter on meshes whose geometry arrays equal those
  * recomputed from the vertex coordinates, whose bed edge values are the
  * interpolation of the bed vertex values and whose bed is single-valued at
  * every mesh node; it gives the same bits as the edge-record step (flux,
