@@ -452,7 +452,6 @@ int enqueue_fused_step(sw_handle *h, int slot) {
 bool vstep_applies(const sw_handle *h, const std::vector<DevState> &d) {
     if (h->step_path == SW_STEP_EDGE || !h->vstep_geometry) return false;
     if (h->params.precision != SW_PRECISION_FP64) return false;
-    if (h->params.limiter != SW_LIMITER_OPTIMISED) return false;
     for (const auto &s : d)
         if (!s.bed_interp || !s.bed_cont) return false;
     return true;
@@ -475,7 +474,11 @@ void launch_update_v(sw_handle *h, int mode, double a, double b) {
 template <int B>
 void launch_extrap_v(sw_handle *h) {
     const unsigned g = (unsigned)((h->N + B - 1) / B);
-    hipLaunchKernelGGL((k_extrap_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
+    if (h->params.limiter != SW_LIMITER_OPTIMISED)
+        hipLaunchKernelGGL((k_extrap_v<B, true>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp,
+                           h->vpar);
+    else
+        hipLaunchKernelGGL((k_extrap_v<B>), dim3(g), dim3(B), 0, h->stream, h->mesh, h->f, h->sf, h->dp, h->vpar);
 }
 
 // Enqueues one vertex-record step (its Runge-Kutta stages, one parity each)

@@ -891,10 +891,17 @@ SW_DEV void reconstruct_limited(const RecIn &r, int k, int nob, double qc, const
             b = ((x1 - x0) * (q2 - q0) - (x2 - x0) * (q1 - q0)) / det;
         }
     } else if (nob == 2) {
-        const int i = (r.sk[0] != k) ? 0 : (r.sk[1] != k) ? 1 : 2;
-        if (r.sk[i] != k) {
-            const double x0 = r.xs[i].x, y0 = r.xs[i].y, x1 = r.xy.x, y1 = r.xy.y;
-            const double q0 = qs[i], q1 = qc;
+        // the one neighbour, chosen among values loaded beforehand: a
+        // run-time index into the records (or a select of their addresses)
+        // would put them in scratch memory
+        const bool n0 = r.sk[0] != k, n1 = r.sk[1] != k, n2 = r.sk[2] != k;
+        const double2 s0 = r.xs[0], s1 = r.xs[1], s2 = r.xs[2];
+        const double qs0 = qs[0], qs1 = qs[1], qs2 = qs[2];
+        if (n0 || n1 || n2) {
+            const double x0 = n0 ? s0.x : n1 ? s1.x : s2.x;
+            const double y0 = n0 ? s0.y : n1 ? s1.y : s2.y;
+            const double x1 = r.xy.x, y1 = r.xy.y;
+            const double q0 = n0 ? qs0 : n1 ? qs1 : qs2, q1 = qc;
             const double det = (x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0);
             a = (x1 - x0) * (q1 - q0) / det;
             b = (y1 - y0) * (q1 - q0) / det;

@@ -405,23 +405,25 @@ SW_DEV StepStart step_start(const StepFields &S, const DevParams &P, const DevSt
     const bool halted = (mode == kStepEuler || mode == kStageFirst) ? (s.event || s.error || s.halt || errw)
                                                                      : (s.halt || s.error || errw);
     StepStart r{s, halted};
-    if (halted) {
-        r.s2.halt = 1;
-        return r;
+    if (!halted && mode != kStageMid) {
+        // the first stage takes the timestep and advances the time; the last
+        // reports its flux timestep and ends the step (k_timestep's modes)
+        r.s2.flux_timestep = bitsd(wave_min_u64(slotv));
+        if (mode != kStageLast) {
+            timestep_logic(&r.s2, P);
+            r.stop = (r.s2.error & kErrSmallsteps) != 0;
+            if (!r.stop) {
+                r.s2.time = r.s2.time + r.s2.timestep;
+                r.s2.csi_live = 0;
+                r.s2.call_steps += 1;
+            }
+        }
+        if (!r.stop && mode != kStageFirst) finish_step(&r.s2);
     }
-    if (mode == kStageMid) return r;
-    // the first stage takes the timestep and advances the time; the last
-    // reports its flux timestep and ends the step (k_timestep's modes)
-    r.s2.flux_timestep = bitsd(wave_min_u64(slotv));
-    if (mode != kStageLast) {
-        timestep_logic(&r.s2, P);
-        r.stop = (r.s2.error & kErrSmallsteps) != 0;
-        if (r.stop) return r;
-        r.s2.time = r.s2.time + r.s2.timestep;
-        r.s2.csi_live = 0;
-        r.s2.call_steps += 1;
-    }
-    if (mode != kStageFirst) finish_step(&r.s2);
+    // written on every path: a store on the halted path alone would be merged
+    // with finish_step's store of the event into one store through a chosen
+    // address, which keeps the state in scratch memory
+    r.s2.halt = halted ? 1 : r.s2.halt;
     return r;
 }
 
@@ -502,10 +504,13 @@ __global__ __launch_bounds__(BLOCK, SW_UPD_MINWAVES) void k_update_v(DevMesh M, 
 // extrapolate_second_order_sw + balance_deep_and_shallow of the updated
 // centroid values (shallow_water_ext.c:2717-3324, :1462-1636): the vertex
 // records of the next step's flux and the centroid depth of its gravity term.
+// LIMITED: the per-quantity limiters instead of the optimised one
+// (extrapolate_second_order_and_limit_by_vertex_or_edge.cu), chosen at
+// compile time as in k_distribute.
 #ifndef SW_EXT_MINWAVES
 #define SW_EXT_MINWAVES 1
 #endif
-template <int BLOCK>
+template <int BLOCK, bool LIMITED = false>
 __global__ __launch_bounds__(BLOCK, SW_EXT_MINWAVES) void k_extrap_v(DevMesh M, DevFields F, StepFields S,
                                                                      DevParams P, int p) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -553,6 +558,10 @@ __global__ __launch_bounds__(BLOCK, SW_EXT_MINWAVES) void k_extrap_v(DevMesh M, 
         wv[0] = wv[1] = wv[2] = c.w;
         uv[0] = uv[1] = uv[2] = c.uh;
         vv[0] = vv[1] = vv[2] = c.vh;
+    } else if (LIMITED) {
+        // the per-quantity limiters (limit by vertex / by edge), each
+        // quantity with its own beta
+        reconstruct_limited_all(P, k, r, wv, uv, vv);
     } else {
         // a dry triangle keeps its previous vertex values
         auto stale = [&](int i, double &w, double &u, double &v) {

@@ -105,12 +105,14 @@ def workload_tag(args):
     return tag if args.locality == "hilbert" else tag + "_keep_order"
 
 
-def kernel_symbol(kernel, precision, launch, vstep=False):
+def kernel_symbol(kernel, precision, launch, vstep=False, limited=False):
     """Name of a fused kernel as rocprofv3 reports it; ``launch`` = workgroup
-    sizes of (flux, update, distribute)."""
+    sizes of (flux, update, distribute); ``limited``: the per-quantity
+    limiters' variant of the vertex-record extrapolation."""
     if vstep:
+        extrap = "k_extrap_v<%d, true>" % launch[2] if limited else "k_extrap_v<%d>" % launch[2]
         return {"flux": "k_flux_v<%d>" % launch[0], "update": "k_update_v<%d>" % launch[1],
-                "distribute": "k_extrap_v<%d>" % launch[2]}[kernel]
+                "distribute": extrap}[kernel]
     r = "double" if precision == "fp64" else "float"
     fb, ub, db = launch
     flux = "k_flux<%s, true, %d>" % (r, fb) if precision == "fp64" else "k_flux_mixed<%d>" % fb
@@ -292,7 +294,8 @@ def main():
     if os.path.exists(tfile):
         kernels = json.load(open(tfile)).get("kernels", {})
         if vstep:
-            recs = {k: kernels.get(kernel_symbol(k, args.precision, launch, True)) for k in kb}
+            recs = {k: kernels.get(kernel_symbol(k, args.precision, launch, True, args.limiter != "optimised"))
+                    for k in kb}
             rec = recs[dominant] or {}
             if all(recs.values()):
                 step_traffic = sum(r["traffic_bytes"] for r in recs.values())
@@ -349,7 +352,8 @@ def main():
                    "triangles": N, "precision": args.precision, "order": 2, "timestepping": args.timestepping,
                    "limiter": args.limiter, "mesh_build": args.mesh_build,
                    "device_numbering": args.locality, "workgroup_sizes": list(launch)},
-        "roofline": {"bound": "hbm", "kernel": kernel_symbol(dominant, args.precision, launch, vstep),
+        "roofline": {"bound": "hbm", "kernel": kernel_symbol(dominant, args.precision, launch, vstep,
+                                                               args.limiter != "optimised"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_triangle": kernel_roof_bytes,

@@ -24,9 +24,9 @@ def _dam(**kw):
     return scenarios.dam_break_channel(24, 24, **kw)
 
 
-def _delaunay(order):
+def _delaunay(order, **kw):
     return scenarios.delaunay_terrain(npoints=3000, order=order, extent=2000.0, mound_radius=200.0,
-                                      seed=7, terrain_seed=11)
+                                      seed=7, terrain_seed=11, **kw)
 
 
 SCENARIOS = {
@@ -143,6 +143,8 @@ def test_per_quantity_limiters_match_oracle(limiter, fused):
     d.quantities["xmomentum"].beta = 0.7
     d.quantities["ymomentum"].beta = 1.3
     assert d.limiter_path() == limiter
+    if fused:
+        assert d._dev().step_path() == _lib.SW_STEP_VERTEX
     stats = {}
     _, dts = _lockstep(d, 16 if fused else 6, fused=fused, stats=stats)
     assert len(dts) == (16 if fused else 6)
@@ -566,16 +568,23 @@ def test_non_default_parameter_branches_match_oracle(case, fused):
                                                   {k: v[0] for k, v in stats.items() if v[0]}))
 
 
-@pytest.mark.parametrize("method", ["euler", "rk2", "rk3"])
-def test_vertex_record_step_equals_edge_record_step(method):
+@pytest.mark.parametrize("method,limiter", [("euler", "optimised"), ("rk2", "optimised"), ("rk3", "optimised"),
+                                            ("euler", "vertex"), ("rk2", "edge")])
+def test_vertex_record_step_equals_edge_record_step(method, limiter):
     """The vertex-record step (flux from the vertex records; timestep, update
     and friction; extrapolation and balance), with Euler steps or the stages
-    of rk2 / rk3, and the edge-record step give the same bits: timesteps,
-    yields and every array of the operator API, over yields that stop a
-    captured 16-step graph part way."""
+    of rk2 / rk3 and with the optimised or a per-quantity limiter (betas that
+    differ per quantity), and the edge-record step give the same bits:
+    timesteps, yields and every array of the operator API, over yields that
+    stop a captured 16-step graph part way."""
+    kw = {"vertex": {"optimised_gradient_limiter": 0}, "edge": {"use_edge_limiter": 1}}.get(limiter, {})
     runs = {}
     for path in (_lib.SW_STEP_AUTO, _lib.SW_STEP_EDGE):
-        d = _delaunay("random")
+        d = _delaunay("random", **kw)
+        if limiter != "optimised":
+            d.quantities["xmomentum"].beta = 0.7
+            d.quantities["ymomentum"].beta = 1.3
+        assert d.limiter_path() == limiter
         d.set_timestepping_method(method)
         d._dev().set_step_path(path)
         assert d._handle.step_path() == (_lib.SW_STEP_VERTEX if path == _lib.SW_STEP_AUTO else _lib.SW_STEP_EDGE)
