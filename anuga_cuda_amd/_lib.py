@@ -109,6 +109,7 @@ _SIGNATURES = {
     "sw_update_timestep": ([C.c_void_p, C.c_double, C.c_double, C.c_int32], C.c_int),
     "sw_update_conserved_quantities": ([C.c_void_p], C.c_int),
     "sw_protect": ([C.c_void_p], C.c_int),
+    "sw_protect_swb2": ([C.c_void_p], C.c_int),
     "sw_extrapolate_second_order_sw": ([C.c_void_p], C.c_int),
     "sw_extrapolate_first_order": ([C.c_void_p], C.c_int),
     "sw_balance_deep_and_shallow": ([C.c_void_p], C.c_int),

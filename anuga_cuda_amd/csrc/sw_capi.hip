@@ -1454,6 +1454,14 @@ int sw_protect(sw_handle *h) {
     return launch_check(h, "protect");
 }
 
+int sw_protect_swb2(sw_handle *h) {
+    if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
+    if (int rc = single(h, "protect_swb2")) return rc;
+    hipLaunchKernelGGL(k_protect_swb2, dim3(grid_for(h->N)), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
+    return launch_check(h, "protect_swb2");
+}
+
 int sw_extrapolate_second_order_sw(sw_handle *h) {
     if (!h) return SW_ERR_ARG;
     if (int rc = touch_api(h)) return rc;

@@ -809,6 +809,39 @@ __global__ __launch_bounds__(kBlock) void k_protect(DevMesh M, DevFields F, DevP
     F.cq[k] = c;
 }
 
+// _protect_swb2, the protect of the 'tsunami' flow algorithm
+// (protect.cu:64-138): below a depth threshold that grows with the rise from
+// the bed centroid to the highest bed edge midpoint (minimum_relative_height
+// 0.1, at least minimum_allowed_height) the momenta become zero; a dry cell
+// whose stage is below its lowest bed vertex less minimum_allowed_height has
+// its stage and stage vertex values raised to that level.  max / min are the
+// fmax / fmin the reference's kernel calls.  Each triangle reads and writes
+// only its own values.
+__global__ __launch_bounds__(kBlock) void k_protect_swb2(DevMesh M, DevFields F, DevParams P) {
+    const int k = tri_index();
+    const int N = M.N;
+    if (k >= N) return;
+    Q4 c = ldq(F.cq + k);
+    double zv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) zv[i] = ld_s(F.zv + i * N + k);
+    const double mah = P.minimum_allowed_height;
+    const double hc = c.w - c.z;
+    const double bmax = 0.5 * fmax(zv[0] + zv[1], fmax(zv[1] + zv[2], zv[2] + zv[0]));
+    if (!(hc < fmax(0.1 * (bmax - c.z), mah))) return;
+    c.uh = 0.0;
+    c.vh = 0.0;
+    if (hc <= 0.0) {
+        const double bmin = fmin(zv[0], fmin(zv[1], zv[2])) - mah;
+        if (c.w < bmin) {
+            c.w = fmax(c.w, bmin);
+#pragma unroll
+            for (int i = 0; i < 3; i++) F.wv[i * N + k] = fmax(F.wv[i * N + k], bmin);
+        }
+    }
+    F.cq[k] = c;
+}
+
 // Centroid quantity as seen by the extrapolation: velocity when
 // extrapolate_velocity_second_order (shallow_water_ext.c:2862-2874).
 SW_DEV void centroid_wuv(const Q4 &c, const DevParams &P, double &w, double &u, double &v) {

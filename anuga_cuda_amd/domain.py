@@ -319,6 +319,9 @@ class Domain:
         self.recorded_min_timestep = self._params["evolve_max_timestep"]
         self._order_ = self._params["default_order"]
         self.smallsteps = 0
+        # any name but 'tsunami' selects the default flow, as the reference's
+        # dispatch does (gpu_domain_advanced.py:1014, utility.py:267-272)
+        self.flow_algorithm = "original"
         # SWW output (ANUGA's names; nothing is written unless store is set)
         self.name = "domain"
         self.datadir = "."
@@ -642,8 +645,17 @@ class Domain:
     def update_conserved_quantities(self):
         self._dev().call("sw_update_conserved_quantities")
 
+    def set_flow_algorithm(self, flow_algorithm):
+        """Names the flow algorithm.  'tsunami' switches
+        protect_against_infinitesimal_and_negative_heights to that
+        algorithm's protect (gpu_domain_advanced.py:1014-1031); its flux and
+        extrapolation are not defined by the reference, so evolve() refuses
+        it."""
+        self.flow_algorithm = str(flow_algorithm)
+
     def protect_against_infinitesimal_and_negative_heights(self):
-        self._dev().call("sw_protect")
+        # gpu_domain_advanced.py:1003-1047: _protect_swb2 for 'tsunami'
+        self._dev().call("sw_protect_swb2" if self.flow_algorithm == "tsunami" else "sw_protect")
 
     def extrapolate_second_order_sw(self):
         self._dev().call("sw_extrapolate_second_order_sw")
@@ -703,6 +715,12 @@ class Domain:
         one-launch-per-operator step instead of the fused device loop."""
         if self.__dict__.get("_in_ensemble"):
             raise RuntimeError("this domain belongs to an Ensemble: evolve the ensemble")
+        if self.flow_algorithm == "tsunami":
+            # the reference's 'tsunami' compute_fluxes only prints and its
+            # distribute call is malformed (gpu_domain_advanced.py:1677-1678,
+            # :1757-1776): there is no step to reproduce
+            raise NotImplementedError("the 'tsunami' flow algorithm has no evolve step; only its protect "
+                                      "(protect_against_infinitesimal_and_negative_heights) is provided")
         h = self._dev()
         if yieldstep is None:
             yieldstep = self.evolve_max_timestep

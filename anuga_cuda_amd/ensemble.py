@@ -35,6 +35,8 @@ class Ensemble:
                 raise ValueError("ensemble members must share their numerical parameters and precision")
             if bytes(d._sw_params()) != bytes(d0._sw_params()):
                 raise ValueError("ensemble members must share their timestepping method and limiter settings")
+            if d.flow_algorithm == "tsunami":
+                raise NotImplementedError("the 'tsunami' flow algorithm has no evolve step")
         self.handle = _lib.Handle.ensemble([d.mesh for d in self.domains], d0._sw_params(), device)
         for i, d in enumerate(self.domains):
             d._attach(self.handle.view(i))

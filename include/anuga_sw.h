@@ -262,6 +262,14 @@ int sw_saxpy_conserved_quantities(sw_handle *h, double a, double b);
 /* protect_against_infinitesimal_and_negative_heights
  * (gpu_domain_advanced.py:1003-1055; shallow_water_ext.c:1638-1705) */
 int sw_protect(sw_handle *h);
+/* _protect_swb2 (protect.cu:64-138), the protect that
+ * protect_against_infinitesimal_and_negative_heights runs for the 'tsunami'
+ * flow algorithm (gpu_domain_advanced.py:1014-1031): momenta zero below
+ * max(0.1 * (highest bed edge midpoint - bed centroid), minimum_allowed_height)
+ * of depth; a dry cell below its lowest bed vertex less minimum_allowed_height
+ * gets stage and stage vertex values raised to that level.  The mass this adds
+ * is not reported (the reference's kernel discards it). */
+int sw_protect_swb2(sw_handle *h);
 /* extrapolate_second_order_sw (gpu_domain_advanced.py:1061-1161;
  * shallow_water_ext.c:2717-3324) */
 int sw_extrapolate_second_order_sw(sw_handle *h);

@@ -91,6 +91,7 @@ def load():
         "or_update": ([C.c_int64, C.c_double, _F64P, _F64P, _F64P], C.c_int),
         "or_update_conserved_quantities": ([P], C.c_int),
         "or_protect": ([P], None),
+        "or_protect_swb2": ([P], None),
         "or_extrapolate_second_order_sw": ([P], C.c_int),
         "or_extrapolate_first_order": ([C.c_int64, _F64P, _F64P, _F64P], None),
         "or_balance_deep_and_shallow": ([P], None),
@@ -292,6 +293,9 @@ class OracleDomain:
 
     def protect(self):
         self.lib.or_protect(self._p())
+
+    def protect_swb2(self):
+        self.lib.or_protect_swb2(self._p())
 
     def extrapolate_second_order_sw(self):
         self._rc(self.lib.or_extrapolate_second_order_sw(self._p()), "extrapolate")

@@ -373,6 +373,39 @@ void or_protect(or_domain *D) {
     }
 }
 
+/* protect.cu:64-138 (_protect_swb2): the protect of the 'tsunami' flow
+ * algorithm (gpu_domain_advanced.py:1014-1031).  The depth threshold scales
+ * with the rise from the bed centroid to the highest bed edge midpoint
+ * (minimum_relative_height 0.1); below it the momenta become zero, and a cell
+ * that is dry with its stage below the lowest bed vertex less
+ * minimum_allowed_height gets stage and stage vertex values raised to that
+ * level.  The kernel's max / min of doubles are CUDA's fmax / fmin; the mass
+ * it adds is summed per thread and discarded, so it is not formed here. */
+void or_protect_swb2(or_domain *D) {
+    const int64_t N = D->number_of_elements;
+    const double mah = D->minimum_allowed_height;
+    const double minimum_relative_height = 0.1;
+    double *wc = D->stage_centroid_values, *wv = D->stage_vertex_values;
+    const double *zc = D->bed_centroid_values, *zv = D->bed_vertex_values;
+    double *xmomc = D->xmom_centroid_values, *ymomc = D->ymom_centroid_values;
+    for (int64_t k = 0; k < N; k++) {
+        const double *z = zv + 3 * k;
+        double hc = wc[k] - zc[k];
+        double bmax = 0.5 * fmax(z[0] + z[1], fmax(z[1] + z[2], z[2] + z[0]));
+        if (hc < fmax(minimum_relative_height * (bmax - zc[k]), mah)) {
+            xmomc[k] = 0.0;
+            ymomc[k] = 0.0;
+            if (hc <= 0.0) {
+                double bmin = fmin(z[0], fmin(z[1], z[2])) - mah;
+                if (wc[k] < bmin) {
+                    wc[k] = fmax(wc[k], bmin);
+                    for (int i = 0; i < 3; i++) wv[3 * k + i] = fmax(wv[3 * k + i], bmin);
+                }
+            }
+        }
+    }
+}
+
 /* shallow_water_ext.c:98-149 */
 static void find_qmin_and_qmax(double dq0, double dq1, double dq2, double *qmin, double *qmax) {
     if (dq0 >= 0.0) {

@@ -103,6 +103,7 @@ int or_update(int64_t N, double timestep, double *centroid_values,
               double *explicit_update, double *semi_implicit_update);
 int or_update_conserved_quantities(or_domain *D);
 void or_protect(or_domain *D);
+void or_protect_swb2(or_domain *D);
 int or_extrapolate_second_order_sw(or_domain *D);
 void or_extrapolate_second_order_and_limit(or_domain *D, const double *centroid, double *vertex, double *edge,
                                            double beta, int by_edge);

@@ -110,6 +110,20 @@ def test_dam_break_scenario_has_wet_and_dry_cells():
     assert d.number_of_elements == 1600
 
 
+def test_tsunami_flow_algorithm_has_no_evolve_step():
+    """'tsunami' selects only its protect: evolve() and ensembles refuse it
+    before any device work (the reference defines no such step)."""
+    from anuga_cuda_amd import Ensemble
+    d = scenarios.dam_break_channel(4, 4)
+    assert d.flow_algorithm != "tsunami"
+    d.set_flow_algorithm("tsunami")
+    with pytest.raises(NotImplementedError):
+        next(d.evolve(yieldstep=0.1, finaltime=0.2))
+    assert d._handle is None
+    with pytest.raises(NotImplementedError):
+        Ensemble([d])
+
+
 def test_product_path_has_no_cpu_fallback(monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libanuga_sw.so")
     monkeypatch.setattr(_lib, "_lib", None)

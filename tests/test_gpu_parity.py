@@ -13,7 +13,7 @@ from anuga_cuda_amd import Ensemble, _lib, scenarios
 from anuga_cuda_amd.domain import Dirichlet_boundary, Domain, Reflective_boundary, Transmissive_boundary
 from anuga_cuda_amd.mesh import Mesh, rectangular_cross
 from oracle.oracle import OracleDomain
-from tests.helpers import compare, compare_state, device_array, wet_mask
+from tests.helpers import FIELDS, compare, compare_state, device_array, wet_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -224,6 +224,50 @@ def test_operators_match_oracle_one_by_one():
         for loc in ("centroid_values", "edge_values", "vertex_values", "boundary_values"):
             compare(device_array(d, q, loc), od.q(q, loc), "%s.%s" % (q, loc), TOL, st)
     print({k: v for k, v in st.items() if v[0]})
+
+
+def test_protect_swb2_matches_oracle():
+    """The 'tsunami' flow algorithm's protect (_protect_swb2,
+    protect.cu:64-138), run by protect_against_infinitesimal_and_negative_heights
+    when flow_algorithm is 'tsunami', bit for bit against the oracle on the
+    3,000-point Delaunay terrain, with depths in every branch: deep, near the
+    threshold, dry above the raise level, dry below it."""
+    d = _delaunay("morton")
+    d.set_flow_algorithm("tsunami")
+    h = d._dev()
+    od = OracleDomain.from_domain(d)
+    N = d.number_of_elements
+    rng = np.random.default_rng(12)
+    zc = od.q("elevation", "centroid_values")
+    low = od.q("elevation", "vertex_values").min(axis=1) - d.minimum_allowed_height
+    pick = rng.integers(0, 5, size=N)
+    depth = np.choose(pick, [np.full(N, 0.5), np.full(N, 2e-3), np.full(N, 0.0), (low - zc) * 0.5,
+                             (low - zc) - 0.2])
+    state = {("stage", _lib.SW_CENTROID, "centroid_values"): zc + depth,
+             ("stage", _lib.SW_VERTEX, "vertex_values"): zc[:, None] + rng.uniform(-0.3, 0.3, size=(N, 3)),
+             ("xmomentum", _lib.SW_CENTROID, "centroid_values"): rng.normal(scale=0.2, size=N),
+             ("ymomentum", _lib.SW_CENTROID, "centroid_values"): rng.normal(scale=0.2, size=N)}
+    code = dict(FIELDS)
+    for (q, loc, name), a in state.items():
+        h.upload(code[q], loc, a)
+        od.q(q, name)[:] = a
+    d.protect_against_infinitesimal_and_negative_heights()
+    od.protect_swb2()
+    for (q, _, name), a in state.items():
+        got = device_array(d, q, name)
+        assert np.array_equal(got, od.q(q, name).ravel()), (q, name)
+    w = od.q("stage", "centroid_values")
+    uh = od.q("xmomentum", "centroid_values")
+    raised = w != state[("stage", _lib.SW_CENTROID, "centroid_values")]
+    zeroed = uh == 0
+    assert raised.any() and (zeroed & ~raised).any() and (~zeroed).any()
+    # the default flow algorithm runs the default protect
+    d.set_flow_algorithm("original")
+    before = device_array(d, "stage", "centroid_values")
+    d.protect_against_infinitesimal_and_negative_heights()
+    od.protect()
+    assert np.array_equal(device_array(d, "stage", "centroid_values"), od.q("stage", "centroid_values"))
+    assert not np.array_equal(before, od.q("stage", "centroid_values"))
 
 
 def test_restart_from_device_buffers_matches_oracle_restart():

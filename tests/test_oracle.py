@@ -18,8 +18,9 @@ in numpy 2 and the file uses the Python-2 C API), so the oracle is pinned by:
     smooth wave at fixed timesteps), lake at rest and mass conservation;
   * the per-quantity limiters: exact linear reconstruction, neighbour bounds,
     beta = 0, lake at rest and mass;
-  * protect's rules, the balance's H0 property, the other quantities' closed
-    form and the boundary values by kind.
+  * protect's rules (and the 'tsunami' protect's, with hand-worked cases),
+    the balance's H0 property, the other quantities' closed form and the
+    boundary values by kind.
 """
 import copy
 import math
@@ -602,6 +603,82 @@ def test_protect_rules(max_speed):
             assert np.all(np.sign(sp) == np.sign(sp0))
             keep = np.abs(sp0) <= max_speed
             assert np.array_equal(new[shallow][keep], old[shallow][keep])
+
+
+def test_protect_swb2_known_answers():
+    """_protect_swb2 (protect.cu:64-138) on hand-worked triangles.  Bed vertex
+    values (0, 1, 2), bed centroid 1, minimum_allowed_height 1e-3: the highest
+    bed edge midpoint is (1 + 2) / 2 = 1.5, so the depth threshold is
+    max(0.1 (1.5 - 1), 1e-3) = 0.05 (a threshold from the highest bed vertex
+    would be 0.1); the raise level is the lowest bed vertex less 1e-3, -1e-3
+    (from the bed centroid it would be 0.999)."""
+    d, od, _ = _random_state_domain(seed=8)
+    mah = d.minimum_allowed_height
+    assert mah == 1e-3
+    zv = od.q("elevation", "vertex_values")
+    zc = od.q("elevation", "centroid_values")
+    w, u, v = (od.q(q, "centroid_values") for q in ("stage", "xmomentum", "ymomentum"))
+    wv = od.q("stage", "vertex_values")
+    # stage -> (stage after, momenta kept, stage vertex values after)
+    cases = [(1.06, True, [0.5, -0.5, 1.2]),           # depth 0.06 >= 0.05: untouched
+             (1.04, False, [0.5, -0.5, 1.2]),          # depth 0.04 < 0.05: momenta zero
+             (0.9, False, [0.5, -0.5, 1.2]),           # dry, above -1e-3: stage kept
+             (-0.5, False, [0.5, -1e-3, 1.2])]         # dry, below -1e-3: raised
+    for k, (wk, _, _) in enumerate(cases):
+        zv[k] = (0.0, 1.0, 2.0)
+        zc[k] = 1.0
+        w[k] = wk
+        u[k], v[k] = 0.3, -0.2
+        wv[k] = (0.5, -0.5, 1.2)
+    rest = slice(len(cases), None)
+    before = {q: od.q(q, "centroid_values")[rest].copy() for q in ("stage", "xmomentum", "ymomentum")}
+    od.protect_swb2()
+    for k, (wk, keep, wv_after) in enumerate(cases):
+        assert w[k] == (-1e-3 if wk < -1e-3 else wk), k
+        assert (u[k], v[k]) == ((0.3, -0.2) if keep else (0.0, 0.0)), k
+        assert list(wv[k]) == wv_after, k
+    # the rest of the mesh follows the same rules (checked in the next test)
+    assert any(not np.array_equal(before[q], od.q(q, "centroid_values")[rest]) for q in before)
+
+
+def test_protect_swb2_rules():
+    """_protect_swb2 over a random state: cells at least the threshold deep
+    (0.1 of the rise from the bed centroid to the highest bed edge midpoint,
+    at least minimum_allowed_height) are untouched; shallower ones lose their
+    momenta; only dry ones below the lowest bed vertex less
+    minimum_allowed_height change stage, to exactly that level, with every
+    stage vertex value at least that level afterwards and those already
+    above it unchanged."""
+    d, od, _ = _random_state_domain(seed=9)
+    mah = d.minimum_allowed_height
+    N = d.number_of_elements
+    rng = np.random.default_rng(10)
+    zc = od.q("elevation", "centroid_values")
+    zv = od.q("elevation", "vertex_values").copy()
+    od.q("stage", "vertex_values")[:] = zc[:, None] + rng.uniform(-0.5, 0.5, size=(N, 3))
+    # depths straddling every branch, the threshold included
+    mid = 0.5 * np.stack([zv[:, 1] + zv[:, 2], zv[:, 2] + zv[:, 0], zv[:, 0] + zv[:, 1]], axis=1)
+    thr = np.maximum(0.1 * (mid.max(axis=1) - zc), mah)
+    low = zv.min(axis=1) - mah
+    pick = rng.integers(0, 5, size=N)
+    h = np.choose(pick, [thr * 1.5, thr * 0.5, np.full(N, -1e-9), (low - zc) * 0.5, (low - zc) - 0.3])
+    od.q("stage", "centroid_values")[:] = zc + h
+    w0, u0, v0, wv0 = (od.q(q, l).copy() for q, l in (("stage", "centroid_values"), ("xmomentum", "centroid_values"),
+                                                       ("ymomentum", "centroid_values"), ("stage", "vertex_values")))
+    od.protect_swb2()
+    w, u, v, wv = (od.q(q, l) for q, l in (("stage", "centroid_values"), ("xmomentum", "centroid_values"),
+                                           ("ymomentum", "centroid_values"), ("stage", "vertex_values")))
+    hc = w0 - zc
+    deep = hc >= thr
+    raised = (hc <= 0) & (w0 < low)
+    assert deep.any() and (~deep & ~raised).any() and raised.any() and (~deep & (hc <= 0) & ~raised).any()
+    assert np.array_equal(u[deep], u0[deep]) and np.array_equal(v[deep], v0[deep])
+    assert np.all(u[~deep] == 0) and np.all(v[~deep] == 0)
+    assert np.array_equal(w[~raised], w0[~raised]) and np.array_equal(wv[~raised], wv0[~raised])
+    assert np.array_equal(w[raised], low[raised])
+    assert np.all(wv[raised] >= low[raised, None])
+    above = wv0[raised] >= low[raised, None]
+    assert np.array_equal(wv[raised][above], wv0[raised][above])
 
 
 def test_balance_keeps_vertex_depths_above_H0_and_centroid_velocity():
