@@ -110,7 +110,7 @@ def kernel_symbol(kernel, precision, launch, vstep=False, limited=False):
     sizes of (flux, update, distribute); ``limited``: the per-quantity
     limiters' variant of the vertex-record extrapolation."""
     if vstep:
-        extrap = "k_extrap_v<%d, true>" % launch[2] if limited else "k_extrap_v<%d>" % launch[2]
+        extrap = "k_extrap_v<%d, %s>" % (launch[2], "true" if limited else "false")
         return {"flux": "k_flux_v<%d>" % launch[0], "update": "k_update_v<%d>" % launch[1],
                 "distribute": extrap}[kernel]
     r = "double" if precision == "fp64" else "float"
