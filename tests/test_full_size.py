@@ -4,7 +4,9 @@
   10 km, fBm terrain, wet/dry mound, transmissive boundaries) in random and in
   Morton order: 10 steps in lockstep with the FP64 oracle (centroid values,
   timestep, time and wet/dry mask every step), then 16 steps in one call (a
-  captured graph) and the whole state compared again;
+  captured graph) and the whole state compared again; the same workload with
+  the limit-by-edge limiter and rk2 steps (6 lockstep steps, then 10 in one
+  call);
 * the 4,000,000-triangle dam break in mixed precision against the FP64 oracle
   at the time the oracle reaches in 20 steps, error stated per quantity;
 * the 64-member ensemble (8.67M triangles): 5 steps, each member's timesteps
@@ -72,6 +74,73 @@ def test_config2_million_triangles_lockstep(order):
     wet = np.count_nonzero(wet_mask(od.q("stage", "centroid_values"), bed, eps))
     print("1M %s: %d triangles, %d wet, t = %.6f s after 26 steps; bit-mismatching fields %s" % (
         order, d.number_of_elements, wet, s.time, {k: v[0] for k, v in stats.items() if v[0]}))
+    d.close()
+
+
+def _same_with_nans(got, exp, what, stats):
+    """The non-finite entries at the same places, the rest to the bar."""
+    got = np.asarray(got, dtype=np.float64).ravel()
+    exp = np.asarray(exp, dtype=np.float64).ravel()
+    bad = ~np.isfinite(exp)
+    assert np.array_equal(~np.isfinite(got), bad), what + ": non-finite entries differ"
+    compare(got[~bad], exp[~bad], what, TOL, stats)
+    return int(np.count_nonzero(bad))
+
+
+def test_config2_million_triangles_edge_limiter_rk2_lockstep():
+    """The same 1M-triangle workload with the per-quantity limiters (limit by
+    edge, betas 1.0 / 0.7 / 1.3) and rk2 steps, both on the vertex-record
+    step: 6 steps in lockstep with the oracle, then 10 in one call.
+
+    The reference's compute_gradients divides by the determinant of the three
+    surrogate centroids with no guard
+    (extrapolate_second_order_and_limit_by_vertex_or_edge.cu:258-266).  This
+    mesh has two boundary triangles whose surrogates (the neighbours on either
+    side, fanned from one interior vertex to consecutive boundary segments,
+    and the triangle itself) have centroids on one line, so with the flat
+    initial stage their gradient is 0 / 0: the oracle and the device give NaN
+    there from the first distribute and it spreads through the neighbours
+    with every stage (about 8,000 cells after 16 rk2 steps).  The non-finite
+    entries must be the same, the others within the bar."""
+    d = scenarios.delaunay_terrain(npoints=500000, order="morton", use_edge_limiter=1)
+    d.quantities["xmomentum"].beta = 0.7
+    d.quantities["ymomentum"].beta = 1.3
+    d.set_timestepping_method("rk2")
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+    od.evolve_prepare(1e9, None)
+    assert h.step_path() == _lib.SW_STEP_VERTEX and d.limiter_path() == "edge"
+    bed = od.q("elevation", "centroid_values")
+    eps = d.minimum_allowed_height
+    stats = {}
+    assert _same_with_nans(device_array(d, "stage", "vertex_values"), od.q("stage", "vertex_values"),
+                           "prepare.stage.vertex_values", stats) > 0
+    for step in range(6):
+        done, ev = h.evolve_steps(1)
+        od.evolve_step()
+        s = h.get_state()
+        assert done == 1 and ev == _lib.SW_EVENT_NONE
+        compare([s.timestep, s.time], [od.d.timestep, od.d.time], "timestep/time", TOL, stats)
+        for q in CONSERVED:
+            _same_with_nans(device_array(d, q, "centroid_values"), od.q(q, "centroid_values"),
+                            "step%d.%s" % (step, q), stats)
+        w = device_array(d, "stage", "centroid_values")
+        assert np.array_equal(wet_mask(w, bed, eps), wet_mask(od.q("stage", "centroid_values"), bed, eps)), step
+    done, ev = h.evolve_steps(10)
+    assert done == 10
+    for _ in range(10):
+        od.evolve_step()
+    s = h.get_state()
+    compare([s.time], [od.d.time], "time after 16 steps", TOL)
+    assert s.number_of_steps == od.d.number_of_steps == 16
+    nan = {}
+    for q in CONSERVED:
+        for loc in ("centroid_values", "vertex_values", "edge_values"):
+            nan[q + "." + loc] = _same_with_nans(device_array(d, q, loc), od.q(q, loc), q + "." + loc, stats)
+    assert nan["stage.centroid_values"] > 0
+    print("1M edge limiter rk2: t = %.6f s after 16 steps; non-finite centroids %d; bit-mismatching fields %s" % (
+        s.time, nan["stage.centroid_values"], {k: v[0] for k, v in stats.items() if v[0]}))
     d.close()
 
 

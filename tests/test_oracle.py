@@ -533,6 +533,28 @@ def test_limited_reconstruction_stays_within_neighbour_bounds(by_edge):
     np.testing.assert_array_equal(vv, np.repeat(cv[:, None], 3, axis=1))
 
 
+@pytest.mark.parametrize("by_edge", [False, True], ids=["vertex", "edge"])
+def test_limited_gradient_of_collinear_surrogates_is_not_finite(by_edge):
+    """compute_gradients divides by the determinant of the three surrogate
+    centroids with no guard (extrapolate_second_order_and_limit_by_vertex_or_edge.cu:258-266).
+    Three boundary triangles fanned from one interior vertex to consecutive
+    segments of a straight boundary have centroids at one height, so the
+    middle one (one boundary edge: its surrogates are the two others and
+    itself) gets 0 / 0 on a flat stage and x / 0 on a sloped one; the limiter
+    cannot repair either, so its values are not finite while the other
+    triangles' are."""
+    points = np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0], [3.0, 0.0], [1.5, 1.0]])
+    tris = np.array([[0, 1, 4], [1, 2, 4], [2, 3, 4]])
+    for f in (lambda x, y: 2.0 + 0 * x, lambda x, y: 2.0 + 0.1 * x):
+        d = Domain(mesh=Mesh(points, tris))
+        d.set_quantity("elevation", 0.0)
+        d.set_quantity("stage", f, location="centroids")
+        assert list(d.mesh.number_of_boundaries) == [2, 1, 2]
+        od, cv, vv, ev = _limited(d, "stage", by_edge, 1.0)
+        assert np.all(np.isfinite(cv))
+        assert not np.any(np.isfinite(vv[1])) and np.all(np.isfinite(vv[[0, 2]]))
+
+
 @pytest.mark.parametrize("limiter", ["vertex", "edge"])
 def test_limiter_paths_keep_lake_at_rest_and_mass(limiter):
     kw = {"use_edge_limiter": 1} if limiter == "edge" else {"optimised_gradient_limiter": 0}
