@@ -3,9 +3,9 @@
 // Two families share the numerics in sw_device.h:
 //   * operator kernels, one per overridden ANUGA operator (the drop-in API:
 //     each mirrors one reference kernel / C routine and its array contract);
-//   * the edge-record fused step used by evolve() (rk2 / rk3, mixed
-//     precision, the per-quantity limiters, and meshes the vertex-record step of
-//     sw_step_vertex.h does not take); per stage
+//   * the edge-record fused step used by evolve() for mixed precision and for
+//     meshes the vertex-record step of sw_step_vertex.h does not take (and on
+//     request: it runs Euler, rk2, rk3 and every limiter too); per stage
 //       flux       = compute_fluxes + boundary evaluation + gravity_wb; each
 //                    wavefront stores the minimum of its CFL candidates
 //                    (wavefront shuffles, a plain store);
