@@ -332,7 +332,7 @@ class Domain:
         self.quantities_to_be_stored = {"elevation": 1, "stage": 2, "xmomentum": 2, "ymomentum": 2}
         # coordinates are relative to this origin (ANUGA's Geo_reference)
         self.geo_reference = {"xllcorner": 0.0, "yllcorner": 0.0, "zone": -1}
-        self.max_speed = np.zeros(self.number_of_elements)
+        self._max_speed = np.zeros(self.number_of_elements)
 
     # ------------------------------------------------------------------ params
     def __getattr__(self, name):
@@ -576,7 +576,14 @@ class Domain:
             self._host_stale = False
             for name, key, arr in self._each_array():
                 self._handle.download(_QID[name], _LOC[key], arr.reshape(-1))
-            self.max_speed = self._handle.max_speed()
+            self._max_speed = self._handle.max_speed()
+
+    @property
+    def max_speed(self):
+        """Per-triangle maximum wave speed of the last flux evaluation (brought
+        from the device when read)."""
+        self._pull()
+        return self._max_speed
 
     def _push(self):
         h = self._ensure_device()
@@ -593,9 +600,13 @@ class Domain:
         self._all_dirty = False
 
     def _dev(self):
-        """Device handle with host changes applied; host copies marked stale."""
-        self._pull()
-        self._push()
+        """Device handle with host changes applied; host copies marked stale.
+        The host copies are brought up to date only before a push (pushing
+        every array needs every copy current): a call with no host changes,
+        such as each yield of evolve(), downloads nothing."""
+        if self._handle is None or self._all_dirty or self._touched:
+            self._pull()
+            self._push()
         self._host_stale = True
         return self._handle
 

@@ -149,7 +149,7 @@ def copy_peak_gbs(mib=1024, reps=20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="fv", choices=["fv", "ensemble", "learned", "merimbula"],
+    ap.add_argument("--workload", default="fv", choices=["fv", "ensemble", "learned", "merimbula", "cpu-plan"],
                     help="fv: the 1M-triangle plain FV step (default); ensemble: the batched ensemble; "
                          "learned: train the edge-GNN surrogate on FV trajectories and roll it out at 1M")
     ap.add_argument("--delta", type=float, default=0.0625, help="learned: seconds per learned step")
@@ -209,6 +209,8 @@ def main():
         return bench_learned(args)
     if args.workload == "merimbula":
         return bench_merimbula(args)
+    if args.workload == "cpu-plan":
+        return bench_cpu_plan(args)
 
     if args.copy_calibration:
         import torch
@@ -319,14 +321,7 @@ def main():
         for _ in range(args.cpu_steps):
             od.evolve_step()
         c_el = time.perf_counter() - c0
-        cpu_model = platform.processor() or ""
-        try:
-            for line in open("/proc/cpuinfo"):
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-        except OSError:
-            pass
+        cpu_model = cpu_model_name()
         cpu = {"value": N * args.cpu_steps / c_el, "unit": "triangle-steps/s", "cores": 1, "kind": "port",
                "sample": "%d %s steps of the same %d-triangle workload, serial FP64 C oracle "
                          "(restated reference C), %.2f s on %s (nproc %d)" % (
@@ -465,6 +460,109 @@ def bench_merimbula(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def cpu_model_name():
+    name = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return name
+
+
+def bench_cpu_plan(args):
+    """The serial FP64 C oracle timed beside the GPU on each plain-FV
+    configuration, in one run on the same host: the 40,000-triangle dam break
+    in full (t = 10 s, yields every 1 s, whole Domain.evolve on the GPU, the
+    second of two runs), the 1M-triangle Delaunay workload in Morton and in
+    random order (20 oracle steps after one untimed step; 200 GPU steps after
+    autotuning and 20 warmup steps), and the 4M-triangle dam break (5 oracle
+    steps; 20 GPU steps in FP64 and in mixed precision).  One JSON line with a
+    record per case."""
+    from anuga_cuda_amd import scenarios
+    from oracle.oracle import OracleDomain
+
+    def oracle_steps(od, n):
+        od.evolve_prepare(1e9, None)
+        od.evolve_step()  # outside the sample
+        c0 = time.perf_counter()
+        for _ in range(n):
+            od.evolve_step()
+        return time.perf_counter() - c0
+
+    def gpu_steps(d, n, warmup):
+        h = d._dev()
+        h.call("sw_evolve_prepare", 1e9, 0.0, 0)
+        h.autotune(10)
+        h.evolve_steps(warmup)
+        h.call("sw_synchronize")
+        t0 = time.perf_counter()
+        done, _ = h.evolve_steps(n)
+        h.call("sw_synchronize")
+        if done != n:
+            raise SystemExit("expected %d steps, ran %d" % (n, done))
+        return time.perf_counter() - t0
+
+    cases = []
+
+    def record(name, N, cpu_n, cpu_s, gpu_n, gpu_s, **extra):
+        rec = dict(config=name, triangles=N, cpu_steps=cpu_n, cpu_seconds=cpu_s, cpu_rate=N * cpu_n / cpu_s,
+                   gpu_steps=gpu_n, gpu_seconds=gpu_s, gpu_rate=N * gpu_n / gpu_s, **extra)
+        rec["gpu_over_cpu"] = rec["gpu_rate"] / rec["cpu_rate"]
+        cases.append(rec)
+        log("%s: CPU %.3e, GPU %.3e triangle-steps/s (%.0fx)" % (name, rec["cpu_rate"], rec["gpu_rate"],
+                                                                 rec["gpu_over_cpu"]))
+
+    # 40,000-triangle dam break, the whole run
+    od = OracleDomain.from_domain(scenarios.dam_break_channel(100, 100))
+    c0 = time.perf_counter()
+    cpu_n = 0
+    for _ in od.evolve(1.0, 10.0):
+        cpu_n += od.d.number_of_steps
+    cpu_s = time.perf_counter() - c0
+    for rep in range(2):
+        d = scenarios.dam_break_channel(100, 100)
+        d._dev()
+        t0 = time.perf_counter()
+        gpu_n = 0
+        for _ in d.evolve(yieldstep=1.0, finaltime=10.0):
+            gpu_n += d.number_of_steps
+        gpu_s = time.perf_counter() - t0
+        d.close()
+    record("40,000-triangle dam break to t = 10 s (whole Domain.evolve, 11 yields)", 40000, cpu_n, cpu_s, gpu_n,
+           gpu_s)
+    del od
+
+    # the 1M-triangle Delaunay workload in both orders
+    for order in ("morton", "random"):
+        d = scenarios.delaunay_terrain(npoints=500000, order=order)
+        od = OracleDomain.from_domain(d)
+        cpu_s = oracle_steps(od, 20)
+        del od
+        gpu_s = gpu_steps(d, 200, 20)
+        record("1M-triangle Delaunay, %s order, FP64" % order, d.number_of_elements, 20, cpu_s, 200, gpu_s)
+        d.close()
+
+    # the 4M-triangle dam break: FP64 and mixed precision on the GPU
+    d = scenarios.dam_break_channel(1000, 1000)
+    od = OracleDomain.from_domain(d)
+    cpu_s = oracle_steps(od, 5)
+    del od
+    gpu_s = gpu_steps(d, 20, 5)
+    N = d.number_of_elements
+    d.close()
+    dm = scenarios.dam_break_channel(1000, 1000, precision="mixed")
+    gpu_m = gpu_steps(dm, 20, 5)
+    dm.close()
+    record("4M-triangle dam break", N, 5, cpu_s, 20, gpu_s, gpu_mixed_seconds=gpu_m,
+           gpu_mixed_rate=N * 20 / gpu_m)
+
+    print(json.dumps({"metric": "serial FP64 C oracle and GPU, triangle-steps/s per configuration",
+                      "unit": "triangle-steps/s", "cores": 1, "kind": "port", "cpu_model": cpu_model_name(),
+                      "nproc": os.cpu_count() or 0, "cases": cases}))
 
 
 def bench_ensemble(args):
