@@ -1625,7 +1625,7 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
 // at most kBatchSteps.  Steps queued past a yield only run the halted early
 // exits, so this bounds that waste; a low estimate costs another check.
 static int64_t batch_estimate(const sw_handle *h, const std::vector<DevState> &d) {
-    int64_t est = kGraphSteps;
+    int64_t est = 0;
     for (int i = 0; i < h->n_inst && i < (int)d.size(); i++) {
         const DevState &s = d[i];
         if (s.event != SW_EVENT_NONE || s.halt || s.error) continue;
@@ -1637,6 +1637,10 @@ static int64_t batch_estimate(const sw_handle *h, const std::vector<DevState> &d
         if (!(n < (double)kBatchSteps)) return kBatchSteps;
         est = std::max<int64_t>(est, (int64_t)(1.1 * n) + 2);
     }
+    // fewer steps than a graph holds: launched one by one, rather than a
+    // whole graph whose later steps would only find the yield and halt
+    if (est == 0) est = kGraphSteps;
+    if (est < kGraphSteps) return est;
     est = (est + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
     return std::min<int64_t>(est, kBatchSteps);
 }
