@@ -1635,13 +1635,12 @@ static int64_t batch_estimate(const sw_handle *h, const std::vector<DevState> &d
         if (s.has_finaltime && s.finaltime < target) target = s.finaltime;
         const double n = (target - s.time) / dt;
         if (!(n < (double)kBatchSteps)) return kBatchSteps;
-        est = std::max<int64_t>(est, (int64_t)(1.1 * n) + 2);
+        est = std::max<int64_t>(est, (int64_t)(1.05 * n) + 2);
     }
-    // fewer steps than a graph holds: launched one by one, rather than a
-    // whole graph whose later steps would only find the yield and halt
+    // whole graphs for the bulk and single launches for the rest (the caller
+    // does both), rather than whole graphs whose last steps would only find
+    // the yield and halt; a short estimate costs one more batch
     if (est == 0) est = kGraphSteps;
-    if (est < kGraphSteps) return est;
-    est = (est + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
     return std::min<int64_t>(est, kBatchSteps);
 }
 
