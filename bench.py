@@ -163,7 +163,7 @@ def main():
     ap.add_argument("--save-net", default=None, help="learned: write the trained network (.npz)")
     ap.add_argument("--load-net", default=None,
                     help="learned: use this trained network (.npz); without --save-net the committed --network is used")
-    ap.add_argument("--judge-precision", default="mixed", choices=["mixed", "fp64"],
+    ap.add_argument("--judge-precision", default="fp64", choices=["fp64", "mixed"],
                     help="learned: precision of the FV judge's step (the reference and fallbacks are FP64; the "
                          "other choice is reported beside it)")
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
@@ -373,15 +373,16 @@ def learned_rollout(args):
     """The learned form of the same workload: the committed network (trained
     by this package's kernels; provenance in anuga_cuda_amd/networks/)
     rolled out for 100 learned steps of 0.0625 s from the workload's initial
-    state, the FV judge (mixed-precision step) every 8 steps with the FP64 FV
-    fallback above 1% stage L2 error, against the FP64 FV path over the same
-    6.25 s.  speedup = FV wall time / judged rollout wall time."""
+    state, the FV judge (--judge-precision: the FP64 or the mixed-precision
+    step) every 8 steps with the FP64 FV fallback above 1% stage L2 error,
+    against the FP64 FV path over the same 6.25 s.  speedup = FV wall time /
+    judged rollout wall time."""
     from anuga_cuda_amd import learned, scenarios
 
     net = learned.EdgeGNN.load(args.network)
     dom = scenarios.delaunay_terrain(npoints=args.points, order=args.order)
-    res = learned.evaluate_rollout(net, dom, 100, 0.0625, judge_every=8, threshold=0.01, judge_precision="mixed",
-                                   log=log)
+    res = learned.evaluate_rollout(net, dom, 100, 0.0625, judge_every=8, threshold=0.01,
+                                   judge_precision=args.judge_precision, log=log)
     dom.close()
     keys = ("speedup", "learned_rate", "fv_rate", "steps", "delta", "fv_steps", "final_stage_rel_l2",
             "max_stage_rel_l2", "mass_balance_rel", "mass_change_vs_fv", "fallback_windows", "judge_precision",
