@@ -180,9 +180,12 @@ class Quantity:
         self.beta = beta
 
     def _get(self, key):
-        self.domain._pull()
-        self.domain._touched.add((self.name, key))
-        return self._arrays[key]
+        d = self.domain
+        d._pull_one(self.name, key)
+        arr = self._arrays[key]
+        if (self.name, key) not in d._touched:
+            d._touched[(self.name, key)] = arr.copy()  # what it held when first handed out
+        return arr
 
     def _set(self, key, value):
         arr = self._get(key)
@@ -304,8 +307,8 @@ class Domain:
         self._kinds = np.zeros(self.number_of_boundary_elements, dtype=np.int32)
         self._dirichlet = np.zeros((self.number_of_boundary_elements, 3))
         self._handle = None
-        self._host_stale = False
-        self._touched = set()
+        self._stale = set()
+        self._touched = {}
         self._all_dirty = True
         self._time = 0.0
         self.starttime = 0.0
@@ -571,18 +574,43 @@ class Domain:
                     continue
                 yield name, key, arr
 
+    # Host copies of the device's arrays.  _stale holds the (quantity,
+    # location) pairs, and "max_speed", whose host copy is out of date: each is
+    # downloaded when it is read, alone.  _touched maps the arrays handed out
+    # since the last push to a copy of what they held then: a push uploads
+    # those that differ from it, so arrays that were only read stay as the
+    # device has them.
+    @property
+    def _host_stale(self):
+        return bool(self._stale)
+
+    @_host_stale.setter
+    def _host_stale(self, value):
+        self._stale = ({(n, k) for n, k, _ in self._each_array()} | {"max_speed"}) if value else set()
+
     def _pull(self):
-        if self._host_stale and self._handle is not None:
-            self._host_stale = False
-            for name, key, arr in self._each_array():
+        """Every out-of-date host copy from the device."""
+        if self._handle is None:
+            return
+        for name, key, arr in self._each_array():
+            if (name, key) in self._stale:
                 self._handle.download(_QID[name], _LOC[key], arr.reshape(-1))
+        if "max_speed" in self._stale:
             self._max_speed = self._handle.max_speed()
+        self._stale = set()
+
+    def _pull_one(self, name, key):
+        if (name, key) in self._stale and self._handle is not None:
+            self._handle.download(_QID[name], _LOC[key], self.quantities[name]._arrays[key].reshape(-1))
+            self._stale.discard((name, key))
 
     @property
     def max_speed(self):
         """Per-triangle maximum wave speed of the last flux evaluation (brought
         from the device when read)."""
-        self._pull()
+        if "max_speed" in self._stale and self._handle is not None:
+            self._max_speed = self._handle.max_speed()
+            self._stale.discard("max_speed")
         return self._max_speed
 
     def _push(self):
@@ -590,8 +618,13 @@ class Domain:
         if self._all_dirty:
             items = list(self._each_array())
         else:
-            items = [(n, k, self.quantities[n]._arrays[k]) for (n, k) in self._touched
-                     if not (n == "friction" and k != "centroid_values")]
+            items = []
+            for (n, k), before in self._touched.items():
+                arr = self.quantities[n]._arrays[k]
+                if n == "friction" and k != "centroid_values":
+                    continue
+                if arr.shape != before.shape or not np.array_equal(arr.view(np.int64), before.view(np.int64)):
+                    items.append((n, k, arr))
         # elevation vertex values last: the device derives the mean bed from them
         items.sort(key=lambda t: (t[0] == "elevation" and t[1] == "vertex_values"))
         for name, key, arr in items:
@@ -605,7 +638,8 @@ class Domain:
         every array needs every copy current): a call with no host changes,
         such as each yield of evolve(), downloads nothing."""
         if self._handle is None or self._all_dirty or self._touched:
-            self._pull()
+            if self._all_dirty:
+                self._pull()
             self._push()
         self._host_stale = True
         return self._handle

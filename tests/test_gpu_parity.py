@@ -722,6 +722,41 @@ def test_upload_between_chained_calls_matches_oracle():
     compare_state(d, od, TOL, stats=stats, tag="after upload.")
 
 
+def test_yield_reads_upload_nothing_and_writes_reach_the_next_step():
+    """Domain.evolve at yields: reading quantities downloads only what is read
+    and uploads nothing (the device keeps its step records); an in-place
+    change to a quantity's array is uploaded, that array only, and the run
+    continues from it as the oracle does given the same values."""
+    d = _delaunay("morton")
+    od = OracleDomain.from_domain(d)
+    h = d._dev()
+    uploads, downloads = [], []
+    up, down = h.upload, h.download
+    h.upload = lambda q, loc, arr: (uploads.append((q, loc)), up(q, loc, arr))[1]
+    h.download = lambda q, loc, out: (downloads.append((q, loc)), down(q, loc, out))[1]
+    bed = od.q("elevation", "centroid_values")
+    gen, ogen = d.evolve(yieldstep=0.01, finaltime=0.05), od.evolve(0.01, 0.05)
+    stats = {}
+    for y, (t, t_o) in enumerate(zip(gen, ogen)):
+        compare([t], [t_o], "yield time", TOL, stats)
+        n_up, n_down = len(uploads), len(downloads)
+        w = d.quantities["stage"].centroid_values
+        assert downloads[n_down:] == [(_lib.SW_STAGE, _lib.SW_CENTROID)]
+        compare(w, od.q("stage", "centroid_values"), "yield %d stage" % y, TOL, stats)
+        uh = d.quantities["xmomentum"].centroid_values
+        compare(uh, od.q("xmomentum", "centroid_values"), "yield %d xmomentum" % y, TOL, stats)
+        if y == 2:
+            # a 1 cm rise of the wet cells, on both sides
+            w[...] = np.where(w - bed > 0.05, w + 0.01, w)
+            od.q("stage", "centroid_values")[:] = w
+        n_dev = len(uploads)
+        d._dev()
+        assert uploads[n_dev:] == ([(_lib.SW_STAGE, _lib.SW_CENTROID)] if y == 2 else [])
+        assert len(uploads) == n_up + (1 if y == 2 else 0)
+    assert y == 5
+    compare_state(d, od, TOL, stats=stats, tag="after the yields.")
+
+
 def test_vertex_step_with_varying_friction_and_centroid_bed_matches_oracle():
     """The vertex-record step reads Manning n and the mean bed per triangle
     when they are not uniform / not the centroid bed (its update skips those
