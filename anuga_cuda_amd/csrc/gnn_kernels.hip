@@ -935,14 +935,30 @@ __global__ __launch_bounds__(kB) void k_limit_groups(int64_t N, const int32_t *g
     theta[i] = out > vol ? vol / out : 1.0;
 }
 
-// Momentum of cells shallower than h_dry set to zero (as the solver's protect).
+// Momentum of cells shallower than h_dry set to zero (as the solver's
+// protect); elsewhere, with froude_max > 0, the speed capped at
+// froude_max sqrt(g h) with its direction kept (as the solver's
+// maximum_allowed_speed caps speeds): a learned step must not leave a cell
+// with a speed the flow cannot have, which the network, far outside the
+// states it was trained on, would amplify in the next step.  Mass is not
+// touched.
 __global__ __launch_bounds__(kB) void k_protect_dry(int64_t N, const double *w, const double *z, double *uh,
-                                                    double *vh, double h_dry) {
+                                                    double *vh, double h_dry, double g, double froude_max) {
     const int64_t i = gidx();
     if (i >= N) return;
-    if (w[i] - z[i] < h_dry) {
+    const double h = w[i] - z[i];
+    if (h < h_dry) {
         uh[i] = 0.0;
         vh[i] = 0.0;
+    } else if (froude_max > 0.0) {
+        const double qu = uh[i], qv = vh[i];
+        const double q2 = qu * qu + qv * qv;
+        const double qmax = froude_max * sqrt(g * h) * h;
+        if (q2 > qmax * qmax) {
+            const double s = qmax / sqrt(q2);
+            uh[i] = qu * s;
+            vh[i] = qv * s;
+        }
     }
 }
 
@@ -1226,8 +1242,10 @@ int gnn_limit(void *stream, int64_t N, const int32_t *slots, const int32_t *sfla
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int gnn_protect(void *stream, int64_t N, const double *w, const double *z, double *uh, double *vh, double h_dry) {
-    hipLaunchKernelGGL(k_protect_dry, dim3(blocks(N)), dim3(kB), 0, S(stream), N, w, z, uh, vh, h_dry);
+int gnn_protect(void *stream, int64_t N, const double *w, const double *z, double *uh, double *vh, double h_dry,
+                double g, double froude_max) {
+    if (!(g > 0.0) || froude_max < 0.0) return -1;
+    hipLaunchKernelGGL(k_protect_dry, dim3(blocks(N)), dim3(kB), 0, S(stream), N, w, z, uh, vh, h_dry, g, froude_max);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

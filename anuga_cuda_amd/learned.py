@@ -48,6 +48,9 @@ FEAT, HID, CAT, OUT = 32, 64, 128, 32
 FOUT = 8  # row of the FP32 edge outputs (5 used)
 GRAVITY = 9.8
 H_DRY = 1.0e-3  # depth below which a cell's velocity is taken as zero (the solver's minimum_allowed_height)
+# Largest Froude number a learned step may leave in a cell (protect caps the
+# speed at FROUDE_MAX sqrt(g h)); 0 switches the cap off.
+FROUDE_MAX = 5.0
 LAYERS = [("enc", FEAT, HID), ("mp1", CAT, HID), ("mp2", CAT, HID), ("mp3", CAT, HID), ("head", HID, OUT)]
 
 _gnn = None
@@ -79,7 +82,7 @@ def _load():
         _gnn.gnn_edge_grad.argtypes = [p, i64, i64, p, p, p, p, p, p]
         _gnn.gnn_apply.argtypes = [p, i64] + [p] * 9 + [f] + [p] * 11 + [C.c_double, p, p, p]
         _gnn.gnn_limit.argtypes = [p, i64] + [p] * 7 + [f, p, p, p, p]
-        _gnn.gnn_protect.argtypes = [p, i64, p, p, p, p, C.c_double]
+        _gnn.gnn_protect.argtypes = [p, i64, p, p, p, p, C.c_double, C.c_double, C.c_double]
         _gnn.gnn_apply_bwd.argtypes = [p, i64, i64] + [p] * 6 + [f, p, p]
         _gnn.gnn_adam.argtypes = [p, i64, p, p, p, p, f, f, f, f, i32]
         _gnn.gnn_cast_weights.argtypes = [p, i32, i32, p, p, p]
@@ -288,7 +291,7 @@ class EdgeGNN:
     """The network (FP32 master weights, BF16 copies) and its scales."""
 
     def __init__(self, seed=0, depth_scale=5.0, momentum_scale=5.0, length_scale=10.0, flux_scale=1.0,
-                 sigma=(1e-2, 1e-2, 1e-2), merge_radius=0.0, physics_base=True):
+                 sigma=(1e-2, 1e-2, 1e-2), merge_radius=0.0, physics_base=True, delta=0.0):
         torch = _torch()
         self.lib = _load()
         g = np.random.default_rng(seed)
@@ -296,10 +299,11 @@ class EdgeGNN:
         # network is trained and run on (small_cell_groups).  physics_base:
         # the edge outputs are the first-order central-upwind flux
         # (gnn_base_flux) plus the network's correction; without it the
-        # network predicts the whole flux.
+        # network predicts the whole flux.  delta: the learned step (seconds)
+        # the network was trained for (0: not recorded).
         self.scales = dict(depth=float(depth_scale), momentum=float(momentum_scale), length=float(length_scale),
                            flux=float(flux_scale), merge_radius=float(merge_radius),
-                           physics_base=1.0 if physics_base else 0.0)
+                           physics_base=1.0 if physics_base else 0.0, delta=float(delta))
         self.sigma = torch.tensor(np.asarray(sigma, dtype=np.float32), device="cuda")
         self.W, self.b = {}, {}
         for name, K, Nn in LAYERS:
@@ -346,7 +350,8 @@ class EdgeGNN:
             self.b[name] = torch.tensor(np.asarray(d[name + ".b"]), dtype=torch.float32, device="cuda")
         self.sigma = torch.tensor(np.asarray(d["sigma"], dtype=np.float32), device="cuda")
         for k in self.scales:
-            self.scales[k] = float(d["scale." + k])
+            if "scale." + k in d:  # files written before a scale existed keep its default
+                self.scales[k] = float(d["scale." + k])
         self.cast()
 
     def save(self, path):
@@ -701,15 +706,23 @@ def limit(net, graph, ws, delta, w, z):
     return ws.theta
 
 
+def protect(net, graph, w, uh, vh, z, h_dry=1e-3, froude_max=None):
+    """Zero momentum where the depth is below h_dry; elsewhere speeds capped
+    at froude_max sqrt(g h) (FROUDE_MAX unless given; 0: no cap)."""
+    fr = FROUDE_MAX if froude_max is None else froude_max
+    _ck(net.lib.gnn_protect(net.stream(), graph.N, _ptr(w), _ptr(z), _ptr(uh), _ptr(vh), float(h_dry), GRAVITY,
+                            float(fr)), "protect")
+
+
 def learned_step(net, graph, ws, delta, w, uh, vh, z, h_dry=1e-3):
     """One learned step in place (inference): edge outputs, positivity
-    limiter, update, then zero momentum where the depth is below h_dry."""
+    limiter, update, then zero momentum where the depth is below h_dry and
+    speeds capped (protect)."""
     forward(net, graph, ws, w, uh, vh, z, save=False)
     theta = limit(net, graph, ws, delta, w, z)
     apply(net, graph, ws, delta, w, uh, vh, w, uh, vh, theta=theta)
     if h_dry > 0.0:
-        _ck(net.lib.gnn_protect(net.stream(), graph.N, _ptr(w), _ptr(z), _ptr(uh), _ptr(vh), float(h_dry)),
-            "protect")
+        protect(net, graph, w, uh, vh, z, h_dry)
 
 
 def mass_balance(net, graph, ws, delta, w, uh, vh, z, steps, h_dry=H_DRY):
@@ -737,8 +750,7 @@ def mass_balance(net, graph, ws, delta, w, uh, vh, z, steps, h_dry=H_DRY):
         out = d * float(torch.sum(c))
         apply(net, g, ws, dl, w, uh, vh, w, uh, vh, theta=theta)
         if h_dry > 0.0:
-            _ck(net.lib.gnn_protect(net.stream(), g.N, _ptr(w), _ptr(z), _ptr(uh), _ptr(vh), float(h_dry)),
-                "protect")
+            protect(net, g, w, uh, vh, z, h_dry)
         m1 = float(torch.sum(g.area * (w - z)))
         worst = max(worst, abs(m1 - m0 + out) / abs(m0))
     return worst
@@ -1016,9 +1028,9 @@ def training_meshes(count, extent=1400.0, density=5000.0, seed=7, first=0):
             for i in range(first, first + count)]
 
 
-def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.0625, iters=3000, batch=16, lr=1e-3,
+def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.1, iters=3000, batch=16, lr=1e-3,
                          eval_points=500000, eval_steps=100, judge_every=8, threshold=0.01, extent=1400.0,
-                         merge_radius=2.5, horizon=None, save_net=None, load_net=None, judge_precision="mixed",
+                         merge_radius=2.5, horizon=None, save_net=None, load_net=None, judge_precision="fp64",
                          log=None, train_trajectories=0, max_train_meshes=400, both_judges=True):
     """Train on FV trajectories of configuration-2-style windows, then roll the
     network out on the configuration-2 workload (1M triangles) against the
@@ -1031,14 +1043,18 @@ def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.0625, 
 
     from . import scenarios
 
+    if load_net:
+        # a network trained earlier: its scales, merge radius and learned step
+        # come with it
+        net = EdgeGNN.load(load_net)
+        if net.scales["delta"] > 0.0:
+            delta = net.scales["delta"]
+        if log:
+            log("network loaded from %s (no training in this run; learned step %g s)" % (load_net, delta))
     horizon = eval_steps * delta if horizon is None else horizon
     train = dict(train_pairs=samples, train_horizon=horizon, train_iters=iters)
     if load_net:
-        # a network trained earlier (its scales and merge radius come with it)
-        net = EdgeGNN.load(load_net)
         train.update(loaded_network=os.path.basename(load_net))
-        if log:
-            log("network loaded from %s (no training in this run)" % load_net)
     else:
         t0 = time.time()
         data, first = [], 0
@@ -1053,7 +1069,7 @@ def run_learned_workload(train_meshes=40, per_mesh=50, samples=8, delta=0.0625, 
         if log:
             log("training data: %d trajectories x %d pairs on %d of %d windows of %d triangles (%.1f s)" % (
                 n_traj, samples, len(data), first, meshes[0].number_of_elements, t_data))
-        net = EdgeGNN(seed=1, merge_radius=merge_radius)
+        net = EdgeGNN(seed=1, merge_radius=merge_radius, delta=delta)
         fit_scales(net, data)
         tr = Trainer(net, data, batch=batch, lr=lr)
         untrained = tr.evaluate(16)
@@ -1112,7 +1128,7 @@ def learned_kernel_bytes(N, E):
     }
 
 
-def time_learned_step(points=500000, steps=20, delta=0.0625, merge_radius=2.5):
+def time_learned_step(points=500000, steps=20, delta=0.1, merge_radius=2.5):
     """Milliseconds per learned inference step on the seeded Delaunay workload
     of ``points`` points (an untrained network: the same kernels and cost as a
     trained one)."""

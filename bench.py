@@ -152,7 +152,8 @@ def main():
     ap.add_argument("--workload", default="fv", choices=["fv", "ensemble", "learned", "merimbula", "cpu-plan"],
                     help="fv: the 1M-triangle plain FV step (default); ensemble: the batched ensemble; "
                          "learned: train the edge-GNN surrogate on FV trajectories and roll it out at 1M")
-    ap.add_argument("--delta", type=float, default=0.0625, help="learned: seconds per learned step")
+    ap.add_argument("--delta", type=float, default=0.1,
+                    help="learned: seconds per learned step when training (a loaded network brings its own)")
     ap.add_argument("--merge-radius", type=float, default=2.5,
                     help="learned: inscribed radius (m) below which triangles are merged for the learned update")
     ap.add_argument("--train-iters", type=int, default=3000, help="learned: training iterations")
@@ -372,23 +373,24 @@ def main():
 def learned_rollout(args):
     """The learned form of the same workload: the committed network (trained
     by this package's kernels; provenance in anuga_cuda_amd/networks/)
-    rolled out for 100 learned steps of 0.0625 s from the workload's initial
-    state, the FV judge (--judge-precision: the FP64 or the mixed-precision
-    step) every 8 steps with the FP64 FV fallback above 1% stage L2 error,
-    against the FP64 FV path over the same 6.25 s.  speedup = FV wall time /
-    judged rollout wall time."""
+    rolled out for 100 learned steps of the length it was trained for (0.1 s)
+    from the workload's initial state, the FV judge (--judge-precision: the
+    FP64 or the mixed-precision step) every 8 steps with the FP64 FV fallback
+    above 1% stage L2 error, against the FP64 FV path over the same simulated
+    time.  speedup = FV wall time / judged rollout wall time."""
     from anuga_cuda_amd import learned, scenarios
 
     net = learned.EdgeGNN.load(args.network)
     dom = scenarios.delaunay_terrain(npoints=args.points, order=args.order)
-    res = learned.evaluate_rollout(net, dom, 100, 0.0625, judge_every=8, threshold=0.01,
+    res = learned.evaluate_rollout(net, dom, 100, net.scales["delta"] or 0.1, judge_every=8, threshold=0.01,
                                    judge_precision=args.judge_precision, log=log)
     dom.close()
     keys = ("speedup", "learned_rate", "fv_rate", "steps", "delta", "fv_steps", "final_stage_rel_l2",
             "max_stage_rel_l2", "mass_balance_rel", "mass_change_vs_fv", "fallback_windows", "judge_precision",
             "learned_step_ms", "fv_seconds", "learned_seconds")
     r = {k: res[k] for k in keys}
-    r.update(unit="FV-equivalent triangle-steps/s (triangles x FV steps for the same simulated time / s)",
+    r.update(froude_max=learned.FROUDE_MAX,
+             unit="FV-equivalent triangle-steps/s (triangles x FV steps for the same simulated time / s)",
              windows_judged=len(res["judged"]), network=os.path.relpath(args.network, ROOT),
              stage_error="area-weighted L2 of the stage error over the depth norm, against FP64 FV",
              mass_balance="(volume change + open-boundary outflow) / volume over one window of 8 learned steps")

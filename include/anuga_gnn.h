@@ -158,8 +158,11 @@ int gnn_base_flux(void *stream, int64_t E, const int32_t *ea, const int32_t *eb,
                   double g, double h_dry, float flux_scale, float *P);
 
 /* uh = vh = 0 where w - z < h_dry (the solver's protect, without its mass
- * change). */
-int gnn_protect(void *stream, int64_t N, const double *w, const double *z, double *uh, double *vh, double h_dry);
+ * change); elsewhere, if froude_max > 0, the speed |(uh, vh)| / h capped at
+ * froude_max * sqrt(g h) with its direction kept.  Returns -1 for g <= 0 or
+ * froude_max < 0. */
+int gnn_protect(void *stream, int64_t N, const double *w, const double *z, double *uh, double *vh, double h_dry,
+                double g, double froude_max);
 
 /* dF[Epad][32] (BF16) from gq: the transpose of gnn_apply's linear map (same
  * group arguments). */

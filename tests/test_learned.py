@@ -373,13 +373,50 @@ def test_mass_balance_with_open_boundaries():
     dom.close()
 
 
+def test_learned_protect_zeroes_dry_cells_and_caps_the_froude_number():
+    """gnn_protect: momentum zero below h_dry; above it, a speed over
+    froude_max sqrt(g h) is scaled down to exactly that with its direction
+    kept, slower cells and the stage are left alone; froude_max = 0 caps
+    nothing."""
+    import torch
+    from anuga_cuda_amd import learned as L
+
+    g, fr = 9.8, 5.0
+    z = torch.zeros(6, dtype=torch.float64, device="cuda")
+    h = torch.tensor([5e-4, 0.2, 0.2, 2.0, 2.0, 10.0], dtype=torch.float64, device="cuda")
+    speed = torch.tensor([3.0, 0.5, 40.0, 4.0, 400.0, 20.0], dtype=torch.float64, device="cuda")
+    ang = torch.tensor([0.3, 1.0, -2.0, 0.7, 2.5, -0.4], dtype=torch.float64, device="cuda")
+    w = z + h
+    uh0, vh0 = speed * h * torch.cos(ang), speed * h * torch.sin(ang)
+
+    class _G:
+        N = 6
+
+    net = L.EdgeGNN(seed=0)
+    for cap, expect_capped in ((fr, [False, False, True, False, True, False]), (0.0, [False] * 6)):
+        uh, vh, w1 = uh0.clone(), vh0.clone(), w.clone()
+        L.protect(net, _G, w1, uh, vh, z, h_dry=1e-3, froude_max=cap)
+        torch.cuda.synchronize()
+        assert torch.equal(w1, w)
+        assert float(uh[0]) == 0.0 and float(vh[0]) == 0.0
+        q = torch.sqrt(uh * uh + vh * vh)
+        qmax = fr * torch.sqrt(g * h) * h
+        for i in range(1, 6):
+            if expect_capped[i]:
+                assert abs(float(q[i] / qmax[i]) - 1.0) < 1e-14
+                assert abs(float(torch.atan2(vh[i], uh[i]) - torch.atan2(vh0[i], uh0[i]))) < 1e-12
+            else:
+                assert float(uh[i]) == float(uh0[i]) and float(vh[i]) == float(vh0[i])
+
+
 def test_trained_network_rollout_meets_the_error_and_mass_bar():
-    """The committed network (trained by this package's kernels on 2,398 FV
-    trajectories; provenance in anuga_cuda_amd/networks/edge_gnn_cfg5.txt)
-    on the 1,002,826-triangle workload: 100 learned steps of 0.0625 s, FV
-    judge every 8 steps with fallback above 1%.  Against the FP64 FV path over
-    the same 6.25 s the stage error (area-weighted L2 over the depth norm) stays
-    at most 1% at every window end, no window falls back, and a window of
+    """The committed network (trained by this package's kernels on 2,327 FV
+    trajectories of 0.1 s steps; provenance in
+    anuga_cuda_amd/networks/edge_gnn_cfg5.txt) on the 1,002,826-triangle
+    workload: 100 learned steps of the 0.1 s it was trained for, FV judge
+    every 8 steps with fallback above 1%.  Against the FP64 FV path over the
+    same 10 s the stage error (area-weighted L2 over the depth norm) stays at
+    most 1% at every window end, no window falls back, and a window of
     learned steps changes the volume only by its open-boundary outflow (to
     1e-12 relative).  The rollout is also faster than the FV path it replaces
     (the speed-up is measured and reported by bench.py)."""
@@ -389,7 +426,8 @@ def test_trained_network_rollout_meets_the_error_and_mass_bar():
 
     net = L.EdgeGNN.load(os.path.join(os.path.dirname(L.__file__), "networks", "edge_gnn_cfg5.npz"))
     dom = scenarios.delaunay_terrain(npoints=500000)
-    res = L.evaluate_rollout(net, dom, 100, 0.0625, judge_every=8, threshold=0.01, judge_precision="mixed")
+    assert net.scales["delta"] == 0.1
+    res = L.evaluate_rollout(net, dom, 100, net.scales["delta"], judge_every=8, threshold=0.01)
     dom.close()
     print("trained rollout: %.2fx FP64 FV, stage rel L2 max %.3g final %.3g, mass balance %.2e, %d fallbacks" % (
         res["speedup"], res["max_stage_rel_l2"], res["final_stage_rel_l2"], res["mass_balance_rel"],
