@@ -1621,9 +1621,9 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
 
 // Steps to launch before the next host check: the estimated number of steps
 // to the nearest yield or final time of the running instances (at the CFL
-// timestep of the last step, +10% and two steps of margin), in whole graphs,
-// at most kBatchSteps.  Steps queued past a yield only run the halted early
-// exits, so this bounds that waste; a low estimate costs another check.
+// timestep of the last step, +5% and two steps of margin), at most
+// kBatchSteps.  Steps queued past a yield only run the halted early exits, so
+// this bounds that waste; a low estimate costs another check.
 static int64_t batch_estimate(const sw_handle *h, const std::vector<DevState> &d) {
     int64_t est = 0;
     for (int i = 0; i < h->n_inst && i < (int)d.size(); i++) {
