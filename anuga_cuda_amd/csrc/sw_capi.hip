@@ -130,8 +130,10 @@ int dalloc(sw_handle *h, T **p, size_t count) {
                     hipGetErrorString(e));
     h->allocations.push_back(q);
     *p = static_cast<T *>(q);
-    e = hipMemset(q, 0, count * sizeof(T));
-    if (e != hipSuccess) return fail(h, SW_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    // zeroed on the handle's stream, so that its later work sees the zeros
+    // (a null-stream memset is not ordered with a non-blocking stream)
+    e = hipMemsetAsync(q, 0, count * sizeof(T), h->stream);
+    if (e != hipSuccess) return fail(h, SW_ERR_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
     return SW_OK;
 }
 
@@ -257,9 +259,19 @@ int current_states(sw_handle *h, std::vector<DevState> &out) {
     return states_error(h, out);
 }
 
+// A copy the host waits for, ordered on the handle's stream alone: a plain
+// hipMemcpy runs on the null stream and so would also wait for whatever the
+// caller has queued there (torch's default stream), e.g. learned steps that
+// were meant to keep the GPU busy while the library's next call is set up.
+hipError_t copy_wait(sw_handle *h, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e;
+}
+
 int put_state(sw_handle *h, int i, const DevState &d) {
     h->st_host_ok = false;
-    HIPCHECK(h, hipMemcpy(h->st + i, &d, sizeof d, hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, h->st + i, &d, sizeof d, hipMemcpyHostToDevice));
     return SW_OK;
 }
 
@@ -923,7 +935,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
     }
     auto put = [&](auto **dst, const auto &vec) -> int {
         if (dalloc(h, dst, vec.size())) return -1;
-        if (hipMemcpy(*dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess)
+        if (copy_wait(h, *dst, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice) != hipSuccess)
             return fail(h, SW_ERR_HIP, "mesh upload failed");
         return 0;
     };
@@ -1004,7 +1016,7 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         d.bed_interp = 1;
         d.bed_cont = 1;
     }
-    if (hipMemcpy(h->st, s0.data(), n_inst * sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) {
+    if (copy_wait(h, h->st, s0.data(), n_inst * sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) {
         fail(h, SW_ERR_HIP, "state upload failed");
         return bail();
     }
@@ -1081,9 +1093,9 @@ int sw_set_boundary(sw_handle *h, const int32_t *kind, const double *dirichlet) 
         k[b] = kind[b];
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy((void *)(h->mesh.bkind + I.boff), k.data(), I.nb * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, (void *)(h->mesh.bkind + I.boff), k.data(), I.nb * sizeof(int), hipMemcpyHostToDevice));
     if (dirichlet)
-        HIPCHECK(h, hipMemcpy((void *)(h->mesh.bdir + 3 * I.boff), dirichlet, 3 * I.nb * sizeof(double),
+        HIPCHECK(h, copy_wait(h, (void *)(h->mesh.bdir + 3 * I.boff), dirichlet, 3 * I.nb * sizeof(double),
                               hipMemcpyHostToDevice));
     return SW_OK;
 }
@@ -1102,9 +1114,9 @@ int sw_upload(sw_handle *h, int quantity, int location, const double *host, int6
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     if (L.count == 0) return SW_OK;
     if (!L.per_triangle) {
-        HIPCHECK(h, hipMemcpy(L.ptr, host, L.count * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(h, copy_wait(h, L.ptr, host, L.count * sizeof(double), hipMemcpyHostToDevice));
     } else {
-        HIPCHECK(h, hipMemcpy(h->staging, host, L.count * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(h, copy_wait(h, h->staging, host, L.count * sizeof(double), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_to_device, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
                            (long long)h->N, (long long)I.off, L.slots, L.comp, h->cal_of + I.off, h->staging, L.ptr);
         if (int rc = launch_check(h, "k_to_device")) return rc;
@@ -1142,13 +1154,13 @@ int sw_download(sw_handle *h, int quantity, int location, double *host, int64_t 
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     if (L.count == 0) return SW_OK;
     if (!L.per_triangle) {
-        HIPCHECK(h, hipMemcpy(host, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(h, copy_wait(h, host, L.ptr, L.count * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         hipLaunchKernelGGL(k_to_host, dim3(grid_for(L.count)), dim3(kBlock), 0, h->stream, (long long)I.n,
                            (long long)h->N, (long long)I.off, L.slots, L.comp, h->dev_of + I.poff, L.ptr, h->staging);
         if (int rc = launch_check(h, "k_to_host")) return rc;
         HIPCHECK(h, hipStreamSynchronize(h->stream));
-        HIPCHECK(h, hipMemcpy(host, h->staging, L.count * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(h, copy_wait(h, host, h->staging, L.count * sizeof(double), hipMemcpyDeviceToHost));
     }
     return SW_OK;
 }
@@ -1231,7 +1243,7 @@ int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, con
     d.event = SW_EVENT_NONE;
     d.error = 0;
     d.halt = 0;
-    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, h->st, &d, sizeof d, hipMemcpyHostToDevice));
     return sw_evolve_prepare(h, yieldstep, 0.0, 0);
 }
 
@@ -1258,7 +1270,7 @@ int sw_download_max_speed(sw_handle *h, double *host, int64_t count) {
                        (long long)I.off, 1, -1, h->dev_of + I.poff, h->f.max_speed, h->staging);
     if (int rc = launch_check(h, "k_to_host")) return rc;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy(host, h->staging, I.n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(h, copy_wait(h, host, h->staging, I.n * sizeof(double), hipMemcpyDeviceToHost));
     return SW_OK;
 }
 
@@ -1279,7 +1291,7 @@ int sw_set_state(sw_handle *h, const sw_state *s) {
     if (i < 0) return i;
     DevState d;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    HIPCHECK(h, hipMemcpy(&d, h->st + i, sizeof d, hipMemcpyDeviceToHost));
+    HIPCHECK(h, copy_wait(h, &d, h->st + i, sizeof d, hipMemcpyDeviceToHost));
     std::memcpy(&d, s, sizeof(sw_state));
     d.halt = (s->event != SW_EVENT_NONE) ? 1 : 0;
     return put_state(h, i, d);
@@ -1330,7 +1342,7 @@ int sw_update_timestep(sw_handle *h, double yieldstep, double finaltime, int32_t
     if (int rc = sync_state(h, 0, &d)) return rc;
     d.finaltime = finaltime;
     d.has_finaltime = has_finaltime;
-    HIPCHECK(h, hipMemcpy(h->st, &d, sizeof d, hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, h->st, &d, sizeof d, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_update_timestep, dim3(1), dim3(1), 0, h->stream, h->st, h->dp);
     if (int rc = launch_check(h, "update_timestep")) return rc;
     return sync_state(h, 0, &d);
@@ -1386,7 +1398,7 @@ int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *num
     }
     // processing order: by the device index of each point's first triangle
     std::vector<int> dev_of(I.n);
-    HIPCHECK(h, hipMemcpy(dev_of.data(), h->dev_of + I.poff, I.n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(h, copy_wait(h, dev_of.data(), h->dev_of + I.poff, I.n * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<std::pair<int, int>> keyed(P);
     for (int64_t p = 0; p < P; p++)
         keyed[p] = {start[p] < start[p + 1] ? dev_of[flat[start[p]] / 3] : -1, (int)p};
@@ -1402,9 +1414,9 @@ int sw_set_sww_points(sw_handle *h, int64_t number_of_points, const int64_t *num
     if (dalloc(h, &S.start, P + 1) || dalloc(h, &S.slot, nv) || dalloc(h, &S.order, P) || dalloc(h, &S.out, 4 * P))
         return SW_ERR_HIP;
     S.P = P;
-    HIPCHECK(h, hipMemcpy(S.order, order.data(), P * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(S.start, start.data(), (P + 1) * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(S.slot, flat.data(), nv * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, S.order, order.data(), P * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, S.start, start.data(), (P + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, S.slot, flat.data(), nv * sizeof(int), hipMemcpyHostToDevice));
     // caller vertex 3k+j -> device slot j*N + off + dev_of[k]
     hipLaunchKernelGGL(k_sww_slots, dim3(grid_for(nv)), dim3(kBlock), 0, h->stream, (int)nv, (long long)h->N,
                        (long long)I.off, h->dev_of + I.poff, S.slot);
@@ -1597,7 +1609,7 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
     std::vector<DevState> d;
     if (int rc = sync_all(h, d)) return rc;
     for (auto &s : d) s.order = h->params.default_order;
-    HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
     if (int rc = distribute_states(h, d, false)) return rc;
     for (auto &s : d) {
         s.time = s.starttime;
@@ -1615,7 +1627,7 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
         s.error = 0;
         s.halt = 0;
     }
-    HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
     return distribute_states(h, d, true);
 }
 
@@ -1777,7 +1789,7 @@ int sw_advance_yield(sw_handle *h) {
         s.halt = 0;
     }
     h->st_host_ok = false;
-    HIPCHECK(h, hipMemcpy(h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    HIPCHECK(h, copy_wait(h, h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
     h->st_host = d;
     h->st_host_ok = true;
     return SW_OK;
@@ -1874,7 +1886,7 @@ int sw_get_step_path(sw_handle *h, int32_t *path) {
     std::vector<DevState> d;
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     d.resize(h->n_inst);
-    HIPCHECK(h, hipMemcpy(d.data(), h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost));
+    HIPCHECK(h, copy_wait(h, d.data(), h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost));
     *path = vstep_applies(h, d) ? SW_STEP_VERTEX : SW_STEP_EDGE;
     return SW_OK;
 }
