@@ -26,6 +26,15 @@ namespace sw {
 // inputs and outputs): non-temporal, so they stream past the L2 and leave it to
 // the edge and centroid records that neighbouring triangles gather.
 template <typename T> SW_DEV T ld_s(const T *p) { return __builtin_nontemporal_load(p); }
+
+// Element idx of an array as its base plus a 32-bit byte offset: with a
+// uniform base the compiler can then use the global instructions' scalar-base
+// form instead of forming a 64-bit address in every lane.  Every array here is
+// far below 4 GiB.
+template <typename T> SW_DEV T *at32(T *base, uint32_t idx) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(base) + (uint64_t)(idx * (uint32_t)sizeof(T)));
+}
 template <typename T> SW_DEV void st_s(T *p, T v) { __builtin_nontemporal_store(v, p); }
 typedef double sw_f64x2 __attribute__((ext_vector_type(2)));
 SW_DEV double2 ld_s(const double2 *p) {

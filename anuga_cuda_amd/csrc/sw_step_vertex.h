@@ -174,25 +174,25 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
         V3 vq[3], na[3], nc[3];
         int nb[3], nd[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) nd[i] = ld_s(M.tri_node + i * N + k);
+        for (int i = 0; i < 3; i++) nd[i] = ld_s(at32(M.tri_node, (uint32_t)(i * N + k)));
 #pragma unroll
-        for (int i = 0; i < 3; i++) vq[i] = S.vr[i * N + k];
+        for (int i = 0; i < 3; i++) vq[i] = *at32(S.vr, (uint32_t)(i * N + k));
 #pragma unroll
-        for (int i = 0; i < 3; i++) nb[i] = ld_s(nbw + i * N + k);
-        const int flags = ld_s(M.flags + k);
-        const double avg_h = ld_s(S.hc + k);
+        for (int i = 0; i < 3; i++) nb[i] = ld_s(at32(nbw, (uint32_t)(i * N + k)));
+        const int flags = ld_s(at32(M.flags, (uint32_t)k));
+        const double avg_h = ld_s(at32(S.hc, (uint32_t)k));
         // inscribed radius and area (the caller's arrays, which sw_create
         // checked tri_geometry reproduces: reading them is cheaper than the
         // three square roots and two divisions that form them)
-        const double2 ra = ld_s(M.ra + k);
+        const double2 ra = ld_s(at32(M.ra, (uint32_t)k));
         // the coordinates and bed of the triangle's nodes (every vertex slot
         // of a node holds its value, checked when the bed is uploaded)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double2 v = M.node_xy[nd[i]];
+            const double2 v = *at32(M.node_xy, (uint32_t)nd[i]);
             x[i] = v.x;
             y[i] = v.y;
-            zv[i] = F.node_z[nd[i]];
+            zv[i] = *at32(F.node_z, (uint32_t)nd[i]);
         }
         // the two vertex records of the neighbour's matching edge (a boundary
         // edge reads its own, which are not used)
@@ -203,8 +203,8 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
                 n = nb[i] & 0x1FFFFFFF;
                 m = nb[i] >> 29;
             }
-            na[i] = S.vr[next3(m) * N + n];
-            nc[i] = S.vr[prev3(m) * N + n];
+            na[i] = *at32(S.vr, (uint32_t)(next3(m) * N + n));
+            nc[i] = *at32(S.vr, (uint32_t)(prev3(m) * N + n));
         }
         TriGeom gm;
         tri_geometry(x, y, gm);
@@ -299,8 +299,8 @@ __global__ __launch_bounds__(BLOCK, SW_FLUX_MINWAVES) void k_flux_v(DevMesh M, D
         s1 += -sidex / area;
         s2 += -sidey / area;
         if (!halted) {
-            S.eur[k] = E3{s0, s1, s2};
-            st_s(F.max_speed + k, max_speed_total);
+            *at32(S.eur, (uint32_t)k) = E3{s0, s1, s2};
+            st_s(at32(F.max_speed, (uint32_t)k), max_speed_total);
         }
     }
     // workgroup minimum of the CFL candidates, one atomic into a slot
