@@ -62,6 +62,15 @@ GNN_DEV void wave_tiles(int64_t &t0, int64_t &step) {
 GNN_DEV f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 GNN_DEV bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 GNN_DEV void st8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
+// The same, element idx of a row array as its base plus a 32-bit byte offset:
+// with a uniform base the compiler uses the global instructions' scalar-base
+// form instead of a 64-bit address per lane (the arrays are below 4 GiB).
+GNN_DEV bf16x8 ld8_at(const bf16 *base, uint32_t idx) {
+    return *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(base) + (uint64_t)(idx * 2u));
+}
+GNN_DEV void st8_at(bf16 *base, uint32_t idx, bf16x8 v) {
+    *reinterpret_cast<bf16x8 *>(reinterpret_cast<char *>(base) + (uint64_t)(idx * 2u)) = v;
+}
 
 // ---------------------------------------------------------------------------
 // Edge features of the current state, in the edge's normal frame (zero for
@@ -419,8 +428,8 @@ __global__ __launch_bounds__(kB) void k_mp_direct(int64_t E, int64_t Epad, const
     int4 nl0 = make_int4(-1, -1, -1, -1), nl1 = make_int4(-1, -1, -1, -1);
     if (t0 < t1) {
         const int64_t e = t0 * 16 + r;
-        h0 = ld8(H + e * 64 + g * 8);
-        h1 = ld8(H + e * 64 + 32 + g * 8);
+        h0 = ld8_at(H, (uint32_t)(e * 64 + g * 8));
+        h1 = ld8_at(H, (uint32_t)(e * 64 + 32 + g * 8));
         nl0 = reinterpret_cast<const int4 *>(nbr6)[2 * e];
         nl1 = reinterpret_cast<const int4 *>(nbr6)[2 * e + 1];
     }
@@ -433,14 +442,14 @@ __global__ __launch_bounds__(kB) void k_mp_direct(int64_t E, int64_t Epad, const
         bf16x8 n0[6], n1[6];
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            n0[q] = nbr[q] >= 0 ? ld8(H + (int64_t)nbr[q] * 64 + g * 8) : zero8;
-            n1[q] = nbr[q] >= 0 ? ld8(H + (int64_t)nbr[q] * 64 + 32 + g * 8) : zero8;
+            n0[q] = nbr[q] >= 0 ? ld8_at(H, (uint32_t)nbr[q] * 64u + g * 8) : zero8;
+            n1[q] = nbr[q] >= 0 ? ld8_at(H, (uint32_t)nbr[q] * 64u + 32 + g * 8) : zero8;
         }
         const int64_t tn = t + step;
         if (tn < t1) {
             const int64_t en = tn * 16 + r;
-            h0 = ld8(H + en * 64 + g * 8);
-            h1 = ld8(H + en * 64 + 32 + g * 8);
+            h0 = ld8_at(H, (uint32_t)(en * 64 + g * 8));
+            h1 = ld8_at(H, (uint32_t)(en * 64 + 32 + g * 8));
             nl0 = reinterpret_cast<const int4 *>(nbr6)[2 * en];
             nl1 = reinterpret_cast<const int4 *>(nbr6)[2 * en + 1];
         }
@@ -468,8 +477,8 @@ __global__ __launch_bounds__(kB) void k_mp_direct(int64_t E, int64_t Epad, const
             o1[i] = (bf16)(y1[i] + (float)c1[i]);
         }
         if (Hout) {
-            st8(Hout + e * 64 + 8 * g, o0);
-            st8(Hout + e * 64 + 32 + 8 * g, o1);
+            st8_at(Hout, (uint32_t)(e * 64 + 8 * g), o0);
+            st8_at(Hout, (uint32_t)(e * 64 + 32 + 8 * g), o1);
         }
         if (WTh) {
             f32x4 ah = mfma(o0, bh0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
