@@ -95,6 +95,7 @@ struct sw_handle {
     std::vector<DevState> st_host;
     bool st_host_ok = false;
     DevState *st_pinned = nullptr;
+    DevState *st_up = nullptr;  // [2][n_inst] pinned staging of state writes that are not waited for
 };
 
 namespace {
@@ -990,6 +991,11 @@ sw_handle *create_impl(const sw_mesh *meshes, int n_inst, const sw_params *p, in
         fail(h, SW_ERR_HIP, "hipHostMalloc of the state buffer failed");
         return bail();
     }
+    if (hipHostMalloc((void **)&h->st_up, 2 * n_inst * sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+        h->st_up = nullptr;
+        fail(h, SW_ERR_HIP, "hipHostMalloc of the state staging buffer failed");
+        return bail();
+    }
     h->vstep_geometry = own_normals && geom_same && Ntot < ((int64_t)1 << 29);
     {
         std::vector<int> tri_node;
@@ -1056,6 +1062,7 @@ int sw_destroy(sw_handle *h) {
         if (e) (void)hipEventDestroy(e);
     for (void *p : h->allocations) (void)hipFree(p);
     if (h->st_pinned) (void)hipHostFree(h->st_pinned);
+    if (h->st_up) (void)hipHostFree(h->st_up);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return SW_OK;
@@ -1226,6 +1233,9 @@ int sw_download_device(sw_handle *h, int quantity, int location, double *ddst, i
     return SW_OK;
 }
 
+static int prepare_loop(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime,
+                        const double *restart_time);
+
 int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, const double *ymom, double time,
                       double yieldstep) {
     if (!h || !stage || !xmom || !ymom) return SW_ERR_ARG;
@@ -1235,16 +1245,9 @@ int sw_restart_device(sw_handle *h, const double *stage, const double *xmom, con
     if (int rc = upload_async(h, 0, SW_STAGE, SW_CENTROID, stage, n)) return rc;
     if (int rc = upload_async(h, 0, SW_XMOM, SW_CENTROID, xmom, n)) return rc;
     if (int rc = upload_async(h, 0, SW_YMOM, SW_CENTROID, ymom, n)) return rc;
-    DevState d;  // read after the uploads; a latched error of an earlier run is cleared below
-    HIPCHECK(h, hipMemcpyAsync(&d, h->st, sizeof d, hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
-    d.time = time;
-    d.starttime = time;
-    d.event = SW_EVENT_NONE;
-    d.error = 0;
-    d.halt = 0;
-    HIPCHECK(h, copy_wait(h, h->st, &d, sizeof d, hipMemcpyHostToDevice));
-    return sw_evolve_prepare(h, yieldstep, 0.0, 0);
+    // the start time set and a latched error of an earlier run cleared with
+    // the preamble's own state write
+    return prepare_loop(h, yieldstep, 0.0, 0, &time);
 }
 
 int sw_download_conserved_device(sw_handle *h, double *stage, double *xmom, double *ymom) {
@@ -1571,7 +1574,7 @@ int sw_update_other_quantities(sw_handle *h) {
 // protect, then distribute_to_vertices_and_edges of the edge-record path on
 // every instance (and the boundary values if `boundary`); d receives the
 // states after it (one read).
-int distribute_states(sw_handle *h, std::vector<DevState> &d, bool boundary) {
+int distribute_states(sw_handle *h, std::vector<DevState> &d, bool boundary, bool read_back = true) {
     const unsigned g = grid_for(h->N);
     hipLaunchKernelGGL(k_protect, dim3(g), dim3(kBlock), 0, h->stream, h->mesh, h->f, h->dp);
     const bool limited = h->params.limiter != SW_LIMITER_OPTIMISED;
@@ -1593,24 +1596,58 @@ int distribute_states(sw_handle *h, std::vector<DevState> &d, bool boundary) {
     if (boundary && h->Nb > 0)
         hipLaunchKernelGGL(k_update_boundary, dim3(grid_for(h->Nb)), dim3(kBlock), 0, h->stream, h->mesh, h->f);
     if (int rc = launch_check(h, "distribute")) return rc;
-    return sync_all(h, d);
+    return read_back ? sync_all(h, d) : SW_OK;
 }
 
-int distribute_all(sw_handle *h) {
-    std::vector<DevState> d;
-    return distribute_states(h, d, false);
+// The host's states to the device through staging slot `slot` (0 or 1) of
+// st_up, without waiting: the two slots let one call make two such writes
+// before its next wait, and every call that writes ends with a wait on the
+// stream, so a slot is free again when the next call fills it.
+static int put_states_async(sw_handle *h, const std::vector<DevState> &d, int slot) {
+    DevState *stage = h->st_up + slot * h->n_inst;
+    std::memcpy(stage, d.data(), h->n_inst * sizeof(DevState));
+    HIPCHECK(h, hipMemcpyAsync(h->st, stage, h->n_inst * sizeof(DevState), hipMemcpyHostToDevice, h->stream));
+    h->st_host_ok = false;
+    return SW_OK;
 }
 
-int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
-    if (!h) return SW_ERR_ARG;
-    if (int rc = touch_api(h)) return rc;
-    if (h->n_inst > 1 && h->cur >= 0)
-        return fail(h, SW_ERR_STATE, "an ensemble is prepared as a whole (select instance -1)");
+// evolve()'s preamble (gpu_domain_advanced.py:2473-2545): distribute with the
+// default order, reset the time bookkeeping, distribute again and evaluate
+// the boundary.  With restart_time the instances also start at that time
+// with no event, error or halt latched (sw_restart_device).  One state read
+// at the start and one at the end: the first distribute can only latch the
+// neighbour error, which the reset clears and the second distribute, on the
+// same values, latches again.
+static int prepare_loop(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime,
+                        const double *restart_time) {
     std::vector<DevState> d;
-    if (int rc = sync_all(h, d)) return rc;
-    for (auto &s : d) s.order = h->params.default_order;
-    HIPCHECK(h, copy_wait(h, h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
-    if (int rc = distribute_states(h, d, false)) return rc;
+    if (restart_time) {
+        // as they stand: a latched error of an earlier run is what a restart
+        // clears, so it is not reported
+        if (h->st_host_ok) {
+            d = h->st_host;
+        } else {
+            d.resize(h->n_inst);
+            HIPCHECK(h, hipMemcpyAsync(h->st_pinned, h->st, h->n_inst * sizeof(DevState), hipMemcpyDeviceToHost,
+                                       h->stream));
+            HIPCHECK(h, hipStreamSynchronize(h->stream));
+            std::memcpy(d.data(), h->st_pinned, h->n_inst * sizeof(DevState));
+        }
+    } else if (int rc = sync_all(h, d)) {
+        return rc;
+    }
+    for (auto &s : d) {
+        s.order = h->params.default_order;
+        if (restart_time) {
+            s.time = *restart_time;
+            s.starttime = *restart_time;
+            s.event = SW_EVENT_NONE;
+            s.error = 0;
+            s.halt = 0;
+        }
+    }
+    if (int rc = put_states_async(h, d, 0)) return rc;
+    if (int rc = distribute_states(h, d, false, false)) return rc;
     for (auto &s : d) {
         s.time = s.starttime;
         s.yieldstep = yieldstep;
@@ -1627,8 +1664,21 @@ int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t 
         s.error = 0;
         s.halt = 0;
     }
-    HIPCHECK(h, copy_wait(h, h->st, d.data(), h->n_inst * sizeof(DevState), hipMemcpyHostToDevice));
+    if (int rc = put_states_async(h, d, 1)) return rc;
     return distribute_states(h, d, true);
+}
+
+int distribute_all(sw_handle *h) {
+    std::vector<DevState> d;
+    return distribute_states(h, d, false);
+}
+
+int sw_evolve_prepare(sw_handle *h, double yieldstep, double finaltime, int32_t has_finaltime) {
+    if (!h) return SW_ERR_ARG;
+    if (int rc = touch_api(h)) return rc;
+    if (h->n_inst > 1 && h->cur >= 0)
+        return fail(h, SW_ERR_STATE, "an ensemble is prepared as a whole (select instance -1)");
+    return prepare_loop(h, yieldstep, finaltime, has_finaltime, nullptr);
 }
 
 // Steps to launch before the next host check: the estimated number of steps
