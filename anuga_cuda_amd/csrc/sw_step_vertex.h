@@ -523,26 +523,26 @@ __global__ __launch_bounds__(BLOCK, SW_EXT_MINWAVES) void k_extrap_v(DevMesh M, 
     // own inputs, then the surrogate neighbours' updated centroid records and
     // centroids and the nodes' coordinates and bed, before anything is used
     RecIn r;
-    r.c = ldq(F.cq + kk);
-    r.flags = ld_s(M.flags + kk);
-    r.xy = M.cc[kk];
+    r.c = ldq(at32(F.cq, (uint32_t)kk));
+    r.flags = ld_s(at32(M.flags, (uint32_t)kk));
+    r.xy = *at32(M.cc, (uint32_t)kk);
     int nd[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) nd[i] = ld_s(M.tri_node + i * N + kk);
+    for (int i = 0; i < 3; i++) nd[i] = ld_s(at32(M.tri_node, (uint32_t)(i * N + kk)));
 #pragma unroll
-    for (int i = 0; i < 3; i++) r.sk[i] = ld_s(M.sur + i * N + kk);
+    for (int i = 0; i < 3; i++) r.sk[i] = ld_s(at32(M.sur, (uint32_t)(i * N + kk)));
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        r.cs[i] = ldq(F.cq + r.sk[i]);
-        r.xs[i] = M.cc[r.sk[i]];
+        r.cs[i] = ldq(at32(F.cq, (uint32_t)r.sk[i]));
+        r.xs[i] = *at32(M.cc, (uint32_t)r.sk[i]);
     }
     double x[3], y[3], zv[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const double2 v = M.node_xy[nd[i]];
+        const double2 v = *at32(M.node_xy, (uint32_t)nd[i]);
         x[i] = v.x;
         y[i] = v.y;
-        zv[i] = F.node_z[nd[i]];
+        zv[i] = *at32(F.node_z, (uint32_t)nd[i]);
     }
     const DevState &s2 = S.stv[p ^ 1][bi.inst];  // the state after this step's update
     if (!valid || s2.halt || s2.error) return;
